@@ -1,0 +1,185 @@
+"""Benchmark of the MsFEM basis build on one MI355X.
+
+Workload (default): the batched plain basis build on the unit square, 64x64
+coarse Q1 cells, each refined to a 64x64 fine Q1 grid (4225 nodes, 4 basis
+problems per cell, 16384 local solves), alpha = exp(g) with g ~ N(0,1) i.i.d.
+per fine cell (Philox, seed 2) clipped to contrast 1e3, f = 1, zero Dirichlet.
+
+One step = every cell's 4 basis solves + A^K, F^K + the coarse assembly, with
+all inputs already resident in HBM.  The dominant kernel (the local solves) is
+timed with HIP events on the stream it runs on; algorithmic work per CG
+iteration is F_iter = (2*9 - 1 + 10) * N_I flops (9-point stencil + CG vector
+updates) and B_iter = 11 N w + n^2 w_c bytes.
+
+The CPU baseline is the oracle's restatement of the reference's own recipe
+(CG + SSOR(1.6), absolute tolerance 1e-12, 1000 iterations;
+include/base/diffusion_problem_basis.tpp:295-317) on a sample of the same cells,
+one OpenMP thread per available core.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "mpi-parallel-multiscale-diffusion-fem_amd")
+for p in (PKG, os.path.join(ROOT, "tests")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def workload(name):
+    import msfem_cases as C
+    if name == "lognormal64":
+        return C.lognormal_cells(dim=2, nc=64, nf=64, seed=2), 64
+    if name == "lognormal3d":
+        return C.lognormal_cells(dim=3, nc=16, nf=16, seed=5), 16
+    raise SystemExit(f"unknown workload {name}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="lognormal64")
+    ap.add_argument("--cpu-sample", type=int, default=64, help="cells in the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: the local solves of each coarse cell are independent and "
+              "the multi-GPU split (cells sharded, one coarse-system reduction) is not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import msfem
+
+    dev = torch.device("cuda:0")
+    case, nc = workload(args.workload)
+    dim = case["dim"]
+    n = len(case["n_fine"])
+    nf = int(case["n_fine"][0])
+    nb = 1 << dim
+    nn = (nf + 1) ** dim
+    ni = (nf - 1) ** dim
+    ncf = nf ** dim
+
+    t_box = torch.from_numpy(case["box"].reshape(-1).copy()).to(dev)
+    t_coef = torch.from_numpy(case["coef"]).to(dev)
+    t_Ae = torch.zeros(n * nb * nb, dtype=torch.float64, device=dev)
+    t_Fe = torch.zeros(n * nb, dtype=torch.float64, device=dev)
+    t_bases = torch.zeros(n * nb * nn, dtype=torch.float64, device=dev)
+    t_it = torch.zeros(n * nb, dtype=torch.int32, device=dev)
+    t_st = torch.zeros(n * nb, dtype=torch.int32, device=dev)
+    ncn = (nc + 1) ** dim
+    t_Kc = torch.zeros(ncn * 3 ** dim, dtype=torch.float64, device=dev)
+    t_Fc = torch.zeros(ncn, dtype=torch.float64, device=dev)
+    t_cm = torch.zeros(ncn, dtype=torch.uint8, device=dev)
+    t_cv = torch.zeros(ncn, dtype=torch.float64, device=dev)
+    t_u = torch.zeros(ncn, dtype=torch.float64, device=dev)
+    t_cit = torch.zeros(1, dtype=torch.int32, device=dev)
+
+    ctx = msfem.Context(0)
+    ctx.set_profiling(True)
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step(opts):
+        ctx.batch_build_device(dim, case["n_fine"], t_box.data_ptr(), case["coef_kind"], t_coef.data_ptr(),
+                               t_Ae.data_ptr(), t_Fe.data_ptr(), t_bases.data_ptr(), t_it.data_ptr(), t_st.data_ptr(),
+                               rhs_kind=case["rhs_kind"], rhs_value=case["rhs_value"], opts=opts, stream=stream)
+        ctx.coarse_assemble_device(dim, nc, t_Ae.data_ptr(), t_Fe.data_ptr(), msfem.BC_ZERO_DIRICHLET, t_Kc.data_ptr(),
+                                   t_Fc.data_ptr(), t_cm.data_ptr(), t_cv.data_ptr(), stream=stream)
+
+    def run(opts, steps, warmup):
+        for _ in range(warmup):
+            step(opts)
+        torch.cuda.synchronize()
+        solve_ms = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            step(opts)
+            solve_ms.append(ctx.profile()[0])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        assert int(t_st.sum().item()) == 0, "local solves did not converge"
+        return (t1 - t0) * 1e3 / steps, float(np.mean(solve_ms)), t_it.cpu().numpy().astype(np.int64)
+
+    # plain FP64 path (the parity path)
+    o64 = msfem.default_opts(precision=msfem.PREC_FP64)
+    ms64, solve64, it64 = run(o64, args.steps, args.warmup)
+    # FP32 CG + FP64 refinement stopped where the energy-norm error is <= 1e-6
+    o32 = msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+    ms32, solve32, it32 = run(o32, args.steps, args.warmup)
+    # coarse solve (timed apart: one single-workgroup FP64 CG)
+    step(o64)
+    torch.cuda.synchronize()
+    tc0 = time.perf_counter()
+    ctx.coarse_solve_device(dim, nc, t_Kc.data_ptr(), t_Fc.data_ptr(), t_cm.data_ptr(), t_cv.data_ptr(), 1e-12,
+                            100000, t_u.data_ptr(), t_cit.data_ptr(), stream=stream)
+    torch.cuda.synchronize()
+    coarse_ms = (time.perf_counter() - tc0) * 1e3
+
+    s = 3 ** dim
+    f_iter = (2 * s - 1 + 10) * ni
+    b_iter = 11 * nn * 8 + ncf * 8
+    flops64 = float(it64.sum()) * f_iter
+    achieved = flops64 / (solve64 * 1e-3) / 1e12
+    b_cell = ncf * 8 + nb * nn * 8 + (nb * nb + nb) * 8
+    cells_per_s = n / (ms64 * 1e-3)
+
+    result = {
+        "metric": "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err",
+        "value": round(cells_per_s, 1),
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox-4x32-10 lognormal field, generated on the host, resident in HBM)",
+        "config": {"workload": f"{args.workload}: {nc}^{dim} coarse Q1 cells x {nf}^{dim} fine Q1 per cell, "
+                               f"{n * nb} local solves, FP64, rel. tol 1e-13",
+                   "coarse_cells": n, "fine_per_cell": nf, "dim": dim, "local_solves": n * nb},
+        "roofline": {"bound": "valu", "kernel": "local_solve_2d (FP64 CG)" if dim == 2 else "local_solve_3d",
+                     "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "kernel_ms": round(solve64, 4), "flops_per_launch": flops64,
+                     "lds_model_TBps": round(float(it64.sum()) * b_iter / (solve64 * 1e-3) / 1e12, 2),
+                     "hbm_GBps": round(n * b_cell / (ms64 * 1e-3) / 1e9, 2)},
+        "iterations": {"fp64_mean": round(float(it64.mean()), 1), "fp64_p99": int(np.percentile(it64, 99)),
+                       "fp32_refine_mean": round(float(it32.mean()), 1),
+                       "fp32_refine_p99": int(np.percentile(it32, 99))},
+        "variants": {"fp64_cells_per_s": round(cells_per_s, 1),
+                     "fp32_refine_1e-6_energy_cells_per_s": round(n / (ms32 * 1e-3), 1),
+                     "fp32_refine_ms_per_step": round(ms32, 4), "fp32_refine_solve_kernel_ms": round(solve32, 4),
+                     "coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.item())},
+    }
+
+    if not args.no_cpu_baseline:
+        import msfem_cases as C
+        sample = np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64)
+        sub = C.lognormal_cells(dim=dim, nc=nc, nf=nf, seed=2 if dim == 2 else 5, which=sample)
+        cores = len(os.sched_getaffinity(0))
+        tcpu0 = time.perf_counter()
+        r = C.oracle_build(sub, want_bases=False, solver=1, n_threads=cores)
+        tcpu = time.perf_counter() - tcpu0
+        result["cpu_baseline"] = {
+            "value": round(len(sample) / tcpu, 2), "unit": "cells/s", "cores": cores, "kind": "port",
+            "sample": f"{len(sample)} cells spread over the same batch, reference recipe CG+SSOR(1.6), abs tol 1e-12,"
+                      f" max 1000 it (fails: {r['fails']}, mean it {r['iters'].mean():.1f}), {tcpu:.1f} s wall"}
+    ctx.close()
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

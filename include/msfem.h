@@ -1,0 +1,233 @@
+/*
+ * msfem.h -- C ABI of the MI355X MsFEM basis engine (libmsfem_hip.so).
+ *
+ * The engine replaces the per-coarse-cell local loop of
+ * konsim83/MPI-Parallel-Multiscale-Diffusion-FEM:
+ *
+ *   DiffusionProblemBasis<dim>::run()              include/base/diffusion_problem_basis.tpp:440-474
+ *     make_grid / setup_system / assemble_system   diffusion_problem_basis.tpp:92-242
+ *     2^dim constrained basis solves (CG+SSOR)     diffusion_problem_basis.tpp:295-317, :450-465
+ *     assemble_global_element_matrix               diffusion_problem_basis.tpp:247-285
+ *   get_global_element_matrix / _rhs               diffusion_problem_basis.tpp:320-334
+ *   called per locally owned cell from
+ *   DiffusionProblemMultiscale::initialize_and_compute_basis  include/base/diffusion_problem_ms.tpp:42-88
+ *
+ * and the coarse system around it:
+ *   DiffusionProblemMultiscale::assemble_system    diffusion_problem_ms.tpp:161-255
+ *   DiffusionProblemMultiscale::solve_iterative    diffusion_problem_ms.tpp:260-296
+ *   send_global_weights_to_cell / set_global_weights  diffusion_problem_ms.tpp:301-325,
+ *                                                  diffusion_problem_basis.tpp:354-377
+ *
+ * One call builds a whole batch of coarse cells on one GPU.  All functions
+ * return an int status (MSFEM_OK on success), never throw, and keep the
+ * message of the last failure in msfem_last_error(ctx).  Caller buffers are
+ * borrowed for the duration of the call.  One host thread per context.
+ *
+ * Data conventions (shared by every entry point):
+ *   - dim is 2 or 3; nb = 2^dim bases per coarse cell; basis i belongs to
+ *     coarse vertex i, lexicographic with x fastest (deal.II vertex order).
+ *   - a coarse cell is an axis-aligned box: box[c] = {lo_x, hi_x, lo_y, hi_y
+ *     [, lo_z, hi_z]}.
+ *   - a cell with nf fine cells per side has (nf+1)^dim fine nodes,
+ *     lexicographic, x fastest.  Its bases are stored as [nb][(nf+1)^dim],
+ *     cells concatenated in batch order.
+ *   - coefficient tiles are concatenated in batch order, one per cell, AoS per
+ *     fine cell (fine cells lexicographic):
+ *       MSFEM_COEF_SCALAR_CELL  [nf^d]            one value per fine cell
+ *       MSFEM_COEF_TENSOR_CELL  [nf^d][ns]        symmetric tensor per fine cell
+ *       MSFEM_COEF_SCALAR_QP    [nf^d][2^d]       scalar at the QGauss(2) points
+ *       MSFEM_COEF_TENSOR_QP    [nf^d][2^d][ns]   tensor at the QGauss(2) points
+ *     with ns = 3 (a11,a12,a22) in 2D and 6 (a11,a12,a13,a22,a23,a33) in 3D,
+ *     quadrature points tensor-ordered, x fastest.  The QP kinds reproduce the
+ *     reference's MatrixCoeff::value_list at the quadrature points
+ *     (include/coefficients/matrix_coeff.tpp:66-91).
+ */
+#ifndef MSFEM_H_
+#define MSFEM_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MSFEM_API_VERSION 1
+
+/* status codes */
+#define MSFEM_OK 0
+#define MSFEM_ERR_ARG 1         /* invalid argument or unsupported shape */
+#define MSFEM_ERR_HIP 2         /* a HIP runtime call failed */
+#define MSFEM_ERR_NOCONV 3      /* some local or coarse solve did not converge */
+#define MSFEM_ERR_NODEVICE 4    /* no usable GPU */
+
+/* coefficient kinds */
+#define MSFEM_COEF_SCALAR_CELL 0
+#define MSFEM_COEF_TENSOR_CELL 1
+#define MSFEM_COEF_SCALAR_QP 2
+#define MSFEM_COEF_TENSOR_QP 3
+
+/* right-hand side kinds (load vector b, hence F^K) */
+#define MSFEM_RHS_CONST 0       /* f = rhs_value (reference default: 2.0,
+                                   include/coefficients/right_hand_side.tpp:17-40) */
+#define MSFEM_RHS_SINSIN 1      /* f = prod_k sin(pi x_k), evaluated at QGauss(2) points */
+
+/* arithmetic of the local solves */
+#define MSFEM_PREC_FP64 0         /* plain FP64 CG */
+#define MSFEM_PREC_FP32_REFINE 1  /* FP32 CG inside FP64 residual refinement */
+
+/* coarse boundary conditions */
+#define MSFEM_BC_ZERO_DIRICHLET 0 /* u = 0 on every face */
+#define MSFEM_BC_REFERENCE 1      /* reference default: Dirichlet g=(x-1/2)^2+(y-1/2)^2 on
+                                     x=0,y=0(,z=0), Neumann h=cos(2 PI_D x)cos(2 PI_D y) on
+                                     x=1,y=1(,z=1); diffusion_problem_ms.tpp:128-137,:206-235 */
+
+typedef struct msfem_ctx msfem_ctx;
+
+/* Local-solve options.  Stopping rule of every local solve: the true FP64
+ * residual of the interior equations satisfies
+ *     ||r||_2 <= max(abs_tol, rel_tol * ||b||_2),   b = -K_IB g_i.
+ * The reference stops at ||r||_2 <= 1e-12 (absolute), 1000 iterations
+ * (include/base/diffusion_problem_basis.tpp:297). */
+typedef struct msfem_solver_opts {
+  int32_t precision;          /* MSFEM_PREC_* */
+  int32_t max_iter;           /* cap on inner CG iterations per basis, summed over cycles */
+  int32_t max_refine;         /* cap on FP64 residual-refinement cycles */
+  int32_t partition_of_unity; /* 1: solve nb-1 bases, phi_{nb-1} = 1 - sum of the others */
+  double rel_tol;
+  double abs_tol;
+  double inner_rel_tol;       /* FP32 inner solve: residual reduction per cycle */
+  int32_t variant;            /* launch-shape table entry: 0 = built-in default,
+                                 >0 = explicit variant id (see msfem_variants) */
+  int32_t reserved[7];
+} msfem_solver_opts;
+
+typedef struct msfem_batch {
+  int32_t dim;
+  int32_t n_cells;
+  const int32_t *n_fine;      /* HOST array [n_cells]: fine cells per side (2D: 16,32,64,128;
+                                 3D: 8,16) */
+  const double *box;          /* [n_cells][dim][2] */
+  int32_t coef_kind;          /* MSFEM_COEF_* */
+  const double *coef;         /* concatenated coefficient tiles */
+  int32_t rhs_kind;           /* MSFEM_RHS_* */
+  double rhs_value;
+  const double *x0;           /* optional initial bases (bases layout) or NULL */
+} msfem_batch;
+
+typedef struct msfem_batch_out {
+  double *Ae;       /* [n_cells][nb][nb]  A^K_ij = phi_i^T K phi_j */
+  double *Fe;       /* [n_cells][nb]      F^K_i  = phi_i^T b       */
+  double *bases;    /* optional (NULL): [sum_c nb (nf_c+1)^dim] basis nodal values */
+  int32_t *iters;   /* optional: [n_cells][nb] inner CG iterations */
+  int32_t *status;  /* optional: [n_cells][nb] 0 converged, 1 not converged */
+  double *resid;    /* optional: [n_cells][nb] final true residual norm */
+} msfem_batch_out;
+
+/* ---- context --------------------------------------------------------- */
+/* Replaces: the per-rank state of DiffusionProblemMultiscale (diffusion_problem_ms.tpp:18-37). */
+int msfem_ctx_create(int device, msfem_ctx **out);
+int msfem_ctx_destroy(msfem_ctx *ctx);
+const char *msfem_last_error(const msfem_ctx *ctx); /* ctx may be NULL (creation errors) */
+void msfem_default_opts(msfem_solver_opts *opts);
+int msfem_version(void);
+
+/* Kernel timing of the batch builds (HIP events on the build's stream around
+ * the local-solve and element launches).  After msfem_ctx_set_profiling(ctx, 1),
+ * msfem_ctx_profile returns the device milliseconds of the last completed
+ * build (it waits for that build's events).  No counterpart in the reference,
+ * whose only timer is the TimerOutput scope "basis initialization and
+ * computation" (include/base/diffusion_problem_ms.tpp:44-45). */
+int msfem_ctx_set_profiling(msfem_ctx *ctx, int on);
+int msfem_ctx_profile(msfem_ctx *ctx, double *ms_local_solve, double *ms_element);
+
+/* ---- the hot path ------------------------------------------------------ */
+/* Builds every cell of the batch: fine assembly, nb basis solves, A^K, F^K.
+ * Replaces DiffusionProblemBasis<dim>::run() for all cells
+ * (include/base/diffusion_problem_basis.tpp:440-474, driven from
+ * diffusion_problem_ms.tpp:84-87) and the getters
+ * get_global_element_matrix / get_global_element_rhs (:320-334).
+ * Host pointers; the call is synchronous.  Returns MSFEM_ERR_NOCONV (after
+ * writing every output) when a basis solve hit its iteration cap, where the
+ * reference throws SolverControl::NoConvergence. */
+int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts,
+                      msfem_batch_out *out);
+
+/* Same, with every array except in->n_fine in device memory and the work
+ * enqueued on `stream` (a hipStream_t, NULL = default stream) without a host
+ * synchronisation.  out->bases may be NULL (the context keeps scratch).
+ * Convergence is reported only through out->status. */
+int msfem_batch_build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts,
+                             msfem_batch_out *out, void *stream);
+
+/* ---- coarse system ------------------------------------------------------ */
+/* Unit square/cube split into nc^dim coarse cells, lexicographic, x fastest
+ * (GridGenerator::hyper_cube + refine_global, diffusion_problem_ms.tpp:93-103).
+ * Assembles the coarse matrix in structured stencil form Kc[(nc+1)^dim][3^dim]
+ * (entry t couples node n with n + (t%3-1, t/3%3-1[, t/9-1])), the load Fc
+ * (Neumann terms included) and the Dirichlet data (cmask, cval).
+ * Replaces DiffusionProblemMultiscale::assemble_system (diffusion_problem_ms.tpp:161-255)
+ * with setup_system's constraints (:108-156).  Host pointers. */
+int msfem_coarse_assemble(msfem_ctx *ctx, int dim, int nc, const double *Ae, const double *Fe, int bc_kind,
+                          double *Kc, double *Fc, uint8_t *cmask, double *cval);
+int msfem_coarse_assemble_device(msfem_ctx *ctx, int dim, int nc, const double *Ae, const double *Fe,
+                                 int bc_kind, double *Kc, double *Fc, uint8_t *cmask, double *cval,
+                                 void *stream);
+
+/* FP64 CG on the constrained coarse system (u = cval where cmask), stops at
+ * ||r||_2 <= rel_tol*||b||_2 or max_iter.  Replaces
+ * DiffusionProblemMultiscale::solve_iterative (diffusion_problem_ms.tpp:260-296),
+ * which uses AMG-CG to an absolute 1e-12.  *iters receives the iteration count.
+ * Host pointers; returns MSFEM_ERR_NOCONV if max_iter was reached. */
+int msfem_coarse_solve(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc,
+                       const uint8_t *cmask, const double *cval, double rel_tol, int max_iter, double *u,
+                       int32_t *iters);
+int msfem_coarse_solve_device(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc,
+                              const uint8_t *cmask, const double *cval, double rel_tol, int max_iter,
+                              double *u, int32_t *iters, void *stream);
+
+/* Per-cell weights w[c][i] = u[global node of vertex i of cell c]
+ * (send_global_weights_to_cell, diffusion_problem_ms.tpp:301-325).  Host pointers. */
+int msfem_gather_weights(int dim, int nc, const double *u, double *w);
+
+/* Fine reconstruction u|_K = sum_i w_i phi_i for every cell
+ * (DiffusionProblemBasis::set_global_weights, diffusion_problem_basis.tpp:354-377).
+ * bases in the bases layout, w [n_cells][nb], u in the layout of one basis
+ * per cell ([sum_c (nf_c+1)^dim]).  Device pointers except n_fine. */
+int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t *n_fine,
+                             const double *bases, const double *w, double *u, void *stream);
+
+/* ---- synthetic inputs (host, FP64, bit-identical for every consumer) ----- */
+/* Counter-based Philox-4x32-10 (Salmon et al., SC'11): out = Philox(ctr, key). */
+void msfem_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+/* Boxes of the nc^dim coarse cells of the unit square/cube -> box[nc^dim][dim][2]. */
+int msfem_gen_unit_boxes(int dim, int nc, double *box);
+/* Reference default coefficient MatrixCoeff<dim> (PI_D = 3.14592653509793218403,
+ * k = 57, scale 0.9999, rotation by PI_D/3; include/coefficients/matrix_coeff.hpp:45-49,
+ * matrix_coeff.tpp:17-91) at the QGauss(2) points -> TENSOR_QP tiles. */
+int msfem_gen_coef_reference(int dim, int n_cells, const int32_t *n_fine, const double *box, double *coef);
+/* alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps) at the QGauss(2) points -> SCALAR_QP tiles (2D). */
+int msfem_gen_coef_periodic(int n_cells, const int32_t *n_fine, const double *box, double eps, double *coef);
+/* alpha = exp(clamp(g, -clip, clip)), g ~ N(0,1) i.i.d. per fine cell -> SCALAR_CELL tiles.
+ * The normal of fine cell j of cell c comes from Philox(ctr={j, c, 0, 0}, key={seed_lo, seed_hi}). */
+int msfem_gen_coef_lognormal(int dim, int n_cells, const int32_t *n_fine, uint64_t seed, double clip,
+                             double *coef);
+/* Uneven batch: per cell a 16^2 probe field of SPD tensors; its contrast
+ * log(max lambda / min lambda) binned at the quartiles of all cells picks the
+ * fine resolution classes[0..3] (low to high contrast) -> n_fine[nc^2],
+ * contrast[nc^2] (may be NULL).  2D. */
+int msfem_gen_resolution_classes(int nc, uint64_t seed, double sigma, const int32_t classes[4], int32_t *n_fine,
+                                 double *contrast);
+/* SPD tensor per fine cell R(theta) diag(l1,l2) R(theta)^T, theta ~ U[0,pi),
+ * log l ~ N(0, sigma) -> TENSOR_CELL tiles.  2D. */
+int msfem_gen_coef_tensor(int n_cells, const int32_t *n_fine, uint64_t seed, double sigma, double *coef);
+/* Gaussian random field log-coefficient tiles (nf x nf per cell): white
+ * noise convolved with a Gaussian kernel so that the covariance is
+ * exp(-r^2 / (2 corr_len^2)), unit variance, clamped to [-clip, clip].
+ * Writes float32 g [n_cells][nf*nf]; alpha = exp(g). */
+int msfem_gen_grf_log(int n_cells, int nf, uint64_t seed, double corr_len, double clip, float *g);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MSFEM_H_ */

@@ -1,0 +1,309 @@
+// Coarse element matrix and load of every coarse cell from its converged bases:
+//     A^K_ij = phi_i^T K phi_j,    F^K_i = phi_i^T b,
+// DiffusionProblemBasis<dim>::assemble_global_element_matrix
+// (include/base/diffusion_problem_basis.tpp:247-285), evaluated fine cell by
+// fine cell as sum_c phi_i|_c^T K_c phi_j|_c with the QGauss(2) element
+// matrices of assemble_system (:161-242).  One workgroup per coarse cell; the
+// per-thread partial sums are combined in a fixed order (deterministic).
+#include "msfem_device.hpp"
+#include "msfem_kernels.hpp"
+
+namespace msfem {
+
+namespace {
+
+constexpr int kElemThreads = 256;
+
+__device__ __forceinline__ int sym_index(int dim, int i, int j)
+{
+  if (i > j)
+    {
+      const int t = i;
+      i = j;
+      j = t;
+    }
+  if (dim == 2)
+    return i == 0 ? j : 2;
+  if (i == 0)
+    return j;
+  if (i == 1)
+    return 2 + j;
+  return 5;
+}
+
+template <int DIM>
+__global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
+{
+  constexpr int NV = 1 << DIM, NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
+  constexpr int NVAL = NV * NV + NV;
+  __shared__ double sM[NS][NV * NV]; // unit element matrices of each tensor component
+  __shared__ double sred[kElemThreads / kWave][NVAL];
+
+  const int cell = a.cells[blockIdx.x];
+  const int nf = a.nf, np = nf + 1;
+  const double *bx = a.box + (size_t)cell * 2 * DIM;
+  double h[3] = {1, 1, 1}, lo[3] = {0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < DIM; ++k)
+    {
+      lo[k] = bx[2 * k];
+      h[k] = (bx[2 * k + 1] - bx[2 * k]) / nf;
+    }
+  const double jxw = (DIM == 2 ? h[0] * h[1] : h[0] * h[1] * h[2]) / NQ;
+  const double gs = 0.5 / sqrt(3.0);
+
+  // d psi_a / d x_k at quadrature point q
+  auto grad = [&](int aa, int k, int q) -> double {
+    double g = ((aa >> k) & 1) ? 1.0 : -1.0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+      if (m != k)
+        {
+          const double xi = ((q >> m) & 1) ? 0.5 + gs : 0.5 - gs;
+          g *= ((aa >> m) & 1) ? xi : 1.0 - xi;
+        }
+    return g / h[k];
+  };
+  auto shape = [&](int aa, int q) -> double {
+    double v = 1.0;
+#pragma unroll
+    for (int m = 0; m < DIM; ++m)
+      {
+        const double xi = ((q >> m) & 1) ? 0.5 + gs : 0.5 - gs;
+        v *= ((aa >> m) & 1) ? xi : 1.0 - xi;
+      }
+    return v;
+  };
+
+  // Unit matrices: M_s[a][b] = sum_q jxw (d_k psi_a d_l psi_b + [k != l] d_l psi_a d_k psi_b)
+  // for the tensor component s = (k, l), k <= l.
+  for (int t = threadIdx.x; t < NS * NV * NV; t += blockDim.x)
+    {
+      const int s = t / (NV * NV), ab = t % (NV * NV), aa = ab / NV, bb = ab % NV;
+      int k = 0, l = 0;
+      for (int kk = 0; kk < DIM; ++kk)
+        for (int ll = kk; ll < DIM; ++ll)
+          if (sym_index(DIM, kk, ll) == s)
+            {
+              k = kk;
+              l = ll;
+            }
+      double v = 0;
+      for (int q = 0; q < NQ; ++q)
+        {
+          double e = grad(aa, k, q) * grad(bb, l, q);
+          if (k != l)
+            e += grad(aa, l, q) * grad(bb, k, q);
+          v += e * jxw;
+        }
+      sM[s][ab] = v;
+    }
+  __syncthreads();
+
+  const double *cf = a.coef + a.coef_off[cell];
+  const double *phi = a.bases + a.bas_off[cell];
+  const size_t nn = DIM == 2 ? (size_t)np * np : (size_t)np * np * np;
+  const int ncell = DIM == 2 ? nf * nf : nf * nf * nf;
+
+  double acc[NVAL];
+#pragma unroll
+  for (int v = 0; v < NVAL; ++v)
+    acc[v] = 0;
+
+  for (int f = threadIdx.x; f < ncell; f += blockDim.x)
+    {
+      const int fx = f % nf, fy = (f / nf) % nf, fz = DIM == 3 ? f / (nf * nf) : 0;
+      double Ke[NV * NV];
+      if (a.coef_kind == 0 || a.coef_kind == 1)
+        {
+          double cs[NS];
+          if (a.coef_kind == 0)
+            {
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                cs[s] = 0;
+              cs[sym_index(DIM, 0, 0)] = cs[sym_index(DIM, 1, 1)] = cf[f];
+              if (DIM == 3)
+                cs[sym_index(DIM, 2, 2)] = cf[f];
+            }
+          else
+            {
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                cs[s] = cf[(size_t)NS * f + s];
+            }
+#pragma unroll
+          for (int ab = 0; ab < NV * NV; ++ab)
+            {
+              double v = 0;
+#pragma unroll
+              for (int s = 0; s < NS; ++s)
+                v += cs[s] * sM[s][ab];
+              Ke[ab] = v;
+            }
+        }
+      else
+        {
+          // coefficient at the quadrature points: Gauss sum per fine cell
+#pragma unroll
+          for (int ab = 0; ab < NV * NV; ++ab)
+            Ke[ab] = 0;
+          for (int q = 0; q < NQ; ++q)
+            {
+              double A[3][3];
+              for (int k = 0; k < DIM; ++k)
+                for (int l = 0; l < DIM; ++l)
+                  {
+                    if (a.coef_kind == 2)
+                      A[k][l] = k == l ? cf[(size_t)NQ * f + q] : 0.0;
+                    else
+                      A[k][l] = cf[((size_t)NQ * f + q) * NS + sym_index(DIM, k, l)];
+                  }
+              double g[NV][3];
+              for (int aa = 0; aa < NV; ++aa)
+                for (int k = 0; k < DIM; ++k)
+                  g[aa][k] = grad(aa, k, q);
+              for (int aa = 0; aa < NV; ++aa)
+                for (int bb = 0; bb < NV; ++bb)
+                  {
+                    double e = 0;
+                    for (int k = 0; k < DIM; ++k)
+                      for (int l = 0; l < DIM; ++l)
+                        e += g[aa][k] * A[k][l] * g[bb][l];
+                    Ke[aa * NV + bb] += e * jxw;
+                  }
+            }
+        }
+      // load vector of the fine cell
+      double be[NV];
+#pragma unroll
+      for (int aa = 0; aa < NV; ++aa)
+        be[aa] = 0;
+      for (int q = 0; q < NQ; ++q)
+        {
+          double fv = a.rhs_value;
+          if (a.rhs_kind == 1)
+            {
+              fv = 1.0;
+              const int fi[3] = {fx, fy, fz};
+              for (int k = 0; k < DIM; ++k)
+                {
+                  const double xi = ((q >> k) & 1) ? 0.5 + gs : 0.5 - gs;
+                  fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * xi));
+                }
+            }
+#pragma unroll
+          for (int aa = 0; aa < NV; ++aa)
+            be[aa] += shape(aa, q) * fv * jxw;
+        }
+      // bases at the fine cell's vertices
+      double pv[NV][NV]; // [basis][vertex]
+#pragma unroll
+      for (int aa = 0; aa < NV; ++aa)
+        {
+          const size_t node = (size_t)(fx + (aa & 1)) +
+                              (size_t)np * ((fy + ((aa >> 1) & 1)) + (size_t)np * (fz + ((aa >> 2) & 1)));
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+            pv[i][aa] = phi[(size_t)i * nn + node];
+        }
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        {
+          double y[NV];
+#pragma unroll
+          for (int aa = 0; aa < NV; ++aa)
+            {
+              double s = 0;
+#pragma unroll
+              for (int bb = 0; bb < NV; ++bb)
+                s += Ke[aa * NV + bb] * pv[j][bb];
+              y[aa] = s;
+            }
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+            {
+              double s = 0;
+#pragma unroll
+              for (int aa = 0; aa < NV; ++aa)
+                s += pv[i][aa] * y[aa];
+              acc[i * NV + j] += s;
+            }
+        }
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        {
+          double s = 0;
+#pragma unroll
+          for (int aa = 0; aa < NV; ++aa)
+            s += pv[i][aa] * be[aa];
+          acc[NV * NV + i] += s;
+        }
+    }
+
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int v = 0; v < NVAL; ++v)
+    {
+      const double s = group_sum<kWave>(acc[v]);
+      if (lane == 0)
+        sred[wave][v] = s;
+    }
+  __syncthreads();
+  for (int v = threadIdx.x; v < NVAL; v += blockDim.x)
+    {
+      double s = 0;
+      for (int w = 0; w < kElemThreads / kWave; ++w)
+        s += sred[w][v];
+      if (v < NV * NV)
+        a.Ae[(size_t)cell * NV * NV + v] = s;
+      else
+        a.Fe[(size_t)cell * NV + v - NV * NV] = s;
+    }
+}
+
+__global__ void pou_fixup_kernel(int n_cells_class, const int *cells, int nb, size_t nn, double *bases,
+                                 const long long *bas_off)
+{
+  const size_t total = (size_t)n_cells_class * nn;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    {
+      const int cell = cells[t / nn];
+      const size_t n = t % nn;
+      double *phi = bases + bas_off[cell];
+      double s = 0;
+      for (int i = 0; i < nb - 1; ++i)
+        s += phi[(size_t)i * nn + n];
+      phi[(size_t)(nb - 1) * nn + n] = 1.0 - s;
+    }
+}
+
+} // namespace
+
+hipError_t launch_element(const ElementArgs &a, hipStream_t st)
+{
+  if (a.n_cells_class == 0)
+    return hipSuccess;
+  if (a.dim == 2)
+    hipLaunchKernelGGL(element_kernel<2>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else
+    hipLaunchKernelGGL(element_kernel<3>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_pou_fixup(int n_cells_class, const int *cells, int dim, int nf, double *bases,
+                            const long long *bas_off, hipStream_t st)
+{
+  if (n_cells_class == 0)
+    return hipSuccess;
+  const size_t nn = dim == 2 ? (size_t)(nf + 1) * (nf + 1) : (size_t)(nf + 1) * (nf + 1) * (nf + 1);
+  const size_t total = (size_t)n_cells_class * nn;
+  size_t grid = (total + 255) / 256;
+  if (grid > 65536)
+    grid = 65536;
+  hipLaunchKernelGGL(pou_fixup_kernel, dim3((unsigned)grid), dim3(256), 0, st, n_cells_class, cells, 1 << dim, nn,
+                     bases, bas_off);
+  return hipGetLastError();
+}
+
+} // namespace msfem

@@ -1,0 +1,709 @@
+// Host side of the MsFEM engine: context, batch planning by fine resolution,
+// kernel sequencing and the C ABI of include/msfem.h.
+//
+// One msfem_batch_build call does, for all cells at once, what the reference
+// does per cell in DiffusionProblemBasis<dim>::run()
+// (include/base/diffusion_problem_basis.tpp:440-474), driven by the cell loop
+// of DiffusionProblemMultiscale::initialize_and_compute_basis
+// (include/base/diffusion_problem_ms.tpp:42-88).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "msfem.h"
+#include "msfem_kernels.hpp"
+
+using namespace msfem;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+};
+
+// Batch layout derived from n_fine: classes by fine resolution and offsets.
+struct Plan {
+  int dim = 0, coef_kind = -1, nb = 0;
+  std::vector<int32_t> n_fine;
+  std::vector<int> class_nf;               // distinct resolutions
+  std::vector<std::vector<int>> class_cells;
+  std::vector<long long> coef_off, bas_off, cond_off;
+  long long coef_total = 0, bas_total = 0, cond_total = 0;
+  size_t cells_dev_off[8] = {0};           // offsets (ints) of each class list in d_cells
+  bool uploaded = false;
+};
+
+} // namespace
+
+struct msfem_ctx {
+  int device = 0;
+  std::string err;
+  hipStream_t stream = nullptr;
+  std::map<std::string, DevBuf> bufs;
+  Plan plan;
+  bool profiling = false;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // solve begin/end, element begin/end
+  bool ev_recorded = false;
+};
+
+namespace {
+
+int fail(msfem_ctx *ctx, int code, const char *fmt, ...)
+{
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  if (ctx)
+    ctx->err = buf;
+  else
+    g_create_error = buf;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                                  \
+  do                                                                                                        \
+    {                                                                                                       \
+      hipError_t e_ = (expr);                                                                               \
+      if (e_ != hipSuccess)                                                                                 \
+        return fail(ctx, MSFEM_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                                              \
+    }                                                                                                       \
+  while (0)
+
+// Grow-only device scratch owned by the context.
+int scratch(msfem_ctx *ctx, const char *name, size_t bytes, void **out)
+{
+  DevBuf &b = ctx->bufs[name];
+  if (b.bytes < bytes)
+    {
+      if (b.p)
+        HIP_TRY(ctx, hipFree(b.p));
+      b.p = nullptr;
+      b.bytes = 0;
+      HIP_TRY(ctx, hipMalloc(&b.p, bytes > 0 ? bytes : 1));
+      b.bytes = bytes;
+    }
+  *out = b.p;
+  return MSFEM_OK;
+}
+
+int coef_stride(int dim, int kind)
+{
+  const int nq = 1 << dim, ns = dim == 2 ? 3 : 6;
+  switch (kind)
+    {
+      case MSFEM_COEF_SCALAR_CELL: return 1;
+      case MSFEM_COEF_TENSOR_CELL: return ns;
+      case MSFEM_COEF_SCALAR_QP: return nq;
+      case MSFEM_COEF_TENSOR_QP: return nq * ns;
+      default: return -1;
+    }
+}
+
+long long ipow(long long b, int e)
+{
+  long long r = 1;
+  while (e-- > 0)
+    r *= b;
+  return r;
+}
+
+bool nf_supported(int dim, int nf)
+{
+  if (dim == 2)
+    return nf == 16 || nf == 32 || nf == 64 || nf == 128;
+  return nf == 8 || nf == 16;
+}
+
+// Local-solve storage mode of a class.
+int class_mode(int dim, int coef_kind, int nf)
+{
+  if (dim == 3)
+    return kModeScalar;
+  if (coef_kind == MSFEM_COEF_SCALAR_CELL)
+    return kModeScalar;
+  if (coef_kind == MSFEM_COEF_TENSOR_CELL && nf <= 64)
+    return kModeTensor;
+  return kModeGlobal;
+}
+
+int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
+{
+  if (!in || !o)
+    return fail(ctx, MSFEM_ERR_ARG, "null batch or options");
+  if (in->dim != 2 && in->dim != 3)
+    return fail(ctx, MSFEM_ERR_ARG, "dim must be 2 or 3 (got %d)", in->dim);
+  if (in->n_cells < 0 || (in->n_cells > 0 && (!in->n_fine || !in->box || !in->coef)))
+    return fail(ctx, MSFEM_ERR_ARG, "n_cells < 0 or a required array is NULL");
+  if (coef_stride(in->dim, in->coef_kind) < 0)
+    return fail(ctx, MSFEM_ERR_ARG, "unknown coefficient kind %d", in->coef_kind);
+  if (in->dim == 3 && in->coef_kind != MSFEM_COEF_SCALAR_CELL)
+    return fail(ctx, MSFEM_ERR_ARG, "3D supports MSFEM_COEF_SCALAR_CELL only");
+  if (in->rhs_kind != MSFEM_RHS_CONST && in->rhs_kind != MSFEM_RHS_SINSIN)
+    return fail(ctx, MSFEM_ERR_ARG, "unknown rhs kind %d", in->rhs_kind);
+  if (o->precision != MSFEM_PREC_FP64 && o->precision != MSFEM_PREC_FP32_REFINE)
+    return fail(ctx, MSFEM_ERR_ARG, "unknown precision %d", o->precision);
+  if (o->max_iter < 1 || o->max_refine < 1 || !(o->rel_tol >= 0) || !(o->abs_tol >= 0) ||
+      (o->rel_tol == 0 && o->abs_tol == 0))
+    return fail(ctx, MSFEM_ERR_ARG, "invalid tolerances or iteration caps");
+  for (int c = 0; c < in->n_cells; ++c)
+    if (!nf_supported(in->dim, in->n_fine[c]))
+      return fail(ctx, MSFEM_ERR_ARG, "cell %d: unsupported fine resolution %d for dim %d", c, in->n_fine[c],
+                  in->dim);
+  return MSFEM_OK;
+}
+
+int make_plan(msfem_ctx *ctx, const msfem_batch *in)
+{
+  Plan &p = ctx->plan;
+  const bool same = p.uploaded && p.dim == in->dim && p.coef_kind == in->coef_kind &&
+                    p.n_fine.size() == (size_t)in->n_cells &&
+                    std::equal(p.n_fine.begin(), p.n_fine.end(), in->n_fine);
+  if (same)
+    return MSFEM_OK;
+  p = Plan();
+  p.dim = in->dim;
+  p.coef_kind = in->coef_kind;
+  p.nb = 1 << in->dim;
+  p.n_fine.assign(in->n_fine, in->n_fine + in->n_cells);
+  const int stride = coef_stride(in->dim, in->coef_kind);
+  p.coef_off.resize(in->n_cells);
+  p.bas_off.resize(in->n_cells);
+  p.cond_off.resize(in->n_cells);
+  std::map<int, std::vector<int>> cls;
+  for (int c = 0; c < in->n_cells; ++c)
+    {
+      const int nf = in->n_fine[c];
+      p.coef_off[c] = p.coef_total;
+      p.coef_total += ipow(nf, in->dim) * stride;
+      p.bas_off[c] = p.bas_total;
+      p.bas_total += (long long)p.nb * ipow(nf + 1, in->dim);
+      p.cond_off[c] = p.cond_total;
+      if (in->dim == 2 && class_mode(2, in->coef_kind, nf) == kModeGlobal)
+        p.cond_total += 6 * ipow(nf, 2);
+      cls[nf].push_back(c);
+    }
+  std::vector<int> all_cells;
+  int k = 0;
+  for (auto &kv : cls)
+    {
+      p.class_nf.push_back(kv.first);
+      p.cells_dev_off[k++] = all_cells.size();
+      all_cells.insert(all_cells.end(), kv.second.begin(), kv.second.end());
+      p.class_cells.push_back(kv.second);
+    }
+  // device metadata: cells | coef_off | bas_off | cond_off
+  const size_t n = in->n_cells;
+  const size_t bytes = n * sizeof(int) + 3 * n * sizeof(long long) + 64;
+  void *d;
+  int rc = scratch(ctx, "meta", bytes, &d);
+  if (rc)
+    return rc;
+  std::vector<char> host(bytes);
+  size_t off = 0;
+  memcpy(host.data(), all_cells.data(), n * sizeof(int));
+  off = ((n * sizeof(int) + 7) / 8) * 8;
+  memcpy(host.data() + off, p.coef_off.data(), n * sizeof(long long));
+  memcpy(host.data() + off + n * sizeof(long long), p.bas_off.data(), n * sizeof(long long));
+  memcpy(host.data() + off + 2 * n * sizeof(long long), p.cond_off.data(), n * sizeof(long long));
+  HIP_TRY(ctx, hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice));
+  p.uploaded = true;
+  return MSFEM_OK;
+}
+
+const int *d_cells(msfem_ctx *ctx) { return (const int *)ctx->bufs["meta"].p; }
+const long long *d_off(msfem_ctx *ctx, int which) // 0 coef, 1 bas, 2 cond
+{
+  const size_t n = ctx->plan.n_fine.size();
+  const size_t off = ((n * sizeof(int) + 7) / 8) * 8;
+  return (const long long *)((const char *)ctx->bufs["meta"].p + off + which * n * sizeof(long long));
+}
+
+__global__ void fill_kernel(void *p, size_t n_words, uint32_t v)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n_words; i += (size_t)gridDim.x * blockDim.x)
+    ((uint32_t *)p)[i] = v;
+}
+
+hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
+{
+  if (!p || bytes == 0)
+    return hipSuccess;
+  const size_t words = bytes / 4;
+  size_t grid = (words + 255) / 256;
+  if (grid > 4096)
+    grid = 4096;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)grid), dim3(256), 0, st, p, words, 0u);
+  return hipGetLastError();
+}
+
+int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+                 hipStream_t st)
+{
+  int rc = validate(ctx, in, o);
+  if (rc)
+    return rc;
+  if (!out || !out->Ae || !out->Fe)
+    return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
+  if (in->n_cells == 0)
+    return MSFEM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rc = make_plan(ctx, in);
+  if (rc)
+    return rc;
+  Plan &p = ctx->plan;
+  double *bases = out->bases;
+  if (!bases)
+    {
+      rc = scratch(ctx, "bases", p.bas_total * sizeof(double), (void **)&bases);
+      if (rc)
+        return rc;
+    }
+  double *cond = nullptr;
+  if (p.cond_total > 0)
+    {
+      rc = scratch(ctx, "cond", p.cond_total * sizeof(double), (void **)&cond);
+      if (rc)
+        return rc;
+    }
+  const int nb = p.nb;
+  const bool pou = o->partition_of_unity != 0;
+  const size_t nstat = (size_t)in->n_cells * nb;
+  if (pou)
+    {
+      HIP_TRY(ctx, fill_zero(out->iters, nstat * sizeof(int32_t), st));
+      HIP_TRY(ctx, fill_zero(out->status, nstat * sizeof(int32_t), st));
+      HIP_TRY(ctx, fill_zero(out->resid, nstat * sizeof(double), st));
+    }
+  if (ctx->profiling)
+    HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
+  // all classes' local solves first, then all element matrices
+  for (size_t k = 0; k < p.class_nf.size(); ++k)
+    {
+      const int nf = p.class_nf[k];
+      const int ncls = (int)p.class_cells[k].size();
+      const int *cells = d_cells(ctx) + p.cells_dev_off[k];
+      const int mode = class_mode(in->dim, in->coef_kind, nf);
+      if (mode == kModeGlobal)
+        HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
+                                         d_off(ctx, 2), st));
+      LocalArgs a;
+      a.nb_solve = pou ? nb - 1 : nb;
+      a.n_problems = ncls * a.nb_solve;
+      a.nb = nb;
+      a.cells = cells;
+      a.box = in->box;
+      a.coef = in->coef;
+      a.coef_off = d_off(ctx, 0);
+      a.cond = cond;
+      a.cond_off = d_off(ctx, 2);
+      a.bases = bases;
+      a.bas_off = d_off(ctx, 1);
+      a.x0 = in->x0;
+      a.iters = out->iters;
+      a.status = out->status;
+      a.resid = out->resid;
+      a.rel_tol = o->rel_tol;
+      a.abs_tol = o->abs_tol;
+      a.inner_rel_tol = o->inner_rel_tol;
+      a.max_iter = o->max_iter;
+      a.max_refine = o->max_refine;
+      const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
+      if (in->dim == 2)
+        HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, o->variant, a, st));
+      else
+        HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, o->variant, a, st));
+      if (pou)
+        HIP_TRY(ctx, launch_pou_fixup(ncls, cells, in->dim, nf, bases, d_off(ctx, 1), st));
+    }
+  if (ctx->profiling)
+    {
+      HIP_TRY(ctx, hipEventRecord(ctx->ev[1], st));
+      HIP_TRY(ctx, hipEventRecord(ctx->ev[2], st));
+    }
+  for (size_t k = 0; k < p.class_nf.size(); ++k)
+    {
+      const int nf = p.class_nf[k];
+      const int ncls = (int)p.class_cells[k].size();
+      const int *cells = d_cells(ctx) + p.cells_dev_off[k];
+      ElementArgs e;
+      e.n_cells_class = ncls;
+      e.cells = cells;
+      e.dim = in->dim;
+      e.nf = nf;
+      e.nb = nb;
+      e.coef_kind = in->coef_kind;
+      e.box = in->box;
+      e.coef = in->coef;
+      e.coef_off = d_off(ctx, 0);
+      e.cond = cond;
+      e.cond_off = d_off(ctx, 2);
+      e.bases = bases;
+      e.bas_off = d_off(ctx, 1);
+      e.rhs_kind = in->rhs_kind;
+      e.rhs_value = in->rhs_value;
+      e.Ae = out->Ae;
+      e.Fe = out->Fe;
+      HIP_TRY(ctx, launch_element(e, st));
+    }
+  if (ctx->profiling)
+    {
+      HIP_TRY(ctx, hipEventRecord(ctx->ev[3], st));
+      ctx->ev_recorded = true;
+    }
+  return MSFEM_OK;
+}
+
+struct HostStage {
+  msfem_ctx *ctx;
+  int rc = MSFEM_OK;
+  template <typename T>
+  T *up(const char *name, const T *src, size_t n)
+  {
+    void *d = nullptr;
+    if (rc || !src)
+      return nullptr;
+    rc = scratch(ctx, name, n * sizeof(T), &d);
+    if (rc)
+      return nullptr;
+    hipError_t e = hipMemcpyAsync(d, src, n * sizeof(T), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess)
+      rc = fail(ctx, MSFEM_ERR_HIP, "hipMemcpyAsync H2D: %s", hipGetErrorString(e));
+    return (T *)d;
+  }
+  template <typename T>
+  T *alloc(const char *name, size_t n, bool want)
+  {
+    void *d = nullptr;
+    if (rc || !want)
+      return nullptr;
+    rc = scratch(ctx, name, n * sizeof(T), &d);
+    return rc ? nullptr : (T *)d;
+  }
+  template <typename T>
+  void down(T *dst, const T *src, size_t n)
+  {
+    if (rc || !dst || !src)
+      return;
+    hipError_t e = hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess)
+      rc = fail(ctx, MSFEM_ERR_HIP, "hipMemcpyAsync D2H: %s", hipGetErrorString(e));
+  }
+};
+
+} // namespace
+
+extern "C" {
+
+int msfem_version(void) { return MSFEM_API_VERSION; }
+
+const char *msfem_last_error(const msfem_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+void msfem_default_opts(msfem_solver_opts *o)
+{
+  if (!o)
+    return;
+  memset(o, 0, sizeof(*o));
+  o->precision = MSFEM_PREC_FP64;
+  o->max_iter = 20000;
+  o->max_refine = 20;
+  o->partition_of_unity = 0;
+  o->rel_tol = 1e-13;
+  o->abs_tol = 0.0;
+  o->inner_rel_tol = 1e-4;
+  o->variant = 0;
+}
+
+int msfem_ctx_create(int device, msfem_ctx **out)
+{
+  if (!out)
+    return fail(nullptr, MSFEM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess || n <= 0)
+    return fail(nullptr, MSFEM_ERR_NODEVICE, "no HIP device (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= n)
+    return fail(nullptr, MSFEM_ERR_ARG, "device %d out of range (%d devices)", device, n);
+  e = hipSetDevice(device);
+  if (e != hipSuccess)
+    return fail(nullptr, MSFEM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  msfem_ctx *ctx = new msfem_ctx;
+  ctx->device = device;
+  e = hipStreamCreate(&ctx->stream);
+  if (e != hipSuccess)
+    {
+      delete ctx;
+      return fail(nullptr, MSFEM_ERR_HIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+  *out = ctx;
+  return MSFEM_OK;
+}
+
+int msfem_ctx_set_profiling(msfem_ctx *ctx, int on)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (on && !ctx->ev[0])
+    for (auto &e : ctx->ev)
+      HIP_TRY(ctx, hipEventCreate(&e));
+  ctx->profiling = on != 0;
+  ctx->ev_recorded = false;
+  return MSFEM_OK;
+}
+
+int msfem_ctx_profile(msfem_ctx *ctx, double *ms_local_solve, double *ms_element)
+{
+  if (!ctx || !ms_local_solve || !ms_element)
+    return fail(ctx, MSFEM_ERR_ARG, "null argument");
+  if (!ctx->ev_recorded)
+    return fail(ctx, MSFEM_ERR_ARG, "no profiled build recorded");
+  HIP_TRY(ctx, hipEventSynchronize(ctx->ev[3]));
+  float a = 0, b = 0;
+  HIP_TRY(ctx, hipEventElapsedTime(&a, ctx->ev[0], ctx->ev[1]));
+  HIP_TRY(ctx, hipEventElapsedTime(&b, ctx->ev[2], ctx->ev[3]));
+  *ms_local_solve = a;
+  *ms_element = b;
+  return MSFEM_OK;
+}
+
+int msfem_ctx_destroy(msfem_ctx *ctx)
+{
+  if (!ctx)
+    return MSFEM_OK;
+  hipSetDevice(ctx->device);
+  if (ctx->stream)
+    hipStreamSynchronize(ctx->stream);
+  for (auto &e : ctx->ev)
+    if (e)
+      hipEventDestroy(e);
+  for (auto &kv : ctx->bufs)
+    if (kv.second.p)
+      hipFree(kv.second.p);
+  if (ctx->stream)
+    hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return MSFEM_OK;
+}
+
+int msfem_batch_build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts,
+                             msfem_batch_out *out, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  return build_device(ctx, in, opts, out, (hipStream_t)stream);
+}
+
+int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts, msfem_batch_out *out)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  int rc = validate(ctx, in, opts);
+  if (rc)
+    return rc;
+  if (!out || !out->Ae || !out->Fe)
+    return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
+  if (in->n_cells == 0)
+    return MSFEM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int dim = in->dim, nb = 1 << dim, n = in->n_cells;
+  long long coef_total = 0, bas_total = 0;
+  for (int c = 0; c < n; ++c)
+    {
+      coef_total += ipow(in->n_fine[c], dim) * coef_stride(dim, in->coef_kind);
+      bas_total += nb * ipow(in->n_fine[c] + 1, dim);
+    }
+  HostStage s{ctx};
+  msfem_batch din = *in;
+  din.box = s.up("h_box", in->box, (size_t)n * 2 * dim);
+  din.coef = s.up("h_coef", in->coef, (size_t)coef_total);
+  din.x0 = s.up("h_x0", in->x0, (size_t)bas_total);
+  msfem_batch_out dout;
+  dout.Ae = s.alloc<double>("h_Ae", (size_t)n * nb * nb, true);
+  dout.Fe = s.alloc<double>("h_Fe", (size_t)n * nb, true);
+  dout.bases = s.alloc<double>("h_bases", (size_t)bas_total, true);
+  dout.iters = s.alloc<int32_t>("h_iters", (size_t)n * nb, true);
+  dout.status = s.alloc<int32_t>("h_status", (size_t)n * nb, true);
+  dout.resid = s.alloc<double>("h_resid", (size_t)n * nb, out->resid != nullptr);
+  if (s.rc)
+    return s.rc;
+  rc = build_device(ctx, &din, opts, &dout, ctx->stream);
+  if (rc)
+    return rc;
+  std::vector<int32_t> status((size_t)n * nb);
+  s.down(out->Ae, dout.Ae, (size_t)n * nb * nb);
+  s.down(out->Fe, dout.Fe, (size_t)n * nb);
+  s.down(out->bases, dout.bases, (size_t)bas_total);
+  s.down(out->iters, dout.iters, (size_t)n * nb);
+  s.down(status.data(), dout.status, (size_t)n * nb);
+  s.down(out->resid, dout.resid, (size_t)n * nb);
+  if (s.rc)
+    return s.rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (out->status)
+    memcpy(out->status, status.data(), status.size() * sizeof(int32_t));
+  int bad = 0;
+  for (int32_t v : status)
+    bad += v != 0;
+  if (bad)
+    return fail(ctx, MSFEM_ERR_NOCONV, "%d basis solve(s) did not converge", bad);
+  return MSFEM_OK;
+}
+
+int msfem_coarse_assemble_device(msfem_ctx *ctx, int dim, int nc, const double *Ae, const double *Fe, int bc_kind,
+                                 double *Kc, double *Fc, uint8_t *cmask, double *cval, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || nc < 1 || (bc_kind != 0 && bc_kind != 1) || !Ae || !Fe || !Kc || !Fc || !cmask ||
+      !cval)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid coarse assembly arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, launch_coarse_assemble(dim, nc, Ae, Fe, bc_kind, Kc, Fc, cmask, cval, (hipStream_t)stream));
+  return MSFEM_OK;
+}
+
+int msfem_coarse_assemble(msfem_ctx *ctx, int dim, int nc, const double *Ae, const double *Fe, int bc_kind,
+                          double *Kc, double *Fc, uint8_t *cmask, double *cval)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || nc < 1)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid coarse assembly arguments");
+  const int nb = 1 << dim;
+  const size_t ncell = ipow(nc, dim), nn = ipow(nc + 1, dim), ns3 = ipow(3, dim);
+  HostStage s{ctx};
+  const double *dA = s.up("c_Ae", Ae, ncell * nb * nb);
+  const double *dF = s.up("c_Fe", Fe, ncell * nb);
+  double *dK = s.alloc<double>("c_Kc", nn * ns3, true);
+  double *dFc = s.alloc<double>("c_Fc", nn, true);
+  uint8_t *dm = s.alloc<uint8_t>("c_cm", nn, true);
+  double *dv = s.alloc<double>("c_cv", nn, true);
+  if (s.rc)
+    return s.rc;
+  int rc = msfem_coarse_assemble_device(ctx, dim, nc, dA, dF, bc_kind, dK, dFc, dm, dv, ctx->stream);
+  if (rc)
+    return rc;
+  s.down(Kc, dK, nn * ns3);
+  s.down(Fc, dFc, nn);
+  s.down(cmask, dm, nn);
+  s.down(cval, dv, nn);
+  if (s.rc)
+    return s.rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MSFEM_OK;
+}
+
+int msfem_coarse_solve_device(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc,
+                              const uint8_t *cmask, const double *cval, double rel_tol, int max_iter, double *u,
+                              int32_t *iters, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || nc < 1 || !(rel_tol > 0) || max_iter < 1 || !Kc || !Fc || !cmask || !cval || !u ||
+      !iters)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid coarse solve arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  double *work;
+  int rc = scratch(ctx, "coarse_work", coarse_solve_work_doubles(dim, nc) * sizeof(double), (void **)&work);
+  if (rc)
+    return rc;
+  HIP_TRY(ctx, launch_coarse_solve(dim, nc, Kc, Fc, cmask, cval, rel_tol, max_iter, u, iters, work,
+                                   (hipStream_t)stream));
+  return MSFEM_OK;
+}
+
+int msfem_coarse_solve(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc, const uint8_t *cmask,
+                       const double *cval, double rel_tol, int max_iter, double *u, int32_t *iters)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || nc < 1 || !iters)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid coarse solve arguments");
+  const size_t nn = ipow(nc + 1, dim), ns3 = ipow(3, dim);
+  HostStage s{ctx};
+  const double *dK = s.up("s_Kc", Kc, nn * ns3);
+  const double *dF = s.up("s_Fc", Fc, nn);
+  const uint8_t *dm = s.up("s_cm", cmask, nn);
+  const double *dv = s.up("s_cv", cval, nn);
+  double *du = s.alloc<double>("s_u", nn, true);
+  int32_t *dit = s.alloc<int32_t>("s_it", 1, true);
+  if (s.rc)
+    return s.rc;
+  int rc = msfem_coarse_solve_device(ctx, dim, nc, dK, dF, dm, dv, rel_tol, max_iter, du, dit, ctx->stream);
+  if (rc)
+    return rc;
+  s.down(u, du, nn);
+  s.down(iters, dit, 1);
+  if (s.rc)
+    return s.rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (*iters >= max_iter)
+    return fail(ctx, MSFEM_ERR_NOCONV, "coarse CG reached %d iterations", max_iter);
+  return MSFEM_OK;
+}
+
+int msfem_gather_weights(int dim, int nc, const double *u, double *w)
+{
+  if ((dim != 2 && dim != 3) || nc < 1 || !u || !w)
+    return MSFEM_ERR_ARG;
+  const int nb = 1 << dim, np = nc + 1;
+  const long long ncell = ipow(nc, dim);
+  for (long long c = 0; c < ncell; ++c)
+    {
+      const int ci[3] = {(int)(c % nc), (int)((c / nc) % nc), (int)(c / ((long long)nc * nc))};
+      for (int v = 0; v < nb; ++v)
+        {
+          const long long X = ci[0] + (v & 1), Y = ci[1] + ((v >> 1) & 1), Z = dim == 3 ? ci[2] + ((v >> 2) & 1) : 0;
+          w[c * nb + v] = u[X + np * (Y + (long long)np * Z)];
+        }
+    }
+  return MSFEM_OK;
+}
+
+int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t *n_fine, const double *bases,
+                             const double *w, double *u, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || n_cells < 0 || (n_cells > 0 && (!n_fine || !bases || !w || !u)))
+    return fail(ctx, MSFEM_ERR_ARG, "invalid reconstruction arguments");
+  if (n_cells == 0)
+    return MSFEM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int nb = 1 << dim;
+  std::vector<long long> boff(n_cells), ooff(n_cells);
+  std::vector<int> nn(n_cells);
+  long long b = 0, o = 0;
+  for (int c = 0; c < n_cells; ++c)
+    {
+      nn[c] = (int)ipow(n_fine[c] + 1, dim);
+      boff[c] = b;
+      ooff[c] = o;
+      b += (long long)nb * nn[c];
+      o += nn[c];
+    }
+  HostStage s{ctx};
+  const long long *db = s.up("r_boff", boff.data(), n_cells);
+  const long long *dq = s.up("r_ooff", ooff.data(), n_cells);
+  const int *dn = s.up("r_nn", nn.data(), n_cells);
+  if (s.rc)
+    return s.rc;
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream)); // metadata in place before the caller's stream reads it
+  HIP_TRY(ctx, launch_reconstruct(n_cells, nb, db, dq, dn, bases, w, u, (hipStream_t)stream));
+  return MSFEM_OK;
+}
+
+} // extern "C"

@@ -1,0 +1,566 @@
+// 2D local basis solves: for every (coarse cell, basis i) solve
+//     K_II x = -K_IB g_i,   g_i = Q1 nodal function N_i on the cell boundary,
+// i.e. the constrained problems of DiffusionProblemBasis<2>::run()
+// (include/base/diffusion_problem_basis.tpp:450-465, solver :295-317), to the
+// same converged values, with a matrix-free Q1 operator on the GPU.
+//
+// One lane group owns one problem.  Lanes map to node columns (CPL columns per
+// lane), SW stacked waves split the node rows (R rows per lane); a workgroup of
+// SW waves holds PPW = 64 / lane-group-width problems side by side.  The CG
+// iterate, residual, direction and Jacobi diagonal of the owned nodes live in
+// registers, the search direction of the whole cell in LDS (the only values a
+// lane needs from its neighbours), and the fine-cell coefficients in registers
+// (kModeScalar / kModeTensor) or global memory (kModeGlobal).
+//
+// Precision: an FP64 outer loop computes the true residual r = -(K phi)_I from
+// phi in global memory (FP64 coefficients) and checks
+//     ||r|| <= max(abs_tol, rel_tol ||b||);
+// the inner CG in type T solves K d = r (T = double: to the final tolerance,
+// one cycle; T = float: to inner_rel_tol, refined in FP64 until converged).
+#include "msfem_device.hpp"
+#include "msfem_kernels.hpp"
+
+namespace msfem {
+
+namespace {
+
+// 6 conductances of a fine cell: c01, c02, c03, c12, c13, c23.
+template <typename U>
+struct Cond6 {
+  U c[6];
+};
+
+// Contribution of the four cells around node C to (K p)_C.
+//   NE cell: node is vertex 0, NW: vertex 1, SE: vertex 2, SW: vertex 3.
+template <typename U>
+__device__ __forceinline__ U gather6(U pC, U pE, U pW, U pN, U pS, U pNE, U pNW, U pSE, U pSW, const Cond6<U> &ne,
+                                     const Cond6<U> &nw, const Cond6<U> &se, const Cond6<U> &sw)
+{
+  const U d = (ne.c[0] + ne.c[1] + ne.c[2]) + (nw.c[0] + nw.c[4] + nw.c[3]) + (se.c[5] + se.c[1] + se.c[3]) +
+              (sw.c[5] + sw.c[4] + sw.c[2]);
+  U y = d * pC;
+  y -= (ne.c[0] + se.c[5]) * pE;
+  y -= (nw.c[0] + sw.c[5]) * pW;
+  y -= (ne.c[1] + nw.c[4]) * pN;
+  y -= (se.c[1] + sw.c[4]) * pS;
+  y -= ne.c[2] * pNE;
+  y -= nw.c[3] * pNW;
+  y -= se.c[3] * pSE;
+  y -= sw.c[2] * pSW;
+  return y;
+}
+
+template <typename U>
+__device__ __forceinline__ Cond6<U> from4(const Cond4 &c)
+{
+  return {{(U)c.ch, (U)c.cv, (U)c.cd, (U)c.ca, (U)c.cv, (U)c.ch}};
+}
+
+// FP64 conductances of fine cell (cx, cy) straight from global memory.
+template <int MODE, int NF>
+__device__ __forceinline__ Cond6<double> cell_cond64(const double *cf, const double *cg, int cx, int cy, double rho)
+{
+  const int f = cy * NF + cx;
+  if constexpr (MODE == kModeScalar)
+    return from4<double>(cond_scalar(cf[f], rho));
+  else if constexpr (MODE == kModeTensor)
+    return from4<double>(cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho));
+  else
+    {
+      Cond6<double> r;
+#pragma unroll
+      for (int k = 0; k < 6; ++k)
+        r.c[k] = cg[6 * f + k];
+      return r;
+    }
+}
+
+// Per-problem sums of two values over LW lanes x SW waves.
+template <int LW, int SW, int PPW>
+__device__ __forceinline__ void problem_sum2(double &a, double &b, double *red, int wave, int slot, int pl)
+{
+  a = group_sum<LW>(a);
+  b = group_sum<LW>(b);
+  if constexpr (SW > 1)
+    {
+      if (pl == 0)
+        {
+          red[(wave * PPW + slot) * 2] = a;
+          red[(wave * PPW + slot) * 2 + 1] = b;
+        }
+      __syncthreads();
+      a = 0;
+      b = 0;
+#pragma unroll
+      for (int w = 0; w < SW; ++w)
+        {
+          a += red[(w * PPW + slot) * 2];
+          b += red[(w * PPW + slot) * 2 + 1];
+        }
+    }
+}
+
+} // namespace
+
+template <typename T, int NF, int CPL, int SW, int R, int MODE>
+__global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
+{
+  constexpr int NP = NF + 1;
+  constexpr int LW = NF / CPL;
+  static_assert(LW == 16 || LW == 32 || LW == 64, "lane group must be 16, 32 or 64 lanes");
+  constexpr int PPW = kWave / LW;
+  static_assert(SW * R >= NF - 1, "rows not covered");
+  // LDS grid of one problem: pitch GP, GR rows; the padding column NF+1 and
+  // rows beyond NF stay zero so that every window read is in bounds.
+  constexpr int GP = NF + 2;
+  constexpr int GR = (SW * R + 2 > NP) ? SW * R + 2 : NP;
+  constexpr int GSZ = GP * GR;
+  constexpr int NCACHE = MODE == kModeScalar ? 1 : (MODE == kModeTensor ? 4 : 1);
+
+  __shared__ T sgrid[PPW * GSZ];
+  __shared__ double sred[3][SW * PPW * 2];
+
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int slot = lane / LW, pl = lane % LW;
+  const int prob = blockIdx.x * PPW + slot;
+  const bool pvalid = prob < a.n_problems;
+  int cell = 0, basis = 0;
+  if (pvalid)
+    {
+      cell = a.cells[prob / a.nb_solve];
+      basis = prob % a.nb_solve;
+    }
+  T *grid = sgrid + slot * GSZ;
+  const int X0 = 1 + pl * CPL, Y0 = 1 + wave * R;
+  T *gown = grid + Y0 * GP + X0;       // own node (row 0, column 0)
+  T *gwin = grid + (Y0 - 1) * GP + X0 - 1; // window origin
+
+  double rho = 1.0;
+  const double *cf = a.coef;
+  const double *cg = a.cond;
+  double *phi = a.bases;
+  if (pvalid)
+    {
+      const double *bx = a.box + (size_t)cell * 4;
+      rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      cf = a.coef + a.coef_off[cell];
+      if (MODE == kModeGlobal)
+        cg = a.cond + a.cond_off[cell];
+      phi = a.bases + a.bas_off[cell] + (size_t)basis * NP * NP;
+    }
+
+  // ---- fine-cell data cached in registers ----------------------------------
+  T cc[R + 1][CPL + 1][NCACHE];
+#pragma unroll
+  for (int i = 0; i <= R; ++i)
+#pragma unroll
+    for (int j = 0; j <= CPL; ++j)
+      {
+        const int cx = X0 - 1 + j, cy = Y0 - 1 + i;
+        const bool ok = pvalid && cx < NF && cy < NF;
+        if constexpr (MODE == kModeScalar)
+          cc[i][j][0] = ok ? (T)cf[cy * NF + cx] : (T)0;
+        else if constexpr (MODE == kModeTensor)
+          {
+            Cond4 c4 = {0, 0, 0, 0};
+            if (ok)
+              {
+                const int f = cy * NF + cx;
+                c4 = cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho);
+              }
+            cc[i][j][0] = (T)c4.ch;
+            cc[i][j][1] = (T)c4.cv;
+            cc[i][j][2] = (T)c4.cd;
+            cc[i][j][3] = (T)c4.ca;
+          }
+        else
+          cc[i][j][0] = 0;
+      }
+  const Cond4 k4 = cond_scalar(1.0, rho); // scalar-mode unit conductances
+  const T kh = (T)k4.ch, kv = (T)k4.cv, kd = (T)k4.cd, kc = (T)(k4.ch + k4.cv + k4.cd);
+
+  auto cond_T = [&](int i, int j) -> Cond6<T> {
+    if constexpr (MODE == kModeScalar)
+      {
+        const T s = cc[i][j][0];
+        return {{s * kh, s * kv, s * kd, s * kd, s * kv, s * kh}};
+      }
+    else if constexpr (MODE == kModeTensor)
+      return {{cc[i][j][0], cc[i][j][1], cc[i][j][2], cc[i][j][3], cc[i][j][1], cc[i][j][0]}};
+    else
+      {
+        const int cx = X0 - 1 + j, cy = Y0 - 1 + i;
+        Cond6<T> r = {{0, 0, 0, 0, 0, 0}};
+        if (pvalid && cx < NF && cy < NF)
+          {
+            const double *q = cg + 6 * (cy * NF + cx);
+#pragma unroll
+            for (int k = 0; k < 6; ++k)
+              r.c[k] = (T)q[k];
+          }
+        return r;
+      }
+  };
+
+  // Jacobi diagonal; 0 on slots that are not interior nodes of a valid
+  // problem, which keeps every vector of those slots at exactly 0.
+  T dinv[R][CPL];
+#pragma unroll
+  for (int i = 0; i < R; ++i)
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      {
+        const bool act = pvalid && (X0 + c <= NF - 1) && (Y0 + i <= NF - 1);
+        const Cond6<T> ne = cond_T(i + 1, c + 1), nw = cond_T(i + 1, c), se = cond_T(i, c + 1), sw = cond_T(i, c);
+        const T d = (ne.c[0] + ne.c[1] + ne.c[2]) + (nw.c[0] + nw.c[4] + nw.c[3]) + (se.c[5] + se.c[1] + se.c[3]) +
+                    (sw.c[5] + sw.c[4] + sw.c[2]);
+        dinv[i][c] = act ? (T)1 / d : (T)0;
+      }
+
+  // ---- initial state: LDS grid zero, phi = g on the boundary, x0 inside ----
+  for (int k = tid; k < PPW * GSZ; k += blockDim.x)
+    sgrid[k] = 0;
+  if (pvalid)
+    for (int k = pl + LW * wave; k < NP * NP; k += LW * SW)
+      {
+        const int X = k % NP, Y = k / NP;
+        const bool bnd = X == 0 || X == NF || Y == 0 || Y == NF;
+        double v;
+        if (bnd)
+          v = nodal_q1(basis, 2, X, Y, 0, NF);
+        else
+          v = a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP + k] : 0.0;
+        phi[k] = v;
+      }
+  __syncthreads();
+
+  // (K phi)_C in FP64 from global memory; bonly: interior values taken as 0.
+  auto apply64 = [&](int X, int Y, bool bonly) -> double {
+    auto v = [&](int XX, int YY) -> double {
+      const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF;
+      return (bonly && !bnd) ? 0.0 : phi[YY * NP + XX];
+    };
+    const Cond6<double> ne = cell_cond64<MODE, NF>(cf, cg, X, Y, rho);
+    const Cond6<double> nw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y, rho);
+    const Cond6<double> se = cell_cond64<MODE, NF>(cf, cg, X, Y - 1, rho);
+    const Cond6<double> sw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y - 1, rho);
+    return gather6<double>(v(X, Y), v(X + 1, Y), v(X - 1, Y), v(X, Y + 1), v(X, Y - 1), v(X + 1, Y + 1),
+                           v(X - 1, Y + 1), v(X + 1, Y - 1), v(X - 1, Y - 1), ne, nw, se, sw);
+  };
+
+  double tol_eff = 0.0, bnorm = 0.0;
+  if (a.x0)
+    {
+      double s = 0, dummy = 0;
+#pragma unroll 1
+      for (int k = 0; k < R * CPL; ++k)
+        {
+          const int X = X0 + k % CPL, Y = Y0 + k / CPL;
+          if (pvalid && X <= NF - 1 && Y <= NF - 1)
+            {
+              const double b = apply64(X, Y, true);
+              s += b * b;
+            }
+        }
+      problem_sum2<LW, SW, PPW>(s, dummy, sred[2], wave, slot, pl);
+      bnorm = sqrt(s);
+      tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+      __syncthreads();
+    }
+
+  T xr[R][CPL], rr[R][CPL], pr[R][CPL], q[R][CPL];
+  int it_total = 0;
+  int state = pvalid ? 0 : 1; // 0 running, 1 done
+  int conv = 0;
+  double rnorm = 0.0;
+
+  for (int cyc = 0;; ++cyc)
+    {
+      // ---- true FP64 residual of the current phi ----------------------------
+      // (rolled loop; the values pass through this thread's own LDS slots so
+      // the FP64 gathers do not hold registers across rows)
+      double s = 0, dummy = 0;
+#pragma unroll 1
+      for (int k = 0; k < R * CPL; ++k)
+        {
+          const int X = X0 + k % CPL, Y = Y0 + k / CPL;
+          if (pvalid && X <= NF - 1 && Y <= NF - 1)
+            {
+              const double r64 = -apply64(X, Y, false);
+              grid[Y * GP + X] = (T)r64;
+              s += r64 * r64;
+            }
+        }
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          rr[i][c] = dinv[i][c] != (T)0 ? gown[i * GP + c] : (T)0;
+      problem_sum2<LW, SW, PPW>(s, dummy, sred[2], wave, slot, pl);
+      rnorm = sqrt(s);
+      if (cyc == 0 && !a.x0)
+        {
+          bnorm = rnorm;
+          tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+        }
+      if (state == 0)
+        {
+          if (rnorm <= tol_eff)
+            {
+              state = 1;
+              conv = 1;
+            }
+          else if (it_total >= a.max_iter || cyc >= a.max_refine)
+            state = 1;
+        }
+      if (!__syncthreads_or(state == 0))
+        break;
+
+      // ---- inner CG on K d = r ------------------------------------------------
+      const bool run = state == 0;
+      double inner_tol = 0.5 * tol_eff;
+      if (sizeof(T) < sizeof(double))
+        inner_tol = fmax(a.inner_rel_tol * rnorm, inner_tol);
+      double rz = 0, t0 = 0;
+#pragma unroll
+      for (int i = 0; i < R; ++i)
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          {
+            xr[i][c] = 0;
+            const T z = run ? rr[i][c] * dinv[i][c] : (T)0;
+            pr[i][c] = z;
+            rz += (double)(rr[i][c] * z);
+            gown[i * GP + c] = z;
+          }
+      problem_sum2<LW, SW, PPW>(rz, t0, sred[1], wave, slot, pl);
+      bool irun = run;
+      __syncthreads();
+      while (true)
+        {
+          // q = K p from the LDS grid, rows slid through a 3-row window
+          T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
+#pragma unroll
+          for (int k = 0; k < CPL + 2; ++k)
+            {
+              w0[k] = gwin[k];
+              w1[k] = gwin[GP + k];
+            }
+          double pq = 0;
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+            {
+#pragma unroll
+              for (int k = 0; k < CPL + 2; ++k)
+                w2[k] = gwin[(i + 2) * GP + k];
+#pragma unroll
+              for (int c = 0; c < CPL; ++c)
+                {
+                  T y;
+                  if constexpr (MODE == kModeScalar)
+                    {
+                      const T aNE = cc[i + 1][c + 1][0], aNW = cc[i + 1][c][0], aSE = cc[i][c + 1][0],
+                              aSW = cc[i][c][0];
+                      y = kc * ((aNE + aNW) + (aSE + aSW)) * w1[c + 1];
+                      y -= kh * ((aNE + aSE) * w1[c + 2] + (aNW + aSW) * w1[c]);
+                      y -= kv * ((aNE + aNW) * w2[c + 1] + (aSE + aSW) * w0[c + 1]);
+                      y -= kd * (aNE * w2[c + 2] + aNW * w2[c] + aSE * w0[c + 2] + aSW * w0[c]);
+                    }
+                  else
+                    y = gather6<T>(w1[c + 1], w1[c + 2], w1[c], w2[c + 1], w0[c + 1], w2[c + 2], w2[c], w0[c + 2],
+                                   w0[c], cond_T(i + 1, c + 1), cond_T(i + 1, c), cond_T(i, c + 1), cond_T(i, c));
+                  q[i][c] = dinv[i][c] != (T)0 ? y : (T)0;
+                  pq += (double)(pr[i][c] * q[i][c]);
+                }
+#pragma unroll
+              for (int k = 0; k < CPL + 2; ++k)
+                {
+                  w0[k] = w1[k];
+                  w1[k] = w2[k];
+                }
+            }
+          problem_sum2<LW, SW, PPW>(pq, t0, sred[0], wave, slot, pl);
+          const T alpha = irun ? (T)(rz / pq) : (T)0;
+          double rz_new = 0, rn2 = 0;
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              {
+                xr[i][c] += alpha * pr[i][c];
+                rr[i][c] -= alpha * q[i][c];
+                const T z = rr[i][c] * dinv[i][c];
+                rz_new += (double)(rr[i][c] * z);
+                rn2 += (double)(rr[i][c] * rr[i][c]);
+                q[i][c] = z;
+              }
+          problem_sum2<LW, SW, PPW>(rz_new, rn2, sred[1], wave, slot, pl);
+          if (irun)
+            {
+              ++it_total;
+              if (sqrt(rn2) <= inner_tol || it_total >= a.max_iter)
+                irun = false;
+              else
+                {
+                  const T beta = (T)(rz_new / rz);
+                  rz = rz_new;
+#pragma unroll
+                  for (int i = 0; i < R; ++i)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c)
+                      {
+                        pr[i][c] = q[i][c] + beta * pr[i][c];
+                        gown[i * GP + c] = pr[i][c];
+                      }
+                }
+            }
+          if (!__syncthreads_or(irun))
+            break;
+        }
+      // ---- phi += d in FP64 ------------------------------------------------------
+      if (run)
+        {
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              if (dinv[i][c] != (T)0)
+                phi[(Y0 + i) * NP + X0 + c] += (double)xr[i][c];
+        }
+      __syncthreads();
+    }
+
+  if (pvalid && pl == 0 && wave == 0)
+    {
+      const size_t o = (size_t)cell * a.nb + basis;
+      if (a.iters)
+        a.iters[o] = it_total;
+      if (a.status)
+        a.status[o] = conv ? 0 : 1;
+      if (a.resid)
+        a.resid[o] = rnorm;
+    }
+}
+
+// ---- coefficient preparation for kModeGlobal: 6 FP64 conductances per fine cell
+// from any 2D coefficient kind, QGauss(2) on the box fine cell
+// (diffusion_problem_basis.tpp:207-222).
+__global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                             const double *coef, const long long *coef_off, double *cond, const long long *cond_off)
+{
+  const int ncell = nf * nf;
+  const long long total = (long long)n_cells_class * ncell;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
+    {
+      const int cell = cells[t / ncell];
+      const int f = (int)(t % ncell);
+      const double *bx = box + (size_t)cell * 4;
+      const double hx = (bx[1] - bx[0]) / nf, hy = (bx[3] - bx[2]) / nf;
+      const double *cf = coef + coef_off[cell];
+      double K[4][4] = {{0}};
+      const double s = 0.5 / sqrt(3.0);
+      for (int q = 0; q < 4; ++q)
+        {
+          const double xi = (q & 1) ? 0.5 + s : 0.5 - s, eta = (q & 2) ? 0.5 + s : 0.5 - s;
+          double A11, A12, A22;
+          switch (coef_kind)
+            {
+              case 0: A11 = A22 = cf[f]; A12 = 0; break;
+              case 1: A11 = cf[3 * f]; A12 = cf[3 * f + 1]; A22 = cf[3 * f + 2]; break;
+              case 2: A11 = A22 = cf[4 * f + q]; A12 = 0; break;
+              default: A11 = cf[12 * f + 3 * q]; A12 = cf[12 * f + 3 * q + 1]; A22 = cf[12 * f + 3 * q + 2]; break;
+            }
+          double gx[4], gy[4];
+          for (int v = 0; v < 4; ++v)
+            {
+              const double fx = (v & 1) ? xi : 1.0 - xi, fy = (v & 2) ? eta : 1.0 - eta;
+              gx[v] = ((v & 1) ? 1.0 : -1.0) * fy / hx;
+              gy[v] = ((v & 2) ? 1.0 : -1.0) * fx / hy;
+            }
+          const double jxw = 0.25 * hx * hy;
+          for (int u = 0; u < 4; ++u)
+            for (int v = 0; v < 4; ++v)
+              K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
+        }
+      double *o = cond + cond_off[cell] + 6 * (size_t)f;
+      o[0] = -K[0][1];
+      o[1] = -K[0][2];
+      o[2] = -K[0][3];
+      o[3] = -K[1][2];
+      o[4] = -K[1][3];
+      o[5] = -K[2][3];
+    }
+}
+
+// ---- launch table -----------------------------------------------------------------
+namespace {
+
+template <typename T, int NF, int CPL, int SW, int R, int MODE>
+hipError_t launch_one(const LocalArgs &a, hipStream_t st)
+{
+  constexpr int PPW = kWave / (NF / CPL);
+  const int grid = (a.n_problems + PPW - 1) / PPW;
+  if (grid == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE>), dim3(grid), dim3(kWave * SW), 0, st, a);
+  return hipGetLastError();
+}
+
+template <typename T, int MODE>
+hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
+{
+  switch (nf)
+    {
+      case 16:
+        return variant == 1 ? launch_one<T, 16, 1, 1, 15, MODE>(a, st) : launch_one<T, 16, 1, 2, 8, MODE>(a, st);
+      case 32:
+        return variant == 1 ? launch_one<T, 32, 1, 2, 16, MODE>(a, st) : launch_one<T, 32, 1, 4, 8, MODE>(a, st);
+      case 64:
+        return variant == 1 ? launch_one<T, 64, 1, 4, 16, MODE>(a, st) : launch_one<T, 64, 1, 8, 8, MODE>(a, st);
+      case 128:
+        return launch_one<T, 128, 2, 16, 8, MODE>(a, st);
+      default:
+        return hipErrorInvalidValue;
+    }
+}
+
+} // namespace
+
+int local2d_variants(int nf) { return nf == 128 ? 1 : 2; }
+
+hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st)
+{
+  if (variant > 0)
+    variant -= 1; // 1-based explicit ids -> table index
+  if (fp32)
+    {
+      if (mode == kModeScalar)
+        return launch_mode<float, kModeScalar>(nf, variant, a, st);
+      if (mode == kModeTensor)
+        return launch_mode<float, kModeTensor>(nf, variant, a, st);
+      return launch_mode<float, kModeGlobal>(nf, variant, a, st);
+    }
+  if (mode == kModeScalar)
+    return launch_mode<double, kModeScalar>(nf, variant, a, st);
+  if (mode == kModeTensor)
+    return launch_mode<double, kModeTensor>(nf, variant, a, st);
+  return launch_mode<double, kModeGlobal>(nf, variant, a, st);
+}
+
+hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                               const double *coef, const long long *coef_off, double *cond,
+                               const long long *cond_off, hipStream_t st)
+{
+  const long long total = (long long)n_cells_class * nf * nf;
+  if (total == 0)
+    return hipSuccess;
+  const int block = 256;
+  long long grid = (total + block - 1) / block;
+  if (grid > 65536)
+    grid = 65536;
+  hipLaunchKernelGGL(prep_cond_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, coef_kind,
+                     box, coef, coef_off, cond, cond_off);
+  return hipGetLastError();
+}
+
+} // namespace msfem

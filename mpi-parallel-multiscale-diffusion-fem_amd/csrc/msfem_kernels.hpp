@@ -1,0 +1,52 @@
+// Host-side launchers of the MsFEM kernels (implemented in the .hip files).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "msfem_device.hpp"
+
+namespace msfem {
+
+// local2d.hip
+int local2d_variants(int nf);
+hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st);
+hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                               const double *coef, const long long *coef_off, double *cond,
+                               const long long *cond_off, hipStream_t st);
+
+// local3d.hip
+int local3d_variants(int nf);
+hipError_t launch_local_solve_3d(bool fp32, int nf, int variant, const LocalArgs &a, hipStream_t st);
+
+// element.hip: A^K, F^K of every cell of one size class; partition-of-unity fix-up.
+struct ElementArgs {
+  int n_cells_class;
+  const int *cells;
+  int dim, nf, nb;
+  int coef_kind;              // coefficient kind of the batch (MSFEM_COEF_*)
+  const double *box;
+  const double *coef;
+  const long long *coef_off;
+  const double *cond;         // 2D conductance tiles when the class used kModeGlobal (else nullptr)
+  const long long *cond_off;
+  const double *bases;
+  const long long *bas_off;
+  int rhs_kind;
+  double rhs_value;
+  double *Ae, *Fe;
+};
+hipError_t launch_element(const ElementArgs &a, hipStream_t st);
+hipError_t launch_pou_fixup(int n_cells_class, const int *cells, int dim, int nf, double *bases,
+                            const long long *bas_off, hipStream_t st);
+
+// coarse.hip
+hipError_t launch_coarse_assemble(int dim, int nc, const double *Ae, const double *Fe, int bc_kind, double *Kc,
+                                  double *Fc, unsigned char *cmask, double *cval, hipStream_t st);
+hipError_t launch_coarse_solve(int dim, int nc, const double *Kc, const double *Fc, const unsigned char *cmask,
+                               const double *cval, double rel_tol, int max_iter, double *u, int *iters,
+                               double *work, hipStream_t st);
+size_t coarse_solve_work_doubles(int dim, int nc);
+hipError_t launch_reconstruct(int n_cells, int nb, const long long *bas_off, const long long *out_off,
+                              const int *n_nodes, const double *bases, const double *w, double *u, hipStream_t st);
+
+} // namespace msfem

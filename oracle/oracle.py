@@ -1,0 +1,250 @@
+"""ctypes front end of the CPU FP64 oracle (msfem_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, always as the checker.  The product library never
+imports this module.
+
+Every function restates one piece of the reference (file:line citations are in
+msfem_oracle.c):
+  basis_q1_coeffs / basis_q1_value   include/coefficients/basis_q1.tpp:26-113
+  matrix_coeff_ref                   include/coefficients/matrix_coeff.tpp:17-91
+  fine_element / local_assemble      include/base/diffusion_problem_basis.tpp:161-242
+  local_bases_direct                 diffusion_problem_basis.tpp:450-465 (exact solve)
+  local_bases_ssor_pcg               diffusion_problem_basis.tpp:295-317 (the recipe)
+  element                            diffusion_problem_basis.tpp:247-285
+  cell_build / batch_build           diffusion_problem_basis.tpp:440-474,
+                                     diffusion_problem_ms.tpp:42-88
+  coarse_assemble / coarse_solve     diffusion_problem_ms.tpp:108-296
+  reconstruct                        diffusion_problem_basis.tpp:354-377
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmsfem_oracle.so")
+
+COEF_SCALAR_CELL, COEF_TENSOR_CELL, COEF_SCALAR_QP, COEF_TENSOR_QP = 0, 1, 2, 3
+RHS_CONST, RHS_SINSIN = 0, 1
+SOLVER_DIRECT, SOLVER_SSOR_PCG = 0, 1
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_up = ctypes.POINTER(ctypes.c_ubyte)
+_i, _d = ctypes.c_int, ctypes.c_double
+
+
+def _declare(L):
+    sig = {
+        "orc_n_sym": (_i, [_i]),
+        "orc_coef_stride": (_i, [_i, _i]),
+        "orc_basis_q1_coeffs": (_i, [_i, _dp, _dp]),
+        "orc_basis_q1_value": (_d, [_i, _dp, _i, _dp]),
+        "orc_box_vertices": (None, [_i, _dp, _dp]),
+        "orc_matrix_coeff_ref": (None, [_i, _dp, _dp]),
+        "orc_fine_element": (None, [_i, _dp, _dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_n_nodes": (_i, [_i, _i]),
+        "orc_local_assemble": (None, [_i, _i, _dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_stencil_apply": (None, [_i, _i, _dp, _dp, _dp]),
+        "orc_dirichlet_data": (None, [_i, _i, _dp, _i, _dp]),
+        "orc_local_bases_direct": (_i, [_i, _i, _dp, _dp, _dp]),
+        "orc_local_bases_ssor_pcg": (_i, [_i, _i, _dp, _dp, _d, _d, _i, _dp, _ip]),
+        "orc_element": (None, [_i, _i, _dp, _dp, _dp, _dp, _dp]),
+        "orc_cell_build": (_i, [_i, _i, _dp, _i, _dp, _i, _d, _i, _d, _d, _i, _dp, _dp, _dp, _ip]),
+        "orc_batch_build": (_i, [_i, _i, _dp, _ip, _i, _dp, _i, _d, _i, _d, _d, _i, _i, _dp, _dp, _dp, _ip]),
+        "orc_max_threads": (_i, []),
+        "orc_coarse_assemble": (None, [_i, _i, _dp, _dp, _i, _dp, _dp, _up, _dp]),
+        "orc_coarse_solve": (_i, [_i, _i, _dp, _dp, _up, _dp, _dp]),
+        "orc_reconstruct": (None, [_i, _i, _dp, _dp, _dp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _d64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _ptr(a, t=_dp):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def n_nodes(dim, nf):
+    return (nf + 1) ** dim
+
+
+def coef_stride(dim, kind):
+    return lib().orc_coef_stride(dim, kind)
+
+
+def basis_q1_coeffs(dim, verts):
+    v = _d64(verts).reshape(-1)
+    c = np.zeros((1 << dim) ** 2)
+    rc = lib().orc_basis_q1_coeffs(dim, _ptr(v), _ptr(c))
+    if rc:
+        raise ValueError("singular vertex monomial matrix")
+    return c.reshape(1 << dim, 1 << dim)
+
+
+def basis_q1_value(dim, coeffs, index, p):
+    c = _d64(coeffs).reshape(-1)
+    pp = _d64(p)
+    return lib().orc_basis_q1_value(dim, _ptr(c), index, _ptr(pp))
+
+
+def box_vertices(dim, box):
+    b = _d64(box).reshape(-1)
+    v = np.zeros((1 << dim) * dim)
+    lib().orc_box_vertices(dim, _ptr(b), _ptr(v))
+    return v.reshape(1 << dim, dim)
+
+
+def matrix_coeff_ref(dim, p):
+    pp = _d64(p)
+    a = np.zeros(3 if dim == 2 else 6)
+    lib().orc_matrix_coeff_ref(dim, _ptr(pp), _ptr(a))
+    return a
+
+
+def fine_element(dim, x0, h, coef_kind, cdata, rhs_kind=RHS_CONST, rhs_value=1.0):
+    nv = 1 << dim
+    Ke = np.zeros(nv * nv)
+    be = np.zeros(nv)
+    lib().orc_fine_element(dim, _ptr(_d64(x0)), _ptr(_d64(h)), coef_kind, _ptr(_d64(cdata)), rhs_kind,
+                           rhs_value, _ptr(Ke), _ptr(be))
+    return Ke.reshape(nv, nv), be
+
+
+def local_assemble(dim, nf, box, coef_kind, tile, rhs_kind=RHS_CONST, rhs_value=1.0):
+    nn = n_nodes(dim, nf)
+    K = np.zeros(nn * 3 ** dim)
+    b = np.zeros(nn)
+    lib().orc_local_assemble(dim, nf, _ptr(_d64(box).reshape(-1)), coef_kind, _ptr(_d64(tile).reshape(-1)),
+                             rhs_kind, rhs_value, _ptr(K), _ptr(b))
+    return K.reshape(nn, 3 ** dim), b
+
+
+def stencil_apply(dim, nf, K, x):
+    y = np.zeros(n_nodes(dim, nf))
+    lib().orc_stencil_apply(dim, nf, _ptr(_d64(K).reshape(-1)), _ptr(_d64(x)), _ptr(y))
+    return y
+
+
+def stencil_to_dense(dim, nf, K):
+    """Dense matrix of the structured stencil (small grids only)."""
+    np1 = nf + 1
+    nn = n_nodes(dim, nf)
+    A = np.zeros((nn, nn))
+    for n in range(nn):
+        X, Y, Z = n % np1, (n // np1) % np1, (n // (np1 * np1) if dim == 3 else 0)
+        for t in range(3 ** dim):
+            ox, oy, oz = t % 3 - 1, (t // 3) % 3 - 1, (t // 9 - 1 if dim == 3 else 0)
+            XX, YY, ZZ = X + ox, Y + oy, Z + oz
+            if not (0 <= XX <= nf and 0 <= YY <= nf and 0 <= ZZ <= (nf if dim == 3 else 0)):
+                continue
+            A[n, XX + np1 * (YY + np1 * ZZ)] += K[n, t]
+    return A
+
+
+def dirichlet_data(dim, nf, box, index):
+    g = np.zeros(n_nodes(dim, nf))
+    lib().orc_dirichlet_data(dim, nf, _ptr(_d64(box).reshape(-1)), index, _ptr(g))
+    return g
+
+
+def local_bases_direct(dim, nf, box, K):
+    phi = np.zeros((1 << dim) * n_nodes(dim, nf))
+    rc = lib().orc_local_bases_direct(dim, nf, _ptr(_d64(box).reshape(-1)), _ptr(_d64(K).reshape(-1)), _ptr(phi))
+    if rc:
+        raise ValueError("local matrix not SPD")
+    return phi.reshape(1 << dim, -1)
+
+
+def local_bases_ssor_pcg(dim, nf, box, K, omega=1.6, abs_tol=1e-12, max_iter=1000):
+    nb = 1 << dim
+    phi = np.zeros(nb * n_nodes(dim, nf))
+    its = np.zeros(nb, dtype=np.int32)
+    fails = lib().orc_local_bases_ssor_pcg(dim, nf, _ptr(_d64(box).reshape(-1)), _ptr(_d64(K).reshape(-1)), omega,
+                                           abs_tol, max_iter, _ptr(phi), _ptr(its, _ip))
+    return phi.reshape(nb, -1), its, fails
+
+
+def element(dim, nf, K, b, phi):
+    nb = 1 << dim
+    Ae = np.zeros(nb * nb)
+    Fe = np.zeros(nb)
+    lib().orc_element(dim, nf, _ptr(_d64(K).reshape(-1)), _ptr(_d64(b)), _ptr(_d64(phi).reshape(-1)), _ptr(Ae),
+                      _ptr(Fe))
+    return Ae.reshape(nb, nb), Fe
+
+
+def batch_build(dim, box, nf, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, solver=SOLVER_DIRECT,
+                omega=1.6, abs_tol=1e-12, max_iter=1000, n_threads=0, want_bases=False):
+    """All cells of a batch.  box [n][dim][2], nf [n] (int), coef: concatenated tiles.
+    Returns dict(Ae [n][nb][nb], Fe [n][nb], iters [n][nb], bases (flat or None), fails)."""
+    box = _d64(box).reshape(-1)
+    nf = np.ascontiguousarray(nf, dtype=np.int32)
+    n = nf.shape[0]
+    nb = 1 << dim
+    Ae = np.zeros(n * nb * nb)
+    Fe = np.zeros(n * nb)
+    its = np.zeros(n * nb, dtype=np.int32)
+    bases = None
+    if want_bases:
+        bases = np.zeros(int(sum(nb * (int(k) + 1) ** dim for k in nf)))
+    fails = lib().orc_batch_build(dim, n, _ptr(box), _ptr(nf, _ip), coef_kind, _ptr(_d64(coef).reshape(-1)),
+                                  rhs_kind, rhs_value, solver, omega, abs_tol, max_iter, n_threads, _ptr(bases),
+                                  _ptr(Ae), _ptr(Fe), _ptr(its, _ip))
+    return {"Ae": Ae.reshape(n, nb, nb), "Fe": Fe.reshape(n, nb), "iters": its.reshape(n, nb), "bases": bases,
+            "fails": fails}
+
+
+def max_threads():
+    return lib().orc_max_threads()
+
+
+def coarse_assemble(dim, nc, Ae, Fe, bc_kind):
+    nn = (nc + 1) ** dim
+    Kc = np.zeros(nn * 3 ** dim)
+    Fc = np.zeros(nn)
+    cm = np.zeros(nn, dtype=np.uint8)
+    cv = np.zeros(nn)
+    lib().orc_coarse_assemble(dim, nc, _ptr(_d64(Ae).reshape(-1)), _ptr(_d64(Fe).reshape(-1)), bc_kind, _ptr(Kc),
+                              _ptr(Fc), _ptr(cm, _up), _ptr(cv))
+    return Kc.reshape(nn, 3 ** dim), Fc, cm, cv
+
+
+def coarse_solve(dim, nc, Kc, Fc, cmask, cval):
+    u = np.zeros((nc + 1) ** dim)
+    rc = lib().orc_coarse_solve(dim, nc, _ptr(_d64(Kc).reshape(-1)), _ptr(_d64(Fc)),
+                                _ptr(np.ascontiguousarray(cmask, dtype=np.uint8), _up), _ptr(_d64(cval)), _ptr(u))
+    if rc:
+        raise ValueError("coarse matrix not SPD")
+    return u
+
+
+def reconstruct(dim, nf, phi, w):
+    u = np.zeros(n_nodes(dim, nf))
+    lib().orc_reconstruct(dim, nf, _ptr(_d64(phi).reshape(-1)), _ptr(_d64(w)), _ptr(u))
+    return u

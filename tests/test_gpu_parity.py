@@ -1,0 +1,217 @@
+"""GPU parity: the HIP engine against the CPU oracle and the golden fixtures.
+
+Bars (stated per test): plain FP64 path within 1e-10 relative of the exact
+(direct-solve) values on basis nodal values, A^K and F^K; FP32+FP64-refinement
+path to the same bar when refined to the same tolerance, and to <= 1e-6
+relative energy norm at its loose setting.
+"""
+import glob
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import msfem
+import msfem_cases as C
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TOL = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = msfem.Context(0)
+    yield c
+    c.close()
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def check_against_oracle(case, out, ref, tol=TOL):
+    assert (out["status"] == 0).all()
+    assert rel(out["bases"], ref["bases"]) < tol
+    assert rel(out["Ae"], ref["Ae"]) < tol
+    assert rel(out["Fe"], ref["Fe"]) < tol
+
+
+GPU_GOLDEN = [p for p in sorted(glob.glob(os.path.join(GOLDEN, "*.npz")))
+              if int(np.load(p)["nf"]) in ((16, 32, 64, 128) if int(np.load(p)["dim"]) == 2 else (8, 16))]
+
+
+@pytest.mark.parametrize("path", GPU_GOLDEN, ids=[os.path.basename(p) for p in GPU_GOLDEN])
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_golden(ctx, path, prec):
+    g = np.load(path)
+    say(f"golden {os.path.basename(path)} prec={prec}")
+    dim = int(g["dim"])
+    out = ctx.batch_build(dim, [int(g["nf"])], g["box"][None], int(g["coef_kind"]), g["coef"], int(g["rhs_kind"]),
+                          float(g["rhs_value"]), opts=msfem.default_opts(precision=prec))
+    assert rel(out["bases"], g["bases"].reshape(-1)) < TOL
+    assert rel(out["Ae"][0], g["Ae"]) < TOL
+    assert rel(out["Fe"][0], g["Fe"]) < TOL
+
+
+def test_periodic_8x8_nf32_and_coarse(ctx):
+    """8x8 coarse, 32^2 fine, alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps), eps = 1/64,
+    f = 1, zero Dirichlet: bases, A^K, F^K, coarse system and coarse solution."""
+    say("periodic 8x8 nf32")
+    case = C.periodic_cells()
+    out = C.gpu_build(ctx, case)
+    ref = C.oracle_build(case)
+    check_against_oracle(case, out, ref)
+    Kc, Fc, cm, cv = ctx.coarse_assemble(2, 8, out["Ae"], out["Fe"], msfem.BC_ZERO_DIRICHLET)
+    Kr, Fr, cmr, cvr = O.coarse_assemble(2, 8, ref["Ae"], ref["Fe"], 0)
+    assert rel(Kc, Kr) < TOL and rel(Fc, Fr) < TOL and (cm == cmr).all()
+    u, it = ctx.coarse_solve(2, 8, Kc, Fc, cm, cv, rel_tol=1e-14)
+    ur = O.coarse_solve(2, 8, Kr, Fr, cmr, cvr)
+    assert rel(u, ur) < TOL, it
+
+
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_lognormal_nf64(ctx, prec):
+    """Cells of the 64x64-coarse lognormal batch (64^2 fine, contrast 1e3, f = 1)."""
+    say(f"lognormal nf64 prec={prec}")
+    case = C.lognormal_cells(which=np.array([0, 1, 63, 64, 2080, 4095, 1234, 777, 3000, 17, 4031, 99]))
+    out = C.gpu_build(ctx, case, precision=prec)
+    ref = C.oracle_build(case)
+    check_against_oracle(case, out, ref)
+    assert (out["iters"] > 50).all()
+
+
+def test_lognormal_mixed_loose_energy_error(ctx):
+    """FP32 CG + FP64 refinement stopped at rel. residual 1e-8: energy-norm error <= 1e-6."""
+    say("lognormal nf64 mixed loose")
+    case = C.lognormal_cells(which=np.arange(0, 4096, 512))
+    out = C.gpu_build(ctx, case, precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+    ref = C.oracle_build(case)
+    assert (out["status"] == 0).all()
+    err = C.energy_error(case, out["bases"], ref["bases"])
+    say(f"energy error {err:.3e}")
+    assert err <= 1e-6
+    assert rel(out["Ae"], ref["Ae"]) < 1e-6
+
+
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_tensor_uneven_classes(ctx, prec):
+    """Uneven batch: SPD tensor per fine cell, every resolution class 16..128, f = sin sin."""
+    say(f"tensor uneven prec={prec}")
+    case = C.tensor_cells(nc=128, which=np.arange(0, 16384, 1024), n_fine=[16, 32, 64, 128] * 4)
+    out = C.gpu_build(ctx, case, precision=prec)
+    ref = C.oracle_build(case)
+    check_against_oracle(case, out, ref)
+
+
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_lognormal_3d(ctx, prec):
+    """3D: 16^3 coarse, 16^3 fine per cell, lognormal, 8 bases per cell."""
+    say(f"lognormal 3d nf16 prec={prec}")
+    case = C.lognormal_cells(dim=3, nc=16, nf=16, seed=5, which=np.array([0, 15, 273, 4095]))
+    out = C.gpu_build(ctx, case, precision=prec)
+    ref = C.oracle_build(case)
+    check_against_oracle(case, out, ref)
+
+
+def test_reference_default_cells(ctx):
+    """Reference default problem: MatrixCoeff with PI_D at the quadrature points,
+    f = 2, 128^2 fine per cell (source/main.cxx:23)."""
+    say("reference default nf128")
+    case = C.reference_cells(which=np.array([0, 27]))
+    out = C.gpu_build(ctx, case)
+    ref = C.oracle_build(case)
+    check_against_oracle(case, out, ref)
+
+
+def test_partition_of_unity_option(ctx):
+    say("partition of unity")
+    case = C.lognormal_cells(which=np.array([5, 6, 7]))
+    full = C.gpu_build(ctx, case)
+    pou = C.gpu_build(ctx, case, partition_of_unity=1)
+    assert rel(pou["bases"], full["bases"]) < TOL
+    assert rel(pou["Ae"], full["Ae"]) < TOL
+    assert (pou["iters"][:, 3] == 0).all() and (pou["iters"][:, :3] > 0).all()
+
+
+def test_initial_guess_shortens_solve(ctx):
+    say("initial guess")
+    case = C.lognormal_cells(which=np.array([10, 11]))
+    ref = C.oracle_build(case)
+    rng = np.random.default_rng(0)
+    x0 = ref["bases"] + 1e-6 * rng.standard_normal(ref["bases"].shape)
+    cold = C.gpu_build(ctx, case)
+    o = msfem.default_opts()
+    warm = ctx.batch_build(2, case["n_fine"], case["box"], case["coef_kind"], case["coef"], x0=x0, opts=o)
+    check_against_oracle(case, warm, ref)
+    assert warm["iters"].sum() < cold["iters"].sum()
+
+
+def test_iteration_cap_reports_noconvergence(ctx):
+    say("iteration cap")
+    case = C.lognormal_cells(which=np.array([3]))
+    with pytest.raises(msfem.MsfemError) as e:
+        C.gpu_build(ctx, case, max_iter=5)
+    assert e.value.code == msfem.ERR_NOCONV
+    o = msfem.default_opts(max_iter=5)
+    out = ctx.batch_build(2, case["n_fine"], case["box"], case["coef_kind"], case["coef"], opts=o, raise_noconv=False)
+    assert (out["status"] == 1).all() and (out["iters"] == 5).all()
+
+
+def test_invalid_arguments_are_rejected(ctx):
+    say("invalid args")
+    case = C.lognormal_cells(which=np.array([0]))
+    with pytest.raises(msfem.MsfemError) as e:
+        ctx.batch_build(2, [48], case["box"], case["coef_kind"], case["coef"])
+    assert e.value.code == msfem.ERR_ARG
+    with pytest.raises(msfem.MsfemError):
+        ctx.batch_build(3, [16], C.unit_cells(3, 2)[:1], msfem.COEF_TENSOR_CELL, np.ones(16 ** 3 * 6))
+
+
+def test_full_lognormal_batch_properties(ctx):
+    """The whole 64x64-coarse batch (4096 cells, 64^2 fine): size-independent
+    properties on every cell, oracle parity on a sample."""
+    say("full lognormal 4096 cells")
+    case = C.lognormal_cells()
+    out = C.gpu_build(ctx, case)
+    assert (out["status"] == 0).all()
+    Ae, Fe = out["Ae"], out["Fe"]
+    assert np.abs(Ae - Ae.transpose(0, 2, 1)).max() < 1e-12
+    assert np.abs(Ae.sum(2)).max() < 1e-10
+    assert np.abs(Fe.sum(1) - (1 / 64) ** 2).max() < 1e-15
+    b = out["bases"].reshape(4096, 4, 65 * 65)
+    assert np.abs(b.sum(1) - 1).max() < 1e-10
+    assert b.min() > -1e-10 and b.max() < 1 + 1e-10
+    sample = np.array([0, 2047, 4095])
+    ref = C.oracle_build(C.lognormal_cells(which=sample))
+    assert rel(b[sample].reshape(-1), ref["bases"]) < TOL
+    assert rel(Ae[sample], ref["Ae"]) < TOL
+    # coarse system on the device path vs oracle
+    Kc, Fc, cm, cv = ctx.coarse_assemble(2, 64, Ae, Fe, msfem.BC_ZERO_DIRICHLET)
+    u, it = ctx.coarse_solve(2, 64, Kc, Fc, cm, cv, rel_tol=1e-14)
+    ur = O.coarse_solve(2, 64, *O.coarse_assemble(2, 64, Ae, Fe, 0))
+    assert rel(u, ur) < TOL, it
+
+
+def test_reconstruction_device(ctx):
+    say("reconstruction")
+    import torch
+    case = C.lognormal_cells(which=np.array([0, 1, 2]))
+    out = C.gpu_build(ctx, case)
+    w = np.random.default_rng(1).standard_normal((3, 4))
+    dev = torch.device("cuda:0")
+    tb = torch.from_numpy(out["bases"]).to(dev)
+    tw = torch.from_numpy(w).to(dev)
+    tu = torch.zeros(3 * 65 * 65, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    ctx.reconstruct_device(2, case["n_fine"], tb.data_ptr(), tw.data_ptr(), tu.data_ptr())
+    torch.cuda.synchronize()
+    want = np.concatenate([O.reconstruct(2, 64, out["bases"][c * 4 * 4225:(c + 1) * 4 * 4225], w[c]) for c in range(3)])
+    assert rel(tu.cpu().numpy(), want) < 1e-14
