@@ -57,11 +57,8 @@ def main():
               "the multi-GPU split (cells sharded, one coarse-system reduction) is not built.", file=sys.stderr)
         sys.exit(2)
 
-    import torch
-
     import msfem
 
-    dev = torch.device("cuda:0")
     case, nc = workload(args.workload)
     dim = case["dim"]
     n = len(case["n_fine"])
@@ -71,45 +68,44 @@ def main():
     ni = (nf - 1) ** dim
     ncf = nf ** dim
 
-    t_box = torch.from_numpy(case["box"].reshape(-1).copy()).to(dev)
-    t_coef = torch.from_numpy(case["coef"]).to(dev)
-    t_Ae = torch.zeros(n * nb * nb, dtype=torch.float64, device=dev)
-    t_Fe = torch.zeros(n * nb, dtype=torch.float64, device=dev)
-    t_bases = torch.zeros(n * nb * nn, dtype=torch.float64, device=dev)
-    t_it = torch.zeros(n * nb, dtype=torch.int32, device=dev)
-    t_st = torch.zeros(n * nb, dtype=torch.int32, device=dev)
-    ncn = (nc + 1) ** dim
-    t_Kc = torch.zeros(ncn * 3 ** dim, dtype=torch.float64, device=dev)
-    t_Fc = torch.zeros(ncn, dtype=torch.float64, device=dev)
-    t_cm = torch.zeros(ncn, dtype=torch.uint8, device=dev)
-    t_cv = torch.zeros(ncn, dtype=torch.float64, device=dev)
-    t_u = torch.zeros(ncn, dtype=torch.float64, device=dev)
-    t_cit = torch.zeros(1, dtype=torch.int32, device=dev)
-
     ctx = msfem.Context(0)
+    t_box = ctx.to_device(case["box"].reshape(-1))
+    t_coef = ctx.to_device(case["coef"])
+    t_Ae = ctx.alloc(n * nb * nb, np.float64)
+    t_Fe = ctx.alloc(n * nb, np.float64)
+    t_bases = ctx.alloc(n * nb * nn, np.float64)
+    t_it = ctx.alloc(n * nb, np.int32)
+    t_st = ctx.alloc(n * nb, np.int32)
+    ncn = (nc + 1) ** dim
+    t_Kc = ctx.alloc(ncn * 3 ** dim, np.float64)
+    t_Fc = ctx.alloc(ncn, np.float64)
+    t_cm = ctx.alloc(ncn, np.uint8)
+    t_cv = ctx.alloc(ncn, np.float64)
+    t_u = ctx.alloc(ncn, np.float64)
+    t_cit = ctx.alloc(1, np.int32)
     ctx.set_profiling(True)
-    stream = torch.cuda.current_stream().cuda_stream
+    stream = ctx.stream
 
     def step(opts):
-        ctx.batch_build_device(dim, case["n_fine"], t_box.data_ptr(), case["coef_kind"], t_coef.data_ptr(),
-                               t_Ae.data_ptr(), t_Fe.data_ptr(), t_bases.data_ptr(), t_it.data_ptr(), t_st.data_ptr(),
+        ctx.batch_build_device(dim, case["n_fine"], t_box.ptr, case["coef_kind"], t_coef.ptr,
+                               t_Ae.ptr, t_Fe.ptr, t_bases.ptr, t_it.ptr, t_st.ptr,
                                rhs_kind=case["rhs_kind"], rhs_value=case["rhs_value"], opts=opts, stream=stream)
-        ctx.coarse_assemble_device(dim, nc, t_Ae.data_ptr(), t_Fe.data_ptr(), msfem.BC_ZERO_DIRICHLET, t_Kc.data_ptr(),
-                                   t_Fc.data_ptr(), t_cm.data_ptr(), t_cv.data_ptr(), stream=stream)
+        ctx.coarse_assemble_device(dim, nc, t_Ae.ptr, t_Fe.ptr, msfem.BC_ZERO_DIRICHLET, t_Kc.ptr,
+                                   t_Fc.ptr, t_cm.ptr, t_cv.ptr, stream=stream)
 
     def run(opts, steps, warmup):
         for _ in range(warmup):
             step(opts)
-        torch.cuda.synchronize()
+        ctx.synchronize()
         solve_ms = []
         t0 = time.perf_counter()
         for _ in range(steps):
             step(opts)
             solve_ms.append(ctx.profile()[0])
-        torch.cuda.synchronize()
+        ctx.synchronize()
         t1 = time.perf_counter()
-        assert int(t_st.sum().item()) == 0, "local solves did not converge"
-        return (t1 - t0) * 1e3 / steps, float(np.mean(solve_ms)), t_it.cpu().numpy().astype(np.int64)
+        assert int(t_st.numpy().sum()) == 0, "local solves did not converge"
+        return (t1 - t0) * 1e3 / steps, float(np.mean(solve_ms)), t_it.numpy().astype(np.int64)
 
     # plain FP64 path (the parity path)
     o64 = msfem.default_opts(precision=msfem.PREC_FP64)
@@ -119,11 +115,11 @@ def main():
     ms32, solve32, it32 = run(o32, args.steps, args.warmup)
     # coarse solve (timed apart: one single-workgroup FP64 CG)
     step(o64)
-    torch.cuda.synchronize()
+    ctx.synchronize()
     tc0 = time.perf_counter()
-    ctx.coarse_solve_device(dim, nc, t_Kc.data_ptr(), t_Fc.data_ptr(), t_cm.data_ptr(), t_cv.data_ptr(), 1e-12,
-                            100000, t_u.data_ptr(), t_cit.data_ptr(), stream=stream)
-    torch.cuda.synchronize()
+    ctx.coarse_solve_device(dim, nc, t_Kc.ptr, t_Fc.ptr, t_cm.ptr, t_cv.ptr, 1e-12,
+                            100000, t_u.ptr, t_cit.ptr, stream=stream)
+    ctx.synchronize()
     coarse_ms = (time.perf_counter() - tc0) * 1e3
 
     s = 3 ** dim
@@ -162,10 +158,11 @@ def main():
         "variants": {"fp64_cells_per_s": round(cells_per_s, 1),
                      "fp32_refine_1e-6_energy_cells_per_s": round(n / (ms32 * 1e-3), 1),
                      "fp32_refine_ms_per_step": round(ms32, 4), "fp32_refine_solve_kernel_ms": round(solve32, 4),
-                     "coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.item())},
+                     "coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0])},
     }
 
     if not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
         import msfem_cases as C
         sample = np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64)
         sub = C.lognormal_cells(dim=dim, nc=nc, nf=nf, seed=2 if dim == 2 else 5, which=sample)

@@ -140,6 +140,17 @@ int msfem_version(void);
 int msfem_ctx_set_profiling(msfem_ctx *ctx, int on);
 int msfem_ctx_profile(msfem_ctx *ctx, double *ms_local_solve, double *ms_element);
 
+/* Device-memory plumbing for callers without an allocator of their own
+ * (no counterpart in the reference, whose data never leaves the host).
+ * Memory comes from hipMalloc on the context's device; copies are synchronous;
+ * msfem_ctx_stream returns the context's own stream for the *_device entry points. */
+int msfem_dev_alloc(msfem_ctx *ctx, uint64_t bytes, void **ptr);
+int msfem_dev_free(msfem_ctx *ctx, void *ptr);
+int msfem_memcpy_h2d(msfem_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+int msfem_memcpy_d2h(msfem_ctx *ctx, void *dst, const void *src, uint64_t bytes);
+void *msfem_ctx_stream(msfem_ctx *ctx);
+int msfem_ctx_synchronize(msfem_ctx *ctx);
+
 /* ---- the hot path ------------------------------------------------------ */
 /* Builds every cell of the batch: fine assembly, nb basis solves, A^K, F^K.
  * Replaces DiffusionProblemBasis<dim>::run() for all cells

@@ -451,6 +451,59 @@ int msfem_ctx_create(int device, msfem_ctx **out)
   return MSFEM_OK;
 }
 
+int msfem_dev_alloc(msfem_ctx *ctx, uint64_t bytes, void **ptr)
+{
+  if (!ctx || !ptr)
+    return fail(ctx, MSFEM_ERR_ARG, "null argument");
+  *ptr = nullptr;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMalloc(ptr, bytes > 0 ? bytes : 1));
+  return MSFEM_OK;
+}
+
+int msfem_dev_free(msfem_ctx *ctx, void *ptr)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if (!ptr)
+    return MSFEM_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipFree(ptr));
+  return MSFEM_OK;
+}
+
+int msfem_memcpy_h2d(msfem_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+  if (!ctx || (bytes && (!dst || !src)))
+    return fail(ctx, MSFEM_ERR_ARG, "null argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MSFEM_OK;
+}
+
+int msfem_memcpy_d2h(msfem_ctx *ctx, void *dst, const void *src, uint64_t bytes)
+{
+  if (!ctx || (bytes && (!dst || !src)))
+    return fail(ctx, MSFEM_ERR_ARG, "null argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MSFEM_OK;
+}
+
+void *msfem_ctx_stream(msfem_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int msfem_ctx_synchronize(msfem_ctx *ctx)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MSFEM_OK;
+}
+
 int msfem_ctx_set_profiling(msfem_ctx *ctx, int on)
 {
   if (!ctx)

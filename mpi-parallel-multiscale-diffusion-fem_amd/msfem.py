@@ -25,7 +25,8 @@ BC_ZERO_DIRICHLET, BC_REFERENCE = 0, 1
 # Every symbol include/msfem.h declares.
 EXPORTS = [
     "msfem_ctx_create", "msfem_ctx_destroy", "msfem_last_error", "msfem_default_opts", "msfem_version",
-    "msfem_ctx_set_profiling", "msfem_ctx_profile",
+    "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
+    "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
@@ -72,6 +73,13 @@ def lib():
         L.msfem_ctx_destroy.argtypes = [vp]
         L.msfem_ctx_set_profiling.argtypes = [vp, i32]
         L.msfem_ctx_profile.argtypes = [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.msfem_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
+        L.msfem_dev_free.argtypes = [vp, vp]
+        L.msfem_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
+        L.msfem_memcpy_d2h.argtypes = [vp, vp, vp, ctypes.c_uint64]
+        L.msfem_ctx_stream.argtypes = [vp]
+        L.msfem_ctx_stream.restype = vp
+        L.msfem_ctx_synchronize.argtypes = [vp]
         L.msfem_default_opts.argtypes = [ctypes.POINTER(SolverOpts)]
         L.msfem_default_opts.restype = None
         L.msfem_batch_build.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchOut)]
@@ -157,6 +165,24 @@ class Context:
         _check(lib().msfem_ctx_profile(self.h, ctypes.byref(a), ctypes.byref(b)), self.h)
         return a.value, b.value
 
+    # ---- device memory --------------------------------------------------------------
+    def alloc(self, shape, dtype):
+        """A DeviceArray of the given shape/dtype in HBM (freed with the array)."""
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    @property
+    def stream(self):
+        return lib().msfem_ctx_stream(self.h)
+
+    def synchronize(self):
+        _check(lib().msfem_ctx_synchronize(self.h), self.h)
+
     # ---- the hot path, host arrays ------------------------------------------------
     def batch_build(self, dim, n_fine, box, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, x0=None,
                     opts=None, want_bases=True, raise_noconv=True):
@@ -227,6 +253,40 @@ class Context:
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         _check(lib().msfem_reconstruct_device(self.h, dim, int(n_fine.shape[0]), _p(n_fine), bases_ptr, w_ptr, u_ptr,
                                               stream), self.h)
+
+
+class DeviceArray:
+    """A buffer in device memory with a numpy shape/dtype (host copies on demand)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(np.atleast_1d(shape)) if not isinstance(shape, tuple) else shape
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().msfem_dev_alloc(ctx.h, self.nbytes, ctypes.byref(p)), ctx.h)
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        _check(lib().msfem_memcpy_h2d(self.ctx.h, self.ptr, a.ctypes.data, self.nbytes), self.ctx.h)
+
+    def numpy(self):
+        out = np.empty(self.shape, self.dtype)
+        _check(lib().msfem_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), self.ctx.h)
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            lib().msfem_dev_free(self.ctx.h, self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def gather_weights(dim, nc, u):

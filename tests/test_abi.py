@@ -64,15 +64,10 @@ def test_default_opts_and_version():
 
 def test_no_device_is_a_clean_error():
     try:
-        import torch
-        has_gpu = torch.cuda.is_available()
-    except Exception:
-        has_gpu = False
-    if has_gpu:
+        msfem.Context(0).close()
         pytest.skip("a GPU is present")
-    with pytest.raises(msfem.MsfemError) as e:
-        msfem.Context(0)
-    assert e.value.code == msfem.ERR_NODEVICE
+    except msfem.MsfemError as e:
+        assert e.code == msfem.ERR_NODEVICE
 
 
 def test_gather_weights_host():

@@ -202,16 +202,13 @@ def test_full_lognormal_batch_properties(ctx):
 
 def test_reconstruction_device(ctx):
     say("reconstruction")
-    import torch
     case = C.lognormal_cells(which=np.array([0, 1, 2]))
     out = C.gpu_build(ctx, case)
     w = np.random.default_rng(1).standard_normal((3, 4))
-    dev = torch.device("cuda:0")
-    tb = torch.from_numpy(out["bases"]).to(dev)
-    tw = torch.from_numpy(w).to(dev)
-    tu = torch.zeros(3 * 65 * 65, dtype=torch.float64, device=dev)
-    torch.cuda.synchronize()
-    ctx.reconstruct_device(2, case["n_fine"], tb.data_ptr(), tw.data_ptr(), tu.data_ptr())
-    torch.cuda.synchronize()
+    tb = ctx.to_device(out["bases"])
+    tw = ctx.to_device(w)
+    tu = ctx.alloc(3 * 65 * 65, np.float64)
+    ctx.reconstruct_device(2, case["n_fine"], tb.ptr, tw.ptr, tu.ptr, stream=ctx.stream)
+    ctx.synchronize()
     want = np.concatenate([O.reconstruct(2, 64, out["bases"][c * 4 * 4225:(c + 1) * 4 * 4225], w[c]) for c in range(3)])
-    assert rel(tu.cpu().numpy(), want) < 1e-14
+    assert rel(tu.numpy(), want) < 1e-14
