@@ -83,6 +83,7 @@ def main():
     t_cv = ctx.alloc(ncn, np.float64)
     t_u = ctx.alloc(ncn, np.float64)
     t_cit = ctx.alloc(1, np.int32)
+    tuned = ctx.load_variant_table()  # launch shapes picked by autotune.py on this GPU
     ctx.set_profiling(True)
     stream = ctx.stream
 
@@ -145,7 +146,8 @@ def main():
         "data": "synthetic (Philox-4x32-10 lognormal field, generated on the host, resident in HBM)",
         "config": {"workload": f"{args.workload}: {nc}^{dim} coarse Q1 cells x {nf}^{dim} fine Q1 per cell, "
                                f"{n * nb} local solves, FP64, rel. tol 1e-13",
-                   "coarse_cells": n, "fine_per_cell": nf, "dim": dim, "local_solves": n * nb},
+                   "coarse_cells": n, "fine_per_cell": nf, "dim": dim, "local_solves": n * nb,
+                   "autotuned_classes": tuned},
         "roofline": {"bound": "valu", "kernel": "local_solve_2d (FP64 CG)" if dim == 2 else "local_solve_3d",
                      "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,

@@ -49,6 +49,7 @@ struct msfem_ctx {
   std::string err;
   hipStream_t stream = nullptr;
   std::map<std::string, DevBuf> bufs;
+  std::map<long long, int> variants; // (dim, nf, precision, coef_kind) -> launch-shape id
   Plan plan;
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // solve begin/end, element begin/end
@@ -119,6 +120,11 @@ long long ipow(long long b, int e)
   return r;
 }
 
+long long variant_key(int dim, int nf, int precision, int coef_kind)
+{
+  return ((long long)dim * 1024 + nf) * 64 + precision * 8 + coef_kind;
+}
+
 bool nf_supported(int dim, int nf)
 {
   if (dim == 2)
@@ -154,6 +160,15 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
     return fail(ctx, MSFEM_ERR_ARG, "unknown rhs kind %d", in->rhs_kind);
   if (o->precision != MSFEM_PREC_FP64 && o->precision != MSFEM_PREC_FP32_REFINE)
     return fail(ctx, MSFEM_ERR_ARG, "unknown precision %d", o->precision);
+  if (o->variant < 0)
+    return fail(ctx, MSFEM_ERR_ARG, "variant must be >= 0");
+  for (int c = 0; c < in->n_cells && o->variant > 0; ++c)
+    {
+      const int n = in->dim == 2 ? local2d_variants(in->n_fine[c]) : local3d_variants(in->n_fine[c]);
+      if (o->variant > n)
+        return fail(ctx, MSFEM_ERR_ARG, "variant %d does not exist for %d fine cells per side", o->variant,
+                    in->n_fine[c]);
+    }
   if (o->max_iter < 1 || o->max_refine < 1 || !(o->rel_tol >= 0) || !(o->abs_tol >= 0) ||
       (o->rel_tol == 0 && o->abs_tol == 0))
     return fail(ctx, MSFEM_ERR_ARG, "invalid tolerances or iteration caps");
@@ -320,10 +335,17 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       a.max_iter = o->max_iter;
       a.max_refine = o->max_refine;
       const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
+      int variant = o->variant;
+      if (variant == 0)
+        {
+          auto it = ctx->variants.find(variant_key(in->dim, nf, o->precision, in->coef_kind));
+          if (it != ctx->variants.end())
+            variant = it->second;
+        }
       if (in->dim == 2)
-        HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, o->variant, a, st));
+        HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
       else
-        HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, o->variant, a, st));
+        HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, variant, a, st));
       if (pou)
         HIP_TRY(ctx, launch_pou_fixup(ncls, cells, in->dim, nf, bases, d_off(ctx, 1), st));
     }
@@ -501,6 +523,28 @@ int msfem_ctx_synchronize(msfem_ctx *ctx)
     return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return MSFEM_OK;
+}
+
+int msfem_variant_count(int dim, int nf)
+{
+  if (!nf_supported(dim, nf))
+    return 0;
+  return dim == 2 ? local2d_variants(nf) : local3d_variants(nf);
+}
+
+int msfem_set_variant(msfem_ctx *ctx, int dim, int nf, int precision, int coef_kind, int variant)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if (!nf_supported(dim, nf) || variant < 0 || variant > msfem_variant_count(dim, nf) ||
+      (precision != MSFEM_PREC_FP64 && precision != MSFEM_PREC_FP32_REFINE) || coef_kind < 0 || coef_kind > 3)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid variant table entry");
+  const long long key = variant_key(dim, nf, precision, coef_kind);
+  if (variant == 0)
+    ctx->variants.erase(key);
+  else
+    ctx->variants[key] = variant;
   return MSFEM_OK;
 }
 

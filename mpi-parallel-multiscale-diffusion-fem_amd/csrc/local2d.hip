@@ -7,16 +7,23 @@
 // One lane group owns one problem.  Lanes map to node columns (CPL columns per
 // lane), SW stacked waves split the node rows (R rows per lane); a workgroup of
 // SW waves holds PPW = 64 / lane-group-width problems side by side.  The CG
-// iterate, residual, direction and Jacobi diagonal of the owned nodes live in
-// registers, the search direction of the whole cell in LDS (the only values a
-// lane needs from its neighbours), and the fine-cell coefficients in registers
+// vectors of the owned nodes live in registers, the preconditioned residual z
+// of the whole cell in a zero-padded LDS grid (the only values a lane needs
+// from its neighbours), the fine-cell coefficients in registers
 // (kModeScalar / kModeTensor) or global memory (kModeGlobal).
+//
+// Inner solver: Jacobi-preconditioned CG in the single-reduction form of
+// Chronopoulos and Gear (J. Comput. Appl. Math. 25, 1989): per iteration one
+// stencil apply w = K z and ONE fused reduction of (r.z, w.z, r.r), so two
+// workgroup barriers per iteration (z exchange, reduction).  s = K p is
+// carried by recurrence.
 //
 // Precision: an FP64 outer loop computes the true residual r = -(K phi)_I from
 // phi in global memory (FP64 coefficients) and checks
 //     ||r|| <= max(abs_tol, rel_tol ||b||);
 // the inner CG in type T solves K d = r (T = double: to the final tolerance,
-// one cycle; T = float: to inner_rel_tol, refined in FP64 until converged).
+// normally one cycle; T = float: to inner_rel_tol, refined in FP64 until
+// converged).
 #include "msfem_device.hpp"
 #include "msfem_kernels.hpp"
 
@@ -30,7 +37,7 @@ struct Cond6 {
   U c[6];
 };
 
-// Contribution of the four cells around node C to (K p)_C.
+// (K p)_C from the four cells around node C.
 //   NE cell: node is vertex 0, NW: vertex 1, SE: vertex 2, SW: vertex 3.
 template <typename U>
 __device__ __forceinline__ U gather6(U pC, U pE, U pW, U pN, U pS, U pNE, U pNW, U pSE, U pSW, const Cond6<U> &ne,
@@ -75,35 +82,63 @@ __device__ __forceinline__ Cond6<double> cell_cond64(const double *cf, const dou
     }
 }
 
-// Per-problem sums of two values over LW lanes x SW waves.
-template <int LW, int SW, int PPW>
-__device__ __forceinline__ void problem_sum2(double &a, double &b, double *red, int wave, int slot, int pl)
+// Per-problem sums of NV values (type S) over LW lanes x SW waves, plus the OR
+// of a per-problem flag over the whole workgroup (the loop-continuation test).
+// red: [SW][PPW][NV + 1] values of type S.  Identical results in every lane.
+template <typename S, int LW, int SW, int PPW, int NV>
+__device__ __forceinline__ bool problem_sums(S (&v)[NV], bool flag, S *red, int wave, int slot, int pl)
 {
-  a = group_sum<LW>(a);
-  b = group_sum<LW>(b);
-  if constexpr (SW > 1)
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = group_sum_any<S, LW>(v[k]);
+  if constexpr (SW == 1)
+    return __syncthreads_or(flag);
+  else
     {
       if (pl == 0)
         {
-          red[(wave * PPW + slot) * 2] = a;
-          red[(wave * PPW + slot) * 2 + 1] = b;
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            red[(wave * PPW + slot) * (NV + 1) + k] = v[k];
+          red[(wave * PPW + slot) * (NV + 1) + NV] = flag ? (S)1 : (S)0;
         }
       __syncthreads();
-      a = 0;
-      b = 0;
+      // lane j of the group holds wave j's partials (j < SW <= 16 <= LW), then a
+      // DPP sum over those lanes: few live registers, identical totals in every lane
+      static_assert(SW <= 16, "at most 16 stacked waves");
+      const bool has = pl < SW;
+      S f = 0;
 #pragma unroll
-      for (int w = 0; w < SW; ++w)
+      for (int k = 0; k <= NV; ++k)
         {
-          a += red[(w * PPW + slot) * 2];
-          b += red[(w * PPW + slot) * 2 + 1];
+          const S part = has ? red[(pl * PPW + slot) * (NV + 1) + k] : (S)0;
+          S tot;
+          if constexpr (PPW == 1)
+            tot = first_lanes_sum<S, SW>(part);
+          else
+            tot = group_sum_any<S, LW>(part);
+          if (k < NV)
+            v[k] = tot;
+          else
+            f = tot;
         }
+      return __ballot(f != (S)0) != 0;
     }
+}
+
+template <typename S>
+__device__ __forceinline__ S quotient(S a, S b)
+{
+  if constexpr (sizeof(S) == 4)
+    return __fdividef(a, b);
+  else
+    return a / b;
 }
 
 } // namespace
 
-template <typename T, int NF, int CPL, int SW, int R, int MODE>
-__global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
+template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE>
+__global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 {
   constexpr int NP = NF + 1;
   constexpr int LW = NF / CPL;
@@ -118,7 +153,8 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
   constexpr int NCACHE = MODE == kModeScalar ? 1 : (MODE == kModeTensor ? 4 : 1);
 
   __shared__ T sgrid[PPW * GSZ];
-  __shared__ double sred[3][SW * PPW * 2];
+  __shared__ T sred[SW * PPW * 4];        // inner-CG partial sums (type T)
+  __shared__ double sred64[SW * PPW * 2]; // FP64 residual norms
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -132,7 +168,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
     }
   T *grid = sgrid + slot * GSZ;
   const int X0 = 1 + pl * CPL, Y0 = 1 + wave * R;
-  T *gown = grid + Y0 * GP + X0;       // own node (row 0, column 0)
+  T *gown = grid + Y0 * GP + X0;           // own node (row 0, column 0)
   T *gwin = grid + (Y0 - 1) * GP + X0 - 1; // window origin
 
   double rho = 1.0;
@@ -202,9 +238,10 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
       }
   };
 
-  // Jacobi diagonal; 0 on slots that are not interior nodes of a valid
-  // problem, which keeps every vector of those slots at exactly 0.
-  T dinv[R][CPL];
+  // Jacobi diagonal (FP32 suffices for a preconditioner); 0 on slots that are
+  // not interior nodes of a valid problem, which keeps every vector of those
+  // slots at exactly 0.
+  float dinv[R][CPL];
 #pragma unroll
   for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -214,7 +251,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
         const Cond6<T> ne = cond_T(i + 1, c + 1), nw = cond_T(i + 1, c), se = cond_T(i, c + 1), sw = cond_T(i, c);
         const T d = (ne.c[0] + ne.c[1] + ne.c[2]) + (nw.c[0] + nw.c[4] + nw.c[3]) + (se.c[5] + se.c[1] + se.c[3]) +
                     (sw.c[5] + sw.c[4] + sw.c[2]);
-        dinv[i][c] = act ? (T)1 / d : (T)0;
+        dinv[i][c] = act ? (float)((T)1 / d) : 0.0f;
       }
 
   // ---- initial state: LDS grid zero, phi = g on the boundary, x0 inside ----
@@ -251,7 +288,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
   double tol_eff = 0.0, bnorm = 0.0;
   if (a.x0)
     {
-      double s = 0, dummy = 0;
+      double s[1] = {0};
 #pragma unroll 1
       for (int k = 0; k < R * CPL; ++k)
         {
@@ -259,16 +296,16 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
           if (pvalid && X <= NF - 1 && Y <= NF - 1)
             {
               const double b = apply64(X, Y, true);
-              s += b * b;
+              s[0] += b * b;
             }
         }
-      problem_sum2<LW, SW, PPW>(s, dummy, sred[2], wave, slot, pl);
-      bnorm = sqrt(s);
+      problem_sums<double, LW, SW, PPW, 1>(s, false, sred64, wave, slot, pl);
+      bnorm = sqrt(s[0]);
       tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
       __syncthreads();
     }
 
-  T xr[R][CPL], rr[R][CPL], pr[R][CPL], q[R][CPL];
+  T xr[R][CPL], rr[R][CPL], pr[R][CPL], sr[R][CPL];
   int it_total = 0;
   int state = pvalid ? 0 : 1; // 0 running, 1 done
   int conv = 0;
@@ -279,7 +316,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
       // ---- true FP64 residual of the current phi ----------------------------
       // (rolled loop; the values pass through this thread's own LDS slots so
       // the FP64 gathers do not hold registers across rows)
-      double s = 0, dummy = 0;
+      double s[1] = {0};
 #pragma unroll 1
       for (int k = 0; k < R * CPL; ++k)
         {
@@ -288,16 +325,16 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
             {
               const double r64 = -apply64(X, Y, false);
               grid[Y * GP + X] = (T)r64;
-              s += r64 * r64;
+              s[0] += r64 * r64;
             }
         }
 #pragma unroll
       for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
-          rr[i][c] = dinv[i][c] != (T)0 ? gown[i * GP + c] : (T)0;
-      problem_sum2<LW, SW, PPW>(s, dummy, sred[2], wave, slot, pl);
-      rnorm = sqrt(s);
+          rr[i][c] = dinv[i][c] != 0.0f ? gown[i * GP + c] : (T)0;
+      problem_sums<double, LW, SW, PPW, 1>(s, false, sred64, wave, slot, pl);
+      rnorm = sqrt(s[0]);
       if (cyc == 0 && !a.x0)
         {
           bnorm = rnorm;
@@ -316,29 +353,34 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
       if (!__syncthreads_or(state == 0))
         break;
 
-      // ---- inner CG on K d = r ------------------------------------------------
+      // ---- inner CG (single reduction) on K d = r --------------------------------
       const bool run = state == 0;
       double inner_tol = 0.5 * tol_eff;
       if (sizeof(T) < sizeof(double))
         inner_tol = fmax(a.inner_rel_tol * rnorm, inner_tol);
-      double rz = 0, t0 = 0;
+      const T inner_tol2 = (T)(inner_tol * inner_tol);
 #pragma unroll
       for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
           {
             xr[i][c] = 0;
-            const T z = run ? rr[i][c] * dinv[i][c] : (T)0;
-            pr[i][c] = z;
-            rz += (double)(rr[i][c] * z);
-            gown[i * GP + c] = z;
+            pr[i][c] = 0;
+            sr[i][c] = 0;
           }
-      problem_sum2<LW, SW, PPW>(rz, t0, sred[1], wave, slot, pl);
-      bool irun = run;
-      __syncthreads();
+      bool irun = run, first = true;
+      T gamma_prev = 1, alpha_prev = 1;
       while (true)
         {
-          // q = K p from the LDS grid, rows slid through a 3-row window
+          // z = M r into the LDS grid (zero on padding/boundary slots)
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              gown[i * GP + c] = irun ? rr[i][c] * (T)dinv[i][c] : (T)0;
+          __syncthreads();
+          // w = K z, rows slid through a 3-row window; fused partial sums
+          T wv[R][CPL];
           T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
 #pragma unroll
           for (int k = 0; k < CPL + 2; ++k)
@@ -346,7 +388,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
               w0[k] = gwin[k];
               w1[k] = gwin[GP + k];
             }
-          double pq = 0;
+          T sums[3] = {0, 0, 0}; // (r, z), (w, z), (r, r)
 #pragma unroll
           for (int i = 0; i < R; ++i)
             {
@@ -369,8 +411,11 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
                   else
                     y = gather6<T>(w1[c + 1], w1[c + 2], w1[c], w2[c + 1], w0[c + 1], w2[c + 2], w2[c], w0[c + 2],
                                    w0[c], cond_T(i + 1, c + 1), cond_T(i + 1, c), cond_T(i, c + 1), cond_T(i, c));
-                  q[i][c] = dinv[i][c] != (T)0 ? y : (T)0;
-                  pq += (double)(pr[i][c] * q[i][c]);
+                  const T z = w1[c + 1];
+                  wv[i][c] = dinv[i][c] != 0.0f ? y : (T)0;
+                  sums[0] += rr[i][c] * z;
+                  sums[1] += wv[i][c] * z;
+                  sums[2] += rr[i][c] * rr[i][c];
                 }
 #pragma unroll
               for (int k = 0; k < CPL + 2; ++k)
@@ -379,42 +424,42 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
                   w1[k] = w2[k];
                 }
             }
-          problem_sum2<LW, SW, PPW>(pq, t0, sred[0], wave, slot, pl);
-          const T alpha = irun ? (T)(rz / pq) : (T)0;
-          double rz_new = 0, rn2 = 0;
-#pragma unroll
-          for (int i = 0; i < R; ++i)
-#pragma unroll
-            for (int c = 0; c < CPL; ++c)
-              {
-                xr[i][c] += alpha * pr[i][c];
-                rr[i][c] -= alpha * q[i][c];
-                const T z = rr[i][c] * dinv[i][c];
-                rz_new += (double)(rr[i][c] * z);
-                rn2 += (double)(rr[i][c] * rr[i][c]);
-                q[i][c] = z;
-              }
-          problem_sum2<LW, SW, PPW>(rz_new, rn2, sred[1], wave, slot, pl);
+          // Converged (or capped) problems stop here with x the current iterate.
+          // The flag sent with the partials is the state before this test, so
+          // the loop makes one more (idle) pass after the last problem stops.
+          T v3[3] = {sums[0], sums[1], sums[2]};
+          const bool any = problem_sums<T, LW, SW, PPW, 3>(v3, irun, sred, wave, slot, pl);
+          if (irun && (v3[2] <= inner_tol2 || it_total >= a.max_iter))
+            irun = false;
           if (irun)
             {
-              ++it_total;
-              if (sqrt(rn2) <= inner_tol || it_total >= a.max_iter)
-                irun = false;
+              const T gamma = v3[0], delta = v3[1];
+              T beta = 0, alpha;
+              if (first)
+                alpha = quotient<T>(gamma, delta);
               else
                 {
-                  const T beta = (T)(rz_new / rz);
-                  rz = rz_new;
-#pragma unroll
-                  for (int i = 0; i < R; ++i)
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c)
-                      {
-                        pr[i][c] = q[i][c] + beta * pr[i][c];
-                        gown[i * GP + c] = pr[i][c];
-                      }
+                  beta = quotient<T>(gamma, gamma_prev);
+                  alpha = quotient<T>(gamma, delta - beta * quotient<T>(gamma, alpha_prev));
                 }
+              const T tb = beta, ta = alpha;
+#pragma unroll
+              for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                  {
+                    const T z = rr[i][c] * (T)dinv[i][c];
+                    pr[i][c] = z + tb * pr[i][c];
+                    sr[i][c] = wv[i][c] + tb * sr[i][c];
+                    xr[i][c] += ta * pr[i][c];
+                    rr[i][c] -= ta * sr[i][c];
+                  }
+              gamma_prev = gamma;
+              alpha_prev = alpha;
+              first = false;
+              ++it_total;
             }
-          if (!__syncthreads_or(irun))
+          if (!any)
             break;
         }
       // ---- phi += d in FP64 ------------------------------------------------------
@@ -424,7 +469,7 @@ __global__ void __launch_bounds__(kWave *SW) local_solve_2d(LocalArgs a)
           for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-              if (dinv[i][c] != (T)0)
+              if (dinv[i][c] != 0.0f)
                 phi[(Y0 + i) * NP + X0 + c] += (double)xr[i][c];
         }
       __syncthreads();
@@ -493,16 +538,23 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
 }
 
 // ---- launch table -----------------------------------------------------------------
+// Variant ids per fine resolution (the autotuner times every id and keeps the
+// fastest per class; id 0 is the built-in default):
+//   16:  1 = (SW 2, R 8, 2 waves/EU)   2 = (SW 1, R 15, 2)    3 = (SW 4, R 4, 4)
+//   32:  1 = (SW 4, R 8, 2)            2 = (SW 2, R 16, 1)    3 = (SW 8, R 4, 4)
+//   64:  1 = (SW 8, R 8, 2)            2 = (SW 4, R 16, 1)    3 = (SW 16, R 4, 4)   4 = (SW 8, R 8, 4)
+//        5 = (SW 4, R 16, 3)
+//   128: 1 = (SW 16, R 8, CPL 2, 4)
 namespace {
 
-template <typename T, int NF, int CPL, int SW, int R, int MODE>
+template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE>
 hipError_t launch_one(const LocalArgs &a, hipStream_t st)
 {
   constexpr int PPW = kWave / (NF / CPL);
   const int grid = (a.n_problems + PPW - 1) / PPW;
   if (grid == 0)
     return hipSuccess;
-  hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE>), dim3(grid), dim3(kWave * SW), 0, st, a);
+  hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE, WPE>), dim3(grid), dim3(kWave * SW), 0, st, a);
   return hipGetLastError();
 }
 
@@ -512,13 +564,30 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
   switch (nf)
     {
       case 16:
-        return variant == 1 ? launch_one<T, 16, 1, 1, 15, MODE>(a, st) : launch_one<T, 16, 1, 2, 8, MODE>(a, st);
+        switch (variant)
+          {
+            case 2: return launch_one<T, 16, 1, 1, 15, MODE, 2>(a, st);
+            case 3: return launch_one<T, 16, 1, 4, 4, MODE, 4>(a, st);
+            default: return launch_one<T, 16, 1, 2, 8, MODE, 2>(a, st);
+          }
       case 32:
-        return variant == 1 ? launch_one<T, 32, 1, 2, 16, MODE>(a, st) : launch_one<T, 32, 1, 4, 8, MODE>(a, st);
+        switch (variant)
+          {
+            case 2: return launch_one<T, 32, 1, 2, 16, MODE, 1>(a, st);
+            case 3: return launch_one<T, 32, 1, 8, 4, MODE, 4>(a, st);
+            default: return launch_one<T, 32, 1, 4, 8, MODE, 2>(a, st);
+          }
       case 64:
-        return variant == 1 ? launch_one<T, 64, 1, 4, 16, MODE>(a, st) : launch_one<T, 64, 1, 8, 8, MODE>(a, st);
+        switch (variant)
+          {
+            case 2: return launch_one<T, 64, 1, 4, 16, MODE, 1>(a, st);
+            case 3: return launch_one<T, 64, 1, 16, 4, MODE, 4>(a, st);
+            case 4: return launch_one<T, 64, 1, 8, 8, MODE, 4>(a, st);
+            case 5: return launch_one<T, 64, 1, 4, 16, MODE, 3>(a, st);
+            default: return launch_one<T, 64, 1, 8, 8, MODE, 2>(a, st);
+          }
       case 128:
-        return launch_one<T, 128, 2, 16, 8, MODE>(a, st);
+        return launch_one<T, 128, 2, 16, 8, MODE, 4>(a, st);
       default:
         return hipErrorInvalidValue;
     }
@@ -526,12 +595,20 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
 
 } // namespace
 
-int local2d_variants(int nf) { return nf == 128 ? 1 : 2; }
+int local2d_variants(int nf)
+{
+  switch (nf)
+    {
+      case 16: return 3;
+      case 32: return 3;
+      case 64: return 5;
+      case 128: return 1;
+      default: return 0;
+    }
+}
 
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st)
 {
-  if (variant > 0)
-    variant -= 1; // 1-based explicit ids -> table index
   if (fp32)
     {
       if (mode == kModeScalar)

@@ -357,8 +357,8 @@ int local3d_variants(int nf) { return nf == 16 ? 2 : 1; }
 
 hipError_t launch_local_solve_3d(bool fp32, int nf, int variant, const LocalArgs &a, hipStream_t st)
 {
-  if (variant > 0)
-    variant -= 1;
+  // ids: 16 -> 1 = (2 stacks of 8 z-levels), 2 = (1 stack of 15); 8 -> 1
+  variant = variant == 2 ? 1 : 0;
   return fp32 ? launch3t<float>(nf, variant, a, st) : launch3t<double>(nf, variant, a, st);
 }
 

@@ -47,6 +47,113 @@ __device__ __forceinline__ double group_sum(double v)
   return v;
 }
 
+// DPP move of a double (two 32-bit v_mov_dpp); CTRL is a GFX9 dpp_ctrl code.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+__device__ __forceinline__ double readlane_f64(double v, int lane)
+{
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float readlane_f32(float v, int lane)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
+template <typename S, int CTRL>
+__device__ __forceinline__ S dpp_any(S v)
+{
+  if constexpr (sizeof(S) == 8)
+    return dpp_f64<CTRL>(v);
+  else
+    return dpp_f32<CTRL>(v);
+}
+
+template <typename S>
+__device__ __forceinline__ S readlane_any(S v, int lane)
+{
+  if constexpr (sizeof(S) == 8)
+    return readlane_f64(v, lane);
+  else
+    return readlane_f32(v, lane);
+}
+
+// Sum of the values of lanes 0..N-1 of the wave (N <= 16, a power of two),
+// returned identically in every lane: DPP steps inside row 0, then readlane.
+template <typename S, int N>
+__device__ __forceinline__ S first_lanes_sum(S v)
+{
+  static_assert(N >= 1 && N <= 16 && (N & (N - 1)) == 0, "N must be a power of two <= 16");
+  if constexpr (N >= 2)
+    v += dpp_any<S, 0xB1>(v); // pairs
+  if constexpr (N >= 4)
+    v += dpp_any<S, 0x4E>(v); // quads
+  if constexpr (N >= 8)
+    {
+      // two row rotations: every lane of row 0 then holds the whole row
+      // (lanes N..15 contribute zeros), whichever way the rotation turns
+      v += dpp_any<S, 0x124>(v);
+      v += dpp_any<S, 0x128>(v);
+    }
+  return readlane_any<S>(v, 0);
+}
+
+// Generic-type version of group_sum_dpp (FP32 partial sums move with one DPP
+// instruction per step instead of two).
+template <typename S, int LW>
+__device__ __forceinline__ S group_sum_any(S v)
+{
+  v += dpp_any<S, 0xB1>(v);
+  v += dpp_any<S, 0x4E>(v);
+  v += dpp_any<S, 0x124>(v);
+  v += dpp_any<S, 0x128>(v);
+  if constexpr (LW == 64)
+    return (readlane_any<S>(v, 0) + readlane_any<S>(v, 16)) + (readlane_any<S>(v, 32) + readlane_any<S>(v, 48));
+  else if constexpr (LW == 32)
+    {
+      const int base = (int)(threadIdx.x & (kWave - 1)) & ~31;
+      return __shfl(v, base, kWave) + __shfl(v, base + 16, kWave);
+    }
+  else
+    return __shfl(v, (int)(threadIdx.x & (kWave - 1)) & ~15, kWave);
+}
+
+// Sum over a lane group of LW = 16, 32 or 64 lanes, identical in every lane of
+// the group: DPP within each 16-lane row (quad_perm [1,0,3,2], [2,3,0,1],
+// row_ror 4, row_ror 8), then the row totals of one lane per row combined in a
+// fixed order (readlane for a whole wave, ds_bpermute otherwise).
+template <int LW>
+__device__ __forceinline__ double group_sum_dpp(double v)
+{
+  v += dpp_f64<0xB1>(v);
+  v += dpp_f64<0x4E>(v);
+  v += dpp_f64<0x124>(v);
+  v += dpp_f64<0x128>(v);
+  if constexpr (LW == 64)
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+  else if constexpr (LW == 32)
+    {
+      const int base = (int)(threadIdx.x & (kWave - 1)) & ~31;
+      return __shfl(v, base, kWave) + __shfl(v, base + 16, kWave);
+    }
+  else
+    return __shfl(v, (int)(threadIdx.x & (kWave - 1)) & ~15, kWave);
+}
+
 // Q1 nodal function N_i of the coarse cell at fine node (X, Y[, Z]) of an
 // nf-grid: tensor product of 1D hats, exact for axis-aligned boxes (the
 // closed form of include/coefficients/basis_q1.tpp:26-158 on boxes).

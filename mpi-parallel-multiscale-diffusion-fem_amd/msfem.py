@@ -26,7 +26,7 @@ BC_ZERO_DIRICHLET, BC_REFERENCE = 0, 1
 EXPORTS = [
     "msfem_ctx_create", "msfem_ctx_destroy", "msfem_last_error", "msfem_default_opts", "msfem_version",
     "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
-    "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize",
+    "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
@@ -80,6 +80,8 @@ def lib():
         L.msfem_ctx_stream.argtypes = [vp]
         L.msfem_ctx_stream.restype = vp
         L.msfem_ctx_synchronize.argtypes = [vp]
+        L.msfem_variant_count.argtypes = [i32, i32]
+        L.msfem_set_variant.argtypes = [vp, i32, i32, i32, i32, i32]
         L.msfem_default_opts.argtypes = [ctypes.POINTER(SolverOpts)]
         L.msfem_default_opts.restype = None
         L.msfem_batch_build.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchOut)]
@@ -155,6 +157,21 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def set_variant(self, dim, nf, precision, coef_kind, variant):
+        """Launch-shape id for one class (0 = built-in default)."""
+        _check(lib().msfem_set_variant(self.h, dim, nf, precision, coef_kind, variant), self.h)
+
+    def load_variant_table(self, path=None):
+        """Applies a table written by autotune.py (default: autotune_table.json here)."""
+        import json
+        path = path or os.path.join(HERE, "autotune_table.json")
+        if not os.path.exists(path):
+            return 0
+        table = json.load(open(path))
+        for e in table["entries"]:
+            self.set_variant(e["dim"], e["nf"], e["precision"], e["coef_kind"], e["variant"])
+        return len(table["entries"])
 
     def set_profiling(self, on=True):
         _check(lib().msfem_ctx_set_profiling(self.h, 1 if on else 0), self.h)
@@ -287,6 +304,10 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def variant_count(dim, nf):
+    return lib().msfem_variant_count(dim, nf)
 
 
 def gather_weights(dim, nc, u):

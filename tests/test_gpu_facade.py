@@ -1,0 +1,39 @@
+"""The C++ facade (include/msfem_basis.hpp) driving the whole reference pipeline
+(DiffusionProblemMultiscale<2>(n_refine, n_refine_local).run(), source/main.cxx:37-39)
+on the reference default problem, checked against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import msfem
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "test_facade")
+
+
+def test_facade_reference_pipeline(tmp_path):
+    print("[gpu-case] facade n_refine=2 n_refine_local=5", flush=True)
+    out = tmp_path / "facade.bin"
+    p = subprocess.run([EXE, "2", "5", str(out)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    raw = out.read_bytes()
+    nu = int(np.frombuffer(raw[:8], np.int64)[0])
+    u = np.frombuffer(raw[8:8 + 8 * nu], np.float64)
+    nc_cells = int(np.frombuffer(raw[8 + 8 * nu:16 + 8 * nu], np.int64)[0])
+    rest = np.frombuffer(raw[16 + 8 * nu:], np.float64).reshape(nc_cells, 20)
+    Ae, Fe = rest[:, :16].reshape(-1, 4, 4), rest[:, 16:]
+    # oracle: same cells, reference coefficient (PI_D) at the quadrature points, f = 2,
+    # Dirichlet g on x=0,y=0 and Neumann h on x=1,y=1
+    nc, nf = 4, 32
+    box = msfem.unit_boxes(2, nc)
+    n_fine = np.full(nc * nc, nf, np.int32)
+    coef = msfem.coef_reference(2, n_fine, box)
+    r = O.batch_build(2, box, n_fine, O.COEF_TENSOR_QP, coef, O.RHS_CONST, 2.0)
+    assert np.abs(Ae - r["Ae"]).max() / np.abs(r["Ae"]).max() < 1e-10
+    assert np.abs(Fe - r["Fe"]).max() / np.abs(r["Fe"]).max() < 1e-10
+    Kc, Fc, cm, cv = O.coarse_assemble(2, nc, r["Ae"], r["Fe"], 1)
+    ur = O.coarse_solve(2, nc, Kc, Fc, cm, cv)
+    assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-10

@@ -212,3 +212,22 @@ def test_reconstruction_device(ctx):
     ctx.synchronize()
     want = np.concatenate([O.reconstruct(2, 64, out["bases"][c * 4 * 4225:(c + 1) * 4 * 4225], w[c]) for c in range(3)])
     assert rel(tu.numpy(), want) < 1e-14
+
+
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_every_launch_shape(ctx, prec):
+    """Every compiled launch shape (autotuner candidates) of every 2D class gives the exact values."""
+    case = C.tensor_cells(nc=128, which=np.arange(0, 16384, 2048), n_fine=[16, 32, 64, 128] * 2)
+    ref = C.oracle_build(case)
+    for nf in (16, 32, 64, 128):
+        for v in range(1, msfem.variant_count(2, nf) + 1):
+            say(f"launch shape nf={nf} variant={v} prec={prec}")
+            sel = np.where(case["n_fine"] == nf)[0]
+            sub = C.tensor_cells(nc=128, which=np.arange(0, 16384, 2048)[sel], n_fine=[nf] * len(sel))
+            out = C.gpu_build(ctx, sub, precision=prec, variant=v)
+            r = C.oracle_build(sub)
+            check_against_oracle(sub, out, r)
+    for v in range(1, msfem.variant_count(3, 16) + 1):
+        say(f"launch shape 3d nf=16 variant={v} prec={prec}")
+        c3 = C.lognormal_cells(dim=3, nc=16, nf=16, seed=5, which=np.array([7, 100]))
+        check_against_oracle(c3, C.gpu_build(ctx, c3, precision=prec, variant=v), C.oracle_build(c3))
