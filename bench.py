@@ -1,15 +1,21 @@
 """Benchmark of the MsFEM basis build on one MI355X.
 
-Workload (default): the batched plain basis build on the unit square, 64x64
-coarse Q1 cells, each refined to a 64x64 fine Q1 grid (4225 nodes, 4 basis
-problems per cell, 16384 local solves), alpha = exp(g) with g ~ N(0,1) i.i.d.
-per fine cell (Philox, seed 2) clipped to contrast 1e3, f = 1, zero Dirichlet.
+Default workload (the one the headline line is quoted on): the batched plain
+basis build on the unit square, 64x64 coarse Q1 cells, each refined to a 64x64
+fine Q1 grid (4225 nodes, 4 basis problems per cell, 16384 local solves),
+alpha = exp(g) with g ~ N(0,1) i.i.d. per fine cell (Philox, seed 2) clipped
+to contrast 1e3, f = 1, zero Dirichlet.  Other workloads (--workload):
+  tensor128    128x128 coarse cells, fine 16..128 per cell by probe contrast,
+               SPD tensor per fine cell, f = sin(pi x) sin(pi y)   (uneven batch)
+  lognormal3d  16^3 coarse cells x 16^3 fine, lognormal alpha, f = 1
+  periodic8    8x8 coarse cells x 32^2 fine, alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps)
+  reference    the reference default: 8x8 x 128^2, MatrixCoeff (PI_D), f = 2
 
-One step = every cell's 4 basis solves + A^K, F^K + the coarse assembly, with
-all inputs already resident in HBM.  The dominant kernel (the local solves) is
-timed with HIP events on the stream it runs on; algorithmic work per CG
-iteration is F_iter = (2*9 - 1 + 10) * N_I flops (9-point stencil + CG vector
-updates) and B_iter = 11 N w + n^2 w_c bytes.
+One step = every cell's basis solves + A^K, F^K + the coarse assembly, with all
+inputs already resident in HBM.  The dominant kernel (the local solves) is timed
+with HIP events on the stream it runs on.  Algorithmic work per CG iteration of
+a problem with N_I interior nodes: F_iter = (2*3^d - 1 + 10) N_I flops (stencil +
+CG vector updates), B_iter = 11 N w + n^d w_c bytes (survey model).
 
 The CPU baseline is the oracle's restatement of the reference's own recipe
 (CG + SSOR(1.6), absolute tolerance 1e-12, 1000 iterations;
@@ -31,15 +37,31 @@ for p in (PKG, os.path.join(ROOT, "tests")):
         sys.path.insert(0, p)
 
 PEAK_FP64_TFLOPS = 78.6
-PEAK_HBM_GBS = 8000.0
+PEAK_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA issue rate equals FP64 (packed FP32 is disabled)
+METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 
 
 def workload(name):
+    """-> (case, nc, main precision, bc kind, description)."""
+    import msfem
     import msfem_cases as C
     if name == "lognormal64":
-        return C.lognormal_cells(dim=2, nc=64, nf=64, seed=2), 64
+        return (C.lognormal_cells(dim=2, nc=64, nf=64, seed=2), 64, msfem.PREC_FP64, 0,
+                "64^2 coarse Q1 cells x 64^2 fine Q1 per cell, lognormal alpha (contrast 1e3), f=1, 16384 local solves")
+    if name == "tensor128":
+        return (C.tensor_cells(nc=128, seed=3, sigma=1.5), 128, msfem.PREC_FP32_REFINE, 0,
+                "128^2 coarse cells, fine 16/32/64/128 per cell by probe contrast, SPD tensor per fine cell, "
+                "f=sin(pi x)sin(pi y), 65536 local solves")
     if name == "lognormal3d":
-        return C.lognormal_cells(dim=3, nc=16, nf=16, seed=5), 16
+        return (C.lognormal_cells(dim=3, nc=16, nf=16, seed=5), 16, msfem.PREC_FP32_REFINE, 0,
+                "16^3 coarse Q1 cells x 16^3 fine Q1 per cell, lognormal alpha, f=1, 32768 local solves")
+    if name == "periodic8":
+        return (C.periodic_cells(nc=8, nf=32), 8, msfem.PREC_FP64, 0,
+                "8^2 coarse cells x 32^2 fine, alpha=2+sin(2 pi x/eps)cos(2 pi y/eps) eps=1/64, f=1, 256 local solves")
+    if name == "reference":
+        return (C.reference_cells(nc=8, nf=128), 8, msfem.PREC_FP64, 1,
+                "reference default: 8^2 coarse cells x 128^2 fine, MatrixCoeff (PI_D) at QGauss(2) points, f=2, "
+                "Dirichlet/Neumann coarse data")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -48,8 +70,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="lognormal64")
-    ap.add_argument("--cpu-sample", type=int, default=64, help="cells in the CPU-baseline sample")
+    ap.add_argument("--workload", default="lognormal64",
+                    choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference"])
+    ap.add_argument("--cpu-sample", type=int, default=2048, help="cells in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -59,21 +82,19 @@ def main():
 
     import msfem
 
-    case, nc = workload(args.workload)
+    case, nc, main_prec, bc_kind, desc = workload(args.workload)
     dim = case["dim"]
-    n = len(case["n_fine"])
-    nf = int(case["n_fine"][0])
+    nf_arr = np.asarray(case["n_fine"]).astype(np.int64)
+    n = len(nf_arr)
     nb = 1 << dim
-    nn = (nf + 1) ** dim
-    ni = (nf - 1) ** dim
-    ncf = nf ** dim
+    n_bases = msfem.bases_size(dim, nf_arr)
 
     ctx = msfem.Context(0)
-    t_box = ctx.to_device(case["box"].reshape(-1))
+    t_box = ctx.to_device(np.asarray(case["box"]).reshape(-1))
     t_coef = ctx.to_device(case["coef"])
     t_Ae = ctx.alloc(n * nb * nb, np.float64)
     t_Fe = ctx.alloc(n * nb, np.float64)
-    t_bases = ctx.alloc(n * nb * nn, np.float64)
+    t_bases = ctx.alloc(n_bases, np.float64)
     t_it = ctx.alloc(n * nb, np.int32)
     t_st = ctx.alloc(n * nb, np.int32)
     ncn = (nc + 1) ** dim
@@ -88,94 +109,100 @@ def main():
     stream = ctx.stream
 
     def step(opts):
-        ctx.batch_build_device(dim, case["n_fine"], t_box.ptr, case["coef_kind"], t_coef.ptr,
-                               t_Ae.ptr, t_Fe.ptr, t_bases.ptr, t_it.ptr, t_st.ptr,
-                               rhs_kind=case["rhs_kind"], rhs_value=case["rhs_value"], opts=opts, stream=stream)
-        ctx.coarse_assemble_device(dim, nc, t_Ae.ptr, t_Fe.ptr, msfem.BC_ZERO_DIRICHLET, t_Kc.ptr,
-                                   t_Fc.ptr, t_cm.ptr, t_cv.ptr, stream=stream)
+        ctx.batch_build_device(dim, case["n_fine"], t_box.ptr, case["coef_kind"], t_coef.ptr, t_Ae.ptr, t_Fe.ptr,
+                               t_bases.ptr, t_it.ptr, t_st.ptr, rhs_kind=case["rhs_kind"],
+                               rhs_value=case["rhs_value"], opts=opts, stream=stream)
+        ctx.coarse_assemble_device(dim, nc, t_Ae.ptr, t_Fe.ptr, bc_kind, t_Kc.ptr, t_Fc.ptr, t_cm.ptr, t_cv.ptr,
+                                   stream=stream)
 
-    def run(opts, steps, warmup):
-        for _ in range(warmup):
+    def run(opts):
+        for _ in range(args.warmup):
             step(opts)
         ctx.synchronize()
         solve_ms = []
         t0 = time.perf_counter()
-        for _ in range(steps):
+        for _ in range(args.steps):
             step(opts)
             solve_ms.append(ctx.profile()[0])
         ctx.synchronize()
         t1 = time.perf_counter()
         assert int(t_st.numpy().sum()) == 0, "local solves did not converge"
-        return (t1 - t0) * 1e3 / steps, float(np.mean(solve_ms)), t_it.numpy().astype(np.int64)
+        return (t1 - t0) * 1e3 / args.steps, float(np.mean(solve_ms)), t_it.numpy().astype(np.int64).reshape(n, nb)
 
-    # plain FP64 path (the parity path)
-    o64 = msfem.default_opts(precision=msfem.PREC_FP64)
-    ms64, solve64, it64 = run(o64, args.steps, args.warmup)
-    # FP32 CG + FP64 refinement stopped where the energy-norm error is <= 1e-6
-    o32 = msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
-    ms32, solve32, it32 = run(o32, args.steps, args.warmup)
+    opts = {msfem.PREC_FP64: msfem.default_opts(precision=msfem.PREC_FP64),
+            # FP32 CG + FP64 refinement stopped where the energy-norm error is <= 1e-6
+            msfem.PREC_FP32_REFINE: msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)}
+    res = {p: run(o) for p, o in opts.items()}
+    ms, solve_ms, its = res[main_prec]
     # coarse solve (timed apart: one single-workgroup FP64 CG)
-    step(o64)
+    step(opts[main_prec])
     ctx.synchronize()
     tc0 = time.perf_counter()
-    ctx.coarse_solve_device(dim, nc, t_Kc.ptr, t_Fc.ptr, t_cm.ptr, t_cv.ptr, 1e-12,
-                            100000, t_u.ptr, t_cit.ptr, stream=stream)
+    ctx.coarse_solve_device(dim, nc, t_Kc.ptr, t_Fc.ptr, t_cm.ptr, t_cv.ptr, 1e-12, 100000, t_u.ptr, t_cit.ptr,
+                            stream=stream)
     ctx.synchronize()
     coarse_ms = (time.perf_counter() - tc0) * 1e3
 
     s = 3 ** dim
-    f_iter = (2 * s - 1 + 10) * ni
-    b_iter = 11 * nn * 8 + ncf * 8
-    flops64 = float(it64.sum()) * f_iter
-    achieved = flops64 / (solve64 * 1e-3) / 1e12
-    b_cell = ncf * 8 + nb * nn * 8 + (nb * nb + nb) * 8
-    cells_per_s = n / (ms64 * 1e-3)
+    f_iter = (2 * s - 1 + 10) * (nf_arr - 1) ** dim                      # per cell
+    w = 8 if main_prec == msfem.PREC_FP64 else 4
+    stride = msfem.coef_stride(dim, case["coef_kind"])
+    b_iter = 11 * (nf_arr + 1) ** dim * w + nf_arr ** dim * stride * w   # per cell
+    flops = float((its.sum(axis=1) * f_iter).sum())
+    peak = PEAK_FP64_TFLOPS if main_prec == msfem.PREC_FP64 else PEAK_FP32_VALU_TFLOPS
+    achieved = flops / (solve_ms * 1e-3) / 1e12
+    b_cell = nf_arr ** dim * stride * 8 + nb * (nf_arr + 1) ** dim * 8 + (nb * nb + nb) * 8
+    cells_per_s = n / (ms * 1e-3)
+    pname = {msfem.PREC_FP64: "fp64", msfem.PREC_FP32_REFINE: "fp32_refine_1e-6_energy"}
+    classes = {int(k): int(v) for k, v in zip(*np.unique(nf_arr, return_counts=True))}
 
     result = {
-        "metric": "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err",
+        "metric": METRIC,
         "value": round(cells_per_s, 1),
         "unit": "cells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms64, 4),
+        "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic (Philox-4x32-10 lognormal field, generated on the host, resident in HBM)",
-        "config": {"workload": f"{args.workload}: {nc}^{dim} coarse Q1 cells x {nf}^{dim} fine Q1 per cell, "
-                               f"{n * nb} local solves, FP64, rel. tol 1e-13",
-                   "coarse_cells": n, "fine_per_cell": nf, "dim": dim, "local_solves": n * nb,
+        "dtype": "f64" if main_prec == msfem.PREC_FP64 else "f32 (FP64 refinement)",
+        "data": "synthetic (Philox-4x32-10 fields generated on the host, resident in HBM)",
+        "config": {"workload": f"{args.workload}: {desc}", "coarse_cells": n, "dim": dim,
+                   "fine_per_cell": classes, "local_solves": n * nb, "path": pname[main_prec],
                    "autotuned_classes": tuned},
-        "roofline": {"bound": "valu", "kernel": "local_solve_2d (FP64 CG)" if dim == 2 else "local_solve_3d",
-                     "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
-                     "kernel_ms": round(solve64, 4), "flops_per_launch": flops64,
-                     "lds_model_TBps": round(float(it64.sum()) * b_iter / (solve64 * 1e-3) / 1e12, 2),
-                     "hbm_GBps": round(n * b_cell / (ms64 * 1e-3) / 1e9, 2)},
-        "iterations": {"fp64_mean": round(float(it64.mean()), 1), "fp64_p99": int(np.percentile(it64, 99)),
-                       "fp32_refine_mean": round(float(it32.mean()), 1),
-                       "fp32_refine_p99": int(np.percentile(it32, 99))},
-        "variants": {"fp64_cells_per_s": round(cells_per_s, 1),
-                     "fp32_refine_1e-6_energy_cells_per_s": round(n / (ms32 * 1e-3), 1),
-                     "fp32_refine_ms_per_step": round(ms32, 4), "fp32_refine_solve_kernel_ms": round(solve32, 4),
-                     "coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0])},
+        "roofline": {"bound": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
+                     "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
+                     "frac": round(achieved / peak, 4), "traffic": None, "kernel_ms": round(solve_ms, 4),
+                     "flops_per_launch": flops,
+                     "lds_model_TBps": round(float((its.sum(axis=1) * b_iter).sum()) / (solve_ms * 1e-3) / 1e12, 2),
+                     "hbm_GBps": round(float(b_cell.sum()) / (ms * 1e-3) / 1e9, 2)},
+        "iterations": {p + "_mean": round(float(r[2].mean()), 1) for p, r in
+                       ((pname[k], v) for k, v in res.items())},
+        "variants": {pname[p] + "_cells_per_s": round(n / (r[0] * 1e-3), 1) for p, r in res.items()},
     }
+    result["variants"].update({pname[p] + "_solve_kernel_ms": round(r[1], 4) for p, r in res.items()})
+    result["variants"].update({"coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0])})
 
     if not args.no_cpu_baseline:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
         import msfem_cases as C
-        sample = np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64)
-        sub = C.lognormal_cells(dim=dim, nc=nc, nf=nf, seed=2 if dim == 2 else 5, which=sample)
+        import oracle as O
+        sample = np.unique(np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64))
+        stride_c = msfem.coef_stride(dim, case["coef_kind"])
+        offs = np.concatenate([[0], np.cumsum(nf_arr ** dim * stride_c)])
+        sub_coef = np.concatenate([case["coef"][offs[i]:offs[i + 1]] for i in sample])
         cores = len(os.sched_getaffinity(0))
         tcpu0 = time.perf_counter()
-        r = C.oracle_build(sub, want_bases=False, solver=1, n_threads=cores)
+        r = O.batch_build(dim, np.asarray(case["box"])[sample], nf_arr[sample], case["coef_kind"], sub_coef,
+                          case["rhs_kind"], case["rhs_value"], solver=O.SOLVER_SSOR_PCG, n_threads=cores)
         tcpu = time.perf_counter() - tcpu0
         result["cpu_baseline"] = {
             "value": round(len(sample) / tcpu, 2), "unit": "cells/s", "cores": cores, "kind": "port",
-            "sample": f"{len(sample)} cells spread over the same batch, reference recipe CG+SSOR(1.6), abs tol 1e-12,"
-                      f" max 1000 it (fails: {r['fails']}, mean it {r['iters'].mean():.1f}), {tcpu:.1f} s wall"}
+            "sample": f"{len(sample)} of the {n} cells (evenly spaced), reference recipe CG+SSOR(1.6), abs tol "
+                      f"1e-12, max 1000 it (not converged: {r['fails']}, mean it {r['iters'].mean():.1f}), "
+                      f"{tcpu:.1f} s wall"}
     ctx.close()
     print(json.dumps(result))
 
