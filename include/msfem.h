@@ -246,6 +246,38 @@ int msfem_gen_coef_tensor(int n_cells, const int32_t *n_fine, uint64_t seed, dou
  * Writes float32 g [n_cells][nf*nf]; alpha = exp(g). */
 int msfem_gen_grf_log(int n_cells, int nf, uint64_t seed, double corr_len, double clip, float *g);
 
+/* ---- learned basis surrogate: network primitives (device pointers) -------
+ * No counterpart in the reference.  Building blocks of the convolutional
+ * surrogate that predicts a cell's bases from its log-coefficient tile and
+ * serves them as the initial guess of msfem_batch_build_device (x0), whose
+ * FP64 residual test accepts the prediction or continues the solve.
+ * Activations FP32 in [channel][sample][y][x]; BF16 values are raw uint16. */
+/* C[M][N] (+)= op(A)[M][K] op(B)[K][N]; A [M][K] (transA: [K][M]), B [K][N]
+ * (transB: [N][K]) BF16 row-major; C FP32 row-major; MFMA with FP32 accumulate. */
+int msfem_nn_gemm_bf16(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
+                       float *C, int ldc, int accumulate, void *stream);
+/* col[(c*kh+ky)*kw+kx][(b*Ho+y)*Wo+x] = bf16(in[c][b][y+ky-pad][x+kx-pad]) (0 outside) */
+int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const float *in, void *col, void *stream);
+/* adjoint of im2col (FP32, gather form) */
+int msfem_nn_col2im(int C, int B, int H, int W, int kh, int kw, int pad, const float *dcol, float *dx, void *stream);
+int msfem_nn_bias_act(int C, uint64_t N, const float *bias, float *y, int relu, void *stream);
+int msfem_nn_act_bwd(uint64_t n, const float *y, float *dy, void *dy_bf16, int relu, void *stream);
+int msfem_nn_bias_grad(int C, uint64_t N, const float *d, float *db, void *stream);
+int msfem_nn_pool2(int planes, int H, int W, const float *in, float *out, int backward, void *stream);
+int msfem_nn_up2(int planes, int H, int W, const float *in, float *out, int backward, void *stream);
+int msfem_nn_axpy(uint64_t n, float a, const float *x, float *y, void *stream);
+/* out = a x + b y (y unread when b == 0) */
+int msfem_nn_axpby(uint64_t n, float a, const float *x, float b, const float *y, float *out, void *stream);
+int msfem_nn_to_bf16(uint64_t n, const float *x, void *y, void *stream);
+/* grad = 2 (pred - target) / n; partial[b] = per-block sums of squares (n_partial blocks) */
+int msfem_nn_mse(uint64_t n, const float *pred, const float *target, float *grad, float *partial, int n_partial,
+                 void *stream);
+int msfem_nn_adam(uint64_t n, float *w, const float *g, float *m, float *v, void *w_bf16, float lr, float beta1,
+                  float beta2, float eps, int step, float weight_decay, void *stream);
+/* bases [cell][4][(nf+1)^2] FP64 = N_i + delta_i (interior), delta [4][cells][(nf-1)^2] FP32; and the inverse */
+int msfem_nn_delta_to_bases(int n_cells, int nf, const float *delta, double *bases, void *stream);
+int msfem_nn_bases_to_delta(int n_cells, int nf, const double *bases, float *delta, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,237 @@
+"""GPU checks of the surrogate's HIP kernels against numpy / the FP64 network oracle.
+
+Bars: BF16-rounding, pooling, upsampling, im2col/col2im and the delta<->bases
+maps are bit-exact (they are exact in FP32); the MFMA GEMM is within 1e-5
+relative (FP32 accumulation of exact BF16 products, vs FP64 sums); a tiny
+network's output, loss and every layer's weight/bias gradient within 1e-3
+relative Frobenius norm of the oracle (measured: about 1e-7; the margin covers
+activations that land one BF16 step apart after re-rounding on either side).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import msfem
+import surrogate as S
+import surrogate_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = msfem.Context(0)
+    yield c
+    c.close()
+
+
+def bf16_bits(a):
+    """float64 values already on the BF16 grid -> raw uint16."""
+    return (np.asarray(a, np.float32).view(np.uint32) >> 16).astype(np.uint16)
+
+
+def frob(a, b):
+    return np.linalg.norm(np.asarray(a, np.float64) - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K", [(70, 100, 45), (64, 64, 32), (5, 300, 600), (130, 7, 9)])
+def test_gemm_bf16(ctx, ta, tb, M, N, K):
+    say(f"gemm ta={ta} tb={tb} M={M} N={N} K={K}")
+    rng = np.random.default_rng(M * 7 + N + K + 13 * ta + 29 * tb)
+    A = R.bf16(rng.standard_normal((M, K)))
+    B = R.bf16(rng.standard_normal((K, N)))
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    a_store = A.T if ta else A
+    b_store = B.T if tb else B
+    dA, dB = ctx.to_device(bf16_bits(np.ascontiguousarray(a_store))), ctx.to_device(bf16_bits(np.ascontiguousarray(b_store)))
+    for acc in (0, 1):
+        dC = ctx.to_device(C0)
+        rc = S._nn().msfem_nn_gemm_bf16(ta, tb, M, N, K, dA.ptr, a_store.shape[1], dB.ptr, b_store.shape[1], dC.ptr, N,
+                                        acc, ctx.stream)
+        assert rc == 0
+        ctx.synchronize()
+        want = A @ B + (C0 if acc else 0.0)
+        got = dC.numpy().reshape(-1).reshape(M, N)
+        assert np.abs(got - want).max() <= 1e-5 * max(np.abs(want).max(), 1.0), (acc, np.abs(got - want).max())
+
+
+def test_gemm_rejects_bad_sizes(ctx):
+    d = ctx.alloc(16, np.float32)
+    assert S._nn().msfem_nn_gemm_bf16(0, 0, 0, 4, 4, d.ptr, 4, d.ptr, 4, d.ptr, 4, 0, ctx.stream) == msfem.ERR_ARG
+    assert S._nn().msfem_nn_gemm_bf16(0, 0, 4, 4, 4, None, 4, d.ptr, 4, d.ptr, 4, 0, ctx.stream) == msfem.ERR_ARG
+
+
+def test_elementwise_primitives(ctx):
+    say("nn elementwise primitives")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(5)
+    C, B, H, W = 3, 2, 6, 8
+    x = rng.standard_normal((C, B, H, W)).astype(np.float32)
+    x.reshape(-1)[:4] = [1 + 2.0 ** -8, 1 + 3 * 2.0 ** -8, -0.0, 65504.0]
+    dx = ctx.to_device(x)
+    # BF16 rounding
+    dbf = ctx.alloc(x.size, np.uint16)
+    assert L.msfem_nn_to_bf16(x.size, dx.ptr, dbf.ptr, st) == 0
+    assert np.array_equal(dbf.numpy().reshape(-1), bf16_bits(R.bf16(x)).reshape(-1))
+    # im2col (3x3 pad 1, 2x2 pad 0) and col2im
+    for k, pad in ((3, 1), (2, 0)):
+        col, (Ho, Wo) = R.im2col(x.astype(np.float64), k, pad)
+        dcol = ctx.alloc(col.size, np.uint16)
+        assert L.msfem_nn_im2col(C, B, H, W, k, k, pad, dx.ptr, dcol.ptr, st) == 0
+        assert np.array_equal(dcol.numpy().reshape(-1), bf16_bits(R.bf16(col)).reshape(-1))
+        g = rng.standard_normal(col.shape).astype(np.float32)
+        dg, dout = ctx.to_device(g), ctx.alloc(x.size, np.float32)
+        assert L.msfem_nn_col2im(C, B, H, W, k, k, pad, dg.ptr, dout.ptr, st) == 0
+        assert np.allclose(dout.numpy().reshape(-1), R.col2im(g.astype(np.float64), x.shape, k, pad).reshape(-1), rtol=1e-6, atol=1e-5)
+    # pool / up, forward and backward
+    out = ctx.alloc(x.size, np.float32)
+    assert L.msfem_nn_pool2(C * B, H, W, dx.ptr, out.ptr, 0, st) == 0
+    assert np.allclose(out.numpy().reshape(-1)[:x.size // 4], R.pool2(x.astype(np.float64)).reshape(-1), rtol=1e-6, atol=1e-6)
+    small = rng.standard_normal((C, B, H // 2, W // 2)).astype(np.float32)
+    ds = ctx.to_device(small)
+    assert L.msfem_nn_pool2(C * B, H, W, ds.ptr, out.ptr, 1, st) == 0
+    assert np.array_equal(out.numpy().reshape(-1), R.pool2_bwd(small.astype(np.float64)).astype(np.float32).reshape(-1))
+    assert L.msfem_nn_up2(C * B, H // 2, W // 2, ds.ptr, out.ptr, 0, st) == 0
+    assert np.array_equal(out.numpy().reshape(-1), R.up2(small).reshape(-1))
+    assert L.msfem_nn_up2(C * B, H // 2, W // 2, dx.ptr, out.ptr, 1, st) == 0
+    assert np.allclose(out.numpy().reshape(-1)[:x.size // 4], R.up2_bwd(x.astype(np.float64)).reshape(-1), rtol=1e-6, atol=1e-6)
+    # bias + ReLU, masked backward, bias gradient
+    N = B * H * W
+    bias = rng.standard_normal(C).astype(np.float32)
+    dbias, y = ctx.to_device(bias), ctx.to_device(x)
+    assert L.msfem_nn_bias_act(C, N, dbias.ptr, y.ptr, 1, st) == 0
+    yref = np.maximum(x.reshape(C, N) + bias[:, None], 0)
+    assert np.array_equal(y.numpy().reshape(-1).reshape(C, N), yref)
+    d = rng.standard_normal((C, N)).astype(np.float32)
+    dd, dbf2 = ctx.to_device(d), ctx.alloc(C * N, np.uint16)
+    assert L.msfem_nn_act_bwd(C * N, y.ptr, dd.ptr, dbf2.ptr, 1, st) == 0
+    dm = np.where(yref > 0, d, 0).astype(np.float32)
+    assert np.array_equal(dd.numpy().reshape(-1).reshape(C, N), dm)
+    assert np.array_equal(dbf2.numpy().reshape(-1), bf16_bits(R.bf16(dm)).reshape(-1))
+    gb = ctx.alloc(C, np.float32)
+    assert L.msfem_nn_bias_grad(C, N, dd.ptr, gb.ptr, st) == 0
+    assert np.allclose(gb.numpy().reshape(-1), dm.astype(np.float64).sum(axis=1), rtol=1e-5, atol=1e-5)
+    # axpy / axpby
+    a2 = ctx.to_device(x)
+    assert L.msfem_nn_axpby(x.size, 2.0, dx.ptr, -1.0, dx.ptr, a2.ptr, st) == 0
+    assert np.allclose(a2.numpy().reshape(-1), x.reshape(-1))
+    assert L.msfem_nn_axpy(x.size, 0.5, dx.ptr, a2.ptr, st) == 0
+    assert np.allclose(a2.numpy().reshape(-1), 1.5 * x.reshape(-1))
+    assert L.msfem_nn_axpby(x.size, 1.0, dx.ptr, 0.0, None, a2.ptr, st) == 0
+    assert np.array_equal(a2.numpy().reshape(-1), x.reshape(-1))
+
+
+def test_mse_and_adam(ctx):
+    say("nn mse + adam")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(6)
+    n = 10007
+    p, t = rng.standard_normal(n).astype(np.float32), rng.standard_normal(n).astype(np.float32)
+    dp, dt, g, part = ctx.to_device(p), ctx.to_device(t), ctx.alloc(n, np.float32), ctx.alloc(64, np.float32)
+    assert L.msfem_nn_mse(n, dp.ptr, dt.ptr, g.ptr, part.ptr, 64, st) == 0
+    d = p.astype(np.float64) - t
+    assert abs(part.numpy().reshape(-1).astype(np.float64).sum() / n - (d * d).mean()) < 1e-5 * (d * d).mean()
+    assert np.allclose(g.numpy().reshape(-1), 2 * d / n, rtol=1e-5, atol=1e-12)
+    # two Adam steps against the textbook update (bias-corrected, L2 decay added to the gradient)
+    w = rng.standard_normal(n).astype(np.float32)
+    dw, m, v, wbf = ctx.to_device(w), ctx.to_device(np.zeros(n, np.float32)), ctx.to_device(np.zeros(n, np.float32)), \
+        ctx.alloc(n, np.uint16)
+    lr, b1, b2, eps, wd = 1e-2, 0.9, 0.999, 1e-8, 1e-3
+    wr, mr, vr = w.astype(np.float64), np.zeros(n), np.zeros(n)
+    for step in (1, 2):
+        gr = g.numpy().reshape(-1).astype(np.float64) + wd * dw.numpy().reshape(-1).astype(np.float64)
+        assert L.msfem_nn_adam(n, dw.ptr, g.ptr, m.ptr, v.ptr, wbf.ptr, lr, b1, b2, eps, step, wd, st) == 0
+        mr = b1 * mr + (1 - b1) * gr
+        vr = b2 * vr + (1 - b2) * gr * gr
+        mh, vh = mr / (1 - b1 ** step), vr / (1 - b2 ** step)
+        wr = wr - lr * mh / (np.sqrt(vh) + eps)
+    got = dw.numpy().reshape(-1)
+    assert np.abs(got - wr).max() < 1e-5
+    assert np.array_equal(wbf.numpy().reshape(-1), bf16_bits(R.bf16(got)))
+
+
+def tiny_net(ctx, chans, tile, seed):
+    net = S.Surrogate(ctx, tile=tile, chans=chans, seed=seed)
+    rng = np.random.default_rng(seed + 100)
+    for layer in net.layers:  # nonzero biases so every bias path is exercised
+        w, _ = layer.get()
+        layer.set(w, rng.standard_normal(layer.cout).astype(np.float32) * 0.1)
+    return net
+
+
+def test_tiny_net_forward_and_gradients(ctx):
+    """Whole forward/backward of a 3-level net (16x16 tiles, 4/8/8 channels) vs the oracle."""
+    say("tiny surrogate forward/backward vs oracle")
+    chans, tile, B = (4, 8, 8), 16, 3
+    net = tiny_net(ctx, chans, tile, seed=7)
+    rng = np.random.default_rng(8)
+    x = rng.standard_normal((1, B, tile, tile)).astype(np.float32)
+    target = (rng.standard_normal((4, B, tile - 1, tile - 1)) * 0.1).astype(np.float32)
+    dx, dtg = ctx.to_device(x), ctx.to_device(target)
+    out = net.forward(dx, B)
+    n = 4 * B * (tile - 1) ** 2
+    got_out = out.numpy().reshape(-1)[:n].astype(np.float64)
+    loss, grad = net.loss_and_grad(out, dtg, n)
+    net.backward(grad)
+    ctx.synchronize()
+    params = {l.name: (l.get()[0].astype(np.float64), l.get()[1].astype(np.float64)) for l in net.layers}
+    ref = R.Net(params, chans)
+    ref_out = ref.forward(x.astype(np.float64))
+    assert frob(got_out, ref_out.reshape(-1)) < 1e-3
+    ref_loss, dref = R.loss_and_grad(ref_out, target.astype(np.float64))
+    assert abs(loss - ref_loss) < 1e-3 * ref_loss
+    ref.backward(dref)
+    rg = ref.grads()
+    for layer in net.layers:
+        gw, gb = layer.gw.numpy().reshape(-1), layer.gb.numpy().reshape(-1)
+        ew, eb = frob(gw, rg[layer.name][0].reshape(-1)), frob(gb, rg[layer.name][1])
+        say(f"  {layer.name}: dW rel {ew:.2e}  db rel {eb:.2e}")
+        assert ew < 1e-3 and eb < 1e-3, layer.name
+
+
+def test_tiny_net_training_reduces_loss(ctx):
+    """Adam on a fixed batch: 60 steps cut the loss at least 5x (all kernels in the loop)."""
+    say("tiny surrogate training")
+    chans, tile, B = (8, 16, 16), 16, 8
+    net = tiny_net(ctx, chans, tile, seed=9)
+    rng = np.random.default_rng(10)
+    x = rng.standard_normal((1, B, tile, tile)).astype(np.float32)
+    # a smooth target that depends on the input: local means of x, per output channel
+    xs = x[0].astype(np.float64)
+    t = np.stack([0.25 * (xs[:, :-1, :-1] + xs[:, 1:, :-1] + xs[:, :-1, 1:] + xs[:, 1:, 1:]) * (i + 1) * 0.1
+                  for i in range(4)]).astype(np.float32)
+    dx, dt = ctx.to_device(x), ctx.to_device(t)
+    losses = [net.train_step(dx, dt, B, lr=3e-3) for _ in range(60)]
+    say(f"  loss {losses[0]:.3e} -> {losses[-1]:.3e}")
+    assert np.isfinite(losses).all()
+    assert losses[-1] < losses[0] / 5
+
+
+def test_delta_bases_round_trip(ctx):
+    """Solver bases -> delta (FP32) -> bases: boundary exact, interior within FP32 rounding."""
+    say("delta <-> bases")
+    nf, n = 16, 5
+    g = msfem.grf_log(n, nf, 40, 4.0, S.CLIP)
+    boxes = msfem.unit_boxes(2, 4)[:n]
+    out = ctx.batch_build(2, [nf] * n, boxes, msfem.COEF_SCALAR_CELL, np.exp(g.astype(np.float64)).reshape(-1),
+                          msfem.RHS_CONST, 0.0)
+    bases = out["bases"]
+    L, st = S._nn(), ctx.stream
+    db = ctx.to_device(bases)
+    dl = ctx.alloc(4 * n * (nf - 1) ** 2, np.float32)
+    assert L.msfem_nn_bases_to_delta(n, nf, db.ptr, dl.ptr, st) == 0
+    back = ctx.alloc(bases.size, np.float64)
+    assert L.msfem_nn_delta_to_bases(n, nf, dl.ptr, back.ptr, st) == 0
+    b2 = back.numpy().reshape(-1).reshape(n, 4, nf + 1, nf + 1)
+    b1 = bases.reshape(n, 4, nf + 1, nf + 1)
+    edge = np.ones((nf + 1, nf + 1), bool)
+    edge[1:-1, 1:-1] = False
+    assert np.array_equal(b2[:, :, edge], b1[:, :, edge])
+    assert np.abs(b2 - b1).max() < 1e-7
