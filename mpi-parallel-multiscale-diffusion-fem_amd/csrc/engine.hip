@@ -309,10 +309,18 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       const int nf = p.class_nf[k];
       const int ncls = (int)p.class_cells[k].size();
       const int *cells = d_cells(ctx) + p.cells_dev_off[k];
-      const int mode = class_mode(in->dim, in->coef_kind, nf);
+      const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
+      int mode = class_mode(in->dim, in->coef_kind, nf);
+      // the FP32 inner CG of the 128^2 tensor class streams 4 FP32 conductances
+      // per fine cell (planar, in the same scratch: 4 nf^2 floats <= 6 nf^2 doubles)
+      if (mode == kModeGlobal && fp32 && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
+        mode = kModeStream;
       if (mode == kModeGlobal)
         HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
                                          d_off(ctx, 2), st));
+      if (mode == kModeStream)
+        HIP_TRY(ctx, launch_prep_cond32_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), (float *)cond,
+                                           d_off(ctx, 2), st));
       LocalArgs a;
       a.nb_solve = pou ? nb - 1 : nb;
       a.n_problems = ncls * a.nb_solve;
@@ -322,6 +330,7 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       a.coef = in->coef;
       a.coef_off = d_off(ctx, 0);
       a.cond = cond;
+      a.cond32 = (const float *)cond;
       a.cond_off = d_off(ctx, 2);
       a.bases = bases;
       a.bas_off = d_off(ctx, 1);
@@ -334,7 +343,6 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       a.inner_rel_tol = o->inner_rel_tol;
       a.max_iter = o->max_iter;
       a.max_refine = o->max_refine;
-      const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
       int variant = o->variant;
       if (variant == 0)
         {

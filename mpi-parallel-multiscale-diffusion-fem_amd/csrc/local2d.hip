@@ -70,7 +70,7 @@ __device__ __forceinline__ Cond6<double> cell_cond64(const double *cf, const dou
   const int f = cy * NF + cx;
   if constexpr (MODE == kModeScalar)
     return from4<double>(cond_scalar(cf[f], rho));
-  else if constexpr (MODE == kModeTensor)
+  else if constexpr (MODE == kModeTensor || MODE == kModeStream)
     return from4<double>(cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho));
   else
     {
@@ -151,14 +151,19 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr int GR = (SW * R + 2 > NP) ? SW * R + 2 : NP;
   constexpr int GSZ = GP * GR;
   constexpr int NCACHE = MODE == kModeScalar ? 1 : (MODE == kModeTensor ? 4 : 1);
+  // kModeStream keeps q = K p of the own nodes in LDS (thread-private slots)
+  constexpr bool QLDS = MODE == kModeStream;
+  constexpr int NT = kWave * SW;
 
   __shared__ T sgrid[PPW * GSZ];
   __shared__ T sred[SW * PPW * 4];        // inner-CG partial sums (type T)
   __shared__ double sred64[SW * PPW * 2]; // FP64 residual norms
+  __shared__ T sq[QLDS ? R * CPL * NT : 1];
+  __shared__ T sred2[QLDS ? SW * PPW * 2 : 1]; // second reduction of the two-reduction PCG
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
-  const int prob = blockIdx.x * PPW + slot;
+  const int prob = xcd_block(blockIdx.x, gridDim.x) * PPW + slot;
   const bool pvalid = prob < a.n_problems;
   int cell = 0, basis = 0;
   if (pvalid)
@@ -174,6 +179,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   double rho = 1.0;
   const double *cf = a.coef;
   const double *cg = a.cond;
+  const float *c32 = a.cond32;
   double *phi = a.bases;
   if (pvalid)
     {
@@ -182,6 +188,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       cf = a.coef + a.coef_off[cell];
       if (MODE == kModeGlobal)
         cg = a.cond + a.cond_off[cell];
+      if (MODE == kModeStream)
+        c32 = a.cond32 + a.cond_off[cell];
       phi = a.bases + a.bas_off[cell] + (size_t)basis * NP * NP;
     }
 
@@ -223,6 +231,18 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       }
     else if constexpr (MODE == kModeTensor)
       return {{cc[i][j][0], cc[i][j][1], cc[i][j][2], cc[i][j][3], cc[i][j][1], cc[i][j][0]}};
+    else if constexpr (MODE == kModeStream)
+      {
+        // no bounds test: cx, cy <= NF stay inside the padded planes, and the
+        // values read for cells outside the tile only reach nodes that are
+        // not interior (masked by the zero diagonal entry)
+        constexpr int PS = stream_plane_stride(NF);
+        typedef const float __attribute__((address_space(1))) * gptr;
+        const unsigned f = (unsigned)((Y0 - 1 + i) * NF + (X0 - 1 + j));
+        const gptr q = (gptr)c32;
+        const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
+        return {{ch, cv, cd, ca, cv, ch}};
+      }
     else
       {
         const int cx = X0 - 1 + j, cy = Y0 - 1 + i;
@@ -253,6 +273,40 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     (sw.c[5] + sw.c[4] + sw.c[2]);
         dinv[i][c] = act ? (float)((T)1 / d) : 0.0f;
       }
+  // kModeStream keeps the diagonal as packed BF16 pairs (8 registers instead
+  // of 16); any positive diagonal is a valid Jacobi preconditioner, and a
+  // nonzero value stays nonzero, so the active-node mask is unchanged.
+  constexpr bool DPK = MODE == kModeStream;
+  uint32_t dpk[DPK ? (R * CPL + 1) / 2 : 1];
+  if constexpr (DPK)
+    {
+#pragma unroll
+      for (int k = 0; k < (R * CPL + 1) / 2; ++k)
+        {
+          uint32_t lo = 0, hi = 0;
+          const int k0 = 2 * k, k1 = 2 * k + 1;
+          {
+            const uint32_t u = __float_as_uint(dinv[k0 / CPL][k0 % CPL]);
+            lo = (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+          }
+          if (k1 < R * CPL)
+            {
+              const uint32_t u = __float_as_uint(dinv[k1 / CPL][k1 % CPL]);
+              hi = (u + 0x7fffu + ((u >> 16) & 1u)) & 0xffff0000u;
+            }
+          dpk[k] = lo | hi;
+        }
+    }
+  auto dget = [&](int i, int c) -> float {
+    if constexpr (DPK)
+      {
+        const int k = i * CPL + c;
+        const uint32_t w = dpk[k >> 1];
+        return __uint_as_float((k & 1) ? (w & 0xffff0000u) : (w << 16));
+      }
+    else
+      return dinv[i][c];
+  };
 
   // ---- initial state: LDS grid zero, phi = g on the boundary, x0 inside ----
   for (int k = tid; k < PPW * GSZ; k += blockDim.x)
@@ -332,7 +386,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       for (int i = 0; i < R; ++i)
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
-          rr[i][c] = dinv[i][c] != 0.0f ? gown[i * GP + c] : (T)0;
+          rr[i][c] = dget(i, c) != 0.0f ? gown[i * GP + c] : (T)0;
       problem_sums<double, LW, SW, PPW, 1>(s, false, sred64, wave, slot, pl);
       rnorm = sqrt(s[0]);
       if (cyc == 0 && !a.x0)
@@ -370,6 +424,112 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           }
       bool irun = run, first = true;
       T gamma_prev = 1, alpha_prev = 1;
+      if constexpr (MODE == kModeStream)
+        {
+          // Standard Jacobi PCG with two reductions per iteration.  Only x, r
+          // and the Jacobi diagonal stay in registers; p lives in the LDS grid
+          // (the stencil's input, read back from the own slots for the
+          // updates), q = K p in thread-private LDS slots, and the conductances
+          // of the two fine-cell rows around a node row are streamed from
+          // global memory as the stencil slides upward.
+          T v2[2] = {0, 0}; // (r, z), (r, r)
+#pragma unroll
+          for (int i = 0; i < R; ++i)
+#pragma unroll
+            for (int c = 0; c < CPL; ++c)
+              {
+                const T z = rr[i][c] * (T)dget(i, c);
+                gown[i * GP + c] = irun ? z : (T)0;
+                v2[0] += rr[i][c] * z;
+                v2[1] += rr[i][c] * rr[i][c];
+              }
+          bool any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
+          if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
+            irun = false;
+          T rz = v2[0];
+          while (any)
+            {
+              __syncthreads(); // p complete in the grid
+              // opaque per-iteration copy of the (wave-uniform) tile address: keeps
+              // the loads in the loop (hoisted, the 27 cells' conductances would
+              // take 108 registers)
+              static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
+              asm volatile("" : "+s"(c32));
+              T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
+              Cond6<T> cb[CPL + 1], ct[CPL + 1];
+#pragma unroll
+              for (int k = 0; k < CPL + 2; ++k)
+                {
+                  w0[k] = gwin[k];
+                  w1[k] = gwin[GP + k];
+                }
+#pragma unroll
+              for (int j = 0; j <= CPL; ++j)
+                cb[j] = cond_T(0, j);
+              T pq[1] = {0};
+#pragma unroll
+              for (int i = 0; i < R; ++i)
+                {
+#pragma unroll
+                  for (int j = 0; j <= CPL; ++j)
+                    ct[j] = cond_T(i + 1, j);
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    w2[k] = gwin[(i + 2) * GP + k];
+#pragma unroll
+                  for (int c = 0; c < CPL; ++c)
+                    {
+                      const T y = gather6<T>(w1[c + 1], w1[c + 2], w1[c], w2[c + 1], w0[c + 1], w2[c + 2], w2[c],
+                                             w0[c + 2], w0[c], ct[c + 1], ct[c], cb[c + 1], cb[c]);
+                      const T q = dget(i, c) != 0.0f ? y : (T)0;
+                      sq[(i * CPL + c) * NT + tid] = q;
+                      pq[0] += w1[c + 1] * q;
+                    }
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    {
+                      w0[k] = w1[k];
+                      w1[k] = w2[k];
+                    }
+#pragma unroll
+                  for (int j = 0; j <= CPL; ++j)
+                    cb[j] = ct[j];
+                  __builtin_amdgcn_sched_barrier(0);
+                }
+              problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave, slot, pl);
+              v2[0] = 0;
+              v2[1] = 0;
+              if (irun)
+                {
+                  const T alpha = quotient<T>(rz, pq[0]);
+#pragma unroll
+                  for (int i = 0; i < R; ++i)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c)
+                      {
+                        xr[i][c] += alpha * gown[i * GP + c];
+                        rr[i][c] -= alpha * sq[(i * CPL + c) * NT + tid];
+                        const T z = rr[i][c] * (T)dget(i, c);
+                        v2[0] += rr[i][c] * z;
+                        v2[1] += rr[i][c] * rr[i][c];
+                      }
+                  ++it_total;
+                }
+              any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
+              if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
+                irun = false;
+              // every lane is past this iteration's stencil (two barriers ago):
+              // the own slots of the grid take the next direction in place
+              const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
+              rz = v2[0];
+#pragma unroll
+              for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                  gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) + beta * gown[i * GP + c] : (T)0;
+            }
+        }
+      else
       while (true)
         {
           // z = M r into the LDS grid (zero on padding/boundary slots)
@@ -377,7 +537,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-              gown[i * GP + c] = irun ? rr[i][c] * (T)dinv[i][c] : (T)0;
+              gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) : (T)0;
           __syncthreads();
           // w = K z, rows slid through a 3-row window; fused partial sums
           T wv[R][CPL];
@@ -412,7 +572,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     y = gather6<T>(w1[c + 1], w1[c + 2], w1[c], w2[c + 1], w0[c + 1], w2[c + 2], w2[c], w0[c + 2],
                                    w0[c], cond_T(i + 1, c + 1), cond_T(i + 1, c), cond_T(i, c + 1), cond_T(i, c));
                   const T z = w1[c + 1];
-                  wv[i][c] = dinv[i][c] != 0.0f ? y : (T)0;
+                  wv[i][c] = dget(i, c) != 0.0f ? y : (T)0;
                   sums[0] += rr[i][c] * z;
                   sums[1] += wv[i][c] * z;
                   sums[2] += rr[i][c] * rr[i][c];
@@ -448,7 +608,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
                   {
-                    const T z = rr[i][c] * (T)dinv[i][c];
+                    const T z = rr[i][c] * (T)dget(i, c);
                     pr[i][c] = z + tb * pr[i][c];
                     sr[i][c] = wv[i][c] + tb * sr[i][c];
                     xr[i][c] += ta * pr[i][c];
@@ -469,7 +629,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-              if (dinv[i][c] != 0.0f)
+              if (dget(i, c) != 0.0f)
                 phi[(Y0 + i) * NP + X0 + c] += (double)xr[i][c];
         }
       __syncthreads();
@@ -537,6 +697,32 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
     }
 }
 
+// ---- FP32 planar conductances for kModeStream (symmetric tensor per fine cell):
+// plane k of cell c at cond32 + cond_off[c] + k stream_plane_stride(nf),
+// k = (ch, cv, cd, ca); entry cy nf + cx.  The plane padding (never written,
+// read only for cells outside the tile) keeps unguarded reads in bounds.
+__global__ void prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
+                               const long long *coef_off, float *cond32, const long long *cond_off)
+{
+  const int ncell = nf * nf;
+  const long long total = (long long)n_cells_class * ncell;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
+    {
+      const int cell = cells[t / ncell];
+      const int f = (int)(t % ncell);
+      const double *bx = box + (size_t)cell * 4;
+      const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      const double *cf = coef + coef_off[cell];
+      const Cond4 c = cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho);
+      const int ps = stream_plane_stride(nf);
+      float *o = cond32 + cond_off[cell] + f;
+      o[0] = (float)c.ch;
+      o[ps] = (float)c.cv;
+      o[2 * ps] = (float)c.cd;
+      o[3 * ps] = (float)c.ca;
+    }
+}
+
 // ---- launch table -----------------------------------------------------------------
 // Variant ids per fine resolution (the autotuner times every id and keeps the
 // fastest per class; id 0 is the built-in default):
@@ -551,8 +737,10 @@ template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE>
 hipError_t launch_one(const LocalArgs &a, hipStream_t st)
 {
   constexpr int PPW = kWave / (NF / CPL);
-  const int grid = (a.n_problems + PPW - 1) / PPW;
-  if (grid == 0)
+  // a multiple of 8 blocks (xcd_block is a bijection on it); blocks past the
+  // last problem find no valid problem and leave at the first residual test
+  const int grid = ((a.n_problems + PPW - 1) / PPW + 7) / 8 * 8;
+  if (a.n_problems == 0)
     return hipSuccess;
   hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE, WPE>), dim3(grid), dim3(kWave * SW), 0, st, a);
   return hipGetLastError();
@@ -609,6 +797,12 @@ int local2d_variants(int nf)
 
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st)
 {
+  if (mode == kModeStream)
+    {
+      if (!fp32 || nf != 128)
+        return hipErrorInvalidValue;
+      return launch_one<float, 128, 2, 16, 8, kModeStream, 4>(a, st);
+    }
   if (fp32)
     {
       if (mode == kModeScalar)
@@ -622,6 +816,21 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
   if (mode == kModeTensor)
     return launch_mode<double, kModeTensor>(nf, variant, a, st);
   return launch_mode<double, kModeGlobal>(nf, variant, a, st);
+}
+
+hipError_t launch_prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
+                                 const long long *coef_off, float *cond32, const long long *cond_off, hipStream_t st)
+{
+  const long long total = (long long)n_cells_class * nf * nf;
+  if (total == 0)
+    return hipSuccess;
+  const int block = 256;
+  long long grid = (total + block - 1) / block;
+  if (grid > 65536)
+    grid = 65536;
+  hipLaunchKernelGGL(prep_cond32_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, box, coef,
+                     coef_off, cond32, cond_off);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,

@@ -12,8 +12,24 @@ constexpr int kWave = 64;
 enum CondMode : int {
   kModeScalar = 0, // one scalar per fine cell, cached in registers
   kModeTensor = 1, // 2D symmetric tensor per fine cell, 4 conductances cached in registers
-  kModeGlobal = 2  // per-fine-cell element conductances read from global memory (prep buffer)
+  kModeGlobal = 2, // per-fine-cell element conductances read from global memory (prep buffer)
+  kModeStream = 3  // FP32 inner CG: 4 FP32 conductances per fine cell streamed from global memory
+                   // (planar prep buffer), the CG product vector w kept in LDS
 };
+
+// Plane stride (floats) of the kModeStream conductance planes: nf^2 cells plus
+// padding for the unguarded reads of cells (cx, cy) with cx, cy <= nf.  Four
+// planes (4 nf^2 + 8 nf floats) fit the 6 nf^2 doubles reserved per cell.
+__host__ __device__ constexpr int stream_plane_stride(int nf) { return nf * nf + 2 * nf; }
+
+// Workgroup -> block index such that the blocks one XCD receives (hardware
+// dispatch is round-robin over the 8 XCDs) hold consecutive problems: the
+// bases of one coarse cell then run on one XCD at the same time and share its
+// L2.  grid must be a multiple of 8; a bijection on [0, grid).
+__device__ __forceinline__ int xcd_block(int b, int grid)
+{
+  return (b & 7) * (grid >> 3) + (b >> 3);
+}
 
 // Arguments of the local basis solves of one size class (one fine resolution).
 struct LocalArgs {
@@ -25,6 +41,7 @@ struct LocalArgs {
   const double *coef;         // concatenated coefficient tiles
   const long long *coef_off;  // [n_cells] tile offsets (elements)
   const double *cond;         // kModeGlobal: concatenated conductance tiles
+  const float *cond32;        // kModeStream: planar FP32 conductance tiles (same offsets, in floats)
   const long long *cond_off;  // [n_cells]
   double *bases;              // concatenated basis blocks (working FP64 solution + output)
   const long long *bas_off;   // [n_cells]

@@ -297,6 +297,17 @@ class DeviceArray:
         _check(lib().msfem_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes), self.ctx.h)
         return out
 
+    def read(self, start, count):
+        """Host copy of elements start .. start+count of the flattened buffer."""
+        total = self.nbytes // self.dtype.itemsize
+        if start < 0 or count < 0 or start + count > total:
+            raise ValueError(f"range {start}+{count} outside a buffer of {total} elements")
+        out = np.empty(count, self.dtype)
+        if count:
+            _check(lib().msfem_memcpy_d2h(self.ctx.h, out.ctypes.data, self.ptr + start * self.dtype.itemsize,
+                                          count * self.dtype.itemsize), self.ctx.h)
+        return out
+
     def free(self):
         if self.ptr and self.ctx.h:
             lib().msfem_dev_free(self.ctx.h, self.ptr)

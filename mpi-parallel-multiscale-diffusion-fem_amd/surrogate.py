@@ -162,6 +162,7 @@ class Surrogate:
         self.layers = self.enc + self.dec + [self.head]
         self.buf = Buffers(ctx)
         self.step_count = 0
+        self.dscale = 1.0  # network output = delta / dscale
 
     def n_params(self):
         return sum(l.n_params() for l in self.layers)
@@ -273,36 +274,294 @@ class Surrogate:
             l.set(st[l.name + ".w"], st[l.name + ".b"])
 
     def save(self, path):
-        np.savez(path, **self.state(), chans=np.array(self.chans), tile=self.tile)
+        np.savez(path, **self.state(), chans=np.array(self.chans), tile=self.tile, dscale=self.dscale)
 
     @staticmethod
     def load(ctx, path):
         d = np.load(path)
         s = Surrogate(ctx, int(d["tile"]), tuple(int(v) for v in d["chans"]))
         s.load_state({k: d[k] for k in d.files if k.endswith(".w") or k.endswith(".b")})
+        s.dscale = float(d["dscale"])
         return s
 
     # ---- serving ------------------------------------------------------------------------
-    def predict_bases(self, g_tiles, bases_out, batch=128):
-        """g_tiles: host float32 [n][tile*tile] log-coefficients; bases_out: device
-        FP64 bases buffer [n][4][(tile+1)^2] (the x0 of msfem_batch_build_device)."""
+    def predict_bases(self, x, n, bases_ptr, batch=128):
+        """x: device FP32 [n][tile*tile] normalized log-coefficients (g / CLIP);
+        bases_ptr: device FP64 bases [n][4][(tile+1)^2], filled with N_i + dscale *
+        prediction (the x0 of msfem_batch_build_device).  Enqueued on the context
+        stream, not synchronized."""
         L, st = _nn(), self.ctx.stream
-        n = g_tiles.shape[0]
         t = self.tile
-        xin = self.buf.get("serve_x", batch * t * t, np.float32)
         nbase = 4 * (t + 1) ** 2
         for s in range(0, n, batch):
             b = min(batch, n - s)
-            xin.upload(np.concatenate([(g_tiles[s:s + b] / CLIP).astype(np.float32).reshape(-1),
-                                       np.zeros((batch - b) * t * t, np.float32)]))
-            out = self.forward(xin, batch)
-            # head output is [4][batch][(t-1)^2]; the first b samples of each channel are used
-            if b < batch:
-                packed = self.buf.get("serve_pack", 4 * b * (t - 1) ** 2, np.float32)
-                host = out.numpy()[:4 * batch * (t - 1) ** 2].reshape(4, batch, -1)[:, :b]
-                packed.upload(np.ascontiguousarray(host).reshape(-1))
-                src = packed
-            else:
-                src = out
-            _ok(L.msfem_nn_delta_to_bases(b, t, src.ptr, bases_out.ptr + s * nbase * 8, st))
-        self.ctx.synchronize()
+            out = self.forward(Ptr(x.ptr + s * t * t * 4), b)
+            m = 4 * b * (t - 1) ** 2
+            _ok(L.msfem_nn_axpby(m, self.dscale, out.ptr, 0.0, None, out.ptr, st))
+            _ok(L.msfem_nn_delta_to_bases(b, t, out.ptr, bases_ptr + s * nbase * 8, st))
+
+
+class Ptr:
+    """A bare device address (a batch slice of a DeviceArray)."""
+
+    def __init__(self, ptr):
+        self.ptr = ptr
+
+
+CORR_LEN = 4.0  # correlation length of the training/serving fields, in fine cells
+
+
+class TileSet:
+    """n cells of nf x nf fine cells with alpha = exp(g), g a Gaussian random
+    field (msfem_gen_grf_log, clipped to contrast 1e3), resident on the GPU:
+    x = g / CLIP (FP32, the network input), coef = exp(g) (FP64, the solver
+    input), unit boxes (the bases of a scalar coefficient on a square cell do
+    not depend on its size)."""
+
+    def __init__(self, ctx, n, seed, nf=64):
+        self.ctx, self.n, self.nf, self.seed = ctx, n, nf, seed
+        g = msfem.grf_log(n, nf, seed, CORR_LEN, CLIP)
+        self.x = ctx.to_device((g / np.float32(CLIP)).astype(np.float32).reshape(-1))
+        self.coef_host = np.exp(g.astype(np.float64)).reshape(-1)
+        self.coef = ctx.to_device(self.coef_host)
+        side = int(np.ceil(np.sqrt(n)))
+        self.box_host = msfem.unit_boxes(2, side)[:n]
+        self.box = ctx.to_device(np.ascontiguousarray(self.box_host).reshape(-1))
+        self.n_fine = np.full(n, nf, np.int32)
+        nb = 4 * (nf + 1) ** 2
+        self.bases = ctx.alloc(n * nb, np.float64)
+        self.Ae = ctx.alloc(n * 16, np.float64)
+        self.Fe = ctx.alloc(n * 4, np.float64)
+        self.iters = ctx.alloc(n * 4, np.int32)
+        self.status = ctx.alloc(n * 4, np.int32)
+
+    def solve(self, opts, x0_ptr=None):
+        """Batched basis build into self.bases (enqueued; returns nothing)."""
+        self.ctx.batch_build_device(2, self.n_fine, self.box.ptr, msfem.COEF_SCALAR_CELL, self.coef.ptr, self.Ae.ptr,
+                                    self.Fe.ptr, self.bases.ptr, self.iters.ptr, self.status.ptr,
+                                    rhs_kind=msfem.RHS_CONST, rhs_value=1.0, x0_ptr=x0_ptr, opts=opts,
+                                    stream=self.ctx.stream)
+
+    def free(self):
+        for a in (self.x, self.coef, self.box, self.bases, self.Ae, self.Fe, self.iters, self.status):
+            a.free()
+
+
+def targets(net, data, s, b, out):
+    """Scaled training targets of cells s..s+b: delta / dscale, [4][b][(nf-1)^2] FP32."""
+    L, st = _nn(), net.ctx.stream
+    nf = data.nf
+    _ok(L.msfem_nn_bases_to_delta(b, nf, data.bases.ptr + s * 4 * (nf + 1) ** 2 * 8, out.ptr, st))
+    _ok(L.msfem_nn_axpby(4 * b * (nf - 1) ** 2, 1.0 / net.dscale, out.ptr, 0.0, None, out.ptr, st))
+
+
+def evaluate(net, data, start, count, batch=64):
+    """Mean squared error (in scaled units) of the predictions for cells start..start+count."""
+    n_per = 4 * (net.tile - 1) ** 2
+    tgt = net.buf.get("eval_t", batch * n_per, np.float32)
+    tot, m = 0.0, 0
+    for s in range(start, start + count, batch):
+        b = min(batch, start + count - s)
+        targets(net, data, s, b, tgt)
+        out = net.forward(Ptr(data.x.ptr + s * data.nf ** 2 * 4), b)
+        loss, _ = net.loss_and_grad(out, tgt, b * n_per)
+        tot += loss * b
+        m += b
+    return tot / m
+
+
+def train(net, data, n_train, epochs, batch=32, lr=1e-3, lr_min=2e-5, wd=0.0, log=print, ckpt=None, seed=0,
+          n_val=0):
+    """Adam with a cosine learning-rate decay over `epochs` passes of the first
+    n_train cells (minibatches of consecutive cells, visited in a fresh random
+    order every epoch); the next n_val cells are the validation set."""
+    import time
+    rng = np.random.default_rng(seed)
+    n_per = 4 * (net.tile - 1) ** 2
+    tgt = net.buf.get("train_t", batch * n_per, np.float32)
+    nb = n_train // batch
+    total, step, hist = epochs * nb, 0, []
+    t0 = tlast = time.perf_counter()
+    for ep in range(epochs):
+        run = 0.0
+        for k, mb in enumerate(rng.permutation(nb)):
+            s = int(mb) * batch
+            targets(net, data, s, batch, tgt)
+            a = lr_min + 0.5 * (lr - lr_min) * (1 + np.cos(np.pi * step / total))
+            run += net.train_step(Ptr(data.x.ptr + s * data.nf ** 2 * 4), tgt, batch, float(a), wd)
+            step += 1
+            if time.perf_counter() - tlast > 30:
+                tlast = time.perf_counter()
+                log(f"epoch {ep} step {k + 1}/{nb} train mse {run / (k + 1):.4e} lr {a:.2e} "
+                    f"{tlast - t0:.0f} s")
+        val = evaluate(net, data, n_train, n_val) if n_val else float("nan")
+        hist.append({"epoch": ep, "train_mse": run / nb, "val_mse": val, "seconds": time.perf_counter() - t0})
+        log(f"epoch {ep}: train mse {run / nb:.4e} val mse {val:.4e} ({time.perf_counter() - t0:.1f} s)")
+        if ckpt:
+            net.save(ckpt)
+    return hist
+
+
+def delta_std(ctx, data, count=256):
+    """RMS of the interior corrections of the first `count` cells (the target scale)."""
+    nf = data.nf
+    d = ctx.alloc(4 * count * (nf - 1) ** 2, np.float32)
+    _ok(_nn().msfem_nn_bases_to_delta(count, nf, data.bases.ptr, d.ptr, ctx.stream))
+    v = d.numpy().astype(np.float64)
+    d.free()
+    return float(np.sqrt((v * v).mean()))
+
+
+def main(argv=None):
+    """Train the surrogate on GPU solves: python surrogate.py --out weights.npz [--cells 5000 --epochs 30]."""
+    import argparse
+    import json
+    import sys
+    import time
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True, help="checkpoint (.npz), rewritten after every epoch")
+    ap.add_argument("--cells", type=int, default=5000, help="training cells (4 local solves each)")
+    ap.add_argument("--val", type=int, default=256, help="validation cells (drawn after the training cells)")
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--batch", type=int, default=32)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=40, help="Philox seed of the training fields")
+    ap.add_argument("--init-seed", type=int, default=0)
+    ap.add_argument("--history", default=None, help="JSON file for the per-epoch losses")
+    ap.add_argument("--load", default=None, help="skip training: serve this checkpoint")
+    ap.add_argument("--serve", type=int, default=0, help="after training, report serving on an N x N coarse mesh")
+    ap.add_argument("--serve-seed", type=int, default=41)
+    ap.add_argument("--report", default=None, help="JSON file for the serving report")
+    a = ap.parse_args(argv)
+
+    def log(msg):
+        print(msg, flush=True)
+        sys.stdout.flush()
+
+    ctx = msfem.Context(0)
+    if a.load:
+        net = Surrogate.load(ctx, a.load)
+    else:
+        net = train_new(ctx, a, log)
+    if a.serve:
+        rep = serve_report(ctx, net, n_side=a.serve, seed=a.serve_seed, log=log)
+        log(json.dumps(rep))
+        if a.report:
+            with open(a.report, "w") as f:
+                json.dump(rep, f, indent=1)
+    ctx.close()
+
+
+def train_new(ctx, a, log):
+    import json
+    import time
+    t0 = time.perf_counter()
+    data = TileSet(ctx, a.cells + a.val, a.seed)
+    log(f"fields: {a.cells + a.val} cells of 64x64, corr. length {CORR_LEN}, seed {a.seed} "
+        f"({time.perf_counter() - t0:.1f} s)")
+    t1 = time.perf_counter()
+    data.solve(msfem.default_opts(precision=msfem.PREC_FP64))
+    ctx.synchronize()
+    st = data.status.numpy()
+    if st.any():
+        raise RuntimeError(f"{int((st != 0).sum())} training solves did not converge")
+    its = data.iters.numpy()
+    log(f"training solves: {4 * (a.cells + a.val)} FP64 local solves in {time.perf_counter() - t1:.2f} s, "
+        f"mean {its.mean():.1f} it")
+    net = Surrogate(ctx, seed=a.init_seed)
+    net.dscale = delta_std(ctx, data)
+    log(f"network: {net.n_params()} parameters, {net.flops_per_sample() / 1e6:.1f} MFLOP/tile forward, "
+        f"target rms {net.dscale:.4e}")
+    hist = train(net, data, a.cells, a.epochs, batch=a.batch, lr=a.lr, log=log, ckpt=a.out, n_val=a.val)
+    if a.history:
+        with open(a.history, "w") as f:
+            json.dump({"cells": a.cells, "val": a.val, "epochs": a.epochs, "batch": a.batch, "lr": a.lr,
+                       "params": net.n_params(), "dscale": net.dscale, "history": hist}, f, indent=1)
+    data.free()
+    return net
+
+
+# Q1 stiffness of a square fine cell (any size in 2D), nodes (0,0),(1,0),(0,1),(1,1)
+_KQ1 = np.array([[4, -1, -1, -2], [-1, 4, -2, -1], [-1, -2, 4, -1], [-2, -1, -1, 4]], np.float64) / 6.0
+
+
+def energy_norm2(alpha, v):
+    """v^T K v for the fine Q1 stiffness of one cell: alpha [nf][nf] per fine
+    cell, v [..., nf+1, nf+1] nodal values (any leading axes)."""
+    c = [v[..., :-1, :-1], v[..., :-1, 1:], v[..., 1:, :-1], v[..., 1:, 1:]]
+    e = sum(_KQ1[i, j] * c[i] * c[j] for i in range(4) for j in range(4))
+    return (alpha * e).sum(axis=(-2, -1))
+
+
+def energy_error(alpha, phi, ref):
+    """max over the cell's bases of ||phi_i - ref_i||_K / ||ref_i||_K."""
+    return float(np.max(np.sqrt(energy_norm2(alpha, phi - ref) / energy_norm2(alpha, ref))))
+
+
+def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, sample=64, log=None, data=None):
+    """Times the basis build of n_side^2 fresh GRF cells four ways -- plain FP64,
+    FP32+FP64 refinement, and each of them warm-started from the network's
+    prediction -- and reports cells/s, iterations, the fallback rate (share of
+    local problems whose prediction failed the residual gate and needed CG
+    iterations) and the energy-norm error of a cell sample against plain FP64."""
+    import time
+    n = n_side * n_side
+    data = data or TileSet(ctx, n, seed)
+    nf = data.nf
+    per = 4 * (nf + 1) ** 2
+    x0 = ctx.alloc(n * per, np.float64)
+    pick = np.unique(np.linspace(0, n - 1, min(sample, n)).astype(np.int64))
+    legs = [("fp64", msfem.default_opts(precision=msfem.PREC_FP64), False),
+            ("fp32_refine", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), False),
+            ("fp32_refine_surrogate", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), True),
+            ("fp64_surrogate", msfem.default_opts(precision=msfem.PREC_FP64), True)]
+    # inference alone
+    net.predict_bases(data.x, n, x0.ptr)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for _ in range(reps):
+        net.predict_bases(data.x, n, x0.ptr)
+    ctx.synchronize()
+    infer_ms = (time.perf_counter() - t) * 1e3 / reps
+    pred = {int(c): x0.read(int(c) * per, per).reshape(4, nf + 1, nf + 1) for c in pick}
+    out, ref = {}, None
+    for name, opts, use in legs:
+        def step():
+            if use:
+                net.predict_bases(data.x, n, x0.ptr)
+            data.solve(opts, x0.ptr if use else None)
+        step()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            step()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t) * 1e3 / reps
+        st = data.status.numpy()
+        if st.any():
+            raise RuntimeError(f"{name}: {int((st != 0).sum())} local solves did not converge")
+        its = data.iters.numpy().astype(np.int64)
+        bases = {int(c): data.bases.read(int(c) * per, per).reshape(4, nf + 1, nf + 1) for c in pick}
+        alpha = {int(c): data.coef_host[int(c) * nf * nf:(int(c) + 1) * nf * nf].reshape(nf, nf) for c in pick}
+        if ref is None:
+            ref = bases
+        r = {"ms": round(ms, 3), "cells_per_s": round(n / (ms * 1e-3), 1), "iters_mean": round(float(its.mean()), 2),
+             "iters_p99": int(np.percentile(its, 99)),
+             "energy_err_max": float(max(energy_error(alpha[c], bases[c], ref[c]) for c in pick))}
+        if use:
+            r["fallback_rate"] = round(float((its > 0).mean()), 4)
+        out[name] = r
+        if log:
+            log(f"{name}: {r}")
+    alpha = {int(c): data.coef_host[int(c) * nf * nf:(int(c) + 1) * nf * nf].reshape(nf, nf) for c in pick}
+    errs = [energy_error(alpha[c], pred[c], ref[c]) for c in pick]
+    out["prediction"] = {"infer_ms": round(infer_ms, 3), "tiles_per_s": round(n / (infer_ms * 1e-3), 1),
+                         "energy_err_median": float(np.median(errs)), "energy_err_max": float(np.max(errs)),
+                         "tflops": round(net.flops_per_sample() * n / (infer_ms * 1e-3) / 1e12, 2)}
+    out["cells"], out["sample"], out["rel_tol"] = n, len(pick), rel_tol
+    x0.free()
+    return out
+
+
+if __name__ == "__main__":
+    main()

@@ -109,3 +109,24 @@ def test_net_gradient_finite_differences(no_rounding):
                 fd = (lp - lm) / (2 * h)
                 an = g[name][gi][idx]
                 assert abs(fd - an) <= 1e-6 + 1e-5 * abs(fd), (name, gi, idx, fd, an)
+
+
+def test_energy_norm_matches_oracle_stiffness():
+    """surrogate.energy_norm2 (the serving error metric) equals v^T K v with the
+    oracle's assembled fine stiffness, for a random field and random nodal vectors."""
+    import oracle as O
+    import surrogate as S
+    rng = np.random.default_rng(11)
+    nf = 8
+    alpha = np.exp(rng.standard_normal((nf, nf)))
+    box = np.array([[0.0, 0.25], [0.0, 0.25]])
+    K, _ = O.local_assemble(2, nf, box, O.COEF_SCALAR_CELL, alpha.reshape(-1))
+    v = rng.standard_normal((3, nf + 1, nf + 1))
+    got = S.energy_norm2(alpha, v)
+    for i in range(3):
+        want = v[i].reshape(-1) @ O.stencil_apply(2, nf, K, v[i].reshape(-1))
+        assert abs(got[i] - want) < 1e-12 * abs(want)
+    # the Q1 nodal interpolant of a linear function has energy alpha |grad|^2 integrated
+    x = np.linspace(0, 1, nf + 1)
+    lin = np.broadcast_to(2.0 * x[None, :], (nf + 1, nf + 1))
+    assert abs(S.energy_norm2(np.ones((nf, nf)), lin) - 4.0) < 1e-12
