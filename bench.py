@@ -10,6 +10,10 @@ to contrast 1e3, f = 1, zero Dirichlet.  Other workloads (--workload):
   lognormal3d  16^3 coarse cells x 16^3 fine, lognormal alpha, f = 1
   periodic8    8x8 coarse cells x 32^2 fine, alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps)
   reference    the reference default: 8x8 x 128^2, MatrixCoeff (PI_D), f = 2
+  surrogate128 128x128 coarse cells x 64^2 fine, alpha = exp(g), g a Gaussian random
+               field (correlation length 4 fine cells, contrast <= 1e3, seed 41);
+               plain FP64 vs FP32+refinement, each with and without the trained
+               surrogate's prediction as initial guess (inference inside the step)
 
 One step = every cell's basis solves + A^K, F^K + the coarse assembly, with all
 inputs already resident in HBM.  The dominant kernel (the local solves) is timed
@@ -38,6 +42,7 @@ for p in (PKG, os.path.join(ROOT, "tests")):
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA issue rate equals FP64 (packed FP32 is disabled)
+PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
 METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 
 
@@ -71,7 +76,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lognormal64",
-                    choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference"])
+                    choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference", "surrogate128"])
     ap.add_argument("--cpu-sample", type=int, default=2048, help="cells in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -82,6 +87,8 @@ def main():
 
     import msfem
 
+    if args.workload == "surrogate128":
+        return bench_surrogate(args)
     case, nc, main_prec, bc_kind, desc = workload(args.workload)
     dim = case["dim"]
     nf_arr = np.asarray(case["n_fine"]).astype(np.int64)
@@ -133,7 +140,22 @@ def main():
             # FP32 CG + FP64 refinement stopped where the energy-norm error is <= 1e-6
             msfem.PREC_FP32_REFINE: msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)}
     res = {p: run(o) for p, o in opts.items()}
-    ms, solve_ms, its = res[main_prec]
+    stream_ms, solve_ms, its = res[main_prec]
+    # the headline step: the same build as one replayed HIP graph (a concurrent
+    # branch per size class), timed by the wall clock and by device marks
+    gopts = msfem.default_opts(precision=main_prec, rel_tol=opts[main_prec].rel_tol, graph=1)
+    for _ in range(args.warmup):
+        step(gopts)
+    ctx.synchronize()
+    tg0 = time.perf_counter()
+    ctx.mark(0)
+    for _ in range(args.steps):
+        step(gopts)
+    ctx.mark(1)
+    ctx.synchronize()
+    ms = (time.perf_counter() - tg0) * 1e3 / args.steps
+    graph_dev_ms = ctx.elapsed(0, 1) / args.steps
+    assert int(t_st.numpy().sum()) == 0, "local solves did not converge (graph)"
     # coarse solve (timed apart: one single-workgroup FP64 CG)
     step(opts[main_prec])
     ctx.synchronize()
@@ -171,7 +193,7 @@ def main():
         "data": "synthetic (Philox-4x32-10 fields generated on the host, resident in HBM)",
         "config": {"workload": f"{args.workload}: {desc}", "coarse_cells": n, "dim": dim,
                    "fine_per_cell": classes, "local_solves": n * nb, "path": pname[main_prec],
-                   "autotuned_classes": tuned},
+                   "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class"},
         "roofline": {"bound": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": None, "kernel_ms": round(solve_ms, 4),
@@ -183,26 +205,83 @@ def main():
         "variants": {pname[p] + "_cells_per_s": round(n / (r[0] * 1e-3), 1) for p, r in res.items()},
     }
     result["variants"].update({pname[p] + "_solve_kernel_ms": round(r[1], 4) for p, r in res.items()})
-    result["variants"].update({"coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0])})
+    result["variants"].update({"coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0]),
+                               "graph_step_device_ms": round(graph_dev_ms, 4),
+                               "stream_step_ms": round(stream_ms, 4)})
 
     if not args.no_cpu_baseline:
-        sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
-        import msfem_cases as C
-        import oracle as O
-        sample = np.unique(np.linspace(0, n - 1, min(args.cpu_sample, n)).astype(np.int64))
-        stride_c = msfem.coef_stride(dim, case["coef_kind"])
-        offs = np.concatenate([[0], np.cumsum(nf_arr ** dim * stride_c)])
-        sub_coef = np.concatenate([case["coef"][offs[i]:offs[i + 1]] for i in sample])
-        cores = len(os.sched_getaffinity(0))
-        tcpu0 = time.perf_counter()
-        r = O.batch_build(dim, np.asarray(case["box"])[sample], nf_arr[sample], case["coef_kind"], sub_coef,
-                          case["rhs_kind"], case["rhs_value"], solver=O.SOLVER_SSOR_PCG, n_threads=cores)
-        tcpu = time.perf_counter() - tcpu0
-        result["cpu_baseline"] = {
-            "value": round(len(sample) / tcpu, 2), "unit": "cells/s", "cores": cores, "kind": "port",
+        result["cpu_baseline"] = cpu_baseline(dim, case["box"], nf_arr, case["coef_kind"], case["coef"],
+                                              case["rhs_kind"], case["rhs_value"], n, args.cpu_sample)
+    ctx.close()
+    print(json.dumps(result))
+
+
+def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, sample_cells):
+    """The oracle's restatement of the reference recipe (CG + SSOR(1.6), abs tol
+    1e-12, 1000 iterations) on an evenly spaced sample of the cells."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
+    import msfem
+    import oracle as O
+    sample = np.unique(np.linspace(0, n - 1, min(sample_cells, n)).astype(np.int64))
+    stride_c = msfem.coef_stride(dim, coef_kind)
+    offs = np.concatenate([[0], np.cumsum(nf_arr ** dim * stride_c)])
+    sub_coef = np.concatenate([coef[offs[i]:offs[i + 1]] for i in sample])
+    cores = len(os.sched_getaffinity(0))
+    t0 = time.perf_counter()
+    r = O.batch_build(dim, np.asarray(box)[sample], nf_arr[sample], coef_kind, sub_coef, rhs_kind, rhs_value,
+                      solver=O.SOLVER_SSOR_PCG, n_threads=cores)
+    t = time.perf_counter() - t0
+    return {"value": round(len(sample) / t, 2), "unit": "cells/s", "cores": cores, "kind": "port",
             "sample": f"{len(sample)} of the {n} cells (evenly spaced), reference recipe CG+SSOR(1.6), abs tol "
                       f"1e-12, max 1000 it (not converged: {r['fails']}, mean it {r['iters'].mean():.1f}), "
-                      f"{tcpu:.1f} s wall"}
+                      f"{t:.1f} s wall"}
+
+
+def bench_surrogate(args):
+    """surrogate128: the headline comparison on the learned-surrogate workload."""
+    import msfem
+    import surrogate as S
+    ctx = msfem.Context(0)
+    ctx.load_variant_table()
+    net = S.Surrogate.load(ctx, os.path.join(PKG, "surrogate_weights.npz"))
+    data = S.TileSet(ctx, 128 * 128, 41)
+    rep = S.serve_report(ctx, net, n_side=128, reps=args.steps, warmup=args.warmup, data=data)
+    main_leg = rep["fp32_refine_surrogate"]
+    pr = rep["prediction"]
+    flops = net.flops_per_sample() * data.n
+    achieved = flops / (pr["infer_ms"] * 1e-3) / 1e12
+    result = {
+        "metric": METRIC,
+        "value": main_leg["cells_per_s"],
+        "unit": "cells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_leg["ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 network (f32 accumulate) + f32 CG (FP64 refinement)",
+        "data": "synthetic (Philox-4x32-10 Gaussian random fields generated on the host, resident in HBM); "
+                "network trained on the GPU from plain-solver bases of 5000 other cells (seed 40)",
+        "config": {"workload": "surrogate128: 128^2 coarse cells x 64^2 fine, alpha=exp(g), g Gaussian random field "
+                               "(corr. length 4 fine cells, contrast <= 1e3), f=1, 65536 local solves; step = "
+                               "inference + batched basis build + coarse assembly",
+                   "coarse_cells": data.n, "dim": 2, "local_solves": 4 * data.n,
+                   "path": "fp32_refine_1e-6_energy + surrogate initial guess", "network_params": net.n_params()},
+        "roofline": {"bound": "mfma", "kernel": "surrogate inference (gemm_bf16 MFMA + im2col/bias/pool glue)",
+                     "achieved": round(achieved, 3), "peak": PEAK_BF16_MFMA_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_BF16_MFMA_TFLOPS, 5), "traffic": None,
+                     "kernel_ms": pr["infer_ms"], "flops_per_launch": float(flops)},
+        "variants": {k: v for k, v in rep.items() if isinstance(v, dict)},
+        "speedup_vs_plain_fp64": round(main_leg["cells_per_s"] / rep["fp64"]["cells_per_s"], 3),
+        "speedup_vs_fp32_without_surrogate": round(main_leg["cells_per_s"] / rep["fp32_refine"]["cells_per_s"], 3),
+    }
+    if not args.no_cpu_baseline:
+        nf_arr = np.full(data.n, data.nf, np.int64)
+        result["cpu_baseline"] = cpu_baseline(2, data.box_host, nf_arr, msfem.COEF_SCALAR_CELL, data.coef_host,
+                                              msfem.RHS_CONST, 1.0, data.n, args.cpu_sample)
+    data.free()
     ctx.close()
     print(json.dumps(result))
 

@@ -98,7 +98,12 @@ typedef struct msfem_solver_opts {
   double inner_rel_tol;       /* FP32 inner solve: residual reduction per cycle */
   int32_t variant;            /* launch-shape table entry: 0 = built-in default,
                                  >0 = explicit variant id (see msfem_variants) */
-  int32_t reserved[7];
+  int32_t graph;              /* 1: run the build as a HIP graph, one concurrent branch
+                                 per size class, instantiated on first use and replayed
+                                 by later calls with the same batch layout, pointers and
+                                 options (msfem_ctx_profile is not updated; time it with
+                                 msfem_ctx_mark) */
+  int32_t reserved[6];
 } msfem_solver_opts;
 
 typedef struct msfem_batch {
@@ -111,7 +116,10 @@ typedef struct msfem_batch {
   const double *coef;         /* concatenated coefficient tiles */
   int32_t rhs_kind;           /* MSFEM_RHS_* */
   double rhs_value;
-  const double *x0;           /* optional initial bases (bases layout) or NULL */
+  const double *x0;           /* optional initial bases (bases layout) or NULL: interior
+                                 values only are read; a non-finite value is taken as 0.
+                                 A problem whose initial FP64 residual already meets the
+                                 tolerance takes 0 iterations (accepted as given). */
 } msfem_batch;
 
 typedef struct msfem_batch_out {
@@ -139,6 +147,14 @@ int msfem_version(void);
  * computation" (include/base/diffusion_problem_ms.tpp:44-45). */
 int msfem_ctx_set_profiling(msfem_ctx *ctx, int on);
 int msfem_ctx_profile(msfem_ctx *ctx, double *ms_local_solve, double *ms_element);
+
+/* Timing marks (no counterpart in the reference): msfem_ctx_mark records HIP
+ * event `slot` (0 <= slot < MSFEM_N_MARKS) on the context stream;
+ * msfem_ctx_elapsed waits for mark `to` and returns the device milliseconds
+ * between two recorded marks (e.g. around a surrogate inference). */
+#define MSFEM_N_MARKS 8
+int msfem_ctx_mark(msfem_ctx *ctx, int slot);
+int msfem_ctx_elapsed(msfem_ctx *ctx, int from, int to, double *ms);
 
 /* Device-memory plumbing for callers without an allocator of their own
  * (no counterpart in the reference, whose data never leaves the host).

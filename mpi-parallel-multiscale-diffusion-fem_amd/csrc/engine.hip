@@ -44,6 +44,8 @@ struct Plan {
 
 } // namespace
 
+constexpr int kSideStreams = 4;
+
 struct msfem_ctx {
   int device = 0;
   std::string err;
@@ -54,6 +56,14 @@ struct msfem_ctx {
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // solve begin/end, element begin/end
   bool ev_recorded = false;
+  hipEvent_t marks[MSFEM_N_MARKS] = {}; // user timing marks on the context stream
+  // graph builds: side streams (one per size class), fork/join events, and the
+  // instantiated graphs; `generation` changes whenever a buffer a graph may
+  // hold is reallocated or the plan changes, which retires every cached graph
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t fork = nullptr, join[kSideStreams] = {};
+  std::map<std::string, hipGraphExec_t> graphs;
+  unsigned long long generation = 0, graph_generation = 0;
 };
 
 namespace {
@@ -88,6 +98,7 @@ int scratch(msfem_ctx *ctx, const char *name, size_t bytes, void **out)
   DevBuf &b = ctx->bufs[name];
   if (b.bytes < bytes)
     {
+      ++ctx->generation;
       if (b.p)
         HIP_TRY(ctx, hipFree(b.p));
       b.p = nullptr;
@@ -187,6 +198,7 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
                     std::equal(p.n_fine.begin(), p.n_fine.end(), in->n_fine);
   if (same)
     return MSFEM_OK;
+  ++ctx->generation;
   p = Plan();
   p.dim = in->dim;
   p.coef_kind = in->coef_kind;
@@ -263,6 +275,191 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
   return hipGetLastError();
 }
 
+// Local solves (coefficient preparation, solve, partition-of-unity fix-up) of size class k.
+int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+                       size_t k, double *bases, double *cond, hipStream_t st)
+{
+  Plan &p = ctx->plan;
+  const int nb = p.nb;
+  const bool pou = o->partition_of_unity != 0;
+  const int nf = p.class_nf[k];
+  const int ncls = (int)p.class_cells[k].size();
+  const int *cells = d_cells(ctx) + p.cells_dev_off[k];
+  const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
+  int mode = class_mode(in->dim, in->coef_kind, nf);
+  // the FP32 inner CG of the 128^2 tensor class streams 4 FP32 conductances
+  // per fine cell (planar, in the same scratch: 4 nf^2 floats <= 6 nf^2 doubles)
+  if (mode == kModeGlobal && fp32 && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
+    mode = kModeStream;
+  if (mode == kModeGlobal)
+    HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
+                                     d_off(ctx, 2), st));
+  if (mode == kModeStream)
+    HIP_TRY(ctx, launch_prep_cond32_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), (float *)cond,
+                                       d_off(ctx, 2), st));
+  LocalArgs a;
+  a.nb_solve = pou ? nb - 1 : nb;
+  a.n_problems = ncls * a.nb_solve;
+  a.nb = nb;
+  a.cells = cells;
+  a.box = in->box;
+  a.coef = in->coef;
+  a.coef_off = d_off(ctx, 0);
+  a.cond = cond;
+  a.cond32 = (const float *)cond;
+  a.cond_off = d_off(ctx, 2);
+  a.bases = bases;
+  a.bas_off = d_off(ctx, 1);
+  a.x0 = in->x0;
+  a.iters = out->iters;
+  a.status = out->status;
+  a.resid = out->resid;
+  a.rel_tol = o->rel_tol;
+  a.abs_tol = o->abs_tol;
+  a.inner_rel_tol = o->inner_rel_tol;
+  a.max_iter = o->max_iter;
+  a.max_refine = o->max_refine;
+  int variant = o->variant;
+  if (variant == 0)
+    {
+      auto it = ctx->variants.find(variant_key(in->dim, nf, o->precision, in->coef_kind));
+      if (it != ctx->variants.end())
+        variant = it->second;
+    }
+  if (in->dim == 2)
+    HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
+  else
+    HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, variant, a, st));
+  if (pou)
+    HIP_TRY(ctx, launch_pou_fixup(ncls, cells, in->dim, nf, bases, d_off(ctx, 1), st));
+  return MSFEM_OK;
+}
+
+// A^K, F^K of size class k from its bases.
+int launch_class_element(msfem_ctx *ctx, const msfem_batch *in, msfem_batch_out *out, size_t k, double *bases,
+                         double *cond, hipStream_t st)
+{
+  Plan &p = ctx->plan;
+  ElementArgs e;
+  e.n_cells_class = (int)p.class_cells[k].size();
+  e.cells = d_cells(ctx) + p.cells_dev_off[k];
+  e.dim = in->dim;
+  e.nf = p.class_nf[k];
+  e.nb = p.nb;
+  e.coef_kind = in->coef_kind;
+  e.box = in->box;
+  e.coef = in->coef;
+  e.coef_off = d_off(ctx, 0);
+  e.cond = cond;
+  e.cond_off = d_off(ctx, 2);
+  e.bases = bases;
+  e.bas_off = d_off(ctx, 1);
+  e.rhs_kind = in->rhs_kind;
+  e.rhs_value = in->rhs_value;
+  e.Ae = out->Ae;
+  e.Fe = out->Fe;
+  HIP_TRY(ctx, launch_element(e, st));
+  return MSFEM_OK;
+}
+
+void drop_graphs(msfem_ctx *ctx)
+{
+  for (auto &kv : ctx->graphs)
+    hipGraphExecDestroy(kv.second);
+  ctx->graphs.clear();
+}
+
+// The whole build as one HIP graph: a branch per size class (solve, then
+// element matrices), all branches concurrent, joined on `st`.  Instantiated
+// once per (plan, scratch buffers, arguments, options) and replayed after.
+int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+              double *bases, double *cond, hipStream_t st)
+{
+  Plan &p = ctx->plan;
+  if (ctx->graph_generation != ctx->generation)
+    {
+      drop_graphs(ctx);
+      ctx->graph_generation = ctx->generation;
+    }
+  std::string key;
+  auto add = [&key](const void *v, size_t n) { key.append((const char *)v, n); };
+  add(&ctx->generation, sizeof ctx->generation);
+  add(in, sizeof *in);
+  add(o, sizeof *o);
+  add(out, sizeof *out);
+  add(&bases, sizeof bases);
+  add(&st, sizeof st);
+  auto it = ctx->graphs.find(key);
+  if (it == ctx->graphs.end())
+    {
+      const size_t nk = p.class_nf.size();
+      for (size_t k = 0; k < nk && k < kSideStreams; ++k)
+        if (!ctx->side[k])
+          HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->side[k], hipStreamNonBlocking));
+      if (!ctx->fork)
+        {
+          HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+          for (auto &e : ctx->join)
+            HIP_TRY(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        }
+      HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc = MSFEM_OK;
+      hipError_t e = hipEventRecord(ctx->fork, st);
+      const bool pou = o->partition_of_unity != 0;
+      const size_t nstat = (size_t)in->n_cells * p.nb;
+      if (e == hipSuccess && pou)
+        {
+          e = fill_zero(out->iters, nstat * sizeof(int32_t), st);
+          if (e == hipSuccess)
+            e = fill_zero(out->status, nstat * sizeof(int32_t), st);
+          if (e == hipSuccess)
+            e = fill_zero(out->resid, nstat * sizeof(double), st);
+          if (e == hipSuccess)
+            e = hipEventRecord(ctx->fork, st);
+        }
+      for (size_t k = 0; k < nk && e == hipSuccess && rc == MSFEM_OK; ++k)
+        {
+          hipStream_t s = ctx->side[k % kSideStreams];
+          e = hipStreamWaitEvent(s, ctx->fork, 0);
+          if (e != hipSuccess)
+            break;
+          rc = launch_class_solve(ctx, in, o, out, k, bases, cond, s);
+          if (rc == MSFEM_OK)
+            rc = launch_class_element(ctx, in, out, k, bases, cond, s);
+          if (rc == MSFEM_OK)
+            e = hipEventRecord(ctx->join[k % kSideStreams], s);
+          if (e == hipSuccess && rc == MSFEM_OK)
+            e = hipStreamWaitEvent(st, ctx->join[k % kSideStreams], 0);
+        }
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(st, &g);
+      if (rc != MSFEM_OK)
+        {
+          if (g)
+            hipGraphDestroy(g);
+          return rc;
+        }
+      if (e != hipSuccess || ec != hipSuccess)
+        {
+          if (g)
+            hipGraphDestroy(g);
+          return fail(ctx, MSFEM_ERR_HIP, "graph capture failed: %s",
+                      hipGetErrorString(e != hipSuccess ? e : ec));
+        }
+      hipGraphExec_t x = nullptr;
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess)
+        return fail(ctx, MSFEM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+      if (ctx->graphs.size() >= 16)
+        drop_graphs(ctx);
+      it = ctx->graphs.emplace(key, x).first;
+    }
+  ctx->ev_recorded = false; // graph builds are timed from outside (msfem_ctx_mark)
+  HIP_TRY(ctx, hipGraphLaunch(it->second, st));
+  return MSFEM_OK;
+}
+
 int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
                  hipStream_t st)
 {
@@ -292,9 +489,10 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       if (rc)
         return rc;
     }
-  const int nb = p.nb;
+  if (o->graph)
+    return run_graph(ctx, in, o, out, bases, cond, st);
   const bool pou = o->partition_of_unity != 0;
-  const size_t nstat = (size_t)in->n_cells * nb;
+  const size_t nstat = (size_t)in->n_cells * p.nb;
   if (pou)
     {
       HIP_TRY(ctx, fill_zero(out->iters, nstat * sizeof(int32_t), st));
@@ -306,56 +504,9 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
   // all classes' local solves first, then all element matrices
   for (size_t k = 0; k < p.class_nf.size(); ++k)
     {
-      const int nf = p.class_nf[k];
-      const int ncls = (int)p.class_cells[k].size();
-      const int *cells = d_cells(ctx) + p.cells_dev_off[k];
-      const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
-      int mode = class_mode(in->dim, in->coef_kind, nf);
-      // the FP32 inner CG of the 128^2 tensor class streams 4 FP32 conductances
-      // per fine cell (planar, in the same scratch: 4 nf^2 floats <= 6 nf^2 doubles)
-      if (mode == kModeGlobal && fp32 && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
-        mode = kModeStream;
-      if (mode == kModeGlobal)
-        HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
-                                         d_off(ctx, 2), st));
-      if (mode == kModeStream)
-        HIP_TRY(ctx, launch_prep_cond32_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), (float *)cond,
-                                           d_off(ctx, 2), st));
-      LocalArgs a;
-      a.nb_solve = pou ? nb - 1 : nb;
-      a.n_problems = ncls * a.nb_solve;
-      a.nb = nb;
-      a.cells = cells;
-      a.box = in->box;
-      a.coef = in->coef;
-      a.coef_off = d_off(ctx, 0);
-      a.cond = cond;
-      a.cond32 = (const float *)cond;
-      a.cond_off = d_off(ctx, 2);
-      a.bases = bases;
-      a.bas_off = d_off(ctx, 1);
-      a.x0 = in->x0;
-      a.iters = out->iters;
-      a.status = out->status;
-      a.resid = out->resid;
-      a.rel_tol = o->rel_tol;
-      a.abs_tol = o->abs_tol;
-      a.inner_rel_tol = o->inner_rel_tol;
-      a.max_iter = o->max_iter;
-      a.max_refine = o->max_refine;
-      int variant = o->variant;
-      if (variant == 0)
-        {
-          auto it = ctx->variants.find(variant_key(in->dim, nf, o->precision, in->coef_kind));
-          if (it != ctx->variants.end())
-            variant = it->second;
-        }
-      if (in->dim == 2)
-        HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
-      else
-        HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, variant, a, st));
-      if (pou)
-        HIP_TRY(ctx, launch_pou_fixup(ncls, cells, in->dim, nf, bases, d_off(ctx, 1), st));
+      rc = launch_class_solve(ctx, in, o, out, k, bases, cond, st);
+      if (rc)
+        return rc;
     }
   if (ctx->profiling)
     {
@@ -364,28 +515,9 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
     }
   for (size_t k = 0; k < p.class_nf.size(); ++k)
     {
-      const int nf = p.class_nf[k];
-      const int ncls = (int)p.class_cells[k].size();
-      const int *cells = d_cells(ctx) + p.cells_dev_off[k];
-      ElementArgs e;
-      e.n_cells_class = ncls;
-      e.cells = cells;
-      e.dim = in->dim;
-      e.nf = nf;
-      e.nb = nb;
-      e.coef_kind = in->coef_kind;
-      e.box = in->box;
-      e.coef = in->coef;
-      e.coef_off = d_off(ctx, 0);
-      e.cond = cond;
-      e.cond_off = d_off(ctx, 2);
-      e.bases = bases;
-      e.bas_off = d_off(ctx, 1);
-      e.rhs_kind = in->rhs_kind;
-      e.rhs_value = in->rhs_value;
-      e.Ae = out->Ae;
-      e.Fe = out->Fe;
-      HIP_TRY(ctx, launch_element(e, st));
+      rc = launch_class_element(ctx, in, out, k, bases, cond, st);
+      if (rc)
+        return rc;
     }
   if (ctx->profiling)
     {
@@ -549,6 +681,7 @@ int msfem_set_variant(msfem_ctx *ctx, int dim, int nf, int precision, int coef_k
       (precision != MSFEM_PREC_FP64 && precision != MSFEM_PREC_FP32_REFINE) || coef_kind < 0 || coef_kind > 3)
     return fail(ctx, MSFEM_ERR_ARG, "invalid variant table entry");
   const long long key = variant_key(dim, nf, precision, coef_kind);
+  ++ctx->generation;
   if (variant == 0)
     ctx->variants.erase(key);
   else
@@ -566,6 +699,30 @@ int msfem_ctx_set_profiling(msfem_ctx *ctx, int on)
       HIP_TRY(ctx, hipEventCreate(&e));
   ctx->profiling = on != 0;
   ctx->ev_recorded = false;
+  return MSFEM_OK;
+}
+
+int msfem_ctx_mark(msfem_ctx *ctx, int slot)
+{
+  if (!ctx || slot < 0 || slot >= MSFEM_N_MARKS)
+    return fail(ctx, MSFEM_ERR_ARG, "mark slot %d outside [0, %d)", slot, MSFEM_N_MARKS);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!ctx->marks[slot])
+    HIP_TRY(ctx, hipEventCreate(&ctx->marks[slot]));
+  HIP_TRY(ctx, hipEventRecord(ctx->marks[slot], ctx->stream));
+  return MSFEM_OK;
+}
+
+int msfem_ctx_elapsed(msfem_ctx *ctx, int from, int to, double *ms)
+{
+  if (!ctx || !ms || from < 0 || to < 0 || from >= MSFEM_N_MARKS || to >= MSFEM_N_MARKS)
+    return fail(ctx, MSFEM_ERR_ARG, "bad mark slots or null output");
+  if (!ctx->marks[from] || !ctx->marks[to])
+    return fail(ctx, MSFEM_ERR_ARG, "mark slot never recorded");
+  HIP_TRY(ctx, hipEventSynchronize(ctx->marks[to]));
+  float t = 0;
+  HIP_TRY(ctx, hipEventElapsedTime(&t, ctx->marks[from], ctx->marks[to]));
+  *ms = t;
   return MSFEM_OK;
 }
 
@@ -592,6 +749,21 @@ int msfem_ctx_destroy(msfem_ctx *ctx)
   if (ctx->stream)
     hipStreamSynchronize(ctx->stream);
   for (auto &e : ctx->ev)
+    if (e)
+      hipEventDestroy(e);
+  for (auto &e : ctx->marks)
+    if (e)
+      hipEventDestroy(e);
+  drop_graphs(ctx);
+  for (auto &sd : ctx->side)
+    if (sd)
+      {
+        hipStreamSynchronize(sd);
+        hipStreamDestroy(sd);
+      }
+  if (ctx->fork)
+    hipEventDestroy(ctx->fork);
+  for (auto &e : ctx->join)
     if (e)
       hipEventDestroy(e);
   for (auto &kv : ctx->bufs)

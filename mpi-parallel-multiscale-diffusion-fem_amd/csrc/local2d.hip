@@ -320,7 +320,12 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         if (bnd)
           v = nodal_q1(basis, 2, X, Y, 0, NF);
         else
-          v = a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP + k] : 0.0;
+          {
+            // a non-finite initial value (e.g. a failed prediction) starts from 0
+            v = a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP + k] : 0.0;
+            if (!isfinite(v))
+              v = 0.0;
+          }
         phi[k] = v;
       }
   __syncthreads();

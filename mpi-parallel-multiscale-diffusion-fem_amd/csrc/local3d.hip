@@ -138,7 +138,9 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
     {
       const int XX = k % NP, YY = (k / NP) % NP, ZZ = k / (NP * NP);
       const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF || ZZ == 0 || ZZ == NF;
-      phi[k] = bnd ? nodal_q1(basis, 3, XX, YY, ZZ, NF) : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
+      double v = bnd ? nodal_q1(basis, 3, XX, YY, ZZ, NF)
+                     : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
+      phi[k] = isfinite(v) ? v : 0.0; // a non-finite initial value starts from 0
     }
   __syncthreads();
 

@@ -25,7 +25,7 @@ BC_ZERO_DIRICHLET, BC_REFERENCE = 0, 1
 # Every symbol include/msfem.h declares.
 EXPORTS = [
     "msfem_ctx_create", "msfem_ctx_destroy", "msfem_last_error", "msfem_default_opts", "msfem_version",
-    "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
+    "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
     "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
@@ -46,7 +46,8 @@ class MsfemError(RuntimeError):
 class SolverOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("max_refine", ctypes.c_int32),
                 ("partition_of_unity", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double),
-                ("inner_rel_tol", ctypes.c_double), ("variant", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("inner_rel_tol", ctypes.c_double), ("variant", ctypes.c_int32), ("graph", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 6)]
 
 
 class Batch(ctypes.Structure):
@@ -76,6 +77,8 @@ def lib():
         L.msfem_ctx_destroy.argtypes = [vp]
         L.msfem_ctx_set_profiling.argtypes = [vp, i32]
         L.msfem_ctx_profile.argtypes = [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+        L.msfem_ctx_mark.argtypes = [vp, ctypes.c_int]
+        L.msfem_ctx_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(dbl)]
         L.msfem_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
         L.msfem_dev_free.argtypes = [vp, vp]
         L.msfem_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
@@ -184,6 +187,16 @@ class Context:
         a, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().msfem_ctx_profile(self.h, ctypes.byref(a), ctypes.byref(b)), self.h)
         return a.value, b.value
+
+    def mark(self, slot):
+        """Record timing mark `slot` (a HIP event) on the context stream."""
+        _check(lib().msfem_ctx_mark(self.h, slot), self.h)
+
+    def elapsed(self, start, end):
+        """Device milliseconds between two recorded marks (waits for `end`)."""
+        ms = ctypes.c_double()
+        _check(lib().msfem_ctx_elapsed(self.h, start, end, ctypes.byref(ms)), self.h)
+        return ms.value
 
     # ---- device memory --------------------------------------------------------------
     def alloc(self, shape, dtype):

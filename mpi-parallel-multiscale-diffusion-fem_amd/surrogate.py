@@ -48,6 +48,12 @@ def _nn():
     return L
 
 
+def bf16_bits(a):
+    """FP32 values -> BF16 bit patterns (round to nearest even), as uint16."""
+    u = np.ascontiguousarray(a, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
 def _ok(rc):
     if rc != 0:
         raise msfem.MsfemError(rc, "nn primitive failed")
@@ -273,14 +279,26 @@ class Surrogate:
         for l in self.layers:
             l.set(st[l.name + ".w"], st[l.name + ".b"])
 
-    def save(self, path):
-        np.savez(path, **self.state(), chans=np.array(self.chans), tile=self.tile, dscale=self.dscale)
+    def save(self, path, compact=False):
+        """compact: weights stored as their BF16 bits (what the matrix cores use;
+        serving from a compact file is bit-identical), biases FP32."""
+        st = self.state()
+        if compact:
+            st = {(k + "bf" if k.endswith(".w") else k): (bf16_bits(v) if k.endswith(".w") else v) for k, v in st.items()}
+        save = np.savez_compressed if compact else np.savez
+        save(path, **st, chans=np.array(self.chans), tile=self.tile, dscale=self.dscale)
 
     @staticmethod
     def load(ctx, path):
         d = np.load(path)
         s = Surrogate(ctx, int(d["tile"]), tuple(int(v) for v in d["chans"]))
-        s.load_state({k: d[k] for k in d.files if k.endswith(".w") or k.endswith(".b")})
+        st = {}
+        for k in d.files:
+            if k.endswith(".w") or k.endswith(".b"):
+                st[k] = d[k]
+            elif k.endswith(".wbf"):
+                st[k[:-2]] = (d[k].astype(np.uint32) << 16).view(np.float32)
+        s.load_state(st)
         s.dscale = float(d["dscale"])
         return s
 
@@ -498,43 +516,57 @@ def energy_error(alpha, phi, ref):
     return float(np.max(np.sqrt(energy_norm2(alpha, phi - ref) / energy_norm2(alpha, ref))))
 
 
-def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, sample=64, log=None, data=None):
-    """Times the basis build of n_side^2 fresh GRF cells four ways -- plain FP64,
-    FP32+FP64 refinement, and each of them warm-started from the network's
-    prediction -- and reports cells/s, iterations, the fallback rate (share of
-    local problems whose prediction failed the residual gate and needed CG
-    iterations) and the energy-norm error of a cell sample against plain FP64."""
+def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, sample=64, log=None, data=None):
+    """Times one step -- the basis build of n_side^2 fresh GRF cells plus the
+    coarse assembly -- four ways: plain FP64, FP32+FP64 refinement, and each of
+    them warm-started from the network's prediction (inference included in the
+    step).  Reports cells/s (wall clock over `reps` steps), device times of the
+    inference and of the local solves (HIP events on the context stream),
+    iterations, the fallback rate (share of local problems whose prediction
+    failed the residual gate and needed CG iterations) and the energy-norm
+    error of a cell sample against plain FP64."""
     import time
     n = n_side * n_side
     data = data or TileSet(ctx, n, seed)
     nf = data.nf
     per = 4 * (nf + 1) ** 2
     x0 = ctx.alloc(n * per, np.float64)
+    ncn = (n_side + 1) ** 2
+    Kc, Fc = ctx.alloc(ncn * 9, np.float64), ctx.alloc(ncn, np.float64)
+    cm, cv = ctx.alloc(ncn, np.uint8), ctx.alloc(ncn, np.float64)
     pick = np.unique(np.linspace(0, n - 1, min(sample, n)).astype(np.int64))
     legs = [("fp64", msfem.default_opts(precision=msfem.PREC_FP64), False),
             ("fp32_refine", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), False),
             ("fp32_refine_surrogate", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), True),
             ("fp64_surrogate", msfem.default_opts(precision=msfem.PREC_FP64), True)]
-    # inference alone
-    net.predict_bases(data.x, n, x0.ptr)
-    ctx.synchronize()
-    t = time.perf_counter()
+    ctx.set_profiling(True)
+    # inference alone (device time between two marks, per pass over all cells)
+    for _ in range(warmup):
+        net.predict_bases(data.x, n, x0.ptr)
+    ctx.mark(0)
     for _ in range(reps):
         net.predict_bases(data.x, n, x0.ptr)
-    ctx.synchronize()
-    infer_ms = (time.perf_counter() - t) * 1e3 / reps
+    ctx.mark(1)
+    infer_ms = ctx.elapsed(0, 1) / reps
     pred = {int(c): x0.read(int(c) * per, per).reshape(4, nf + 1, nf + 1) for c in pick}
+    alpha = {int(c): data.coef_host[int(c) * nf * nf:(int(c) + 1) * nf * nf].reshape(nf, nf) for c in pick}
     out, ref = {}, None
     for name, opts, use in legs:
+        solve_ms = []
+
         def step():
             if use:
                 net.predict_bases(data.x, n, x0.ptr)
             data.solve(opts, x0.ptr if use else None)
-        step()
+            ctx.coarse_assemble_device(2, n_side, data.Ae.ptr, data.Fe.ptr, msfem.BC_ZERO_DIRICHLET, Kc.ptr, Fc.ptr,
+                                       cm.ptr, cv.ptr, stream=ctx.stream)
+        for _ in range(warmup):
+            step()
         ctx.synchronize()
         t = time.perf_counter()
         for _ in range(reps):
             step()
+            solve_ms.append(ctx.profile()[0])
         ctx.synchronize()
         ms = (time.perf_counter() - t) * 1e3 / reps
         st = data.status.numpy()
@@ -542,10 +574,10 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, sample=64,
             raise RuntimeError(f"{name}: {int((st != 0).sum())} local solves did not converge")
         its = data.iters.numpy().astype(np.int64)
         bases = {int(c): data.bases.read(int(c) * per, per).reshape(4, nf + 1, nf + 1) for c in pick}
-        alpha = {int(c): data.coef_host[int(c) * nf * nf:(int(c) + 1) * nf * nf].reshape(nf, nf) for c in pick}
         if ref is None:
             ref = bases
-        r = {"ms": round(ms, 3), "cells_per_s": round(n / (ms * 1e-3), 1), "iters_mean": round(float(its.mean()), 2),
+        r = {"ms": round(ms, 3), "cells_per_s": round(n / (ms * 1e-3), 1),
+             "solve_kernel_ms": round(float(np.mean(solve_ms)), 3), "iters_mean": round(float(its.mean()), 2),
              "iters_p99": int(np.percentile(its, 99)),
              "energy_err_max": float(max(energy_error(alpha[c], bases[c], ref[c]) for c in pick))}
         if use:
@@ -553,15 +585,14 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, sample=64,
         out[name] = r
         if log:
             log(f"{name}: {r}")
-    alpha = {int(c): data.coef_host[int(c) * nf * nf:(int(c) + 1) * nf * nf].reshape(nf, nf) for c in pick}
     errs = [energy_error(alpha[c], pred[c], ref[c]) for c in pick]
     out["prediction"] = {"infer_ms": round(infer_ms, 3), "tiles_per_s": round(n / (infer_ms * 1e-3), 1),
                          "energy_err_median": float(np.median(errs)), "energy_err_max": float(np.max(errs)),
                          "tflops": round(net.flops_per_sample() * n / (infer_ms * 1e-3) / 1e12, 2)}
     out["cells"], out["sample"], out["rel_tol"] = n, len(pick), rel_tol
-    x0.free()
+    for arr in (x0, Kc, Fc, cm, cv):
+        arr.free()
     return out
-
 
 if __name__ == "__main__":
     main()
