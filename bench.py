@@ -79,6 +79,9 @@ def main():
                     choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference", "surrogate128"])
     ap.add_argument("--cpu-sample", type=int, default=2048, help="cells in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (3 times): the reference kernel that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the local solves of each coarse cell are independent and "
@@ -87,6 +90,8 @@ def main():
 
     import msfem
 
+    if args.copy_calibration:
+        return copy_calibration(args.copy_calibration)
     if args.workload == "surrogate128":
         return bench_surrogate(args)
     case, nc, main_prec, bc_kind, desc = workload(args.workload)
@@ -214,6 +219,22 @@ def main():
                                               case["rhs_kind"], case["rhs_value"], n, args.cpu_sample)
     ctx.close()
     print(json.dumps(result))
+
+
+def copy_calibration(mib):
+    """out = x over mib MiB of FP32 (msfem_nn_axpby with b = 0): reads and writes
+    exactly mib MiB each per launch."""
+    import msfem
+    import surrogate as S
+    ctx = msfem.Context(0)
+    n = mib * (1 << 20) // 4
+    x, y = ctx.alloc(n, np.float32), ctx.alloc(n, np.float32)
+    for _ in range(3):
+        S._ok(S._nn().msfem_nn_axpby(n, 1.0, x.ptr, 0.0, None, y.ptr, ctx.stream))
+    ctx.synchronize()
+    print(json.dumps({"copy_calibration": {"kernel": "axpby_kernel", "bytes_read": n * 4, "bytes_written": n * 4,
+                                           "launches": 3}}))
+    ctx.close()
 
 
 def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, sample_cells):

@@ -276,8 +276,19 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
 }
 
 // Local solves (coefficient preparation, solve, partition-of-unity fix-up) of size class k.
+// Storage mode of size class nf in this build.
+int build_mode(const msfem_batch *in, int nf)
+{
+  int mode = class_mode(in->dim, in->coef_kind, nf);
+  // the 128^2 tensor class streams 4 conductances per fine cell from planar
+  // tiles (FP32 or FP64, in the same scratch: 4 (nf^2 + 2 nf) values <= 6 nf^2 doubles)
+  if (mode == kModeGlobal && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
+    mode = kModeStream;
+  return mode;
+}
+
 int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
-                       size_t k, double *bases, double *cond, hipStream_t st)
+                       size_t k, double *bases, double *cond, double *work, hipStream_t st)
 {
   Plan &p = ctx->plan;
   const int nb = p.nb;
@@ -286,17 +297,15 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   const int ncls = (int)p.class_cells[k].size();
   const int *cells = d_cells(ctx) + p.cells_dev_off[k];
   const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
-  int mode = class_mode(in->dim, in->coef_kind, nf);
-  // the FP32 inner CG of the 128^2 tensor class streams 4 FP32 conductances
-  // per fine cell (planar, in the same scratch: 4 nf^2 floats <= 6 nf^2 doubles)
-  if (mode == kModeGlobal && fp32 && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
-    mode = kModeStream;
+  const int mode = build_mode(in, nf);
   if (mode == kModeGlobal)
     HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
                                      d_off(ctx, 2), st));
-  if (mode == kModeStream)
+  if (mode == kModeStream && fp32)
     HIP_TRY(ctx, launch_prep_cond32_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), (float *)cond,
                                        d_off(ctx, 2), st));
+  if (mode == kModeStream && !fp32)
+    HIP_TRY(ctx, launch_prep_cond64p_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), cond, d_off(ctx, 2), st));
   LocalArgs a;
   a.nb_solve = pou ? nb - 1 : nb;
   a.n_problems = ncls * a.nb_solve;
@@ -307,6 +316,7 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.coef_off = d_off(ctx, 0);
   a.cond = cond;
   a.cond32 = (const float *)cond;
+  a.work = work;
   a.cond_off = d_off(ctx, 2);
   a.bases = bases;
   a.bas_off = d_off(ctx, 1);
@@ -373,7 +383,7 @@ void drop_graphs(msfem_ctx *ctx)
 // element matrices), all branches concurrent, joined on `st`.  Instantiated
 // once per (plan, scratch buffers, arguments, options) and replayed after.
 int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
-              double *bases, double *cond, hipStream_t st)
+              double *bases, double *cond, double *work, hipStream_t st)
 {
   Plan &p = ctx->plan;
   if (ctx->graph_generation != ctx->generation)
@@ -423,7 +433,7 @@ int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o,
           e = hipStreamWaitEvent(s, ctx->fork, 0);
           if (e != hipSuccess)
             break;
-          rc = launch_class_solve(ctx, in, o, out, k, bases, cond, s);
+          rc = launch_class_solve(ctx, in, o, out, k, bases, cond, work, s);
           if (rc == MSFEM_OK)
             rc = launch_class_element(ctx, in, out, k, bases, cond, s);
           if (rc == MSFEM_OK)
@@ -489,8 +499,19 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       if (rc)
         return rc;
     }
+  // global q slots of the FP64 streamed class (allocated before any capture)
+  double *work = nullptr;
+  if (o->precision == MSFEM_PREC_FP64)
+    for (size_t k = 0; k < p.class_nf.size(); ++k)
+      if (build_mode(in, p.class_nf[k]) == kModeStream)
+        {
+          const size_t nprob = p.class_cells[k].size() * (size_t)(o->partition_of_unity ? p.nb - 1 : p.nb);
+          rc = scratch(ctx, "stream_q", nprob * kStreamQSlots * sizeof(double), (void **)&work);
+          if (rc)
+            return rc;
+        }
   if (o->graph)
-    return run_graph(ctx, in, o, out, bases, cond, st);
+    return run_graph(ctx, in, o, out, bases, cond, work, st);
   const bool pou = o->partition_of_unity != 0;
   const size_t nstat = (size_t)in->n_cells * p.nb;
   if (pou)
@@ -504,7 +525,7 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
   // all classes' local solves first, then all element matrices
   for (size_t k = 0; k < p.class_nf.size(); ++k)
     {
-      rc = launch_class_solve(ctx, in, o, out, k, bases, cond, st);
+      rc = launch_class_solve(ctx, in, o, out, k, bases, cond, work, st);
       if (rc)
         return rc;
     }

@@ -186,7 +186,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       const double *bx = a.box + (size_t)cell * 4;
       rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
       cf = a.coef + a.coef_off[cell];
-      if (MODE == kModeGlobal)
+      if (MODE == kModeGlobal || (MODE == kModeStream && sizeof(T) == 8))
         cg = a.cond + a.cond_off[cell];
       if (MODE == kModeStream)
         c32 = a.cond32 + a.cond_off[cell];
@@ -237,11 +237,21 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         // values read for cells outside the tile only reach nodes that are
         // not interior (masked by the zero diagonal entry)
         constexpr int PS = stream_plane_stride(NF);
-        typedef const float __attribute__((address_space(1))) * gptr;
         const unsigned f = (unsigned)((Y0 - 1 + i) * NF + (X0 - 1 + j));
-        const gptr q = (gptr)c32;
-        const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
-        return {{ch, cv, cd, ca, cv, ch}};
+        if constexpr (sizeof(T) == 4)
+          {
+            typedef const float __attribute__((address_space(1))) * gptr;
+            const gptr q = (gptr)c32;
+            const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
+            return {{ch, cv, cd, ca, cv, ch}};
+          }
+        else
+          {
+            typedef const double __attribute__((address_space(1))) * gptr;
+            const gptr q = (gptr)cg;
+            const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
+            return {{ch, cv, cd, ca, cv, ch}};
+          }
       }
     else
       {
@@ -452,14 +462,129 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
             irun = false;
           T rz = v2[0];
+          static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
+          if constexpr (sizeof(T) == 8)
+          while (any)
+            {
+              // FP64: no LDS room beside the FP64 grid, so q = K p goes to this
+              // problem's slots of a global work array (thread-private,
+              // coalesced) and x is accumulated straight into the FP64 solution
+              // phi; the stencil runs in cell form -- cell row cy adds its element
+              // products to node rows cy (then complete) and cy + 1 -- so that one
+              // row of cells is live at a time.
+              __syncthreads(); // p complete in the grid
+              asm volatile("" : "+s"(cg));
+              // uniform bases (SGPRs) + 32-bit lane offsets: no 64-bit address per access
+              typedef double __attribute__((address_space(1))) * gdptr;
+              static_assert(R * CPL * NT == kStreamQSlots, "work slots per problem");
+              double *qb = a.work + (size_t)prob * kStreamQSlots;
+              double *pb = phi;
+              asm volatile("" : "+s"(qb), "+s"(pb));
+              const gdptr qg = (gdptr)qb, ph = (gdptr)pb;
+              // the packed diagonal too: its FP64 conversions would otherwise be
+              // hoisted out of the loop (16 more FP64 values live)
+#pragma unroll
+              for (int k = 0; k < (R * CPL + 1) / 2; ++k)
+                asm volatile("" : "+v"(dpk[k]));
+              T lo[CPL + 2], hi[CPL + 2], yp[CPL];
+              T pq[1] = {0};
+#pragma unroll
+              for (int k = 0; k < CPL + 2; ++k)
+                lo[k] = gwin[k];
+#pragma unroll
+              for (int c = 0; c < CPL; ++c)
+                yp[c] = 0;
+#pragma unroll
+              for (int j = 0; j <= R; ++j)
+                {
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    hi[k] = gwin[(j + 1) * GP + k];
+                  T yb[CPL], yt[CPL];
+#pragma unroll
+                  for (int c = 0; c < CPL; ++c)
+                    {
+                      yb[c] = 0;
+                      yt[c] = 0;
+                    }
+#pragma unroll
+                  for (int m = 0; m <= CPL; ++m)
+                    {
+                      // cell (X0 - 1 + m, Y0 - 1 + j): vertices 0, 1 below, 2, 3 above
+                      const Cond6<T> e = cond_T(j, m);
+                      const T p0 = lo[m], p1 = lo[m + 1], p2 = hi[m], p3 = hi[m + 1];
+                      if (m >= 1) // vertices 0 and 2 are own column m - 1
+                        {
+                          yb[m - 1] += e.c[0] * (p0 - p1) + e.c[1] * (p0 - p2) + e.c[2] * (p0 - p3);
+                          yt[m - 1] += e.c[1] * (p2 - p0) + e.c[3] * (p2 - p1) + e.c[5] * (p2 - p3);
+                        }
+                      if (m < CPL) // vertices 1 and 3 are own column m
+                        {
+                          yb[m] += e.c[0] * (p1 - p0) + e.c[3] * (p1 - p2) + e.c[4] * (p1 - p3);
+                          yt[m] += e.c[2] * (p3 - p0) + e.c[4] * (p3 - p1) + e.c[5] * (p3 - p2);
+                        }
+                      __builtin_amdgcn_sched_barrier(0); // one cell's conductances live at a time
+                    }
+                  if (j >= 1) // own node row j - 1 is complete
+                    {
+#pragma unroll
+                      for (int c = 0; c < CPL; ++c)
+                        {
+                          const T q = dget(j - 1, c) != 0.0f ? yp[c] + yb[c] : (T)0;
+                          qg[(unsigned)(((j - 1) * CPL + c) * NT + tid)] = q;
+                          pq[0] += lo[c + 1] * q;
+                        }
+                    }
+#pragma unroll
+                  for (int c = 0; c < CPL; ++c)
+                    yp[c] = yt[c];
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    lo[k] = hi[k];
+                  __builtin_amdgcn_sched_barrier(0);
+                }
+              problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave, slot, pl);
+              v2[0] = 0;
+              v2[1] = 0;
+              if (irun)
+                {
+                  const T alpha = quotient<T>(rz, pq[0]);
+#pragma unroll
+                  for (int i = 0; i < R; ++i)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c)
+                      {
+                        if (dget(i, c) != 0.0f)
+                          ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
+                        rr[i][c] -= alpha * qg[(unsigned)((i * CPL + c) * NT + tid)];
+                        const T z = rr[i][c] * (T)dget(i, c);
+                        v2[0] += rr[i][c] * z;
+                        v2[1] += rr[i][c] * rr[i][c];
+                      }
+                  ++it_total;
+                }
+              any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
+              if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
+                irun = false;
+              const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
+              rz = v2[0];
+#pragma unroll
+              for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                  gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) + beta * gown[i * GP + c] : (T)0;
+            }
+          else
           while (any)
             {
               __syncthreads(); // p complete in the grid
               // opaque per-iteration copy of the (wave-uniform) tile address: keeps
               // the loads in the loop (hoisted, the 27 cells' conductances would
-              // take 108 registers)
-              static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
+              // take 108 registers); likewise the packed diagonal's conversions
               asm volatile("" : "+s"(c32));
+#pragma unroll
+              for (int k = 0; k < (R * CPL + 1) / 2; ++k)
+                asm volatile("" : "+v"(dpk[k]));
               T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
               Cond6<T> cb[CPL + 1], ct[CPL + 1];
 #pragma unroll
@@ -627,8 +752,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if (!any)
             break;
         }
-      // ---- phi += d in FP64 ------------------------------------------------------
-      if (run)
+      // ---- phi += d in FP64 (the FP64 streamed form has already added it) ---------
+      if (run && !(MODE == kModeStream && sizeof(T) == 8))
         {
 #pragma unroll
           for (int i = 0; i < R; ++i)
@@ -804,9 +929,11 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
 {
   if (mode == kModeStream)
     {
-      if (!fp32 || nf != 128)
+      if (nf != 128)
         return hipErrorInvalidValue;
-      return launch_one<float, 128, 2, 16, 8, kModeStream, 4>(a, st);
+      if (fp32)
+        return launch_one<float, 128, 2, 16, 8, kModeStream, 4>(a, st);
+      return launch_one<double, 128, 2, 16, 8, kModeStream, 4>(a, st);
     }
   if (fp32)
     {
@@ -821,6 +948,44 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
   if (mode == kModeTensor)
     return launch_mode<double, kModeTensor>(nf, variant, a, st);
   return launch_mode<double, kModeGlobal>(nf, variant, a, st);
+}
+
+// FP64 planes of the same layout for the FP64 streamed form.
+__global__ void prep_cond64p_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
+                                const long long *coef_off, double *cond, const long long *cond_off)
+{
+  const int ncell = nf * nf;
+  const long long total = (long long)n_cells_class * ncell;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
+    {
+      const int cell = cells[t / ncell];
+      const int f = (int)(t % ncell);
+      const double *bx = box + (size_t)cell * 4;
+      const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      const double *cf = coef + coef_off[cell];
+      const Cond4 c = cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho);
+      const int ps = stream_plane_stride(nf);
+      double *o = cond + cond_off[cell] + f;
+      o[0] = c.ch;
+      o[ps] = c.cv;
+      o[2 * ps] = c.cd;
+      o[3 * ps] = c.ca;
+    }
+}
+
+hipError_t launch_prep_cond64p_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
+                                  const long long *coef_off, double *cond, const long long *cond_off, hipStream_t st)
+{
+  const long long total = (long long)n_cells_class * nf * nf;
+  if (total == 0)
+    return hipSuccess;
+  const int block = 256;
+  long long grid = (total + block - 1) / block;
+  if (grid > 65536)
+    grid = 65536;
+  hipLaunchKernelGGL(prep_cond64p_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, box, coef,
+                     coef_off, cond, cond_off);
+  return hipGetLastError();
 }
 
 hipError_t launch_prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,

@@ -22,6 +22,10 @@ enum CondMode : int {
 // planes (4 nf^2 + 8 nf floats) fit the 6 nf^2 doubles reserved per cell.
 __host__ __device__ constexpr int stream_plane_stride(int nf) { return nf * nf + 2 * nf; }
 
+// FP64 values of q per problem in the FP64 streamed kernel's global work
+// array (the 128^2 class: 8 rows x 2 columns per thread x 1024 threads).
+constexpr int kStreamQSlots = 8 * 2 * 1024;
+
 // Workgroup -> block index such that the blocks one XCD receives (hardware
 // dispatch is round-robin over the 8 XCDs) hold consecutive problems: the
 // bases of one coarse cell then run on one XCD at the same time and share its
@@ -42,6 +46,7 @@ struct LocalArgs {
   const long long *coef_off;  // [n_cells] tile offsets (elements)
   const double *cond;         // kModeGlobal: concatenated conductance tiles
   const float *cond32;        // kModeStream: planar FP32 conductance tiles (same offsets, in floats)
+  double *work;               // kModeStream FP64: per-problem q slots (R CPL threads doubles each)
   const long long *cond_off;  // [n_cells]
   double *bases;              // concatenated basis blocks (working FP64 solution + output)
   const long long *bas_off;   // [n_cells]
