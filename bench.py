@@ -43,6 +43,10 @@ for p in (PKG, os.path.join(ROOT, "tests")):
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA issue rate equals FP64 (packed FP32 is disabled)
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
+# HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by a plain copy of known size (--copy-calibration):
+# profiles/local_solve_2d_lognormal64_hbm_traffic.md
+TRAFFIC_BYTES = {("lognormal64", 0): 1.640154e9}
 METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 
 
@@ -201,7 +205,8 @@ def main():
                    "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class"},
         "roofline": {"bound": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
-                     "frac": round(achieved / peak, 4), "traffic": None, "kernel_ms": round(solve_ms, 4),
+                     "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get((args.workload, main_prec)),
+                     "kernel_ms": round(solve_ms, 4),
                      "flops_per_launch": flops,
                      "lds_model_TBps": round(float((its.sum(axis=1) * b_iter).sum()) / (solve_ms * 1e-3) / 1e12, 2),
                      "hbm_GBps": round(float(b_cell.sum()) / (ms * 1e-3) / 1e9, 2)},
