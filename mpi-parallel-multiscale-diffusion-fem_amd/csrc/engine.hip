@@ -152,6 +152,9 @@ int class_mode(int dim, int coef_kind, int nf)
     return kModeScalar;
   if (coef_kind == MSFEM_COEF_TENSOR_CELL && nf <= 64)
     return kModeTensor;
+  // 128^2 tiles of a tensor or QP coefficient: node couplings streamed from planes
+  if (dim == 2 && nf == 128)
+    return kModeStream;
   return kModeGlobal;
 }
 
@@ -217,8 +220,11 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
       p.bas_off[c] = p.bas_total;
       p.bas_total += (long long)p.nb * ipow(nf + 1, in->dim);
       p.cond_off[c] = p.cond_total;
-      if (in->dim == 2 && class_mode(2, in->coef_kind, nf) == kModeGlobal)
+      const int mode = in->dim == 2 ? class_mode(2, in->coef_kind, nf) : kModeScalar;
+      if (mode == kModeGlobal) // 6 FP64 conductances per fine cell
         p.cond_total += 6 * ipow(nf, 2);
+      if (mode == kModeStream) // 4 FP64 + 4 FP32 coupling planes
+        p.cond_total += 6 * (long long)coupling_plane_stride(nf);
       cls[nf].push_back(c);
     }
   std::vector<int> all_cells;
@@ -279,12 +285,7 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
 // Storage mode of size class nf in this build.
 int build_mode(const msfem_batch *in, int nf)
 {
-  int mode = class_mode(in->dim, in->coef_kind, nf);
-  // the 128^2 tensor class streams 4 conductances per fine cell from planar
-  // tiles (FP32 or FP64, in the same scratch: 4 (nf^2 + 2 nf) values <= 6 nf^2 doubles)
-  if (mode == kModeGlobal && in->coef_kind == MSFEM_COEF_TENSOR_CELL && nf == 128)
-    mode = kModeStream;
-  return mode;
+  return class_mode(in->dim, in->coef_kind, nf);
 }
 
 int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
@@ -301,11 +302,9 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   if (mode == kModeGlobal)
     HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
                                      d_off(ctx, 2), st));
-  if (mode == kModeStream && fp32)
-    HIP_TRY(ctx, launch_prep_cond32_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), (float *)cond,
-                                       d_off(ctx, 2), st));
-  if (mode == kModeStream && !fp32)
-    HIP_TRY(ctx, launch_prep_cond64p_2d(ncls, cells, nf, in->box, in->coef, d_off(ctx, 0), cond, d_off(ctx, 2), st));
+  if (mode == kModeStream)
+    HIP_TRY(ctx, launch_prep_couplings_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
+                                          d_off(ctx, 2), fp32, st));
   LocalArgs a;
   a.nb_solve = pou ? nb - 1 : nb;
   a.n_problems = ncls * a.nb_solve;

@@ -70,7 +70,7 @@ __device__ __forceinline__ Cond6<double> cell_cond64(const double *cf, const dou
   const int f = cy * NF + cx;
   if constexpr (MODE == kModeScalar)
     return from4<double>(cond_scalar(cf[f], rho));
-  else if constexpr (MODE == kModeTensor || MODE == kModeStream)
+  else if constexpr (MODE == kModeTensor)
     return from4<double>(cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho));
   else
     {
@@ -126,6 +126,18 @@ __device__ __forceinline__ bool problem_sums(S (&v)[NV], bool flag, S *red, int 
     }
 }
 
+// Diagonal of the fine stiffness at node (X, Y) from the node-coupling planes:
+// the sum of its 8 couplings (aE, aN, aNE, aNW of the node and of its W, S,
+// SW, SE neighbours).
+template <int NF>
+__device__ __forceinline__ double node_diag64(const double *cg, int X, int Y)
+{
+  constexpr int NP = NF + 1, PSN = coupling_plane_stride(NF);
+  const double *q = cg + Y * NP + X;
+  return (q[0] + q[-1]) + (q[PSN] + q[PSN - NP]) + (q[2 * PSN] + q[2 * PSN - NP - 1]) +
+         (q[3 * PSN] + q[3 * PSN - NP + 1]);
+}
+
 template <typename S>
 __device__ __forceinline__ S quotient(S a, S b)
 {
@@ -154,6 +166,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // kModeStream keeps q = K p of the own nodes in LDS (thread-private slots)
   constexpr bool QLDS = MODE == kModeStream;
   constexpr int NT = kWave * SW;
+  constexpr int PSN = coupling_plane_stride(NF); // node-coupling plane stride (kModeStream)
 
   __shared__ T sgrid[PPW * GSZ];
   __shared__ T sred[SW * PPW * 4];        // inner-CG partial sums (type T)
@@ -186,10 +199,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       const double *bx = a.box + (size_t)cell * 4;
       rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
       cf = a.coef + a.coef_off[cell];
-      if (MODE == kModeGlobal || (MODE == kModeStream && sizeof(T) == 8))
+      if (MODE == kModeGlobal || MODE == kModeStream)
         cg = a.cond + a.cond_off[cell];
-      if (MODE == kModeStream)
-        c32 = a.cond32 + a.cond_off[cell];
+      if (MODE == kModeStream) // FP32 copies of the planes follow the FP64 ones
+        c32 = (const float *)(cg + 4 * PSN);
       phi = a.bases + a.bas_off[cell] + (size_t)basis * NP * NP;
     }
 
@@ -231,28 +244,6 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       }
     else if constexpr (MODE == kModeTensor)
       return {{cc[i][j][0], cc[i][j][1], cc[i][j][2], cc[i][j][3], cc[i][j][1], cc[i][j][0]}};
-    else if constexpr (MODE == kModeStream)
-      {
-        // no bounds test: cx, cy <= NF stay inside the padded planes, and the
-        // values read for cells outside the tile only reach nodes that are
-        // not interior (masked by the zero diagonal entry)
-        constexpr int PS = stream_plane_stride(NF);
-        const unsigned f = (unsigned)((Y0 - 1 + i) * NF + (X0 - 1 + j));
-        if constexpr (sizeof(T) == 4)
-          {
-            typedef const float __attribute__((address_space(1))) * gptr;
-            const gptr q = (gptr)c32;
-            const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
-            return {{ch, cv, cd, ca, cv, ch}};
-          }
-        else
-          {
-            typedef const double __attribute__((address_space(1))) * gptr;
-            const gptr q = (gptr)cg;
-            const T ch = q[f], cv = q[f + PS], cd = q[f + 2 * PS], ca = q[f + 3 * PS];
-            return {{ch, cv, cd, ca, cv, ch}};
-          }
-      }
     else
       {
         const int cx = X0 - 1 + j, cy = Y0 - 1 + i;
@@ -278,9 +269,16 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     for (int c = 0; c < CPL; ++c)
       {
         const bool act = pvalid && (X0 + c <= NF - 1) && (Y0 + i <= NF - 1);
-        const Cond6<T> ne = cond_T(i + 1, c + 1), nw = cond_T(i + 1, c), se = cond_T(i, c + 1), sw = cond_T(i, c);
-        const T d = (ne.c[0] + ne.c[1] + ne.c[2]) + (nw.c[0] + nw.c[4] + nw.c[3]) + (se.c[5] + se.c[1] + se.c[3]) +
-                    (sw.c[5] + sw.c[4] + sw.c[2]);
+        T d;
+        if constexpr (MODE == kModeStream)
+          d = (T)node_diag64<NF>(cg, X0 + c, Y0 + i);
+        else
+          {
+            const Cond6<T> ne = cond_T(i + 1, c + 1), nw = cond_T(i + 1, c), se = cond_T(i, c + 1),
+                           sw = cond_T(i, c);
+            d = (ne.c[0] + ne.c[1] + ne.c[2]) + (nw.c[0] + nw.c[4] + nw.c[3]) + (se.c[5] + se.c[1] + se.c[3]) +
+                (sw.c[5] + sw.c[4] + sw.c[2]);
+          }
         dinv[i][c] = act ? (float)((T)1 / d) : 0.0f;
       }
   // kModeStream keeps the diagonal as packed BF16 pairs (8 registers instead
@@ -346,12 +344,24 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF;
       return (bonly && !bnd) ? 0.0 : phi[YY * NP + XX];
     };
-    const Cond6<double> ne = cell_cond64<MODE, NF>(cf, cg, X, Y, rho);
-    const Cond6<double> nw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y, rho);
-    const Cond6<double> se = cell_cond64<MODE, NF>(cf, cg, X, Y - 1, rho);
-    const Cond6<double> sw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y - 1, rho);
-    return gather6<double>(v(X, Y), v(X + 1, Y), v(X - 1, Y), v(X, Y + 1), v(X, Y - 1), v(X + 1, Y + 1),
-                           v(X - 1, Y + 1), v(X + 1, Y - 1), v(X - 1, Y - 1), ne, nw, se, sw);
+    if constexpr (MODE == kModeStream)
+      {
+        const double pC = v(X, Y);
+        const double *q = cg + Y * NP + X;
+        return q[0] * (pC - v(X + 1, Y)) + q[-1] * (pC - v(X - 1, Y)) + q[PSN] * (pC - v(X, Y + 1)) +
+               q[PSN - NP] * (pC - v(X, Y - 1)) + q[2 * PSN] * (pC - v(X + 1, Y + 1)) +
+               q[2 * PSN - NP - 1] * (pC - v(X - 1, Y - 1)) + q[3 * PSN] * (pC - v(X - 1, Y + 1)) +
+               q[3 * PSN - NP + 1] * (pC - v(X + 1, Y - 1));
+      }
+    else
+      {
+        const Cond6<double> ne = cell_cond64<MODE, NF>(cf, cg, X, Y, rho);
+        const Cond6<double> nw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y, rho);
+        const Cond6<double> se = cell_cond64<MODE, NF>(cf, cg, X, Y - 1, rho);
+        const Cond6<double> sw = cell_cond64<MODE, NF>(cf, cg, X - 1, Y - 1, rho);
+        return gather6<double>(v(X, Y), v(X + 1, Y), v(X - 1, Y), v(X, Y + 1), v(X, Y - 1), v(X + 1, Y + 1),
+                               v(X - 1, Y + 1), v(X + 1, Y - 1), v(X - 1, Y - 1), ne, nw, se, sw);
+      }
   };
 
   double tol_eff = 0.0, bnorm = 0.0;
@@ -441,12 +451,16 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       T gamma_prev = 1, alpha_prev = 1;
       if constexpr (MODE == kModeStream)
         {
-          // Standard Jacobi PCG with two reductions per iteration.  Only x, r
-          // and the Jacobi diagonal stay in registers; p lives in the LDS grid
-          // (the stencil's input, read back from the own slots for the
-          // updates), q = K p in thread-private LDS slots, and the conductances
-          // of the two fine-cell rows around a node row are streamed from
-          // global memory as the stencil slides upward.
+          // Standard Jacobi PCG with two reductions per iteration.  p lives in
+          // the LDS grid (the stencil's input, read back from the own slots for
+          // the updates); the node couplings of the node row being processed and
+          // of the row below are streamed from global memory as the stencil
+          // slides upward: (K p)_C = sum over the 8 neighbours k of a_Ck (p_C - p_k).
+          // FP32: x, r and the packed diagonal in registers, q in thread-private
+          // LDS slots.  FP64: the FP64 grid leaves no LDS for q, so q goes to this
+          // problem's slots of a global work array (thread-private, coalesced)
+          // and x is added straight into the FP64 solution phi.
+          static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
           T v2[2] = {0, 0}; // (r, z), (r, r)
 #pragma unroll
           for (int i = 0; i < R; ++i)
@@ -462,168 +476,88 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
             irun = false;
           T rz = v2[0];
-          static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
-          if constexpr (sizeof(T) == 8)
+          typedef T __attribute__((address_space(1))) * gtptr;
+          typedef const T __attribute__((address_space(1))) * gcptr;
           while (any)
             {
-              // FP64: no LDS room beside the FP64 grid, so q = K p goes to this
-              // problem's slots of a global work array (thread-private,
-              // coalesced) and x is accumulated straight into the FP64 solution
-              // phi; the stencil runs in cell form -- cell row cy adds its element
-              // products to node rows cy (then complete) and cy + 1 -- so that one
-              // row of cells is live at a time.
               __syncthreads(); // p complete in the grid
-              asm volatile("" : "+s"(cg));
-              // uniform bases (SGPRs) + 32-bit lane offsets: no 64-bit address per access
-              typedef double __attribute__((address_space(1))) * gdptr;
-              static_assert(R * CPL * NT == kStreamQSlots, "work slots per problem");
-              double *qb = a.work + (size_t)prob * kStreamQSlots;
+              // opaque per-iteration copies of the (wave-uniform) addresses and of
+              // the packed diagonal: keeps loads and conversions inside the loop
+              // (hoisted, they would need far more than the 128 registers)
+              const T *cbase;
+              if constexpr (sizeof(T) == 8)
+                cbase = (const T *)cg;
+              else
+                cbase = (const T *)c32;
+              T *qb = sizeof(T) == 8 ? (T *)(a.work + (size_t)prob * kStreamQSlots) : nullptr;
               double *pb = phi;
-              asm volatile("" : "+s"(qb), "+s"(pb));
-              const gdptr qg = (gdptr)qb, ph = (gdptr)pb;
-              // the packed diagonal too: its FP64 conversions would otherwise be
-              // hoisted out of the loop (16 more FP64 values live)
+              asm volatile("" : "+s"(cbase), "+s"(qb), "+s"(pb));
 #pragma unroll
               for (int k = 0; k < (R * CPL + 1) / 2; ++k)
                 asm volatile("" : "+v"(dpk[k]));
-              T lo[CPL + 2], hi[CPL + 2], yp[CPL];
-              T pq[1] = {0};
-#pragma unroll
-              for (int k = 0; k < CPL + 2; ++k)
-                lo[k] = gwin[k];
-#pragma unroll
-              for (int c = 0; c < CPL; ++c)
-                yp[c] = 0;
-#pragma unroll
-              for (int j = 0; j <= R; ++j)
-                {
-#pragma unroll
-                  for (int k = 0; k < CPL + 2; ++k)
-                    hi[k] = gwin[(j + 1) * GP + k];
-                  T yb[CPL], yt[CPL];
-#pragma unroll
-                  for (int c = 0; c < CPL; ++c)
-                    {
-                      yb[c] = 0;
-                      yt[c] = 0;
-                    }
-#pragma unroll
-                  for (int m = 0; m <= CPL; ++m)
-                    {
-                      // cell (X0 - 1 + m, Y0 - 1 + j): vertices 0, 1 below, 2, 3 above
-                      const Cond6<T> e = cond_T(j, m);
-                      const T p0 = lo[m], p1 = lo[m + 1], p2 = hi[m], p3 = hi[m + 1];
-                      if (m >= 1) // vertices 0 and 2 are own column m - 1
-                        {
-                          yb[m - 1] += e.c[0] * (p0 - p1) + e.c[1] * (p0 - p2) + e.c[2] * (p0 - p3);
-                          yt[m - 1] += e.c[1] * (p2 - p0) + e.c[3] * (p2 - p1) + e.c[5] * (p2 - p3);
-                        }
-                      if (m < CPL) // vertices 1 and 3 are own column m
-                        {
-                          yb[m] += e.c[0] * (p1 - p0) + e.c[3] * (p1 - p2) + e.c[4] * (p1 - p3);
-                          yt[m] += e.c[2] * (p3 - p0) + e.c[4] * (p3 - p1) + e.c[5] * (p3 - p2);
-                        }
-                      __builtin_amdgcn_sched_barrier(0); // one cell's conductances live at a time
-                    }
-                  if (j >= 1) // own node row j - 1 is complete
-                    {
-#pragma unroll
-                      for (int c = 0; c < CPL; ++c)
-                        {
-                          const T q = dget(j - 1, c) != 0.0f ? yp[c] + yb[c] : (T)0;
-                          qg[(unsigned)(((j - 1) * CPL + c) * NT + tid)] = q;
-                          pq[0] += lo[c + 1] * q;
-                        }
-                    }
-#pragma unroll
-                  for (int c = 0; c < CPL; ++c)
-                    yp[c] = yt[c];
-#pragma unroll
-                  for (int k = 0; k < CPL + 2; ++k)
-                    lo[k] = hi[k];
-                  __builtin_amdgcn_sched_barrier(0);
-                }
-              problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave, slot, pl);
-              v2[0] = 0;
-              v2[1] = 0;
-              if (irun)
-                {
-                  const T alpha = quotient<T>(rz, pq[0]);
-#pragma unroll
-                  for (int i = 0; i < R; ++i)
-#pragma unroll
-                    for (int c = 0; c < CPL; ++c)
-                      {
-                        if (dget(i, c) != 0.0f)
-                          ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
-                        rr[i][c] -= alpha * qg[(unsigned)((i * CPL + c) * NT + tid)];
-                        const T z = rr[i][c] * (T)dget(i, c);
-                        v2[0] += rr[i][c] * z;
-                        v2[1] += rr[i][c] * rr[i][c];
-                      }
-                  ++it_total;
-                }
-              any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
-              if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
-                irun = false;
-              const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
-              rz = v2[0];
-#pragma unroll
-              for (int i = 0; i < R; ++i)
-#pragma unroll
-                for (int c = 0; c < CPL; ++c)
-                  gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) + beta * gown[i * GP + c] : (T)0;
-            }
-          else
-          while (any)
-            {
-              __syncthreads(); // p complete in the grid
-              // opaque per-iteration copy of the (wave-uniform) tile address: keeps
-              // the loads in the loop (hoisted, the 27 cells' conductances would
-              // take 108 registers); likewise the packed diagonal's conversions
-              asm volatile("" : "+s"(c32));
-#pragma unroll
-              for (int k = 0; k < (R * CPL + 1) / 2; ++k)
-                asm volatile("" : "+v"(dpk[k]));
+              const gcptr cp = (gcptr)cbase;
+              const gtptr qg = (gtptr)qb;
+              typedef double __attribute__((address_space(1))) * gdptr;
+              const gdptr ph = (gdptr)pb;
+              // couplings at window column k (node X0 - 1 + k) of node row Y
+              auto cpl = [&](int plane, int Y, int k) -> T { return cp[(unsigned)(plane * PSN + Y * NP + X0 - 1 + k)]; };
+              // couplings of the node row (E, N, NE, NW) and of the row below
+              // (pN, pNE, pNW) in a sliding register window (in FP64 this spills
+              // a few values, still faster than reloading each node's 8 couplings)
               T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
-              Cond6<T> cb[CPL + 1], ct[CPL + 1];
+              T pN[CPL + 2], pNE[CPL + 2], pNW[CPL + 2];
 #pragma unroll
               for (int k = 0; k < CPL + 2; ++k)
                 {
                   w0[k] = gwin[k];
                   w1[k] = gwin[GP + k];
+                  pN[k] = (k >= 1 && k <= CPL) ? cpl(1, Y0 - 1, k) : (T)0;
+                  pNE[k] = (k <= CPL - 1) ? cpl(2, Y0 - 1, k) : (T)0;
+                  pNW[k] = (k >= 2) ? cpl(3, Y0 - 1, k) : (T)0;
                 }
-#pragma unroll
-              for (int j = 0; j <= CPL; ++j)
-                cb[j] = cond_T(0, j);
               T pq[1] = {0};
 #pragma unroll
               for (int i = 0; i < R; ++i)
                 {
-#pragma unroll
-                  for (int j = 0; j <= CPL; ++j)
-                    ct[j] = cond_T(i + 1, j);
+                  const int Y = Y0 + i;
+                  T E[CPL + 2], N[CPL + 2], NE[CPL + 2], NW[CPL + 2];
 #pragma unroll
                   for (int k = 0; k < CPL + 2; ++k)
-                    w2[k] = gwin[(i + 2) * GP + k];
+                    {
+                      E[k] = k <= CPL ? cpl(0, Y, k) : (T)0;
+                      N[k] = (k >= 1 && k <= CPL) ? cpl(1, Y, k) : (T)0;
+                      NE[k] = k <= CPL ? cpl(2, Y, k) : (T)0;
+                      NW[k] = k >= 1 ? cpl(3, Y, k) : (T)0;
+                      w2[k] = gwin[(i + 2) * GP + k];
+                    }
 #pragma unroll
                   for (int c = 0; c < CPL; ++c)
                     {
-                      const T y = gather6<T>(w1[c + 1], w1[c + 2], w1[c], w2[c + 1], w0[c + 1], w2[c + 2], w2[c],
-                                             w0[c + 2], w0[c], ct[c + 1], ct[c], cb[c + 1], cb[c]);
+                      const T pC = w1[c + 1];
+                      T y = E[c + 1] * (pC - w1[c + 2]);
+                      y += E[c] * (pC - w1[c]);
+                      y += N[c + 1] * (pC - w2[c + 1]);
+                      y += pN[c + 1] * (pC - w0[c + 1]);
+                      y += NE[c + 1] * (pC - w2[c + 2]);
+                      y += pNE[c] * (pC - w0[c]);
+                      y += NW[c + 1] * (pC - w2[c]);
+                      y += pNW[c + 2] * (pC - w0[c + 2]);
                       const T q = dget(i, c) != 0.0f ? y : (T)0;
-                      sq[(i * CPL + c) * NT + tid] = q;
-                      pq[0] += w1[c + 1] * q;
+                      if constexpr (sizeof(T) == 4)
+                        sq[(i * CPL + c) * NT + tid] = q;
+                      else
+                        qg[(unsigned)((i * CPL + c) * NT + tid)] = q;
+                      pq[0] += pC * q;
                     }
 #pragma unroll
                   for (int k = 0; k < CPL + 2; ++k)
                     {
                       w0[k] = w1[k];
                       w1[k] = w2[k];
+                      pN[k] = N[k];
+                      pNE[k] = NE[k];
+                      pNW[k] = NW[k];
                     }
-#pragma unroll
-                  for (int j = 0; j <= CPL; ++j)
-                    cb[j] = ct[j];
                   __builtin_amdgcn_sched_barrier(0);
                 }
               problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave, slot, pl);
@@ -637,8 +571,19 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
                     for (int c = 0; c < CPL; ++c)
                       {
-                        xr[i][c] += alpha * gown[i * GP + c];
-                        rr[i][c] -= alpha * sq[(i * CPL + c) * NT + tid];
+                        T q;
+                        if constexpr (sizeof(T) == 4)
+                          {
+                            q = sq[(i * CPL + c) * NT + tid];
+                            xr[i][c] += alpha * gown[i * GP + c];
+                          }
+                        else
+                          {
+                            q = qg[(unsigned)((i * CPL + c) * NT + tid)];
+                            if (dget(i, c) != 0.0f)
+                              ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
+                          }
+                        rr[i][c] -= alpha * q;
                         const T z = rr[i][c] * (T)dget(i, c);
                         v2[0] += rr[i][c] * z;
                         v2[1] += rr[i][c] * rr[i][c];
@@ -777,9 +722,42 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     }
 }
 
-// ---- coefficient preparation for kModeGlobal: 6 FP64 conductances per fine cell
-// from any 2D coefficient kind, QGauss(2) on the box fine cell
-// (diffusion_problem_basis.tpp:207-222).
+// ---- coefficient preparation -----------------------------------------------------
+// QGauss(2) element matrix of fine cell f of a 2D coefficient tile (any kind) on
+// an hx x hy cell (diffusion_problem_basis.tpp:207-222); vertex order (0,0),
+// (1,0), (0,1), (1,1).
+__device__ void qg_element_2d(int coef_kind, const double *cf, int f, double hx, double hy, double (&K)[4][4])
+{
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      K[u][v] = 0;
+  const double s = 0.5 / sqrt(3.0);
+  for (int q = 0; q < 4; ++q)
+    {
+      const double xi = (q & 1) ? 0.5 + s : 0.5 - s, eta = (q & 2) ? 0.5 + s : 0.5 - s;
+      double A11, A12, A22;
+      switch (coef_kind)
+        {
+          case 0: A11 = A22 = cf[f]; A12 = 0; break;
+          case 1: A11 = cf[3 * f]; A12 = cf[3 * f + 1]; A22 = cf[3 * f + 2]; break;
+          case 2: A11 = A22 = cf[4 * f + q]; A12 = 0; break;
+          default: A11 = cf[12 * f + 3 * q]; A12 = cf[12 * f + 3 * q + 1]; A22 = cf[12 * f + 3 * q + 2]; break;
+        }
+      double gx[4], gy[4];
+      for (int v = 0; v < 4; ++v)
+        {
+          const double fx = (v & 1) ? xi : 1.0 - xi, fy = (v & 2) ? eta : 1.0 - eta;
+          gx[v] = ((v & 1) ? 1.0 : -1.0) * fy / hx;
+          gy[v] = ((v & 2) ? 1.0 : -1.0) * fx / hy;
+        }
+      const double jxw = 0.25 * hx * hy;
+      for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v)
+          K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
+    }
+}
+
+// kModeGlobal: 6 FP64 conductances (c01, c02, c03, c12, c13, c23 = -K_uv) per fine cell.
 __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
                              const double *coef, const long long *coef_off, double *cond, const long long *cond_off)
 {
@@ -790,33 +768,8 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
       const int cell = cells[t / ncell];
       const int f = (int)(t % ncell);
       const double *bx = box + (size_t)cell * 4;
-      const double hx = (bx[1] - bx[0]) / nf, hy = (bx[3] - bx[2]) / nf;
-      const double *cf = coef + coef_off[cell];
-      double K[4][4] = {{0}};
-      const double s = 0.5 / sqrt(3.0);
-      for (int q = 0; q < 4; ++q)
-        {
-          const double xi = (q & 1) ? 0.5 + s : 0.5 - s, eta = (q & 2) ? 0.5 + s : 0.5 - s;
-          double A11, A12, A22;
-          switch (coef_kind)
-            {
-              case 0: A11 = A22 = cf[f]; A12 = 0; break;
-              case 1: A11 = cf[3 * f]; A12 = cf[3 * f + 1]; A22 = cf[3 * f + 2]; break;
-              case 2: A11 = A22 = cf[4 * f + q]; A12 = 0; break;
-              default: A11 = cf[12 * f + 3 * q]; A12 = cf[12 * f + 3 * q + 1]; A22 = cf[12 * f + 3 * q + 2]; break;
-            }
-          double gx[4], gy[4];
-          for (int v = 0; v < 4; ++v)
-            {
-              const double fx = (v & 1) ? xi : 1.0 - xi, fy = (v & 2) ? eta : 1.0 - eta;
-              gx[v] = ((v & 1) ? 1.0 : -1.0) * fy / hx;
-              gy[v] = ((v & 2) ? 1.0 : -1.0) * fx / hy;
-            }
-          const double jxw = 0.25 * hx * hy;
-          for (int u = 0; u < 4; ++u)
-            for (int v = 0; v < 4; ++v)
-              K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
-        }
+      double K[4][4];
+      qg_element_2d(coef_kind, coef + coef_off[cell], f, (bx[1] - bx[0]) / nf, (bx[3] - bx[2]) / nf, K);
       double *o = cond + cond_off[cell] + 6 * (size_t)f;
       o[0] = -K[0][1];
       o[1] = -K[0][2];
@@ -827,29 +780,55 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
     }
 }
 
-// ---- FP32 planar conductances for kModeStream (symmetric tensor per fine cell):
-// plane k of cell c at cond32 + cond_off[c] + k stream_plane_stride(nf),
-// k = (ch, cv, cd, ca); entry cy nf + cx.  The plane padding (never written,
-// read only for cells outside the tile) keeps unguarded reads in bounds.
-__global__ void prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                               const long long *coef_off, float *cond32, const long long *cond_off)
+// kModeStream: node couplings of every node (X, Y) of the tile, summed from the
+// (up to two) fine cells that share each edge; FP64 planes at cond + cond_off,
+// and (want32) FP32 copies right after them.  Plane entries of padding are
+// never written: the kernels read them only for non-interior nodes.
+__global__ void prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                                  const double *coef, const long long *coef_off, double *cond,
+                                  const long long *cond_off, int want32)
 {
-  const int ncell = nf * nf;
-  const long long total = (long long)n_cells_class * ncell;
+  const int np = nf + 1, nnode = np * np, ps = coupling_plane_stride(nf);
+  const long long total = (long long)n_cells_class * nnode;
   for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
     {
-      const int cell = cells[t / ncell];
-      const int f = (int)(t % ncell);
+      const int cell = cells[t / nnode];
+      const int n = (int)(t % nnode), X = n % np, Y = n / np;
       const double *bx = box + (size_t)cell * 4;
-      const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      const double hx = (bx[1] - bx[0]) / nf, hy = (bx[3] - bx[2]) / nf;
       const double *cf = coef + coef_off[cell];
-      const Cond4 c = cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho);
-      const int ps = stream_plane_stride(nf);
-      float *o = cond32 + cond_off[cell] + f;
-      o[0] = (float)c.ch;
-      o[ps] = (float)c.cv;
-      o[2 * ps] = (float)c.cd;
-      o[3 * ps] = (float)c.ca;
+      double aE = 0, aN = 0, aNE = 0, aNW = 0, K[4][4];
+      if (X < nf && Y < nf) // cell (X, Y): the node is its vertex 0
+        {
+          qg_element_2d(coef_kind, cf, Y * nf + X, hx, hy, K);
+          aE -= K[0][1];
+          aN -= K[0][2];
+          aNE -= K[0][3];
+        }
+      if (X < nf && Y > 0) // cell (X, Y-1): vertex 2
+        {
+          qg_element_2d(coef_kind, cf, (Y - 1) * nf + X, hx, hy, K);
+          aE -= K[2][3];
+        }
+      if (X > 0 && Y < nf) // cell (X-1, Y): vertex 1
+        {
+          qg_element_2d(coef_kind, cf, Y * nf + X - 1, hx, hy, K);
+          aN -= K[1][3];
+          aNW -= K[1][2];
+        }
+      double *o = cond + cond_off[cell] + n;
+      o[0] = aE;
+      o[ps] = aN;
+      o[2 * ps] = aNE;
+      o[3 * ps] = aNW;
+      if (want32)
+        {
+          float *o32 = (float *)(cond + cond_off[cell] + 4 * (size_t)ps) + n;
+          o32[0] = (float)aE;
+          o32[ps] = (float)aN;
+          o32[2 * ps] = (float)aNE;
+          o32[3 * ps] = (float)aNW;
+        }
     }
 }
 
@@ -950,56 +929,19 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
   return launch_mode<double, kModeGlobal>(nf, variant, a, st);
 }
 
-// FP64 planes of the same layout for the FP64 streamed form.
-__global__ void prep_cond64p_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                                const long long *coef_off, double *cond, const long long *cond_off)
+hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                                   const double *coef, const long long *coef_off, double *cond,
+                                   const long long *cond_off, bool want32, hipStream_t st)
 {
-  const int ncell = nf * nf;
-  const long long total = (long long)n_cells_class * ncell;
-  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
-    {
-      const int cell = cells[t / ncell];
-      const int f = (int)(t % ncell);
-      const double *bx = box + (size_t)cell * 4;
-      const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
-      const double *cf = coef + coef_off[cell];
-      const Cond4 c = cond_tensor(cf[3 * f], cf[3 * f + 1], cf[3 * f + 2], rho);
-      const int ps = stream_plane_stride(nf);
-      double *o = cond + cond_off[cell] + f;
-      o[0] = c.ch;
-      o[ps] = c.cv;
-      o[2 * ps] = c.cd;
-      o[3 * ps] = c.ca;
-    }
-}
-
-hipError_t launch_prep_cond64p_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                                  const long long *coef_off, double *cond, const long long *cond_off, hipStream_t st)
-{
-  const long long total = (long long)n_cells_class * nf * nf;
+  const long long total = (long long)n_cells_class * (nf + 1) * (nf + 1);
   if (total == 0)
     return hipSuccess;
   const int block = 256;
   long long grid = (total + block - 1) / block;
   if (grid > 65536)
     grid = 65536;
-  hipLaunchKernelGGL(prep_cond64p_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, box, coef,
-                     coef_off, cond, cond_off);
-  return hipGetLastError();
-}
-
-hipError_t launch_prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                                 const long long *coef_off, float *cond32, const long long *cond_off, hipStream_t st)
-{
-  const long long total = (long long)n_cells_class * nf * nf;
-  if (total == 0)
-    return hipSuccess;
-  const int block = 256;
-  long long grid = (total + block - 1) / block;
-  if (grid > 65536)
-    grid = 65536;
-  hipLaunchKernelGGL(prep_cond32_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, box, coef,
-                     coef_off, cond32, cond_off);
+  hipLaunchKernelGGL(prep_couplings_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf,
+                     coef_kind, box, coef, coef_off, cond, cond_off, want32 ? 1 : 0);
   return hipGetLastError();
 }
 

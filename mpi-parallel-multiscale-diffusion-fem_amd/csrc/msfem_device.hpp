@@ -17,10 +17,12 @@ enum CondMode : int {
                    // (planar prep buffer), the CG product vector w kept in LDS
 };
 
-// Plane stride (floats) of the kModeStream conductance planes: nf^2 cells plus
-// padding for the unguarded reads of cells (cx, cy) with cx, cy <= nf.  Four
-// planes (4 nf^2 + 8 nf floats) fit the 6 nf^2 doubles reserved per cell.
-__host__ __device__ constexpr int stream_plane_stride(int nf) { return nf * nf + 2 * nf; }
+// kModeStream node couplings: four planes (aE, aN, aNE, aNW: the couplings of
+// node (X, Y) with (X+1, Y), (X, Y+1), (X+1, Y+1), (X-1, Y+1); K = sum over
+// couplings a (e_C - e_k)(e_C - e_k)^T), entry Y (nf+1) + X, plane stride
+// below (padding keeps the kernels' unguarded reads in bounds); FP64 planes
+// first, then FP32 copies: 6 planes' worth of doubles per cell.
+__host__ __device__ constexpr int coupling_plane_stride(int nf) { return (nf + 1) * (nf + 1) + 2 * (nf + 1); }
 
 // FP64 values of q per problem in the FP64 streamed kernel's global work
 // array (the 128^2 class: 8 rows x 2 columns per thread x 1024 threads).

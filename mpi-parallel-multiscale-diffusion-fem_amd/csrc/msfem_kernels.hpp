@@ -10,10 +10,9 @@ namespace msfem {
 // local2d.hip
 int local2d_variants(int nf);
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st);
-hipError_t launch_prep_cond64p_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                                  const long long *coef_off, double *cond, const long long *cond_off, hipStream_t st);
-hipError_t launch_prep_cond32_2d(int n_cells_class, const int *cells, int nf, const double *box, const double *coef,
-                                 const long long *coef_off, float *cond32, const long long *cond_off, hipStream_t st);
+hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                                   const double *coef, const long long *coef_off, double *cond,
+                                   const long long *cond_off, bool want32, hipStream_t st);
 hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
                                const double *coef, const long long *coef_off, double *cond,
                                const long long *cond_off, hipStream_t st);
