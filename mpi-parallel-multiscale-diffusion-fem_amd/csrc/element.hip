@@ -262,6 +262,155 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
     }
 }
 
+// 3D (scalar coefficient per fine cell, the only 3D kind):
+//     A^K = sum_c alpha_c P_c^T L P_c,   F^K = sum_c P_c^T b_c,
+// P_c[a][i] = phi_i at vertex a of fine cell c, L the QGauss(2) Q1 element
+// matrix of the unit coefficient on the fine box (in LDS).  Row by row of
+// Z = alpha L P, only the upper triangle of A^K is accumulated: 44 running
+// sums per thread instead of a full 8x8 element matrix, 8x8 vertex values
+// and 72 sums (which spilled).
+__global__ void __launch_bounds__(kElemThreads) element3d_kernel(ElementArgs a)
+{
+  constexpr int NV = 8, NQ = 8, NU = NV * (NV + 1) / 2, NACC = NU + NV;
+  __shared__ double sL[NV * NV];
+  __shared__ double sred[kElemThreads / kWave][NACC];
+
+  const int cell = a.cells[blockIdx.x];
+  const int nf = a.nf, np = nf + 1;
+  const double *bx = a.box + (size_t)cell * 6;
+  double h[3], lo[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    {
+      lo[k] = bx[2 * k];
+      h[k] = (bx[2 * k + 1] - bx[2 * k]) / nf;
+    }
+  const double jxw = h[0] * h[1] * h[2] / NQ;
+  const double gs = 0.5 / sqrt(3.0);
+  auto gauss = [&](int q, int m) -> double { return ((q >> m) & 1) ? 0.5 + gs : 0.5 - gs; };
+  auto grad = [&](int aa, int k, int q) -> double {
+    double g = ((aa >> k) & 1) ? 1.0 : -1.0;
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      if (m != k)
+        g *= ((aa >> m) & 1) ? gauss(q, m) : 1.0 - gauss(q, m);
+    return g / h[k];
+  };
+  for (int t = threadIdx.x; t < NV * NV; t += blockDim.x)
+    {
+      const int aa = t / NV, bb = t % NV;
+      double v = 0;
+      for (int q = 0; q < NQ; ++q)
+        for (int k = 0; k < 3; ++k)
+          v += grad(aa, k, q) * grad(bb, k, q) * jxw;
+      sL[t] = v;
+    }
+  __syncthreads();
+
+  const double *cf = a.coef + a.coef_off[cell];
+  const double *phi = a.bases + a.bas_off[cell];
+  const size_t nn = (size_t)np * np * np;
+  const int ncell = nf * nf * nf;
+  double acc[NACC];
+#pragma unroll
+  for (int v = 0; v < NACC; ++v)
+    acc[v] = 0;
+  for (int f = threadIdx.x; f < ncell; f += blockDim.x)
+    {
+      const int fx = f % nf, fy = (f / nf) % nf, fz = f / (nf * nf);
+      const double alpha = cf[f];
+      double P[NV][NV]; // [vertex][basis]
+#pragma unroll
+      for (int aa = 0; aa < NV; ++aa)
+        {
+          const size_t node = (size_t)(fx + (aa & 1)) + (size_t)np * ((fy + ((aa >> 1) & 1)) +
+                                                                       (size_t)np * (fz + ((aa >> 2) & 1)));
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+            P[aa][i] = phi[(size_t)i * nn + node];
+        }
+      // load vector of the fine cell
+      double be[NV];
+#pragma unroll
+      for (int aa = 0; aa < NV; ++aa)
+        be[aa] = 0;
+      for (int q = 0; q < NQ; ++q)
+        {
+          double fv = a.rhs_value;
+          if (a.rhs_kind == 1)
+            {
+              const int fi[3] = {fx, fy, fz};
+              fv = 1.0;
+              for (int k = 0; k < 3; ++k)
+                fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * gauss(q, k)));
+            }
+#pragma unroll
+          for (int aa = 0; aa < NV; ++aa)
+            {
+              double sh = 1.0;
+#pragma unroll
+              for (int m = 0; m < 3; ++m)
+                sh *= ((aa >> m) & 1) ? gauss(q, m) : 1.0 - gauss(q, m);
+              be[aa] += sh * fv * jxw;
+            }
+        }
+#pragma unroll
+      for (int aa = 0; aa < NV; ++aa)
+        {
+          double z[NV];
+#pragma unroll
+          for (int j = 0; j < NV; ++j)
+            z[j] = 0;
+#pragma unroll
+          for (int bb = 0; bb < NV; ++bb)
+            {
+              const double l = alpha * sL[aa * NV + bb];
+#pragma unroll
+              for (int j = 0; j < NV; ++j)
+                z[j] += l * P[bb][j];
+            }
+          int u = 0;
+#pragma unroll
+          for (int i = 0; i < NV; ++i)
+            {
+#pragma unroll
+              for (int j = i; j < NV; ++j)
+                acc[u++] += P[aa][i] * z[j];
+              acc[NU + i] += P[aa][i] * be[aa];
+            }
+        }
+    }
+  const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+#pragma unroll
+  for (int v = 0; v < NACC; ++v)
+    {
+      const double s = group_sum<kWave>(acc[v]);
+      if (lane == 0)
+        sred[wave][v] = s;
+    }
+  __syncthreads();
+  for (int v = threadIdx.x; v < NACC; v += blockDim.x)
+    {
+      double s = 0;
+      for (int w = 0; w < kElemThreads / kWave; ++w)
+        s += sred[w][v];
+      if (v < NU)
+        {
+          int i = 0, r = v;
+          while (r >= NV - i)
+            {
+              r -= NV - i;
+              ++i;
+            }
+          const int j = i + r;
+          a.Ae[(size_t)cell * NV * NV + i * NV + j] = s;
+          a.Ae[(size_t)cell * NV * NV + j * NV + i] = s;
+        }
+      else
+        a.Fe[(size_t)cell * NV + v - NU] = s;
+    }
+}
+
 __global__ void pou_fixup_kernel(int n_cells_class, const int *cells, int nb, size_t nn, double *bases,
                                  const long long *bas_off)
 {
@@ -286,6 +435,8 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
     return hipSuccess;
   if (a.dim == 2)
     hipLaunchKernelGGL(element_kernel<2>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else if (a.coef_kind == 0) // scalar per fine cell
+    hipLaunchKernelGGL(element3d_kernel, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else
     hipLaunchKernelGGL(element_kernel<3>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   return hipGetLastError();
