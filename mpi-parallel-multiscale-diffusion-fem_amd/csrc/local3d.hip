@@ -4,9 +4,20 @@
 //
 // One workgroup owns one problem.  Thread t owns the node column
 // (X, Y) = (1 + t % NF, 1 + (t / NF) % NF) and R consecutive z-levels of
-// stack t / NF^2; the search direction of the cell lives in a zero-padded LDS
-// grid, the coefficients of the 4 x (R+1) fine cells around the column in
-// registers.  Same FP64 refinement structure as the 2D kernel.
+// stack t / NF^2; the preconditioned residual z of the cell lives in a
+// zero-padded LDS grid, the coefficients of the 4 x (R+1) fine cells around
+// the column and the CG vectors of the owned nodes in registers.  Same FP64
+// refinement structure and the same single-reduction Jacobi PCG (Chronopoulos
+// and Gear) as the 2D kernel: one stencil apply and one fused reduction of
+// (r.z, w.z, r.r) per iteration, FP32 sums and scalars in the FP32 kernel.
+//
+// Stencil: (K z)_C = sum over the 8 cells c around C of alpha_c sum over the
+// other vertices b of c of u(v_C xor v_b) (z_C - z_b), with the box's unit
+// conductances u(d).  On cubic fine cells the face conductances u(1), u(2),
+// u(4) vanish, and the kernel then skips those three of the seven terms per
+// cell (a uniform branch per problem).
+#include <type_traits>
+
 #include "msfem_device.hpp"
 #include "msfem_kernels.hpp"
 
@@ -36,31 +47,6 @@ __device__ __forceinline__ void unit_cond3(const double h[3], double c[8])
     }
 }
 
-template <int NW>
-__device__ __forceinline__ void block_sum2(double &a, double &b, double *red)
-{
-  a = group_sum<kWave>(a);
-  b = group_sum<kWave>(b);
-  if constexpr (NW > 1)
-    {
-      const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-      if (lane == 0)
-        {
-          red[2 * wave] = a;
-          red[2 * wave + 1] = b;
-        }
-      __syncthreads();
-      a = 0;
-      b = 0;
-#pragma unroll
-      for (int w = 0; w < NW; ++w)
-        {
-          a += red[2 * w];
-          b += red[2 * w + 1];
-        }
-    }
-}
-
 } // namespace
 
 template <typename T, int NF, int SZ, int R>
@@ -75,14 +61,20 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
   constexpr int GZ = (SZ * R + 2 > NP) ? SZ * R + 2 : NP;
 
   __shared__ T sgrid[GXY * GZ];
-  __shared__ double sred[3][2 * NW];
+  __shared__ T sred[NW * 4];        // inner-CG partial sums
+  __shared__ double sred64[NW * 2]; // FP64 residual norms
 
-  const int tid = threadIdx.x;
-  const int prob = blockIdx.x;
-  const int cell = a.cells[prob / a.nb_solve];
-  const int basis = prob % a.nb_solve;
+  const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
+  const int prob = xcd_block(blockIdx.x, gridDim.x);
+  const bool pvalid = prob < a.n_problems;
+  int cell = 0, basis = 0;
+  if (pvalid)
+    {
+      cell = a.cells[prob / a.nb_solve];
+      basis = prob % a.nb_solve;
+    }
   const int X = 1 + tid % NF, Y = 1 + (tid / NF) % NF, Z0 = 1 + (tid / (NF * NF)) * R;
-  const bool colact = X <= NF - 1 && Y <= NF - 1;
+  const bool colact = pvalid && X <= NF - 1 && Y <= NF - 1;
   T *gwin = sgrid + (Z0 - 1) * GXY + (Y - 1) * GX + (X - 1); // window origin
   T *gown = sgrid + Z0 * GXY + Y * GX + X;
 
@@ -101,6 +93,9 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       ctot64 += uc64[d];
     }
   const T ctot = (T)ctot64;
+  // cubic fine cells: the face conductances vanish (exactly for an exact cube)
+  const double umax = fmax(fabs(uc64[3]), fmax(fabs(uc64[5]), fmax(fabs(uc64[6]), fabs(uc64[7]))));
+  const bool cube = fmax(fabs(uc64[1]), fmax(fabs(uc64[2]), fabs(uc64[4]))) <= 1e-14 * umax;
   const double *cf = a.coef + a.coef_off[cell];
   double *phi = a.bases + a.bas_off[cell] + (size_t)basis * NP * NP * NP;
 
@@ -114,8 +109,9 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       for (int j = 0; j < 2; ++j)
         {
           const int cx = X - 1 + j, cy = Y - 1 + k, cz = Z0 - 1 + i;
-          cc[i][k][j] = (cx < NF && cy < NF && cz < NF) ? (T)cf[(cz * NF + cy) * NF + cx] : (T)0;
+          cc[i][k][j] = (pvalid && cx < NF && cy < NF && cz < NF) ? (T)cf[(cz * NF + cy) * NF + cx] : (T)0;
         }
+  // Jacobi diagonal; 0 on slots that are not interior nodes of a valid problem
   T dinv[R];
 #pragma unroll
   for (int i = 0; i < R; ++i)
@@ -134,17 +130,18 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
 
   for (int k = tid; k < GXY * GZ; k += NT)
     sgrid[k] = 0;
-  for (int k = tid; k < NP * NP * NP; k += NT)
-    {
-      const int XX = k % NP, YY = (k / NP) % NP, ZZ = k / (NP * NP);
-      const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF || ZZ == 0 || ZZ == NF;
-      double v = bnd ? nodal_q1(basis, 3, XX, YY, ZZ, NF)
-                     : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
-      phi[k] = isfinite(v) ? v : 0.0; // a non-finite initial value starts from 0
-    }
+  if (pvalid)
+    for (int k = tid; k < NP * NP * NP; k += NT)
+      {
+        const int XX = k % NP, YY = (k / NP) % NP, ZZ = k / (NP * NP);
+        const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF || ZZ == 0 || ZZ == NF;
+        const double v = bnd ? nodal_q1(basis, 3, XX, YY, ZZ, NF)
+                             : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
+        phi[k] = isfinite(v) ? v : 0.0; // a non-finite initial value starts from 0
+      }
   __syncthreads();
 
-  // (K phi) at node (X, Y, Z) in FP64 from global memory
+  // (K phi) at node (X, Y, Z) in FP64 from global memory; bonly: interior values taken as 0
   auto apply64 = [&](int Z, bool bonly) -> double {
     auto v = [&](int XX, int YY, int ZZ) -> double {
       const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF || ZZ == 0 || ZZ == NF;
@@ -173,39 +170,40 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
   double tol_eff = 0.0, bnorm = 0.0;
   if (a.x0)
     {
-      double s = 0, dummy = 0;
+      double s[1] = {0};
 #pragma unroll 1
       for (int i = 0; i < R; ++i)
         if (colact && Z0 + i <= NF - 1)
           {
             const double b = apply64(Z0 + i, true);
-            s += b * b;
+            s[0] += b * b;
           }
-      block_sum2<NW>(s, dummy, sred[2]);
-      bnorm = sqrt(s);
+      problem_sums<double, kWave, NW, 1, 1>(s, false, sred64, wave, 0, lane);
+      bnorm = sqrt(s[0]);
       tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
       __syncthreads();
     }
 
-  T xr[R], rr[R], pr[R], q[R];
-  int it_total = 0, state = 0, conv = 0;
+  T xr[R], rr[R], pr[R], sr[R];
+  int it_total = 0, state = pvalid ? 0 : 1, conv = 0;
   double rnorm = 0;
   for (int cyc = 0;; ++cyc)
     {
-      double s = 0, dummy = 0;
+      // ---- true FP64 residual of the current phi (own slots of the grid carry it to rr)
+      double s[1] = {0};
 #pragma unroll 1
       for (int i = 0; i < R; ++i)
         if (colact && Z0 + i <= NF - 1)
           {
             const double r64 = -apply64(Z0 + i, false);
             gown[i * GXY] = (T)r64;
-            s += r64 * r64;
+            s[0] += r64 * r64;
           }
 #pragma unroll
       for (int i = 0; i < R; ++i)
         rr[i] = dinv[i] != (T)0 ? gown[i * GXY] : (T)0;
-      block_sum2<NW>(s, dummy, sred[2]);
-      rnorm = sqrt(s);
+      problem_sums<double, kWave, NW, 1, 1>(s, false, sred64, wave, 0, lane);
+      rnorm = sqrt(s[0]);
       if (cyc == 0 && !a.x0)
         {
           bnorm = rnorm;
@@ -226,95 +224,117 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       double inner_tol = 0.5 * tol_eff;
       if (sizeof(T) < sizeof(double))
         inner_tol = fmax(a.inner_rel_tol * rnorm, inner_tol);
-      double rz = 0, t0 = 0;
+      const T inner_tol2 = (T)(inner_tol * inner_tol);
 #pragma unroll
       for (int i = 0; i < R; ++i)
         {
           xr[i] = 0;
-          const T z = rr[i] * dinv[i];
-          pr[i] = z;
-          rz += (double)(rr[i] * z);
-          gown[i * GXY] = z;
+          pr[i] = 0;
+          sr[i] = 0;
         }
-      block_sum2<NW>(rz, t0, sred[1]);
-      __syncthreads();
+      bool irun = true, first = true;
+      T gamma_prev = 1, alpha_prev = 1;
       while (true)
         {
-          T w[3][3][3]; // [z][y][x] window around the current node
-#pragma unroll
-          for (int zz = 0; zz < 2; ++zz)
-#pragma unroll
-            for (int yy = 0; yy < 3; ++yy)
-#pragma unroll
-              for (int xx = 0; xx < 3; ++xx)
-                w[zz][yy][xx] = gwin[zz * GXY + yy * GX + xx];
-          double pq = 0;
+          // z = M r into the LDS grid (zero on padding/boundary slots)
 #pragma unroll
           for (int i = 0; i < R; ++i)
-            {
-#pragma unroll
-              for (int yy = 0; yy < 3; ++yy)
-#pragma unroll
-                for (int xx = 0; xx < 3; ++xx)
-                  w[2][yy][xx] = gwin[(i + 2) * GXY + yy * GX + xx];
-              const T pc = w[1][1][1];
-              T y = 0, sa = 0;
-#pragma unroll
-              for (int c = 0; c < 8; ++c)
-                {
-                  const int dx = (c & 1) - 1, dy = ((c >> 1) & 1) - 1, dz = ((c >> 2) & 1) - 1;
-                  const T al = cc[i + 1 + dz][1 + dy][1 + dx];
-                  const int av = (-dx) | ((-dy) << 1) | ((-dz) << 2);
-                  T s2 = 0;
-#pragma unroll
-                  for (int b = 0; b < 8; ++b)
-                    {
-                      if (b == av)
-                        continue;
-                      const int ox = dx + (b & 1), oy = dy + ((b >> 1) & 1), oz = dz + ((b >> 2) & 1);
-                      s2 += uc[av ^ b] * w[1 + oz][1 + oy][1 + ox];
-                    }
-                  y -= al * s2;
-                  sa += al;
-                }
-              y += ctot * sa * pc;
-              q[i] = dinv[i] != (T)0 ? y : (T)0;
-              pq += (double)(pr[i] * q[i]);
-#pragma unroll
-              for (int yy = 0; yy < 3; ++yy)
-#pragma unroll
-                for (int xx = 0; xx < 3; ++xx)
-                  {
-                    w[0][yy][xx] = w[1][yy][xx];
-                    w[1][yy][xx] = w[2][yy][xx];
-                  }
-            }
-          block_sum2<NW>(pq, t0, sred[0]);
-          const T alpha = (T)(rz / pq);
-          double rz_new = 0, rn2 = 0;
-#pragma unroll
-          for (int i = 0; i < R; ++i)
-            {
-              xr[i] += alpha * pr[i];
-              rr[i] -= alpha * q[i];
-              const T z = rr[i] * dinv[i];
-              rz_new += (double)(rr[i] * z);
-              rn2 += (double)(rr[i] * rr[i]);
-              q[i] = z;
-            }
-          block_sum2<NW>(rz_new, rn2, sred[1]);
-          ++it_total;
-          if (sqrt(rn2) <= inner_tol || it_total >= a.max_iter)
-            break; // uniform: one problem per workgroup
-          const T beta = (T)(rz_new / rz);
-          rz = rz_new;
-#pragma unroll
-          for (int i = 0; i < R; ++i)
-            {
-              pr[i] = q[i] + beta * pr[i];
-              gown[i * GXY] = pr[i];
-            }
+            gown[i * GXY] = irun ? rr[i] * dinv[i] : (T)0;
           __syncthreads();
+          // w = K z with the z-levels slid through a 3x3x3 window; fused partial sums
+          T wv[R];
+          T sums[3] = {0, 0, 0}; // (r, z), (w, z), (r, r)
+          auto stencil = [&](auto cube_tag) {
+            constexpr bool CUBE = decltype(cube_tag)::value;
+            T w[3][3][3]; // [z][y][x]
+#pragma unroll
+            for (int zz = 0; zz < 2; ++zz)
+#pragma unroll
+              for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                for (int xx = 0; xx < 3; ++xx)
+                  w[zz][yy][xx] = gwin[zz * GXY + yy * GX + xx];
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+              {
+#pragma unroll
+                for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                  for (int xx = 0; xx < 3; ++xx)
+                    w[2][yy][xx] = gwin[(i + 2) * GXY + yy * GX + xx];
+                const T pc = w[1][1][1];
+                T y = 0, sa = 0;
+#pragma unroll
+                for (int c = 0; c < 8; ++c)
+                  {
+                    const int dx = (c & 1) - 1, dy = ((c >> 1) & 1) - 1, dz = ((c >> 2) & 1) - 1;
+                    const T al = cc[i + 1 + dz][1 + dy][1 + dx];
+                    const int av = (-dx) | ((-dy) << 1) | ((-dz) << 2);
+                    T s2 = 0;
+#pragma unroll
+                    for (int b = 0; b < 8; ++b)
+                      {
+                        const int d = av ^ b;
+                        if (b == av || (CUBE && (d == 1 || d == 2 || d == 4)))
+                          continue;
+                        const int ox = dx + (b & 1), oy = dy + ((b >> 1) & 1), oz = dz + ((b >> 2) & 1);
+                        s2 += uc[d] * w[1 + oz][1 + oy][1 + ox];
+                      }
+                    y -= al * s2;
+                    sa += al;
+                  }
+                y += ctot * sa * pc;
+                wv[i] = dinv[i] != (T)0 ? y : (T)0;
+                sums[0] += rr[i] * pc;
+                sums[1] += wv[i] * pc;
+                sums[2] += rr[i] * rr[i];
+#pragma unroll
+                for (int yy = 0; yy < 3; ++yy)
+#pragma unroll
+                  for (int xx = 0; xx < 3; ++xx)
+                    {
+                      w[0][yy][xx] = w[1][yy][xx];
+                      w[1][yy][xx] = w[2][yy][xx];
+                    }
+                // keeps the next levels' LDS reads from being hoisted (registers)
+                __builtin_amdgcn_sched_barrier(0);
+              }
+          };
+          if (cube) // uniform per problem
+            stencil(std::true_type());
+          else
+            stencil(std::false_type());
+          // one problem per workgroup: irun and the returned flag are uniform
+          const bool any = problem_sums<T, kWave, NW, 1, 3>(sums, irun, sred, wave, 0, lane);
+          if (irun && (sums[2] <= inner_tol2 || it_total >= a.max_iter))
+            irun = false;
+          if (irun)
+            {
+              const T gamma = sums[0], delta = sums[1];
+              T beta = 0, alpha;
+              if (first)
+                alpha = quotient<T>(gamma, delta);
+              else
+                {
+                  beta = quotient<T>(gamma, gamma_prev);
+                  alpha = quotient<T>(gamma, delta - beta * quotient<T>(gamma, alpha_prev));
+                }
+#pragma unroll
+              for (int i = 0; i < R; ++i)
+                {
+                  const T z = rr[i] * dinv[i];
+                  pr[i] = z + beta * pr[i];
+                  sr[i] = wv[i] + beta * sr[i];
+                  xr[i] += alpha * pr[i];
+                  rr[i] -= alpha * sr[i];
+                }
+              gamma_prev = gamma;
+              alpha_prev = alpha;
+              first = false;
+              ++it_total;
+            }
+          if (!any)
+            break;
         }
 #pragma unroll
       for (int i = 0; i < R; ++i)
@@ -322,7 +342,7 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
           phi[((Z0 + i) * NP + Y) * NP + X] += (double)xr[i];
       __syncthreads();
     }
-  if (tid == 0)
+  if (pvalid && tid == 0)
     {
       const size_t o = (size_t)cell * a.nb + basis;
       if (a.iters)
@@ -340,7 +360,10 @@ hipError_t launch3(const LocalArgs &a, hipStream_t st)
 {
   if (a.n_problems == 0)
     return hipSuccess;
-  hipLaunchKernelGGL((local_solve_3d<T, NF, SZ, R>), dim3(a.n_problems), dim3(NF * NF * SZ), 0, st, a);
+  // a multiple of 8 blocks (xcd_block is a bijection on it); blocks past the
+  // last problem find no valid problem and leave at the first residual test
+  const int grid = (a.n_problems + 7) / 8 * 8;
+  hipLaunchKernelGGL((local_solve_3d<T, NF, SZ, R>), dim3(grid), dim3(NF * NF * SZ), 0, st, a);
   return hipGetLastError();
 }
 template <typename T>

@@ -216,4 +216,58 @@ __device__ __forceinline__ Cond4 cond_tensor(double a, double b, double c, doubl
           d - 0.5 * b};
 }
 
+// Per-problem sums of NV values (type S) over LW lanes x SW waves, plus the OR
+// of a per-problem flag over the whole workgroup (the loop-continuation test).
+// red: [SW][PPW][NV + 1] values of type S.  Identical results in every lane.
+template <typename S, int LW, int SW, int PPW, int NV>
+__device__ __forceinline__ bool problem_sums(S (&v)[NV], bool flag, S *red, int wave, int slot, int pl)
+{
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = group_sum_any<S, LW>(v[k]);
+  if constexpr (SW == 1)
+    return __syncthreads_or(flag);
+  else
+    {
+      if (pl == 0)
+        {
+#pragma unroll
+          for (int k = 0; k < NV; ++k)
+            red[(wave * PPW + slot) * (NV + 1) + k] = v[k];
+          red[(wave * PPW + slot) * (NV + 1) + NV] = flag ? (S)1 : (S)0;
+        }
+      __syncthreads();
+      // lane j of the group holds wave j's partials (j < SW <= 16 <= LW), then a
+      // DPP sum over those lanes: few live registers, identical totals in every lane
+      static_assert(SW <= 16, "at most 16 stacked waves");
+      const bool has = pl < SW;
+      S f = 0;
+#pragma unroll
+      for (int k = 0; k <= NV; ++k)
+        {
+          const S part = has ? red[(pl * PPW + slot) * (NV + 1) + k] : (S)0;
+          S tot;
+          if constexpr (PPW == 1)
+            tot = first_lanes_sum<S, SW>(part);
+          else
+            tot = group_sum_any<S, LW>(part);
+          if (k < NV)
+            v[k] = tot;
+          else
+            f = tot;
+        }
+      return __ballot(f != (S)0) != 0;
+    }
+}
+
+template <typename S>
+__device__ __forceinline__ S quotient(S a, S b)
+{
+  if constexpr (sizeof(S) == 4)
+    return __fdividef(a, b);
+  else
+    return a / b;
+}
+
+
 } // namespace msfem
