@@ -272,6 +272,11 @@ int msfem_gen_grf_log(int n_cells, int nf, uint64_t seed, double corr_len, doubl
  * (transB: [N][K]) BF16 row-major; C FP32 row-major; MFMA with FP32 accumulate. */
 int msfem_nn_gemm_bf16(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
                        float *C, int ldc, int accumulate, void *stream);
+/* The same with the K range split over `splits` workgroup layers (for small
+ * M x N with long K, e.g. weight gradients): work holds splits*M*N floats,
+ * the partial products are summed in a fixed order (deterministic). */
+int msfem_nn_gemm_bf16_splitk(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B,
+                              int ldb, float *C, int ldc, int accumulate, int splits, float *work, void *stream);
 /* col[(c*kh+ky)*kw+kx][(b*Ho+y)*Wo+x] = bf16(in[c][b][y+ky-pad][x+kx-pad]) (0 outside) */
 int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const float *in, void *col, void *stream);
 /* adjoint of im2col (FP32, gather form) */

@@ -27,88 +27,189 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f)
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // C[M][N] (+)= op(A)[M][K] * op(B)[K][N]; A is [M][K] (TA: [K][M]), B is [K][N]
-// (TB: [N][K]), all row-major BF16; C FP32 row-major.  64x64 tile per
-// workgroup, four waves of one 32x32 v_mfma_f32_32x32x8_bf16 block each;
-// K staged through LDS 32 at a time, zero-filled beyond the matrix edges.
-template <bool TA, bool TB>
+// (TB: [N][K]), all row-major BF16; C FP32 row-major.  Tile BM x 128 per
+// workgroup (BM = 32, 64 or 128 after M), four waves of 32x32
+// v_mfma_f32_32x32x8_bf16 blocks; K staged through LDS 32 at a time (k
+// contiguous per row, so each lane's MFMA operand is one 8-byte LDS read),
+// with the next K tile's global loads in flight during the current MFMAs.
+// VEC: 16-byte global loads (leading dimensions multiples of 8, 16-byte
+// aligned bases); otherwise element loads.  Zero fill beyond the edges.
+// Split K (gridDim.z > 1): block z covers K range [z kchunk, (z+1) kchunk) and
+// writes its partial tile to work[z][M][N] for a fixed-order reduction.
+constexpr int kGBN = 128, kGBK = 32, kGPad = 8;
+
+typedef uint16_t bf16x8_t __attribute__((ext_vector_type(8)));
+
+template <int BM, bool TA, bool TB, bool VEC>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, const uint16_t *A, int lda,
-                                                        const uint16_t *B, int ldb, float *C, int ldc, int accumulate)
+                                                        const uint16_t *B, int ldb, float *C, int ldc, int accumulate,
+                                                        int kchunk, float *work)
 {
-  __shared__ uint16_t sA[64][40]; // [m][k], padded rows
-  __shared__ uint16_t sB[64][40]; // [n][k]
-  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  constexpr int LDR = kGBK + kGPad; // LDS row (bf16): 80 bytes
+  __shared__ __align__(16) uint16_t sA[BM][LDR];
+  __shared__ __align__(16) uint16_t sB[kGBN][LDR];
+  constexpr int WM = BM == 32 ? 1 : 2, WN = 4 / WM;          // waves along M, N
+  constexpr int TM = BM / WM / 32, TN = kGBN / WN / 32;      // 32x32 blocks per wave
+  constexpr int ACH = BM * kGBK / 8, BCH = kGBN * kGBK / 8;  // 8-element chunks per tile
+  constexpr int APT = (ACH + 255) / 256, BPT = BCH / 256;     // chunks per thread
+
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * kGBN;
+  const int kb = blockIdx.z * kchunk, ke = min(K, kb + kchunk);
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
-  const int wm = (wave / 2) * 32, wn = (wave % 2) * 32;
-  f32x16 acc;
+  const int wm = (wave / WN) * TM * 32, wn = (wave % WN) * TN * 32;
+
+  // one chunk: 8 consecutive elements of a row of the stored matrix
+  auto load8 = [&](const uint16_t *base, int ld, int row, int rows, int col, int cols) -> bf16x8_t {
+    bf16x8_t v;
+    if (VEC && row < rows && col + 7 < cols)
+      v = *(const bf16x8_t *)(base + (size_t)row * ld + col);
+    else
+      {
 #pragma unroll
-  for (int j = 0; j < 16; ++j)
-    acc[j] = 0.0f;
-  for (int k0 = 0; k0 < K; k0 += 32)
+        for (int e = 0; e < 8; ++e)
+          v[e] = (row < rows && col + e < cols) ? base[(size_t)row * ld + col + e] : (uint16_t)0;
+      }
+    return v;
+  };
+  bf16x8_t ra[APT], rb[BPT];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int t = 0; t < APT; ++t)
+      {
+        const int c = tid + 256 * t;
+        if (c < ACH)
+          {
+            if (!TA) // A[m][k]: chunk along k
+              ra[t] = load8(A, lda, m0 + c / 4, M, k0 + (c % 4) * 8, ke);
+            else // A stored [k][m]: chunk along m
+              ra[t] = load8(A, lda, k0 + c / (BM / 8), ke, m0 + (c % (BM / 8)) * 8, M);
+          }
+      }
+#pragma unroll
+    for (int t = 0; t < BPT; ++t)
+      {
+        const int c = tid + 256 * t;
+        if (TB) // B stored [n][k]: chunk along k
+          rb[t] = load8(B, ldb, n0 + c / 4, N, k0 + (c % 4) * 8, ke);
+        else // B[k][n]: chunk along n
+          rb[t] = load8(B, ldb, k0 + c / (kGBN / 8), ke, n0 + (c % (kGBN / 8)) * 8, N);
+      }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int t = 0; t < APT; ++t)
+      {
+        const int c = tid + 256 * t;
+        if (c < ACH)
+          {
+            if (!TA)
+              *(bf16x8_t *)&sA[c / 4][(c % 4) * 8] = ra[t];
+            else
+              {
+                const int k = c / (BM / 8), mq = (c % (BM / 8)) * 8;
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                  sA[mq + e][k] = ra[t][e];
+              }
+          }
+      }
+#pragma unroll
+    for (int t = 0; t < BPT; ++t)
+      {
+        const int c = tid + 256 * t;
+        if (TB)
+          *(bf16x8_t *)&sB[c / 4][(c % 4) * 8] = rb[t];
+        else
+          {
+            const int k = c / (kGBN / 8), nq = (c % (kGBN / 8)) * 8;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              sB[nq + e][k] = rb[t][e];
+          }
+      }
+  };
+
+  f32x16 acc[TM][TN];
+#pragma unroll
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        acc[i][j][e] = 0.0f;
+
+  if (kb < ke)
     {
-      for (int e = tid; e < 64 * 32; e += 256)
-        {
-          int mm, kk;
-          if (TA)
-            {
-              kk = e / 64;
-              mm = e % 64;
-            }
-          else
-            {
-              mm = e / 32;
-              kk = e % 32;
-            }
-          const int gm = m0 + mm, gk = k0 + kk;
-          uint16_t v = 0;
-          if (gm < M && gk < K)
-            v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-          sA[mm][kk] = v;
-        }
-      for (int e = tid; e < 64 * 32; e += 256)
-        {
-          int nn, kk;
-          if (TB)
-            {
-              nn = e / 32;
-              kk = e % 32;
-            }
-          else
-            {
-              kk = e / 64;
-              nn = e % 64;
-            }
-          const int gn = n0 + nn, gk = k0 + kk;
-          uint16_t v = 0;
-          if (gn < N && gk < K)
-            v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-          sB[nn][kk] = v;
-        }
-      __syncthreads();
+      fetch(kb);
+      stash();
+    }
+  __syncthreads();
+  for (int k0 = kb; k0 < ke; k0 += kGBK)
+    {
+      const bool more = k0 + kGBK < ke;
+      if (more)
+        fetch(k0 + kGBK); // in flight during the MFMAs below
 #pragma unroll
-      for (int ks = 0; ks < 32; ks += 8)
+      for (int ks = 0; ks < kGBK; ks += 8)
         {
-          bf16x4 a, b;
           const int kq = ks + (lane / 32) * 4;
+          bf16x4 av[TM], bv[TN];
 #pragma unroll
-          for (int v = 0; v < 4; ++v)
-            {
-              a[v] = (short)sA[wm + lane % 32][kq + v];
-              b[v] = (short)sB[wn + lane % 32][kq + v];
-            }
-          acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+          for (int i = 0; i < TM; ++i)
+            av[i] = *(const bf16x4 *)&sA[wm + i * 32 + lane % 32][kq];
+#pragma unroll
+          for (int j = 0; j < TN; ++j)
+            bv[j] = *(const bf16x4 *)&sB[wn + j * 32 + lane % 32][kq];
+#pragma unroll
+          for (int i = 0; i < TM; ++i)
+#pragma unroll
+            for (int j = 0; j < TN; ++j)
+              acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
       __syncthreads();
+      if (more)
+        {
+          stash();
+          __syncthreads();
+        }
+    }
+  float *out = C;
+  int ldo = ldc;
+  bool acc_in = accumulate != 0;
+  if (gridDim.z > 1)
+    {
+      out = work + (size_t)blockIdx.z * M * N;
+      ldo = N;
+      acc_in = false;
     }
 #pragma unroll
-  for (int j = 0; j < 16; ++j)
-    {
-      const int row = wm + 8 * (j / 4) + 4 * (lane / 32) + (j % 4), col = wn + lane % 32;
-      const int gm = m0 + row, gn = n0 + col;
-      if (gm < M && gn < N)
+  for (int i = 0; i < TM; ++i)
+#pragma unroll
+    for (int j = 0; j < TN; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
         {
-          float *c = C + (size_t)gm * ldc + gn;
-          *c = accumulate ? *c + acc[j] : acc[j];
+          const int row = m0 + wm + i * 32 + 8 * (e / 4) + 4 * (lane / 32) + (e % 4);
+          const int col = n0 + wn + j * 32 + lane % 32;
+          if (row < M && col < N)
+            {
+              float *c = out + (size_t)row * ldo + col;
+              *c = acc_in ? *c + acc[i][j][e] : acc[i][j][e];
+            }
         }
+}
+
+// C (+)= sum over z of work[z] in fixed order (the split-K partials).
+__global__ void splitk_reduce_kernel(int M, int N, int splits, const float *work, float *C, int ldc, int accumulate)
+{
+  const size_t total = (size_t)M * N;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    {
+      float s = 0.0f;
+      for (int z = 0; z < splits; ++z)
+        s += work[(size_t)z * total + t];
+      float *c = C + (t / N) * (size_t)ldc + t % N;
+      *c = accumulate ? *c + s : s;
     }
 }
 
@@ -130,6 +231,47 @@ __global__ void im2col_kernel(int C, int B, int H, int W, int kh, int kw, int pa
         v = in[(((size_t)ci * B + b) * H + yy) * W + xx];
       col[t] = f32_to_bf16(v);
     }
+}
+
+// The same, 8 consecutive columns n per thread (one 16-byte store) with 32-bit
+// indexing: block row blockIdx.y = col row r = (ci, ky, kx).  When Wo is a
+// multiple of 8 the 8 columns share (b, y) and read one input row segment.
+// Requires N = B Ho Wo to be a multiple of 8 (the host checks).
+__global__ void im2col8_kernel(int B, int H, int W, int kh, int kw, int pad, int Ho, int Wo, const float *in,
+                               uint16_t *col)
+{
+  const int r = blockIdx.y;
+  const int ci = r / (kh * kw), ky = (r / kw) % kh, kx = r % kw;
+  const int N = B * Ho * Wo;
+  const int n0 = 8 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (n0 >= N)
+    return;
+  bf16x8_t v;
+  if (Wo % 8 == 0)
+    {
+      const int x0 = n0 % Wo, rest = n0 / Wo, y = rest % Ho, b = rest / Ho;
+      const int yy = y + ky - pad;
+      const bool rowok = yy >= 0 && yy < H;
+      const float *src = in + ((size_t)(ci * B + b) * H + (rowok ? yy : 0)) * W;
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        {
+          const int xx = x0 + e + kx - pad;
+          v[e] = (rowok && xx >= 0 && xx < W) ? f32_to_bf16(src[xx]) : (uint16_t)0;
+        }
+    }
+  else
+    {
+#pragma unroll
+      for (int e = 0; e < 8; ++e)
+        {
+          const int n = n0 + e, x = n % Wo, rest = n / Wo, y = rest % Ho, b = rest / Ho;
+          const int yy = y + ky - pad, xx = x + kx - pad;
+          v[e] = (yy >= 0 && yy < H && xx >= 0 && xx < W) ? f32_to_bf16(in[((size_t)(ci * B + b) * H + yy) * W + xx])
+                                                         : (uint16_t)0;
+        }
+    }
+  *(bf16x8_t *)(col + (size_t)r * N + n0) = v;
 }
 
 // dx[ci][b][y][x] = sum over (ky, kx) of dcol[(ci*kh+ky)*kw+kx][n(y - ky + pad, x - kx + pad)] (gather: deterministic)
@@ -350,19 +492,67 @@ extern "C" {
 int msfem_nn_gemm_bf16(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B, int ldb,
                        float *C, int ldc, int accumulate, void *stream)
 {
-  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C)
+  return msfem_nn_gemm_bf16_splitk(transA, transB, M, N, K, A, lda, B, ldb, C, ldc, accumulate, 1, nullptr, stream);
+}
+
+int msfem_nn_gemm_bf16_splitk(int transA, int transB, int M, int N, int K, const void *A, int lda, const void *B,
+                              int ldb, float *C, int ldc, int accumulate, int splits, float *work, void *stream)
+{
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C || splits < 1 || (splits > 1 && !work))
     return MSFEM_ERR_ARG;
-  dim3 grid((N + 63) / 64, (M + 63) / 64), block(256);
+  if (lda < (transA ? M : K) || ldb < (transB ? K : N) || ldc < N)
+    return MSFEM_ERR_ARG;
+  int kchunk = K;
+  if (splits > 1)
+    {
+      kchunk = ((K + splits - 1) / splits + kGBK - 1) / kGBK * kGBK;
+      splits = (K + kchunk - 1) / kchunk;
+    }
+  const int BM = M <= 32 ? 32 : (M <= 64 ? 64 : 128);
+  dim3 grid((N + kGBN - 1) / kGBN, (M + BM - 1) / BM, splits), block(256);
+  if (grid.y > 65535 || splits > 65535)
+    return MSFEM_ERR_ARG;
   hipStream_t st = (hipStream_t)stream;
   const uint16_t *a = (const uint16_t *)A, *b = (const uint16_t *)B;
-  if (!transA && !transB)
-    hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, block, 0, st, M, N, K, a, lda, b, ldb, C, ldc, accumulate);
-  else if (!transA && transB)
-    hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, block, 0, st, M, N, K, a, lda, b, ldb, C, ldc, accumulate);
-  else if (transA && !transB)
-    hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, block, 0, st, M, N, K, a, lda, b, ldb, C, ldc, accumulate);
+  const bool vec = lda % 8 == 0 && ldb % 8 == 0 && ((uintptr_t)a % 16) == 0 && ((uintptr_t)b % 16) == 0;
+  float *w = splits > 1 ? work : nullptr;
+  const int sel = (transA ? 1 : 0) | (transB ? 2 : 0) | (vec ? 4 : 0);
+#define MSFEM_GEMM_CASE(BMv)                                                                                           \
+  switch (sel)                                                                                                       \
+    {                                                                                                                \
+      case 0: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, false, false>), grid, block, 0, st, M, N, K, a, lda, b, \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 1: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, true, false, false>), grid, block, 0, st, M, N, K, a, lda, b,  \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 2: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, true, false>), grid, block, 0, st, M, N, K, a, lda, b,  \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 3: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, true, true, false>), grid, block, 0, st, M, N, K, a, lda, b,   \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 4: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, false, true>), grid, block, 0, st, M, N, K, a, lda, b,  \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 5: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, true, false, true>), grid, block, 0, st, M, N, K, a, lda, b,   \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      case 6: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, true, true>), grid, block, 0, st, M, N, K, a, lda, b,   \
+                                 ldb, C, ldc, accumulate, kchunk, w); break;                                         \
+      default: hipLaunchKernelGGL((gemm_bf16_kernel<BMv, true, true, true>), grid, block, 0, st, M, N, K, a, lda, b,   \
+                                  ldb, C, ldc, accumulate, kchunk, w); break;                                        \
+    }
+  if (BM == 32)
+    {
+      MSFEM_GEMM_CASE(32)
+    }
+  else if (BM == 64)
+    {
+      MSFEM_GEMM_CASE(64)
+    }
   else
-    hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, block, 0, st, M, N, K, a, lda, b, ldb, C, ldc, accumulate);
+    {
+      MSFEM_GEMM_CASE(128)
+    }
+#undef MSFEM_GEMM_CASE
+  if (splits > 1)
+    hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for((size_t)M * N)), dim3(256), 0, st, M, N, splits, work, C, ldc,
+                       accumulate);
   return check_launch();
 }
 
@@ -370,9 +560,19 @@ int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const f
 {
   if (C <= 0 || B <= 0 || H <= 0 || W <= 0 || kh <= 0 || kw <= 0 || pad < 0 || !in || !col)
     return MSFEM_ERR_ARG;
-  const size_t n = (size_t)C * kh * kw * B * (H + 2 * pad - kh + 1) * (W + 2 * pad - kw + 1);
-  hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, C, B, H, W, kh, kw, pad, in,
-                     (uint16_t *)col);
+  const int Ho = H + 2 * pad - kh + 1, Wo = W + 2 * pad - kw + 1;
+  if (Ho <= 0 || Wo <= 0)
+    return MSFEM_ERR_ARG;
+  const size_t N = (size_t)B * Ho * Wo, rows = (size_t)C * kh * kw;
+  if (N % 8 == 0 && N < (1u << 30) && rows <= 65535 && ((uintptr_t)col % 16) == 0)
+    {
+      const dim3 grid((unsigned)((N / 8 + 255) / 256), (unsigned)rows);
+      hipLaunchKernelGGL(im2col8_kernel, grid, dim3(256), 0, (hipStream_t)stream, B, H, W, kh, kw, pad, Ho, Wo, in,
+                         (uint16_t *)col);
+    }
+  else
+    hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(rows * N)), dim3(256), 0, (hipStream_t)stream, C, B, H, W, kh, kw,
+                       pad, in, (uint16_t *)col);
   return check_launch();
 }
 

@@ -30,6 +30,7 @@ def _nn():
     if not hasattr(L, "_nn_declared"):
         vp, i32, u64, f32 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint64, ctypes.c_float
         L.msfem_nn_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, vp]
+        L.msfem_nn_gemm_bf16_splitk.argtypes = [i32, i32, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp]
         L.msfem_nn_im2col.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
@@ -127,7 +128,12 @@ class Conv:
         _ok(L.msfem_nn_bias_grad(self.cout, N, dy.ptr, self.gb.ptr, st))
         col = buf.get("col", self.K * N, np.uint16)
         _ok(L.msfem_nn_im2col(self.cin, B, H, W, self.k, self.k, self.pad, x.ptr, col.ptr, st))
-        _ok(L.msfem_nn_gemm_bf16(0, 1, self.cout, self.K, N, dybf.ptr, N, col.ptr, N, self.gw.ptr, self.K, 0, st))
+        # weight gradient: M x N = cout x K small, K = pixels long -> split K
+        tiles = -(-self.cout // (32 if self.cout <= 32 else 64 if self.cout <= 64 else 128)) * -(-self.K // 128)
+        splits = int(max(1, min(512 // tiles, N // 1024)))
+        work = buf.get("splitk", splits * self.cout * self.K, np.float32) if splits > 1 else None
+        _ok(L.msfem_nn_gemm_bf16_splitk(0, 1, self.cout, self.K, N, dybf.ptr, N, col.ptr, N, self.gw.ptr, self.K, 0,
+                                        splits, work.ptr if work is not None else None, st))
         if not want_dx:
             return None
         dcol = buf.get("dcol", self.K * N, np.float32)

@@ -62,6 +62,30 @@ def test_gemm_bf16(ctx, ta, tb, M, N, K):
         assert np.abs(got - want).max() <= 1e-5 * max(np.abs(want).max(), 1.0), (acc, np.abs(got - want).max())
 
 
+@pytest.mark.parametrize("ta,tb", [(0, 0), (0, 1), (1, 0), (1, 1)])
+@pytest.mark.parametrize("M,N,K,splits", [(32, 288, 4100, 4), (256, 2304, 700, 3), (70, 100, 45, 2)])
+def test_gemm_bf16_splitk(ctx, ta, tb, M, N, K, splits):
+    """Split-K form (fixed-order sum of per-slice partials) against numpy."""
+    say(f"gemm split-K ta={ta} tb={tb} M={M} N={N} K={K} splits={splits}")
+    rng = np.random.default_rng(M + N + K + splits + 3 * ta + 5 * tb)
+    A = R.bf16(rng.standard_normal((M, K)))
+    B = R.bf16(rng.standard_normal((K, N)))
+    C0 = rng.standard_normal((M, N)).astype(np.float32)
+    a_store = A.T if ta else A
+    b_store = B.T if tb else B
+    dA, dB = ctx.to_device(bf16_bits(np.ascontiguousarray(a_store))), ctx.to_device(bf16_bits(np.ascontiguousarray(b_store)))
+    work = ctx.alloc(splits * M * N, np.float32)
+    for acc in (0, 1):
+        dC = ctx.to_device(C0)
+        rc = S._nn().msfem_nn_gemm_bf16_splitk(ta, tb, M, N, K, dA.ptr, a_store.shape[1], dB.ptr, b_store.shape[1],
+                                               dC.ptr, N, acc, splits, work.ptr, ctx.stream)
+        assert rc == 0
+        ctx.synchronize()
+        want = A @ B + (C0 if acc else 0.0)
+        got = dC.numpy().reshape(M, N)
+        assert np.abs(got - want).max() <= 1e-5 * max(np.abs(want).max(), 1.0), (acc, np.abs(got - want).max())
+
+
 def test_gemm_rejects_bad_sizes(ctx):
     d = ctx.alloc(16, np.float32)
     assert S._nn().msfem_nn_gemm_bf16(0, 0, 0, 4, 4, d.ptr, 4, d.ptr, 4, d.ptr, 4, 0, ctx.stream) == msfem.ERR_ARG
@@ -126,6 +150,20 @@ def test_elementwise_primitives(ctx):
     assert np.allclose(a2.numpy().reshape(-1), 1.5 * x.reshape(-1))
     assert L.msfem_nn_axpby(x.size, 1.0, dx.ptr, 0.0, None, a2.ptr, st) == 0
     assert np.array_equal(a2.numpy().reshape(-1), x.reshape(-1))
+
+
+@pytest.mark.parametrize("C,B,H,W,k,pad", [(2, 3, 4, 4, 3, 1), (3, 2, 16, 16, 3, 1), (2, 8, 2, 2, 3, 1),
+                                           (4, 2, 8, 8, 2, 0), (1, 5, 6, 10, 3, 1)])
+def test_im2col_geometries(ctx, C, B, H, W, k, pad):
+    """im2col bit-exact against numpy for the wide (8 columns per thread, shared
+    or per-column rows) and the element-wise forms."""
+    say(f"im2col C={C} B={B} H={H} W={W} k={k} pad={pad}")
+    rng = np.random.default_rng(C * 100 + B * 10 + H + W + k)
+    x = rng.standard_normal((C, B, H, W)).astype(np.float32)
+    col, _ = R.im2col(x.astype(np.float64), k, pad)
+    dx, dcol = ctx.to_device(x), ctx.alloc(col.size, np.uint16)
+    assert S._nn().msfem_nn_im2col(C, B, H, W, k, k, pad, dx.ptr, dcol.ptr, ctx.stream) == 0
+    assert np.array_equal(dcol.numpy().reshape(-1), bf16_bits(R.bf16(col)).reshape(-1))
 
 
 def test_mse_and_adam(ctx):
