@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "msfem.h"
+#include "msfem_vtu.hpp"
 
 namespace msfem {
 
@@ -239,6 +240,39 @@ public:
     return filename_;
   }
   void set_output_flag(bool flag) { output_flag_ = flag; }
+  bool output_flag() const { return output_flag_; }
+
+  // diffusion_problem_basis.tpp:393-414: the cell's bases as one VTU file
+  // "2d-basis.<subdomain>.cell-<id>.vtu" (3d-...) in directory `dir`.
+  void output_basis(const std::string &dir = "") const
+  {
+    ensure();
+    char buf[64];
+    snprintf(buf, sizeof buf, "%05u", local_subdomain_);
+    const std::string name = std::string(dim == 2 ? "2d-" : "3d-") + "basis." + buf + ".cell-" + std::to_string(id_) +
+                             ".vtu";
+    const int nf = batch_->n_fine(id_);
+    const size_t nn = (size_t)std::pow(nf + 1, dim);
+    std::vector<vtu::Field> fields;
+    for (int i = 0; i < nb; ++i)
+      fields.push_back({"basis_" + std::to_string(i), batch_->bases(id_) + (size_t)i * nn});
+    vtu::write_cell(join(dir, name), dim, box_.data(), nf, fields);
+  }
+
+  // diffusion_problem_basis.tpp:418-431: the fine solution of the cell
+  // (after set_global_weights) as get_filename_global() in directory `dir`.
+  void output_global_solution_in_cell(const std::string &dir = "")
+  {
+    if (!weights_set_)
+      throw Error(MSFEM_ERR_ARG, "global weights must be set first");
+    vtu::write_cell(join(dir, get_filename_global()), dim, box_.data(), batch_->n_fine(id_),
+                    {{"solution", solution_.data()}});
+  }
+
+  static std::string join(const std::string &dir, const std::string &name)
+  {
+    return dir.empty() ? name : (dir.back() == '/' ? dir + name : dir + "/" + name);
+  }
 
 private:
   void ensure() const
@@ -258,8 +292,8 @@ private:
   std::string filename_;
 };
 
-// The driver around the hot path (include/base/diffusion_problem_ms.tpp:447-491
-// without VTU output): coarse mesh of the unit square/cube, one basis object
+// The driver around the hot path (include/base/diffusion_problem_ms.tpp:447-491):
+// coarse mesh of the unit square/cube, one basis object
 // per cell, coarse assembly with the reference boundary data, FP64 coarse
 // solve, weights back to the cells.
 template <int dim>
@@ -282,7 +316,8 @@ public:
         std::array<double, 2 * dim> b;
         for (int k = 0; k < 2 * dim; ++k)
           b[k] = box[(size_t)c * 2 * dim + k];
-        cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, b, 0, batch_));
+        auto it = cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, b, 0, batch_)).first;
+        it->second.set_output_flag(output_all_);
       }
     for (auto &kv : cells_)
       kv.second.run();
@@ -309,6 +344,31 @@ public:
     for (auto &kv : cells_)
       kv.second.set_global_weights(std::vector<double>(w.begin() + (size_t)kv.first * nb,
                                                        w.begin() + (size_t)(kv.first + 1) * nb));
+    for (auto &kv : cells_) // run() of a cell with the output flag writes its bases
+      if (kv.second.output_flag())
+        kv.second.output_basis(output_dir_);
+  }
+
+  // diffusion_problem_ms.tpp:384-420: every cell's fine solution plus the
+  // master record "solution-ms_fine-2d.pvtu" (3d) naming them, in `dir`.
+  void output_global_fine(const std::string &dir = "")
+  {
+    std::vector<std::string> pieces;
+    for (auto &kv : cells_)
+      {
+        kv.second.output_global_solution_in_cell(dir);
+        pieces.push_back(kv.second.get_filename_global());
+      }
+    vtu::write_pvtu(DiffusionProblemBasis<dim>::join(dir, dim == 2 ? "solution-ms_fine-2d.pvtu"
+                                                                   : "solution-ms_fine-3d.pvtu"),
+                    {"solution"}, pieces);
+  }
+
+  // Bases of every cell written by run() (the reference's per-cell output flag).
+  void set_output_flag(bool flag, const std::string &dir = "")
+  {
+    output_all_ = flag;
+    output_dir_ = dir;
   }
 
   const std::vector<double> &coarse_solution() const { return solution_; }
@@ -323,6 +383,8 @@ private:
   std::map<int, DiffusionProblemBasis<dim>> cells_;
   std::vector<double> solution_;
   int32_t coarse_iterations_ = 0;
+  bool output_all_ = false;
+  std::string output_dir_;
 };
 
 } // namespace msfem

@@ -2,7 +2,8 @@
 // (source/main.cxx:37-39): DiffusionProblemMultiscale<2>(n_refine, n_refine_local).run()
 // on the reference default problem, then writes the coarse solution, the
 // element matrices and CG iteration counts for the Python side to check.
-// Usage: test_facade <n_refine> <n_refine_local> <out.bin>
+// Usage: test_facade <n_refine> <n_refine_local> <out.bin> [<vtu_dir>]
+// (with vtu_dir: also the fine output, DiffusionProblemMultiscale::output_global_fine)
 #include <cstdio>
 #include <cstdlib>
 #include <vector>
@@ -11,9 +12,9 @@
 
 int main(int argc, char **argv)
 {
-  if (argc != 4)
+  if (argc != 4 && argc != 5)
     {
-      std::fprintf(stderr, "usage: %s n_refine n_refine_local out.bin\n", argv[0]);
+      std::fprintf(stderr, "usage: %s n_refine n_refine_local out.bin [vtu_dir]\n", argv[0]);
       return 2;
     }
   const unsigned nr = std::atoi(argv[1]), nrl = std::atoi(argv[2]);
@@ -38,6 +39,8 @@ int main(int argc, char **argv)
           std::fwrite(F.data(), sizeof(double), F.size(), f);
         }
       std::fclose(f);
+      if (argc == 5)
+        ms.output_global_fine(argv[4]);
       std::printf("facade ok: %lld cells, coarse CG %d iterations\n", nc, ms.coarse_iterations());
       // a getter before run() must throw, as the reference asserts on misuse
       msfem::BasisBatch<2> batch(engine);

@@ -37,3 +37,29 @@ def test_facade_reference_pipeline(tmp_path):
     Kc, Fc, cm, cv = O.coarse_assemble(2, nc, r["Ae"], r["Fe"], 1)
     ur = O.coarse_solve(2, nc, Kc, Fc, cm, cv)
     assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-10
+
+
+def test_facade_fine_output(tmp_path):
+    """output_global_fine(): one VTU per cell with the fine MsFEM solution and a
+    PVTU master record naming them; at the coarse vertices the fine solution
+    equals the coarse solution (phi_i(v_j) = delta_ij)."""
+    import xml.etree.ElementTree as ET
+    print("[gpu-case] facade fine output n_refine=1 n_refine_local=4", flush=True)
+    out, vdir = tmp_path / "facade.bin", tmp_path / "vtu"
+    vdir.mkdir()
+    p = subprocess.run([EXE, "1", "4", str(out), str(vdir)], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    raw = out.read_bytes()
+    nu = int(np.frombuffer(raw[:8], np.int64)[0])
+    u = np.frombuffer(raw[8:8 + 8 * nu], np.float64).reshape(3, 3)  # coarse nodes, x fastest
+    master = ET.parse(vdir / "solution-ms_fine-2d.pvtu").getroot()
+    pieces = [pc.get("Source") for pc in master.iter("Piece")]
+    assert len(pieces) == 4
+    for src in pieces:
+        piece = ET.parse(vdir / src).getroot().find("UnstructuredGrid/Piece")
+        assert int(piece.get("NumberOfPoints")) == 289 and int(piece.get("NumberOfCells")) == 256
+        pts = np.array(piece.find("Points/DataArray").text.split(), float).reshape(-1, 3)
+        sol = np.array(piece.find("PointData/DataArray").text.split(), float)
+        for k in (0, 16, 272, 288):  # the cell's corners
+            X, Y = int(round(pts[k, 0] * 2)), int(round(pts[k, 1] * 2))
+            assert abs(sol[k] - u[Y, X]) <= 1e-12 * max(1.0, abs(u[Y, X]))
