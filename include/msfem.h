@@ -232,6 +232,19 @@ int msfem_gather_weights(int dim, int nc, const double *u, double *w);
 int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t *n_fine,
                              const double *bases, const double *w, double *u, void *stream);
 
+/* ---- fine-scale standard FEM ---------------------------------------------
+ * Replaces: DiffusionProblem<2> (include/base/diffusion_problem.tpp:40-279),
+ * the single-scale Q1 solve on the fine mesh that the reference's main runs
+ * beside the MsFEM (source/main.cxx:33-35).  Q1 on the n x n fine mesh of the
+ * unit square with the coefficient tile `coef` (host; one cell of nf = n in
+ * the batch layout of coef_kind), load f (rhs_kind, rhs_value) and boundary
+ * data bc_kind (MSFEM_BC_*); FP64 Jacobi-PCG over many workgroups, stopped at
+ * ||r||_2 <= rel_tol ||r_0||_2 (r_0: residual of the Dirichlet lift) or
+ * max_iter iterations (then MSFEM_ERR_NOCONV, u still written).  u: host
+ * [(n+1)^2] nodal values, lexicographic; 2D only (dim = 2). */
+int msfem_fine_solve(msfem_ctx *ctx, int dim, int n, int coef_kind, const double *coef, int rhs_kind,
+                     double rhs_value, int bc_kind, double rel_tol, int max_iter, double *u, int32_t *iters);
+
 /* ---- synthetic inputs (host, FP64, bit-identical for every consumer) ----- */
 /* Counter-based Philox-4x32-10 (Salmon et al., SC'11): out = Philox(ctr, key). */
 void msfem_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);

@@ -1003,4 +1003,93 @@ int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t
   return MSFEM_OK;
 }
 
+int msfem_fine_solve(msfem_ctx *ctx, int dim, int n, int coef_kind, const double *coef, int rhs_kind,
+                     double rhs_value, int bc_kind, double rel_tol, int max_iter, double *u, int32_t *iters)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if (dim != 2)
+    return fail(ctx, MSFEM_ERR_ARG, "the fine-scale solve is built for dim = 2");
+  if (n < 1 || n > 8192 || !coef || !u || coef_stride(2, coef_kind) < 0 ||
+      (rhs_kind != MSFEM_RHS_CONST && rhs_kind != MSFEM_RHS_SINSIN) ||
+      (bc_kind != MSFEM_BC_ZERO_DIRICHLET && bc_kind != MSFEM_BC_REFERENCE) || !(rel_tol > 0) || max_iter < 0)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid fine-solve arguments");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t ncell = (size_t)n * n, np = n + 1, nn = np * np;
+  HostStage s{ctx};
+  const double *dcoef = s.up("f_coef", coef, ncell * coef_stride(2, coef_kind));
+  double *Ae = s.alloc<double>("f_Ae", ncell * 16, true), *Fe = s.alloc<double>("f_Fe", ncell * 4, true);
+  double *Kc = s.alloc<double>("f_Kc", nn * 9, true), *Fc = s.alloc<double>("f_Fc", nn, true);
+  unsigned char *cm = s.alloc<unsigned char>("f_cm", nn, true);
+  double *cv = s.alloc<double>("f_cv", nn, true);
+  double *vec = s.alloc<double>("f_vec", 5 * nn, true);
+  double *part = s.alloc<double>("f_part", 2 * (size_t)fine_cg_blocks((int)nn), true);
+  FineCgState *state = s.alloc<FineCgState>("f_state", 1, true);
+  if (s.rc)
+    return s.rc;
+  HIP_TRY(ctx, launch_fine_elements_2d(n, coef_kind, dcoef, rhs_kind, rhs_value, Ae, Fe, st));
+  HIP_TRY(ctx, launch_coarse_assemble(2, n, Ae, Fe, bc_kind, Kc, Fc, cm, cv, st));
+  FineCgState init = {};
+  init.rel_tol = rel_tol;
+  init.max_iter = max_iter;
+  HIP_TRY(ctx, hipMemcpyAsync(state, &init, sizeof init, hipMemcpyHostToDevice, st));
+  FineCgArgs a;
+  a.dim = 2;
+  a.np = (int)np;
+  a.nn = (int)nn;
+  a.Kc = Kc;
+  a.Fc = Fc;
+  a.cmask = cm;
+  a.cval = cv;
+  a.x = vec;
+  a.r = vec + nn;
+  a.p = vec + 2 * nn;
+  a.q = vec + 3 * nn;
+  a.dinv = vec + 4 * nn;
+  a.part = part;
+  a.state = state;
+  HIP_TRY(ctx, launch_fine_cg_start(a, st));
+  // CG steps in blocks of kSteps, captured once as a graph and replayed until
+  // the device-side state says done (finished steps turn into no-ops)
+  constexpr int kSteps = 32;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  HIP_TRY(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const hipError_t el = launch_fine_cg_steps(a, kSteps, st);
+  const hipError_t ec = hipStreamEndCapture(st, &g);
+  if (el != hipSuccess || ec != hipSuccess)
+    {
+      if (g)
+        hipGraphDestroy(g);
+      return fail(ctx, MSFEM_ERR_HIP, "fine CG capture: %s", hipGetErrorString(el != hipSuccess ? el : ec));
+    }
+  hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  hipGraphDestroy(g);
+  if (e != hipSuccess)
+    return fail(ctx, MSFEM_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  FineCgState host = {};
+  for (int done_steps = 0;; done_steps += kSteps)
+    {
+      e = hipMemcpyAsync(&host, state, sizeof host, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess)
+        e = hipStreamSynchronize(st);
+      if (e != hipSuccess || host.done || done_steps > max_iter)
+        break;
+      e = hipGraphLaunch(x, st);
+      if (e != hipSuccess)
+        break;
+    }
+  hipGraphExecDestroy(x);
+  if (e != hipSuccess)
+    return fail(ctx, MSFEM_ERR_HIP, "fine CG: %s", hipGetErrorString(e));
+  HIP_TRY(ctx, hipMemcpyAsync(u, a.x, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (iters)
+    *iters = host.iters;
+  if (!host.converged)
+    return fail(ctx, MSFEM_ERR_NOCONV, "fine CG: %d iterations, residual %g > %g", host.iters, host.rnorm, host.tol);
+  return MSFEM_OK;
+}
+
 } // extern "C"

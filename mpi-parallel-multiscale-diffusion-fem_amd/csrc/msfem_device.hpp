@@ -270,4 +270,38 @@ __device__ __forceinline__ S quotient(S a, S b)
 }
 
 
+// QGauss(2) element matrix of fine cell f of a 2D coefficient tile (any kind) on
+// an hx x hy cell (diffusion_problem_basis.tpp:207-222); vertex order (0,0),
+// (1,0), (0,1), (1,1).
+__device__ inline void qg_element_2d(int coef_kind, const double *cf, int f, double hx, double hy, double (&K)[4][4])
+{
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      K[u][v] = 0;
+  const double s = 0.5 / sqrt(3.0);
+  for (int q = 0; q < 4; ++q)
+    {
+      const double xi = (q & 1) ? 0.5 + s : 0.5 - s, eta = (q & 2) ? 0.5 + s : 0.5 - s;
+      double A11, A12, A22;
+      switch (coef_kind)
+        {
+          case 0: A11 = A22 = cf[f]; A12 = 0; break;
+          case 1: A11 = cf[3 * f]; A12 = cf[3 * f + 1]; A22 = cf[3 * f + 2]; break;
+          case 2: A11 = A22 = cf[4 * f + q]; A12 = 0; break;
+          default: A11 = cf[12 * f + 3 * q]; A12 = cf[12 * f + 3 * q + 1]; A22 = cf[12 * f + 3 * q + 2]; break;
+        }
+      double gx[4], gy[4];
+      for (int v = 0; v < 4; ++v)
+        {
+          const double fx = (v & 1) ? xi : 1.0 - xi, fy = (v & 2) ? eta : 1.0 - eta;
+          gx[v] = ((v & 1) ? 1.0 : -1.0) * fy / hx;
+          gy[v] = ((v & 2) ? 1.0 : -1.0) * fx / hy;
+        }
+      const double jxw = 0.25 * hx * hy;
+      for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v)
+          K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
+    }
+}
+
 } // namespace msfem

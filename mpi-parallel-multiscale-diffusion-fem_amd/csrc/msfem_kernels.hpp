@@ -52,4 +52,24 @@ size_t coarse_solve_work_doubles(int dim, int nc);
 hipError_t launch_reconstruct(int n_cells, int nb, const long long *bas_off, const long long *out_off,
                               const int *n_nodes, const double *bases, const double *w, double *u, hipStream_t st);
 
+// fine.hip: fine-scale standard FEM (element matrices + multi-workgroup FP64 CG
+// on a structured stencil matrix, scalars in device memory).
+struct FineCgState {
+  double rz, alpha, beta, rnorm, tol, rel_tol;
+  int iters, max_iter, done, converged;
+};
+struct FineCgArgs {
+  int dim, np, nn;
+  const double *Kc, *Fc;
+  const unsigned char *cmask;
+  const double *cval;
+  double *x, *r, *p, *q, *dinv, *part; // part: 2 * fine_cg_blocks(nn) doubles
+  FineCgState *state;
+};
+int fine_cg_blocks(int nn);
+hipError_t launch_fine_elements_2d(int n, int coef_kind, const double *coef, int rhs_kind, double rhs_value,
+                                   double *Ae, double *Fe, hipStream_t st);
+hipError_t launch_fine_cg_start(const FineCgArgs &a, hipStream_t st);
+hipError_t launch_fine_cg_steps(const FineCgArgs &a, int steps, hipStream_t st);
+
 } // namespace msfem

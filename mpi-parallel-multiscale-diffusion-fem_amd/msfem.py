@@ -31,7 +31,7 @@ EXPORTS = [
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
-    "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
+    "msfem_fine_solve", "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
     "msfem_nn_bias_grad", "msfem_nn_pool2", "msfem_nn_up2", "msfem_nn_axpy", "msfem_nn_axpby", "msfem_nn_to_bf16",
     "msfem_nn_mse", "msfem_nn_adam", "msfem_nn_delta_to_bases", "msfem_nn_bases_to_delta",
 ]
@@ -108,6 +108,7 @@ def lib():
         L.msfem_gen_resolution_classes.argtypes = [i32, ctypes.c_uint64, dbl, vp, vp, vp]
         L.msfem_gen_coef_tensor.argtypes = [i32, vp, ctypes.c_uint64, dbl, vp]
         L.msfem_gen_grf_log.argtypes = [i32, i32, ctypes.c_uint64, dbl, dbl, vp]
+        L.msfem_fine_solve.argtypes = [vp, i32, i32, i32, vp, i32, dbl, i32, dbl, i32, vp, vp]
         _lib = L
     return _lib
 
@@ -286,6 +287,17 @@ class Context:
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         _check(lib().msfem_reconstruct_device(self.h, dim, int(n_fine.shape[0]), _p(n_fine), bases_ptr, w_ptr, u_ptr,
                                               stream), self.h)
+
+    # ---- fine-scale standard FEM (the reference's DiffusionProblem<2>) -----------------
+    def fine_solve(self, n, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, bc_kind=BC_ZERO_DIRICHLET,
+                   rel_tol=1e-12, max_iter=200000, dim=2):
+        """Q1 solve on the n x n fine mesh of the unit square -> ((n+1)^2 nodal values, iterations)."""
+        coef = np.ascontiguousarray(coef, dtype=np.float64)
+        u = np.zeros((n + 1) ** dim)
+        it = ctypes.c_int32()
+        _check(lib().msfem_fine_solve(self.h, dim, n, coef_kind, _p(coef), rhs_kind, rhs_value, bc_kind, rel_tol,
+                                      max_iter, _p(u), ctypes.byref(it)), self.h)
+        return u, it.value
 
 
 class DeviceArray:
