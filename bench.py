@@ -218,6 +218,17 @@ def main():
     result["variants"].update({"coarse_solve_ms": round(coarse_ms, 3), "coarse_cg_iterations": int(t_cit.numpy()[0]),
                                "graph_step_device_ms": round(graph_dev_ms, 4),
                                "stream_step_ms": round(stream_ms, 4)})
+    if args.workload == "reference":
+        # the reference's main also runs the single-scale Q1 solve on the fine mesh
+        # (source/main.cxx:33-35): 8 * 128 = 1024 fine cells per side, same data
+        nfine = nc * int(nf_arr[0])
+        unit = np.array([[[0.0, 1.0], [0.0, 1.0]]])
+        fcoef = msfem.coef_reference(2, np.array([nfine], np.int32), unit)
+        tf0 = time.perf_counter()
+        _, fit = ctx.fine_solve(nfine, msfem.COEF_TENSOR_QP, fcoef, msfem.RHS_CONST, 2.0, msfem.BC_REFERENCE,
+                                rel_tol=1e-12)
+        result["variants"].update({"fine_fem_%d_solve_ms_incl_upload" % nfine:
+                                   round((time.perf_counter() - tf0) * 1e3, 2), "fine_fem_cg_iterations": fit})
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(dim, case["box"], nf_arr, case["coef_kind"], case["coef"],
