@@ -67,6 +67,10 @@ def workload(name):
     if name == "periodic8":
         return (C.periodic_cells(nc=8, nf=32), 8, msfem.PREC_FP64, 0,
                 "8^2 coarse cells x 32^2 fine, alpha=2+sin(2 pi x/eps)cos(2 pi y/eps) eps=1/64, f=1, 256 local solves")
+    if name == "quad64":
+        return (C.quad_cells(nc=64, nf=64, amp=0.24, seed=9), 64, msfem.PREC_FP64, 0,
+                "64^2 general quadrilateral coarse cells (unit-square mesh, interior vertices moved up to 0.24 H) "
+                "x 64^2 fine Q1 per cell with the bilinear mapping, lognormal alpha, f=1, 16384 local solves")
     if name == "reference":
         return (C.reference_cells(nc=8, nf=128), 8, msfem.PREC_FP64, 1,
                 "reference default: 8^2 coarse cells x 128^2 fine, MatrixCoeff (PI_D) at QGauss(2) points, f=2, "
@@ -80,7 +84,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lognormal64",
-                    choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference", "surrogate128"])
+                    choices=["lognormal64", "tensor128", "lognormal3d", "periodic8", "reference", "quad64",
+                             "surrogate128"])
     ap.add_argument("--cpu-sample", type=int, default=2048, help="cells in the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -106,7 +111,9 @@ def main():
     n_bases = msfem.bases_size(dim, nf_arr)
 
     ctx = msfem.Context(0)
-    t_box = ctx.to_device(np.asarray(case["box"]).reshape(-1))
+    quads = case.get("vertices")
+    t_box = None if quads is not None else ctx.to_device(np.asarray(case["box"]).reshape(-1))
+    t_vert = None if quads is None else ctx.to_device(np.asarray(quads).reshape(-1))
     t_coef = ctx.to_device(case["coef"])
     t_Ae = ctx.alloc(n * nb * nb, np.float64)
     t_Fe = ctx.alloc(n * nb, np.float64)
@@ -125,9 +132,10 @@ def main():
     stream = ctx.stream
 
     def step(opts):
-        ctx.batch_build_device(dim, case["n_fine"], t_box.ptr, case["coef_kind"], t_coef.ptr, t_Ae.ptr, t_Fe.ptr,
-                               t_bases.ptr, t_it.ptr, t_st.ptr, rhs_kind=case["rhs_kind"],
-                               rhs_value=case["rhs_value"], opts=opts, stream=stream)
+        ctx.batch_build_device(dim, case["n_fine"], t_box.ptr if t_box is not None else None, case["coef_kind"], t_coef.ptr,
+                               t_Ae.ptr, t_Fe.ptr, t_bases.ptr, t_it.ptr, t_st.ptr, rhs_kind=case["rhs_kind"],
+                               rhs_value=case["rhs_value"], opts=opts, stream=stream,
+                               vertices_ptr=t_vert.ptr if t_vert is not None else None)
         ctx.coarse_assemble_device(dim, nc, t_Ae.ptr, t_Fe.ptr, bc_kind, t_Kc.ptr, t_Fc.ptr, t_cm.ptr, t_cv.ptr,
                                    stream=stream)
 
@@ -232,7 +240,7 @@ def main():
 
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = cpu_baseline(dim, case["box"], nf_arr, case["coef_kind"], case["coef"],
-                                              case["rhs_kind"], case["rhs_value"], n, args.cpu_sample)
+                                              case["rhs_kind"], case["rhs_value"], n, args.cpu_sample, quads)
     ctx.close()
     print(json.dumps(result))
 
@@ -253,7 +261,7 @@ def copy_calibration(mib):
     ctx.close()
 
 
-def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, sample_cells):
+def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, sample_cells, quads=None):
     """The oracle's restatement of the reference recipe (CG + SSOR(1.6), abs tol
     1e-12, 1000 iterations) on an evenly spaced sample of the cells."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
@@ -265,8 +273,12 @@ def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, samp
     sub_coef = np.concatenate([coef[offs[i]:offs[i + 1]] for i in sample])
     cores = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    r = O.batch_build(dim, np.asarray(box)[sample], nf_arr[sample], coef_kind, sub_coef, rhs_kind, rhs_value,
-                      solver=O.SOLVER_SSOR_PCG, n_threads=cores)
+    if quads is not None:
+        r = O.batch_build_quad(np.asarray(quads)[sample], nf_arr[sample], coef_kind, sub_coef, rhs_kind, rhs_value,
+                               solver=O.SOLVER_SSOR_PCG, n_threads=cores)
+    else:
+        r = O.batch_build(dim, np.asarray(box)[sample], nf_arr[sample], coef_kind, sub_coef, rhs_kind, rhs_value,
+                          solver=O.SOLVER_SSOR_PCG, n_threads=cores)
     t = time.perf_counter() - t0
     return {"value": round(len(sample) / t, 2), "unit": "cells/s", "cores": cores, "kind": "port",
             "sample": f"{len(sample)} of the {n} cells (evenly spaced), reference recipe CG+SSOR(1.6), abs tol "

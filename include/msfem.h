@@ -27,7 +27,14 @@
  *   - dim is 2 or 3; nb = 2^dim bases per coarse cell; basis i belongs to
  *     coarse vertex i, lexicographic with x fastest (deal.II vertex order).
  *   - a coarse cell is an axis-aligned box: box[c] = {lo_x, hi_x, lo_y, hi_y
- *     [, lo_z, hi_z]}.
+ *     [, lo_z, hi_z]}, or (2D, msfem_batch.vertices) a general convex
+ *     quadrilateral given by its four vertices in deal.II order; its fine grid
+ *     is the bilinear image of the uniform nf x nf grid (what
+ *     GridGenerator::general_cell + refine_global produce,
+ *     include/base/diffusion_problem_basis.tpp:92-99), every fine cell is
+ *     integrated with the bilinear mapping, and the Dirichlet data is BasisQ1's
+ *     N_i(x) = c_0i + c_1i x + c_2i y + c_3i xy with N_i(v_j) = delta_ij
+ *     (include/coefficients/basis_q1.tpp:26-46, :86-96) at the mapped nodes.
  *   - a cell with nf fine cells per side has (nf+1)^dim fine nodes,
  *     lexicographic, x fastest.  Its bases are stored as [nb][(nf+1)^dim],
  *     cells concatenated in batch order.
@@ -51,7 +58,7 @@
 extern "C" {
 #endif
 
-#define MSFEM_API_VERSION 1
+#define MSFEM_API_VERSION 2
 
 /* status codes */
 #define MSFEM_OK 0
@@ -120,6 +127,11 @@ typedef struct msfem_batch {
                                  values only are read; a non-finite value is taken as 0.
                                  A problem whose initial FP64 residual already meets the
                                  tolerance takes 0 iterations (accepted as given). */
+  const double *vertices;     /* optional [n_cells][4][2]: general quadrilateral cells
+                                 (2D only; box is then ignored and may be NULL), deal.II
+                                 vertex order (0,0),(1,0),(0,1),(1,1) of the parametric
+                                 square; each cell convex with positive orientation.
+                                 NULL: the cells are the boxes. */
 } msfem_batch;
 
 typedef struct msfem_batch_out {
@@ -254,6 +266,15 @@ int msfem_gen_unit_boxes(int dim, int nc, double *box);
  * k = 57, scale 0.9999, rotation by PI_D/3; include/coefficients/matrix_coeff.hpp:45-49,
  * matrix_coeff.tpp:17-91) at the QGauss(2) points -> TENSOR_QP tiles. */
 int msfem_gen_coef_reference(int dim, int n_cells, const int32_t *n_fine, const double *box, double *coef);
+/* Unit square split into nc x nc general quadrilaterals: the interior vertices
+ * of the uniform mesh are moved by amp/nc * (2u - 1) per axis, u uniform from
+ * Philox(ctr={vertex, 0, 2, 0}, key=seed) (0 <= amp < 0.25 keeps every cell
+ * convex) -> vertices[nc*nc][4][2], deal.II vertex order, cells x fastest.
+ * Test and benchmark meshes for the general-cell path. */
+int msfem_gen_perturbed_quads(int nc, double amp, uint64_t seed, double *vertices);
+/* MatrixCoeff<2> at the mapped QGauss(2) points of the fine cells of general
+ * quadrilaterals (vertices[n_cells][4][2]) -> TENSOR_QP tiles. */
+int msfem_gen_coef_reference_quad(int n_cells, const int32_t *n_fine, const double *vertices, double *coef);
 /* alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps) at the QGauss(2) points -> SCALAR_QP tiles (2D). */
 int msfem_gen_coef_periodic(int n_cells, const int32_t *n_fine, const double *box, double eps, double *coef);
 /* alpha = exp(clamp(g, -clip, clip)), g ~ N(0,1) i.i.d. per fine cell -> SCALAR_CELL tiles.

@@ -12,8 +12,10 @@
 // registered cell in one msfem_batch_build call on the GPU.  The call sequence
 // of the reference driver is unchanged.
 //
-// Differences forced by the data path: a cell is described by its box instead
-// of a deal.II cell iterator, the communicator argument is gone (the local
+// Differences forced by the data path: a cell is described by its box, or (2D)
+// by its four vertices for a general quadrilateral, instead of a deal.II cell
+// iterator (the reference reads the vertices from the iterator,
+// diffusion_problem_basis.tpp:43-49), the communicator argument is gone (the local
 // problem never used it, diffusion_problem_basis.hpp:198), FullMatrix<double>
 // and Vector<double> become row-major std::vector<double>, and a failed solve
 // throws msfem::NoConvergence (the reference throws deal.II's
@@ -21,6 +23,7 @@
 #ifndef MSFEM_BASIS_HPP_
 #define MSFEM_BASIS_HPP_
 
+#include <algorithm>
 #include <array>
 #include <cmath>
 #include <map>
@@ -76,6 +79,9 @@ struct ProblemData {
   // Fills the concatenated coefficient tiles of the given cells; nullptr =
   // the reference coefficient.
   void (*fill)(int dim, int n_cells, const int32_t *n_fine, const double *box, double *coef) = nullptr;
+  // The same for a batch of general quadrilaterals (vertices [n][4][2]); a
+  // batch with such cells and a custom coefficient needs it.
+  void (*fill_quad)(int n_cells, const int32_t *n_fine, const double *vertices, double *coef) = nullptr;
 };
 
 template <int dim>
@@ -98,7 +104,37 @@ public:
   int add(unsigned n_refine_local, const std::array<double, 2 * dim> &box)
   {
     boxes_.push_back(box);
+    std::array<double, nb * dim> v;
+    for (int i = 0; i < nb; ++i)
+      for (int k = 0; k < dim; ++k)
+        v[i * dim + k] = box[2 * k + ((i >> k) & 1)];
+    verts_.push_back(v);
     n_fine_.push_back(1 << n_refine_local);
+    built_ = false;
+    return (int)n_fine_.size() - 1;
+  }
+  // A general quadrilateral (2D): its vertices in deal.II order.  The whole
+  // batch is then built as general cells (boxes as their four vertices).
+  int add_general(unsigned n_refine_local, const std::vector<std::array<double, dim>> &corner_points)
+  {
+    if (dim != 2 || (int)corner_points.size() != nb)
+      throw Error(MSFEM_ERR_ARG, "general cells: 2D only, four corner points");
+    std::array<double, 2 * dim> bbox;
+    std::array<double, nb * dim> v;
+    for (int k = 0; k < dim; ++k)
+      {
+        bbox[2 * k] = bbox[2 * k + 1] = corner_points[0][k];
+        for (int i = 0; i < nb; ++i)
+          {
+            v[i * dim + k] = corner_points[i][k];
+            bbox[2 * k] = std::min(bbox[2 * k], corner_points[i][k]);
+            bbox[2 * k + 1] = std::max(bbox[2 * k + 1], corner_points[i][k]);
+          }
+      }
+    boxes_.push_back(bbox); // bounding box: unused by a general batch
+    verts_.push_back(v);
+    n_fine_.push_back(1 << n_refine_local);
+    general_ = true;
     built_ = false;
     return (int)n_fine_.size() - 1;
   }
@@ -126,8 +162,20 @@ public:
         offsets_[c + 1] = offsets_[c] + (size_t)nb * (size_t)std::pow(n_fine_[c] + 1, dim);
       }
     nbases = offsets_[n];
-    std::vector<double> coef(ncoef);
-    if (data_.fill)
+    std::vector<double> coef(ncoef), vert;
+    if (general_)
+      {
+        vert.resize((size_t)n * nb * dim);
+        for (int c = 0; c < n; ++c)
+          std::copy(verts_[c].begin(), verts_[c].end(), vert.begin() + (size_t)c * nb * dim);
+        if (data_.fill_quad)
+          data_.fill_quad(n, n_fine_.data(), vert.data(), coef.data());
+        else if (data_.fill)
+          throw Error(MSFEM_ERR_ARG, "a batch of general cells needs ProblemData::fill_quad");
+        else
+          check(msfem_gen_coef_reference_quad(n, n_fine_.data(), vert.data(), coef.data()), nullptr);
+      }
+    else if (data_.fill)
       data_.fill(dim, n, n_fine_.data(), box.data(), coef.data());
     else
       check(msfem_gen_coef_reference(dim, n, n_fine_.data(), box.data(), coef.data()), nullptr);
@@ -136,14 +184,16 @@ public:
     bases_.assign(nbases, 0.0);
     iters_.assign((size_t)n * nb, 0);
     msfem_batch in{dim, n, n_fine_.data(), box.data(), data_.coef_kind, coef.data(), data_.rhs_kind, data_.rhs_value,
-                   nullptr};
+                   nullptr, general_ ? vert.data() : nullptr};
     msfem_batch_out out{Ae_.data(), Fe_.data(), bases_.data(), iters_.data(), nullptr, nullptr};
     check(msfem_batch_build(engine_.get(), &in, &opts_, &out), engine_.get());
     built_ = true;
   }
 
   bool built() const { return built_; }
+  bool general() const { return general_; }
   int n_fine(int id) const { return n_fine_[id]; }
+  const double *vertices(int id) const { return verts_[id].data(); }
   const double *element_matrix(int id) { return Ae_.data() + (size_t)id * nb * nb; }
   const double *element_rhs(int id) { return Fe_.data() + (size_t)id * nb; }
   const double *bases(int id) { return bases_.data() + offsets_[id]; }
@@ -154,6 +204,8 @@ private:
   ProblemData data_;
   msfem_solver_opts opts_;
   std::vector<std::array<double, 2 * dim>> boxes_;
+  std::vector<std::array<double, nb * dim>> verts_;
+  bool general_ = false;
   std::vector<int32_t> n_fine_;
   std::vector<size_t> offsets_;
   std::vector<double> Ae_, Fe_, bases_;
@@ -171,6 +223,14 @@ public:
   DiffusionProblemBasis(unsigned n_refine_local, const std::array<double, 2 * dim> &box, unsigned local_subdomain,
                         BasisBatch<dim> &batch)
     : batch_(&batch), id_(batch.add(n_refine_local, box)), local_subdomain_(local_subdomain), box_(box)
+  {
+  }
+  // A general quadrilateral (2D): the cell's vertices in deal.II order, as
+  // corner_points in the reference (diffusion_problem_basis.tpp:43-49).
+  DiffusionProblemBasis(unsigned n_refine_local, const std::vector<std::array<double, dim>> &corner_points,
+                        unsigned local_subdomain, BasisBatch<dim> &batch)
+    : batch_(&batch), id_(batch.add_general(n_refine_local, corner_points)), local_subdomain_(local_subdomain),
+      box_()
   {
   }
   DiffusionProblemBasis(const DiffusionProblemBasis &) = default;
@@ -256,7 +316,7 @@ public:
     std::vector<vtu::Field> fields;
     for (int i = 0; i < nb; ++i)
       fields.push_back({"basis_" + std::to_string(i), batch_->bases(id_) + (size_t)i * nn});
-    vtu::write_cell(join(dir, name), dim, box_.data(), nf, fields);
+    vtu::write_cell(join(dir, name), dim, box_.data(), nf, fields, geometry());
   }
 
   // diffusion_problem_basis.tpp:418-431: the fine solution of the cell
@@ -266,7 +326,7 @@ public:
     if (!weights_set_)
       throw Error(MSFEM_ERR_ARG, "global weights must be set first");
     vtu::write_cell(join(dir, get_filename_global()), dim, box_.data(), batch_->n_fine(id_),
-                    {{"solution", solution_.data()}});
+                    {{"solution", solution_.data()}}, geometry());
   }
 
   static std::string join(const std::string &dir, const std::string &name)
@@ -275,6 +335,9 @@ public:
   }
 
 private:
+  // vertices of a general cell for the VTU writer (nullptr: the box)
+  const double *geometry() const { return batch_->general() ? batch_->vertices(id_) : nullptr; }
+
   void ensure() const
   {
     if (!ran_)
@@ -308,16 +371,32 @@ public:
   void run()
   {
     const int nc = 1 << n_refine_, n = (int)std::pow(nc, dim), nb = 1 << dim;
-    std::vector<double> box((size_t)n * 2 * dim);
+    std::vector<double> box((size_t)n * 2 * dim), quad;
     check(msfem_gen_unit_boxes(dim, nc, box.data()), nullptr);
+    if (distortion_ > 0)
+      {
+        quad.resize((size_t)n * 8);
+        check(msfem_gen_perturbed_quads(nc, distortion_, distortion_seed_, quad.data()), nullptr);
+      }
     cells_.clear();
     for (int c = 0; c < n; ++c) // initialize_and_compute_basis (diffusion_problem_ms.tpp:42-88)
       {
-        std::array<double, 2 * dim> b;
-        for (int k = 0; k < 2 * dim; ++k)
-          b[k] = box[(size_t)c * 2 * dim + k];
-        auto it = cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, b, 0, batch_)).first;
-        it->second.set_output_flag(output_all_);
+        if (distortion_ > 0)
+          {
+            std::vector<std::array<double, dim>> corners(nb);
+            for (int i = 0; i < nb; ++i)
+              for (int k = 0; k < dim; ++k)
+                corners[i][k] = quad[(size_t)c * 8 + 2 * i + k];
+            cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, corners, 0, batch_));
+          }
+        else
+          {
+            std::array<double, 2 * dim> b;
+            for (int k = 0; k < 2 * dim; ++k)
+              b[k] = box[(size_t)c * 2 * dim + k];
+            cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, b, 0, batch_));
+          }
+        cells_.at(c).set_output_flag(output_all_);
       }
     for (auto &kv : cells_)
       kv.second.run();
@@ -364,6 +443,18 @@ public:
                     {"solution"}, pieces);
   }
 
+  // Coarse mesh with its interior vertices moved by up to amp/nc per axis
+  // (msfem_gen_perturbed_quads; 2D, 0 < amp < 0.25): every cell is then a
+  // general quadrilateral.  The boundary stays the unit square's, so the
+  // coarse boundary data is unchanged.  amp = 0: the uniform mesh.
+  void set_mesh_distortion(double amp, uint64_t seed)
+  {
+    if (amp != 0 && (dim != 2 || !(amp > 0 && amp < 0.25)))
+      throw Error(MSFEM_ERR_ARG, "mesh distortion: 2D only, 0 < amp < 0.25");
+    distortion_ = amp;
+    distortion_seed_ = seed;
+  }
+
   // Bases of every cell written by run() (the reference's per-cell output flag).
   void set_output_flag(bool flag, const std::string &dir = "")
   {
@@ -385,6 +476,8 @@ private:
   int32_t coarse_iterations_ = 0;
   bool output_all_ = false;
   std::string output_dir_;
+  double distortion_ = 0;
+  uint64_t distortion_seed_ = 0;
 };
 
 } // namespace msfem

@@ -1,4 +1,4 @@
-// msfem_vtu.hpp -- VTK XML output of fine-scale fields on a box cell.
+// msfem_vtu.hpp -- VTK XML output of fine-scale fields on a coarse cell.
 //
 // Host code only (no GPU).  Replaces the reference's deal.II DataOut output of
 // the per-cell fine solution and bases and of the parallel master record:
@@ -9,7 +9,8 @@
 // A cell with nf fine cells per side is written as an UnstructuredGrid of
 // nf^dim VTK_QUAD (2D) or VTK_HEXAHEDRON (3D) elements on its (nf+1)^dim
 // lexicographic fine nodes (x fastest), with the given nodal fields as
-// Float64 PointData, ASCII.
+// Float64 PointData, ASCII.  The cell is a box or (2D) a general quadrilateral
+// whose fine nodes are the bilinear image of the uniform grid.
 #ifndef MSFEM_VTU_HPP_
 #define MSFEM_VTU_HPP_
 
@@ -27,8 +28,10 @@ struct Field {
   const double *values; // (nf+1)^dim nodal values, lexicographic
 };
 
-// box: {lo_x, hi_x, lo_y, hi_y[, lo_z, hi_z]}
-inline void write_cell(const std::string &path, int dim, const double *box, int nf, const std::vector<Field> &fields)
+// box: {lo_x, hi_x, lo_y, hi_y[, lo_z, hi_z]}; quad (2D, or nullptr): four
+// vertices {x0, y0, ..., x3, y3} in deal.II order, used instead of the box.
+inline void write_cell(const std::string &path, int dim, const double *box, int nf, const std::vector<Field> &fields,
+                       const double *quad = nullptr)
 {
   if ((dim != 2 && dim != 3) || nf <= 0)
     throw std::invalid_argument("vtu::write_cell: dim must be 2 or 3 and nf > 0");
@@ -43,11 +46,25 @@ inline void write_cell(const std::string &path, int dim, const double *box, int 
                   "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n");
   std::fprintf(f, "<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", npts, ncells);
   std::fprintf(f, "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
-  const double hx = (box[1] - box[0]) / nf, hy = (box[3] - box[2]) / nf, hz = dim == 3 ? (box[5] - box[4]) / nf : 0.0;
   for (long long k = 0; k < npts; ++k)
     {
       const int X = (int)(k % np), Y = (int)((k / np) % np), Z = dim == 3 ? (int)(k / ((long long)np * np)) : 0;
-      std::fprintf(f, "%.17g %.17g %.17g\n", box[0] + X * hx, box[2] + Y * hy, dim == 3 ? box[4] + Z * hz : 0.0);
+      double x, y, z = 0.0;
+      if (quad)
+        {
+          const double xi = (double)X / nf, eta = (double)Y / nf;
+          const double w[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta};
+          x = w[0] * quad[0] + w[1] * quad[2] + w[2] * quad[4] + w[3] * quad[6];
+          y = w[0] * quad[1] + w[1] * quad[3] + w[2] * quad[5] + w[3] * quad[7];
+        }
+      else
+        {
+          x = box[0] + X * ((box[1] - box[0]) / nf);
+          y = box[2] + Y * ((box[3] - box[2]) / nf);
+          if (dim == 3)
+            z = box[4] + Z * ((box[5] - box[4]) / nf);
+        }
+      std::fprintf(f, "%.17g %.17g %.17g\n", x, y, z);
     }
   std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" "
                   "format=\"ascii\">\n");

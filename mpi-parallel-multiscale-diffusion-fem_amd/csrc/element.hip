@@ -31,7 +31,8 @@ __device__ __forceinline__ int sym_index(int dim, int i, int j)
   return 5;
 }
 
-template <int DIM>
+// QUAD: general quadrilaterals (2D, a.verts), geometry per fine cell
+template <int DIM, bool QUAD>
 __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
 {
   constexpr int NV = 1 << DIM, NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
@@ -41,7 +42,11 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
 
   const int cell = a.cells[blockIdx.x];
   const int nf = a.nf, np = nf + 1;
-  const double *bx = a.box + (size_t)cell * 2 * DIM;
+  // general quadrilateral (2D): geometry per fine cell from the bilinear map;
+  // the box data below is then unused (unit placeholders)
+  const double *vq = QUAD ? a.verts + (size_t)cell * 8 : nullptr;
+  const double unit_box[6] = {0, 1, 0, 1, 0, 1};
+  const double *bx = vq ? unit_box : a.box + (size_t)cell * 2 * DIM;
   double h[3] = {1, 1, 1}, lo[3] = {0, 0, 0};
 #pragma unroll
   for (int k = 0; k < DIM; ++k)
@@ -114,7 +119,17 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
     {
       const int fx = f % nf, fy = (f / nf) % nf, fz = DIM == 3 ? f / (nf * nf) : 0;
       double Ke[NV * NV];
-      if (a.coef_kind == 0 || a.coef_kind == 1)
+      double be[NV];
+      if constexpr (QUAD)
+        {
+          static_assert(DIM == 2, "general cells are 2D");
+          double K4[4][4];
+          qg_element_quad(a.coef_kind, cf, f, nf, QuadMap(vq), K4, a.rhs_kind, a.rhs_value, be);
+#pragma unroll
+          for (int ab = 0; ab < 16; ++ab)
+            Ke[ab] = K4[ab / 4][ab % 4];
+        }
+      else if (a.coef_kind == 0 || a.coef_kind == 1)
         {
           double cs[NS];
           if (a.coef_kind == 0)
@@ -174,27 +189,29 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
                   }
             }
         }
-      // load vector of the fine cell
-      double be[NV];
-#pragma unroll
-      for (int aa = 0; aa < NV; ++aa)
-        be[aa] = 0;
-      for (int q = 0; q < NQ; ++q)
+      // load vector of the fine cell (box cells)
+      if constexpr (!QUAD)
         {
-          double fv = a.rhs_value;
-          if (a.rhs_kind == 1)
-            {
-              fv = 1.0;
-              const int fi[3] = {fx, fy, fz};
-              for (int k = 0; k < DIM; ++k)
-                {
-                  const double xi = ((q >> k) & 1) ? 0.5 + gs : 0.5 - gs;
-                  fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * xi));
-                }
-            }
 #pragma unroll
           for (int aa = 0; aa < NV; ++aa)
-            be[aa] += shape(aa, q) * fv * jxw;
+            be[aa] = 0;
+          for (int q = 0; q < NQ; ++q)
+            {
+              double fv = a.rhs_value;
+              if (a.rhs_kind == 1)
+                {
+                  fv = 1.0;
+                  const int fi[3] = {fx, fy, fz};
+                  for (int k = 0; k < DIM; ++k)
+                    {
+                      const double xi = ((q >> k) & 1) ? 0.5 + gs : 0.5 - gs;
+                      fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * xi));
+                    }
+                }
+#pragma unroll
+              for (int aa = 0; aa < NV; ++aa)
+                be[aa] += shape(aa, q) * fv * jxw;
+            }
         }
       // bases at the fine cell's vertices
       double pv[NV][NV]; // [basis][vertex]
@@ -433,12 +450,14 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
 {
   if (a.n_cells_class == 0)
     return hipSuccess;
-  if (a.dim == 2)
-    hipLaunchKernelGGL(element_kernel<2>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  if (a.dim == 2 && a.verts)
+    hipLaunchKernelGGL((element_kernel<2, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else if (a.dim == 2)
+    hipLaunchKernelGGL((element_kernel<2, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.coef_kind == 0) // scalar per fine cell
     hipLaunchKernelGGL(element3d_kernel, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else
-    hipLaunchKernelGGL(element_kernel<3>, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+    hipLaunchKernelGGL((element_kernel<3, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   return hipGetLastError();
 }
 

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -33,6 +34,7 @@ struct DevBuf {
 // Batch layout derived from n_fine: classes by fine resolution and offsets.
 struct Plan {
   int dim = 0, coef_kind = -1, nb = 0;
+  bool general = false;                    // general quadrilaterals (msfem_batch.vertices)
   std::vector<int32_t> n_fine;
   std::vector<int> class_nf;               // distinct resolutions
   std::vector<std::vector<int>> class_cells;
@@ -143,19 +145,38 @@ bool nf_supported(int dim, int nf)
   return nf == 8 || nf == 16;
 }
 
-// Local-solve storage mode of a class.
-int class_mode(int dim, int coef_kind, int nf)
+// Local-solve storage mode of a class.  Scalar and tensor coefficients per
+// fine cell of box cells are cached in registers (up to 64^2); everything
+// else -- QP coefficients, general quadrilaterals (whose geometry lives in the
+// precomputed couplings) and the 128^2 tensor tiles -- streams node couplings
+// from planes (64^2 and 128^2) or reads per-fine-cell conductances (16^2,
+// 32^2, whose lane groups hold several problems per workgroup).
+int class_mode(int dim, int coef_kind, int nf, bool general)
 {
   if (dim == 3)
     return kModeScalar;
-  if (coef_kind == MSFEM_COEF_SCALAR_CELL)
+  if (!general && coef_kind == MSFEM_COEF_SCALAR_CELL)
     return kModeScalar;
-  if (coef_kind == MSFEM_COEF_TENSOR_CELL && nf <= 64)
+  if (!general && coef_kind == MSFEM_COEF_TENSOR_CELL && nf <= 64)
     return kModeTensor;
-  // 128^2 tiles of a tensor or QP coefficient: node couplings streamed from planes
-  if (dim == 2 && nf == 128)
+  if (nf >= 64)
     return kModeStream;
   return kModeGlobal;
+}
+
+// A quadrilateral (deal.II vertex order) whose bilinear map has a positive
+// Jacobian determinant everywhere: the determinant is affine in each parametric
+// variable, so its four corner values decide (convex, positively oriented).
+bool quad_ok(const double *v)
+{
+  const double ax = v[2] - v[0], ay = v[3] - v[1], bx = v[4] - v[0], by = v[5] - v[1];
+  const double cx = v[6] - v[4] - v[2] + v[0], cy = v[7] - v[5] - v[3] + v[1];
+  auto det = [&](double xi, double eta) { return (ax + eta * cx) * (by + xi * cy) - (ay + eta * cy) * (bx + xi * cx); };
+  const double d[4] = {det(0, 0), det(1, 0), det(0, 1), det(1, 1)};
+  for (double x : d)
+    if (!(x > 0) || !std::isfinite(x))
+      return false;
+  return true;
 }
 
 int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
@@ -164,8 +185,10 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
     return fail(ctx, MSFEM_ERR_ARG, "null batch or options");
   if (in->dim != 2 && in->dim != 3)
     return fail(ctx, MSFEM_ERR_ARG, "dim must be 2 or 3 (got %d)", in->dim);
-  if (in->n_cells < 0 || (in->n_cells > 0 && (!in->n_fine || !in->box || !in->coef)))
+  if (in->n_cells < 0 || (in->n_cells > 0 && (!in->n_fine || (!in->box && !in->vertices) || !in->coef)))
     return fail(ctx, MSFEM_ERR_ARG, "n_cells < 0 or a required array is NULL");
+  if (in->vertices && in->dim != 2)
+    return fail(ctx, MSFEM_ERR_ARG, "general (vertex) cells are supported in 2D only");
   if (coef_stride(in->dim, in->coef_kind) < 0)
     return fail(ctx, MSFEM_ERR_ARG, "unknown coefficient kind %d", in->coef_kind);
   if (in->dim == 3 && in->coef_kind != MSFEM_COEF_SCALAR_CELL)
@@ -196,7 +219,8 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
 int make_plan(msfem_ctx *ctx, const msfem_batch *in)
 {
   Plan &p = ctx->plan;
-  const bool same = p.uploaded && p.dim == in->dim && p.coef_kind == in->coef_kind &&
+  const bool general = in->vertices != nullptr;
+  const bool same = p.uploaded && p.dim == in->dim && p.coef_kind == in->coef_kind && p.general == general &&
                     p.n_fine.size() == (size_t)in->n_cells &&
                     std::equal(p.n_fine.begin(), p.n_fine.end(), in->n_fine);
   if (same)
@@ -205,6 +229,7 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
   p = Plan();
   p.dim = in->dim;
   p.coef_kind = in->coef_kind;
+  p.general = general;
   p.nb = 1 << in->dim;
   p.n_fine.assign(in->n_fine, in->n_fine + in->n_cells);
   const int stride = coef_stride(in->dim, in->coef_kind);
@@ -220,7 +245,7 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
       p.bas_off[c] = p.bas_total;
       p.bas_total += (long long)p.nb * ipow(nf + 1, in->dim);
       p.cond_off[c] = p.cond_total;
-      const int mode = in->dim == 2 ? class_mode(2, in->coef_kind, nf) : kModeScalar;
+      const int mode = in->dim == 2 ? class_mode(2, in->coef_kind, nf, general) : kModeScalar;
       if (mode == kModeGlobal) // 6 FP64 conductances per fine cell
         p.cond_total += 6 * ipow(nf, 2);
       if (mode == kModeStream) // 4 FP64 + 4 FP32 coupling planes
@@ -285,7 +310,23 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
 // Storage mode of size class nf in this build.
 int build_mode(const msfem_batch *in, int nf)
 {
-  return class_mode(in->dim, in->coef_kind, nf);
+  return class_mode(in->dim, in->coef_kind, nf, in->vertices != nullptr);
+}
+
+// Doubles of the FP64 streamed classes' q slots before size class k_end (each
+// such class owns a slice of one work array, so concurrent graph branches never
+// share slots).
+size_t stream_work_doubles(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k_end)
+{
+  const Plan &p = ctx->plan;
+  size_t n = 0;
+  if (o->precision != MSFEM_PREC_FP64)
+    return 0;
+  for (size_t k = 0; k < k_end && k < p.class_nf.size(); ++k)
+    if (build_mode(in, p.class_nf[k]) == kModeStream)
+      n += p.class_cells[k].size() * (size_t)(o->partition_of_unity ? p.nb - 1 : p.nb) *
+           stream_q_slots(p.class_nf[k]);
+  return n;
 }
 
 int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
@@ -300,22 +341,25 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
   const int mode = build_mode(in, nf);
   if (mode == kModeGlobal)
-    HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
-                                     d_off(ctx, 2), st));
+    HIP_TRY(ctx, launch_prep_cond_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef, d_off(ctx, 0),
+                                     cond, d_off(ctx, 2), st));
   if (mode == kModeStream)
-    HIP_TRY(ctx, launch_prep_couplings_2d(ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0), cond,
-                                          d_off(ctx, 2), fp32, st));
+    HIP_TRY(ctx, launch_prep_couplings_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
+                                          d_off(ctx, 0), cond, d_off(ctx, 2), fp32, st));
+  if (in->vertices) // the solve kernels read the boundary data from the bases
+    HIP_TRY(ctx, launch_quad_dirichlet_2d(ncls, cells, nf, in->vertices, bases, d_off(ctx, 1), st));
   LocalArgs a;
   a.nb_solve = pou ? nb - 1 : nb;
   a.n_problems = ncls * a.nb_solve;
   a.nb = nb;
   a.cells = cells;
   a.box = in->box;
+  a.verts = in->vertices;
   a.coef = in->coef;
   a.coef_off = d_off(ctx, 0);
   a.cond = cond;
   a.cond32 = (const float *)cond;
-  a.work = work;
+  a.work = work ? work + stream_work_doubles(ctx, in, o, k) : nullptr;
   a.cond_off = d_off(ctx, 2);
   a.bases = bases;
   a.bas_off = d_off(ctx, 1);
@@ -357,6 +401,7 @@ int launch_class_element(msfem_ctx *ctx, const msfem_batch *in, msfem_batch_out 
   e.nb = p.nb;
   e.coef_kind = in->coef_kind;
   e.box = in->box;
+  e.verts = in->vertices;
   e.coef = in->coef;
   e.coef_off = d_off(ctx, 0);
   e.cond = cond;
@@ -498,17 +543,15 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       if (rc)
         return rc;
     }
-  // global q slots of the FP64 streamed class (allocated before any capture)
+  // global q slots of the FP64 streamed classes (allocated before any capture)
   double *work = nullptr;
-  if (o->precision == MSFEM_PREC_FP64)
-    for (size_t k = 0; k < p.class_nf.size(); ++k)
-      if (build_mode(in, p.class_nf[k]) == kModeStream)
-        {
-          const size_t nprob = p.class_cells[k].size() * (size_t)(o->partition_of_unity ? p.nb - 1 : p.nb);
-          rc = scratch(ctx, "stream_q", nprob * kStreamQSlots * sizeof(double), (void **)&work);
-          if (rc)
-            return rc;
-        }
+  const size_t work_doubles = stream_work_doubles(ctx, in, o, p.class_nf.size());
+  if (work_doubles > 0)
+    {
+      rc = scratch(ctx, "stream_q", work_doubles * sizeof(double), (void **)&work);
+      if (rc)
+        return rc;
+    }
   if (o->graph)
     return run_graph(ctx, in, o, out, bases, cond, work, st);
   const bool pou = o->partition_of_unity != 0;
@@ -814,6 +857,11 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
     return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
   if (in->n_cells == 0)
     return MSFEM_OK;
+  if (in->vertices)
+    for (int c = 0; c < in->n_cells; ++c)
+      if (!quad_ok(in->vertices + (size_t)c * 8))
+        return fail(ctx, MSFEM_ERR_ARG, "cell %d: the vertices do not form a convex, positively oriented "
+                    "quadrilateral (deal.II vertex order)", c);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int dim = in->dim, nb = 1 << dim, n = in->n_cells;
   long long coef_total = 0, bas_total = 0;
@@ -825,6 +873,7 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   HostStage s{ctx};
   msfem_batch din = *in;
   din.box = s.up("h_box", in->box, (size_t)n * 2 * dim);
+  din.vertices = s.up("h_vertices", in->vertices, (size_t)n * 8);
   din.coef = s.up("h_coef", in->coef, (size_t)coef_total);
   din.x0 = s.up("h_x0", in->x0, (size_t)bas_total);
   msfem_batch_out dout;

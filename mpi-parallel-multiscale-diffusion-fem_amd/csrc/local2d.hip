@@ -143,8 +143,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   double *phi = a.bases;
   if (pvalid)
     {
-      const double *bx = a.box + (size_t)cell * 4;
-      rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      if constexpr (MODE == kModeScalar || MODE == kModeTensor) // boxes only
+        {
+          const double *bx = a.box + (size_t)cell * 4;
+          rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+        }
       cf = a.coef + a.coef_off[cell];
       if (MODE == kModeGlobal || MODE == kModeStream)
         cg = a.cond + a.cond_off[cell];
@@ -271,6 +274,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       {
         const int X = k % NP, Y = k / NP;
         const bool bnd = X == 0 || X == NF || Y == 0 || Y == NF;
+        if (bnd && a.verts) // general quadrilateral: boundary data already in place
+          continue;
         double v;
         if (bnd)
           v = nodal_q1(basis, 2, X, Y, 0, NF);
@@ -436,7 +441,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 cbase = (const T *)cg;
               else
                 cbase = (const T *)c32;
-              T *qb = sizeof(T) == 8 ? (T *)(a.work + (size_t)prob * kStreamQSlots) : nullptr;
+              static_assert(R * CPL * NT <= stream_q_slots(NF), "q slots of the FP64 streamed form");
+              T *qb = sizeof(T) == 8 ? (T *)(a.work + (size_t)prob * stream_q_slots(NF)) : nullptr;
               double *pb = phi;
               asm volatile("" : "+s"(cbase), "+s"(qb), "+s"(pb));
 #pragma unroll
@@ -670,9 +676,20 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 }
 
 // ---- coefficient preparation -----------------------------------------------------
+// Element matrix of fine cell f of a box cell (bx) or a general quadrilateral (vq).
+__device__ inline void fine_element_2d(int coef_kind, const double *cf, int f, int nf, const double *bx,
+                                       const double *vq, double (&K)[4][4])
+{
+  if (vq)
+    qg_element_quad(coef_kind, cf, f, nf, QuadMap(vq), K, 0, 0.0, nullptr);
+  else
+    qg_element_2d(coef_kind, cf, f, (bx[1] - bx[0]) / nf, (bx[3] - bx[2]) / nf, K);
+}
+
 // kModeGlobal: 6 FP64 conductances (c01, c02, c03, c12, c13, c23 = -K_uv) per fine cell.
 __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                             const double *coef, const long long *coef_off, double *cond, const long long *cond_off)
+                             const double *verts, const double *coef, const long long *coef_off, double *cond,
+                             const long long *cond_off)
 {
   const int ncell = nf * nf;
   const long long total = (long long)n_cells_class * ncell;
@@ -680,9 +697,9 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
     {
       const int cell = cells[t / ncell];
       const int f = (int)(t % ncell);
-      const double *bx = box + (size_t)cell * 4;
       double K[4][4];
-      qg_element_2d(coef_kind, coef + coef_off[cell], f, (bx[1] - bx[0]) / nf, (bx[3] - bx[2]) / nf, K);
+      fine_element_2d(coef_kind, coef + coef_off[cell], f, nf, verts ? nullptr : box + (size_t)cell * 4,
+                      verts ? verts + (size_t)cell * 8 : nullptr, K);
       double *o = cond + cond_off[cell] + 6 * (size_t)f;
       o[0] = -K[0][1];
       o[1] = -K[0][2];
@@ -698,7 +715,7 @@ __global__ void prep_cond_2d(int n_cells_class, const int *cells, int nf, int co
 // and (want32) FP32 copies right after them.  Plane entries of padding are
 // never written: the kernels read them only for non-interior nodes.
 __global__ void prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                                  const double *coef, const long long *coef_off, double *cond,
+                                  const double *verts, const double *coef, const long long *coef_off, double *cond,
                                   const long long *cond_off, int want32)
 {
   const int np = nf + 1, nnode = np * np, ps = coupling_plane_stride(nf);
@@ -707,25 +724,25 @@ __global__ void prep_couplings_2d(int n_cells_class, const int *cells, int nf, i
     {
       const int cell = cells[t / nnode];
       const int n = (int)(t % nnode), X = n % np, Y = n / np;
-      const double *bx = box + (size_t)cell * 4;
-      const double hx = (bx[1] - bx[0]) / nf, hy = (bx[3] - bx[2]) / nf;
+      const double *bx = verts ? nullptr : box + (size_t)cell * 4;
+      const double *vq = verts ? verts + (size_t)cell * 8 : nullptr;
       const double *cf = coef + coef_off[cell];
       double aE = 0, aN = 0, aNE = 0, aNW = 0, K[4][4];
       if (X < nf && Y < nf) // cell (X, Y): the node is its vertex 0
         {
-          qg_element_2d(coef_kind, cf, Y * nf + X, hx, hy, K);
+          fine_element_2d(coef_kind, cf, Y * nf + X, nf, bx, vq, K);
           aE -= K[0][1];
           aN -= K[0][2];
           aNE -= K[0][3];
         }
       if (X < nf && Y > 0) // cell (X, Y-1): vertex 2
         {
-          qg_element_2d(coef_kind, cf, (Y - 1) * nf + X, hx, hy, K);
+          fine_element_2d(coef_kind, cf, (Y - 1) * nf + X, nf, bx, vq, K);
           aE -= K[2][3];
         }
       if (X > 0 && Y < nf) // cell (X-1, Y): vertex 1
         {
-          qg_element_2d(coef_kind, cf, Y * nf + X - 1, hx, hy, K);
+          fine_element_2d(coef_kind, cf, Y * nf + X - 1, nf, bx, vq, K);
           aN -= K[1][3];
           aNW -= K[1][2];
         }
@@ -745,6 +762,79 @@ __global__ void prep_couplings_2d(int n_cells_class, const int *cells, int nf, i
     }
 }
 
+// Dirichlet data of general quadrilaterals: BasisQ1<2> (basis_q1.tpp:26-46,
+// :86-96) is N_i(x) = sum_m C[m][i] mono_m(x), mono = (1, x, y, xy), with
+// C = M^{-1}, M[j][m] = mono_m(v_j).  One thread per (cell, boundary node):
+// Gauss-Jordan inversion of M with partial pivoting (as deal.II's
+// FullMatrix::invert), then N_i at the node's mapped position F(X/nf, Y/nf),
+// written into the node's entry of every basis block.
+__global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, const double *verts, double *bases,
+                                  const long long *bas_off)
+{
+  const int nbnd = 4 * nf, np = nf + 1;
+  const long long total = (long long)n_cells_class * nbnd;
+  for (long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x; t < total; t += (long long)gridDim.x * blockDim.x)
+    {
+      const int cell = cells[t / nbnd];
+      const int k = (int)(t % nbnd), side = k / nf, j = k % nf;
+      // boundary walk: y = 0 (X = j), x = nf (Y = j), y = nf (X = nf - j), x = 0 (Y = nf - j)
+      const int X = side == 0 ? j : side == 1 ? nf : side == 2 ? nf - j : 0;
+      const int Y = side == 0 ? 0 : side == 1 ? j : side == 2 ? nf : nf - j;
+      const double *v = verts + (size_t)cell * 8;
+      double a[4][8];
+      for (int r = 0; r < 4; ++r)
+        {
+          const double x = v[2 * r], y = v[2 * r + 1];
+          a[r][0] = 1;
+          a[r][1] = x;
+          a[r][2] = y;
+          a[r][3] = x * y;
+          for (int c = 0; c < 4; ++c)
+            a[r][4 + c] = r == c ? 1.0 : 0.0;
+        }
+      // fully unrolled with the row swap as selects: every index is static, so
+      // the 4 x 8 system stays in registers
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        {
+#pragma unroll
+          for (int r = c + 1; r < 4; ++r)
+            if (fabs(a[r][c]) > fabs(a[c][c]))
+#pragma unroll
+              for (int m = 0; m < 8; ++m)
+                {
+                  const double tmp = a[c][m];
+                  a[c][m] = a[r][m];
+                  a[r][m] = tmp;
+                }
+          const double d = 1.0 / a[c][c];
+#pragma unroll
+          for (int m = 0; m < 8; ++m)
+            a[c][m] *= d;
+#pragma unroll
+          for (int r = 0; r < 4; ++r)
+            if (r != c)
+              {
+                const double fct = a[r][c];
+#pragma unroll
+                for (int m = 0; m < 8; ++m)
+                  a[r][m] -= fct * a[c][m];
+              }
+        }
+      double x, y;
+      QuadMap(v).point((double)X / nf, (double)Y / nf, x, y);
+      const double mono[4] = {1.0, x, y, x * y};
+      double *phi = bases + bas_off[cell] + (size_t)Y * np + X;
+      for (int i = 0; i < 4; ++i)
+        {
+          double val = 0;
+          for (int m = 0; m < 4; ++m)
+            val += a[m][4 + i] * mono[m];
+          phi[(size_t)i * np * np] = val;
+        }
+    }
+}
+
 // ---- launch table -----------------------------------------------------------------
 // Variant ids per fine resolution (the autotuner times every id and keeps the
 // fastest per class; id 0 is the built-in default):
@@ -752,6 +842,9 @@ __global__ void prep_couplings_2d(int n_cells_class, const int *cells, int nf, i
 //   32:  1 = (SW 4, R 8, 2)            2 = (SW 2, R 16, 1)    3 = (SW 8, R 4, 4)
 //   64:  1 = (SW 8, R 8, 2)            2 = (SW 4, R 16, 1)    3 = (SW 16, R 4, 4)   4 = (SW 8, R 8, 4)
 //        5 = (SW 4, R 16, 3)
+//   128: 1 = (SW 16, R 8, CPL 2, 4)
+// kModeStream (one problem per workgroup, 64 lanes across):
+//   64:  1 = (SW 8, R 8, 2)            2 = (SW 4, R 16, 1)    3 = (SW 16, R 4, 4)
 //   128: 1 = (SW 16, R 8, CPL 2, 4)
 namespace {
 
@@ -805,6 +898,25 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
 
 } // namespace
 
+template <typename T>
+hipError_t launch_stream(int nf, int variant, const LocalArgs &a, hipStream_t st)
+{
+  switch (nf)
+    {
+      case 64:
+        switch (variant)
+          {
+            case 2: return launch_one<T, 64, 1, 4, 16, kModeStream, 1>(a, st);
+            case 3: return launch_one<T, 64, 1, 16, 4, kModeStream, 4>(a, st);
+            default: return launch_one<T, 64, 1, 8, 8, kModeStream, 2>(a, st);
+          }
+      case 128:
+        return launch_one<T, 128, 2, 16, 8, kModeStream, 4>(a, st);
+      default:
+        return hipErrorInvalidValue;
+    }
+}
+
 int local2d_variants(int nf)
 {
   switch (nf)
@@ -821,11 +933,10 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
 {
   if (mode == kModeStream)
     {
-      if (nf != 128)
-        return hipErrorInvalidValue;
+      // variant ids beyond the streamed table of a resolution take its default
       if (fp32)
-        return launch_one<float, 128, 2, 16, 8, kModeStream, 4>(a, st);
-      return launch_one<double, 128, 2, 16, 8, kModeStream, 4>(a, st);
+        return launch_stream<float>(nf, variant, a, st);
+      return launch_stream<double>(nf, variant, a, st);
     }
   if (fp32)
     {
@@ -843,8 +954,8 @@ hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const
 }
 
 hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                                   const double *coef, const long long *coef_off, double *cond,
-                                   const long long *cond_off, bool want32, hipStream_t st)
+                                   const double *verts, const double *coef, const long long *coef_off,
+                                   double *cond, const long long *cond_off, bool want32, hipStream_t st)
 {
   const long long total = (long long)n_cells_class * (nf + 1) * (nf + 1);
   if (total == 0)
@@ -854,12 +965,12 @@ hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf,
   if (grid > 65536)
     grid = 65536;
   hipLaunchKernelGGL(prep_couplings_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf,
-                     coef_kind, box, coef, coef_off, cond, cond_off, want32 ? 1 : 0);
+                     coef_kind, box, verts, coef, coef_off, cond, cond_off, want32 ? 1 : 0);
   return hipGetLastError();
 }
 
 hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                               const double *coef, const long long *coef_off, double *cond,
+                               const double *verts, const double *coef, const long long *coef_off, double *cond,
                                const long long *cond_off, hipStream_t st)
 {
   const long long total = (long long)n_cells_class * nf * nf;
@@ -870,7 +981,22 @@ hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int 
   if (grid > 65536)
     grid = 65536;
   hipLaunchKernelGGL(prep_cond_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, coef_kind,
-                     box, coef, coef_off, cond, cond_off);
+                     box, verts, coef, coef_off, cond, cond_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, const double *verts, double *bases,
+                                    const long long *bas_off, hipStream_t st)
+{
+  const long long total = (long long)n_cells_class * 4 * nf;
+  if (total == 0)
+    return hipSuccess;
+  const int block = 256;
+  long long grid = (total + block - 1) / block;
+  if (grid > 65536)
+    grid = 65536;
+  hipLaunchKernelGGL(quad_dirichlet_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, verts,
+                     bases, bas_off);
   return hipGetLastError();
 }
 

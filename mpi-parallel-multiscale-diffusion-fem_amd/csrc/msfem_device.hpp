@@ -25,8 +25,9 @@ enum CondMode : int {
 __host__ __device__ constexpr int coupling_plane_stride(int nf) { return (nf + 1) * (nf + 1) + 2 * (nf + 1); }
 
 // FP64 values of q per problem in the FP64 streamed kernel's global work
-// array (the 128^2 class: 8 rows x 2 columns per thread x 1024 threads).
-constexpr int kStreamQSlots = 8 * 2 * 1024;
+// array: one per owned node slot (R rows x CPL columns x threads), which is
+// nf^2 for every launch shape with SW * R = nf and nf = 64 CPL lanes.
+__host__ __device__ constexpr int stream_q_slots(int nf) { return nf * nf; }
 
 // Workgroup -> block index such that the blocks one XCD receives (hardware
 // dispatch is round-robin over the 8 XCDs) hold consecutive problems: the
@@ -43,7 +44,8 @@ struct LocalArgs {
   int nb_solve;               // bases solved per cell
   int nb;                     // 2^dim
   const int *cells;           // [cells_in_class] batch cell index
-  const double *box;          // [n_cells][dim][2]
+  const double *box;          // [n_cells][dim][2] (unused for general quadrilaterals)
+  const double *verts;        // [n_cells][4][2] general quadrilaterals (2D) or nullptr: boxes
   const double *coef;         // concatenated coefficient tiles
   const long long *coef_off;  // [n_cells] tile offsets (elements)
   const double *cond;         // kModeGlobal: concatenated conductance tiles
@@ -301,6 +303,85 @@ __device__ inline void qg_element_2d(int coef_kind, const double *cf, int f, dou
       for (int u = 0; u < 4; ++u)
         for (int v = 0; v < 4; ++v)
           K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
+    }
+}
+
+// Bilinear map F(xi, eta) = v0 + xi a + eta b + xi eta c of a quadrilateral
+// with vertices v[4][2] in deal.II order (a = v1 - v0, b = v2 - v0,
+// c = v3 - v2 - v1 + v0).
+struct QuadMap {
+  double x0, y0, ax, ay, bx, by, cx, cy;
+  __device__ explicit QuadMap(const double *v)
+    : x0(v[0]), y0(v[1]), ax(v[2] - v[0]), ay(v[3] - v[1]), bx(v[4] - v[0]), by(v[5] - v[1]),
+      cx(v[6] - v[4] - v[2] + v[0]), cy(v[7] - v[5] - v[3] + v[1])
+  {
+  }
+  __device__ void point(double xi, double eta, double &x, double &y) const
+  {
+    x = x0 + xi * ax + eta * bx + xi * eta * cx;
+    y = y0 + xi * ay + eta * by + xi * eta * cy;
+  }
+};
+
+// QGauss(2) element matrix and load of fine cell f of a general quadrilateral
+// refined nf times per side (diffusion_problem_basis.tpp:161-242 on the mesh of
+// GridGenerator::general_cell + refine_global, :92-99).  The fine cell is the
+// image of the parametric square [fx, fx+1] x [fy, fy+1] / nf under the coarse
+// bilinear map F, which is also its own bilinear mapping, so its Jacobian at
+// reference point (s, t) is dF/d(xi, eta) / nf at xi = (fx + s)/nf,
+// eta = (fy + t)/nf; grad psi = J^{-T} grad_(s,t) psi, JxW = det J / 4;
+// coefficient and f at the mapped point.  be may be nullptr.
+__device__ inline void qg_element_quad(int coef_kind, const double *cf, int f, int nf, const QuadMap &m,
+                                       double (&K)[4][4], int rhs_kind, double rhs_value, double *be)
+{
+  for (int u = 0; u < 4; ++u)
+    for (int v = 0; v < 4; ++v)
+      K[u][v] = 0;
+  if (be)
+    for (int u = 0; u < 4; ++u)
+      be[u] = 0;
+  const double gs = 0.5 / sqrt(3.0), inv = 1.0 / nf;
+  const int fx = f % nf, fy = f / nf;
+  for (int q = 0; q < 4; ++q)
+    {
+      const double s = (q & 1) ? 0.5 + gs : 0.5 - gs, t = (q & 2) ? 0.5 + gs : 0.5 - gs;
+      const double xi = (fx + s) * inv, eta = (fy + t) * inv;
+      const double J00 = (m.ax + eta * m.cx) * inv, J01 = (m.bx + xi * m.cx) * inv;
+      const double J10 = (m.ay + eta * m.cy) * inv, J11 = (m.by + xi * m.cy) * inv;
+      const double det = J00 * J11 - J01 * J10, rdet = 1.0 / det;
+      double A11, A12, A22;
+      switch (coef_kind)
+        {
+          case 0: A11 = A22 = cf[f]; A12 = 0; break;
+          case 1: A11 = cf[3 * f]; A12 = cf[3 * f + 1]; A22 = cf[3 * f + 2]; break;
+          case 2: A11 = A22 = cf[4 * f + q]; A12 = 0; break;
+          default: A11 = cf[12 * f + 3 * q]; A12 = cf[12 * f + 3 * q + 1]; A22 = cf[12 * f + 3 * q + 2]; break;
+        }
+      double gx[4], gy[4], val[4];
+      for (int v = 0; v < 4; ++v)
+        {
+          const double fs = (v & 1) ? s : 1.0 - s, ft = (v & 2) ? t : 1.0 - t;
+          const double ds = ((v & 1) ? 1.0 : -1.0) * ft, dt = ((v & 2) ? 1.0 : -1.0) * fs;
+          gx[v] = (J11 * ds - J10 * dt) * rdet;
+          gy[v] = (J00 * dt - J01 * ds) * rdet;
+          val[v] = fs * ft;
+        }
+      const double jxw = 0.25 * det;
+      for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v)
+          K[u][v] += (gx[u] * (A11 * gx[v] + A12 * gy[v]) + gy[u] * (A12 * gx[v] + A22 * gy[v])) * jxw;
+      if (be)
+        {
+          double fv = rhs_value;
+          if (rhs_kind == 1)
+            {
+              double x, y;
+              m.point(xi, eta, x, y);
+              fv = sin(M_PI * x) * sin(M_PI * y);
+            }
+          for (int u = 0; u < 4; ++u)
+            be[u] += val[u] * fv * jxw;
+        }
     }
 }
 

@@ -172,6 +172,62 @@ int msfem_gen_coef_reference(int dim, int n_cells, const int32_t *n_fine, const 
   return MSFEM_OK;
 }
 
+int msfem_gen_perturbed_quads(int nc, double amp, uint64_t seed, double *vertices)
+{
+  if (nc <= 0 || !(amp >= 0) || !(amp < 0.25) || !vertices)
+    return MSFEM_ERR_ARG;
+  const int np = nc + 1;
+  std::vector<double> xy((size_t)np * np * 2);
+  for (int j = 0; j < np; ++j)
+    for (int i = 0; i < np; ++i)
+      {
+        double x = (double)i / nc, y = (double)j / nc;
+        if (i > 0 && i < nc && j > 0 && j < nc)
+          {
+            const Draw d = draw(seed, (uint32_t)(j * np + i), 0u, 2u);
+            x += amp * (2.0 * d.u2 - 1.0) / nc;
+            y += amp * (2.0 * d.u3 - 1.0) / nc;
+          }
+        xy[2 * ((size_t)j * np + i)] = x;
+        xy[2 * ((size_t)j * np + i) + 1] = y;
+      }
+  for (int c = 0; c < nc * nc; ++c)
+    {
+      const int cx = c % nc, cy = c / nc;
+      for (int v = 0; v < 4; ++v)
+        {
+          const size_t n = (size_t)(cy + (v >> 1)) * np + cx + (v & 1);
+          vertices[(size_t)c * 8 + 2 * v] = xy[2 * n];
+          vertices[(size_t)c * 8 + 2 * v + 1] = xy[2 * n + 1];
+        }
+    }
+  return MSFEM_OK;
+}
+
+int msfem_gen_coef_reference_quad(int n_cells, const int32_t *n_fine, const double *vertices, double *coef)
+{
+  if (n_cells < 0 || !n_fine || !vertices || !coef)
+    return MSFEM_ERR_ARG;
+  size_t off = 0;
+  for (int c = 0; c < n_cells; ++c)
+    {
+      const int nf = n_fine[c];
+      const double *v = vertices + (size_t)c * 8;
+      for (int f = 0; f < nf * nf; ++f)
+        for (int q = 0; q < 4; ++q)
+          {
+            const double xi = ((f % nf) + gauss_pt(q & 1)) / nf, eta = ((f / nf) + gauss_pt(q >> 1)) / nf;
+            double p[2];
+            for (int k = 0; k < 2; ++k)
+              p[k] = (1 - xi) * (1 - eta) * v[k] + xi * (1 - eta) * v[2 + k] + (1 - xi) * eta * v[4 + k] +
+                     xi * eta * v[6 + k];
+            matrix_coeff(2, p, coef + off);
+            off += 3;
+          }
+    }
+  return MSFEM_OK;
+}
+
 int msfem_gen_coef_periodic(int n_cells, const int32_t *n_fine, const double *box, double eps, double *coef)
 {
   if (n_cells < 0 || !n_fine || !box || !coef || !(eps > 0))

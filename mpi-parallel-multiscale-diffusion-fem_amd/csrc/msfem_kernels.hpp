@@ -10,12 +10,17 @@ namespace msfem {
 // local2d.hip
 int local2d_variants(int nf);
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st);
+// verts: general quadrilaterals [n_cells][4][2] or nullptr (boxes)
 hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                                   const double *coef, const long long *coef_off, double *cond,
-                                   const long long *cond_off, bool want32, hipStream_t st);
+                                   const double *verts, const double *coef, const long long *coef_off,
+                                   double *cond, const long long *cond_off, bool want32, hipStream_t st);
 hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
-                               const double *coef, const long long *coef_off, double *cond,
+                               const double *verts, const double *coef, const long long *coef_off, double *cond,
                                const long long *cond_off, hipStream_t st);
+// Dirichlet data N_i (BasisQ1 by monomial inversion) of general quadrilaterals
+// at their boundary fine nodes, written into every basis block.
+hipError_t launch_quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, const double *verts, double *bases,
+                                    const long long *bas_off, hipStream_t st);
 
 // local3d.hip
 int local3d_variants(int nf);
@@ -28,6 +33,7 @@ struct ElementArgs {
   int dim, nf, nb;
   int coef_kind;              // coefficient kind of the batch (MSFEM_COEF_*)
   const double *box;
+  const double *verts;        // general quadrilaterals [n_cells][4][2] (2D) or nullptr
   const double *coef;
   const long long *coef_off;
   const double *cond;         // 2D conductance tiles when the class used kModeGlobal (else nullptr)

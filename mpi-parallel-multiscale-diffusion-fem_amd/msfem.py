@@ -31,6 +31,7 @@ EXPORTS = [
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
+    "msfem_gen_perturbed_quads", "msfem_gen_coef_reference_quad",
     "msfem_fine_solve", "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
     "msfem_nn_bias_grad", "msfem_nn_pool2", "msfem_nn_up2", "msfem_nn_axpy", "msfem_nn_axpby", "msfem_nn_to_bf16",
     "msfem_nn_mse", "msfem_nn_adam", "msfem_nn_delta_to_bases", "msfem_nn_bases_to_delta",
@@ -53,7 +54,8 @@ class SolverOpts(ctypes.Structure):
 class Batch(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("n_cells", ctypes.c_int32), ("n_fine", ctypes.c_void_p),
                 ("box", ctypes.c_void_p), ("coef_kind", ctypes.c_int32), ("coef", ctypes.c_void_p),
-                ("rhs_kind", ctypes.c_int32), ("rhs_value", ctypes.c_double), ("x0", ctypes.c_void_p)]
+                ("rhs_kind", ctypes.c_int32), ("rhs_value", ctypes.c_double), ("x0", ctypes.c_void_p),
+                ("vertices", ctypes.c_void_p)]
 
 
 class BatchOut(ctypes.Structure):
@@ -108,6 +110,8 @@ def lib():
         L.msfem_gen_resolution_classes.argtypes = [i32, ctypes.c_uint64, dbl, vp, vp, vp]
         L.msfem_gen_coef_tensor.argtypes = [i32, vp, ctypes.c_uint64, dbl, vp]
         L.msfem_gen_grf_log.argtypes = [i32, i32, ctypes.c_uint64, dbl, dbl, vp]
+        L.msfem_gen_perturbed_quads.argtypes = [i32, dbl, ctypes.c_uint64, vp]
+        L.msfem_gen_coef_reference_quad.argtypes = [i32, vp, vp, vp]
         L.msfem_fine_solve.argtypes = [vp, i32, i32, i32, vp, i32, dbl, i32, dbl, i32, vp, vp]
         _lib = L
     return _lib
@@ -219,18 +223,20 @@ class Context:
 
     # ---- the hot path, host arrays ------------------------------------------------
     def batch_build(self, dim, n_fine, box, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, x0=None,
-                    opts=None, want_bases=True, raise_noconv=True):
+                    opts=None, want_bases=True, raise_noconv=True, vertices=None):
+        """vertices: [n][4][2] general quadrilaterals (2D; box is then ignored, may be None)."""
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         n = int(n_fine.shape[0])
         nb = 1 << dim
-        box = np.ascontiguousarray(box, dtype=np.float64)
+        box = None if box is None else np.ascontiguousarray(box, dtype=np.float64)
+        vertices = None if vertices is None else np.ascontiguousarray(vertices, dtype=np.float64)
         coef = np.ascontiguousarray(coef, dtype=np.float64)
         x0 = None if x0 is None else np.ascontiguousarray(x0, dtype=np.float64)
         opts = opts or default_opts()
         out = {"Ae": np.zeros((n, nb, nb)), "Fe": np.zeros((n, nb)), "iters": np.zeros((n, nb), np.int32),
                "status": np.zeros((n, nb), np.int32), "resid": np.zeros((n, nb)),
                "bases": np.zeros(bases_size(dim, n_fine)) if want_bases else None}
-        b = Batch(dim, n, _p(n_fine), _p(box), coef_kind, _p(coef), rhs_kind, rhs_value, _p(x0))
+        b = Batch(dim, n, _p(n_fine), _p(box), coef_kind, _p(coef), rhs_kind, rhs_value, _p(x0), _p(vertices))
         o = BatchOut(_p(out["Ae"]), _p(out["Fe"]), _p(out["bases"]), _p(out["iters"]), _p(out["status"]),
                      _p(out["resid"]))
         rc = lib().msfem_batch_build(self.h, ctypes.byref(b), ctypes.byref(opts), ctypes.byref(o))
@@ -242,10 +248,11 @@ class Context:
     # ---- the hot path, device pointers (e.g. torch tensors' data_ptr()) --------------
     def batch_build_device(self, dim, n_fine, box_ptr, coef_kind, coef_ptr, Ae_ptr, Fe_ptr, bases_ptr=None,
                            iters_ptr=None, status_ptr=None, resid_ptr=None, rhs_kind=RHS_CONST, rhs_value=1.0,
-                           x0_ptr=None, opts=None, stream=None):
+                           x0_ptr=None, opts=None, stream=None, vertices_ptr=None):
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         opts = opts or default_opts()
-        b = Batch(dim, int(n_fine.shape[0]), _p(n_fine), box_ptr, coef_kind, coef_ptr, rhs_kind, rhs_value, x0_ptr)
+        b = Batch(dim, int(n_fine.shape[0]), _p(n_fine), box_ptr, coef_kind, coef_ptr, rhs_kind, rhs_value, x0_ptr,
+                  vertices_ptr)
         o = BatchOut(Ae_ptr, Fe_ptr, bases_ptr, iters_ptr, status_ptr, resid_ptr)
         _check(lib().msfem_batch_build_device(self.h, ctypes.byref(b), ctypes.byref(opts), ctypes.byref(o), stream),
                self.h)
@@ -380,6 +387,21 @@ def coef_reference(dim, n_fine, box):
     box = np.ascontiguousarray(box, dtype=np.float64)
     out = np.zeros(_tile_total(dim, n_fine, COEF_TENSOR_QP))
     _check(lib().msfem_gen_coef_reference(dim, len(n_fine), _p(n_fine), _p(box), _p(out)))
+    return out
+
+
+def perturbed_quads(nc, amp, seed):
+    """[nc*nc][4][2] vertices of the unit square's nc x nc mesh with moved interior vertices."""
+    out = np.zeros((nc * nc, 4, 2))
+    _check(lib().msfem_gen_perturbed_quads(nc, amp, seed, _p(out)))
+    return out
+
+
+def coef_reference_quad(n_fine, vertices):
+    n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64)
+    out = np.zeros(_tile_total(2, n_fine, COEF_TENSOR_QP))
+    _check(lib().msfem_gen_coef_reference_quad(len(n_fine), _p(n_fine), _p(vertices), _p(out)))
     return out
 
 

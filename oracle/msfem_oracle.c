@@ -26,6 +26,11 @@
  *       diffusion_problem_ms.tpp:260-296
  *   - fine reconstruction u|_K = sum_i w_i phi_i
  *       diffusion_problem_basis.tpp:354-377
+ *   - general (non-box) quadrilateral coarse cells, 2D: the fine mesh of
+ *       GridGenerator::general_cell + refine_global (diffusion_problem_basis.tpp:
+ *       92-99), FE_Q(1) with the bilinear mapping and QGauss(2) on every fine
+ *       quadrilateral (:161-242), Dirichlet data N_i of BasisQ1 at the mapped
+ *       boundary nodes (basis_q1.tpp:26-46, :86-96)
  *   - the reference default coefficient MatrixCoeff (with PI_D, k=57,
  *       scale 0.9999) in long double: include/coefficients/matrix_coeff.hpp:45-49,
  *       matrix_coeff.tpp:17-91, coefficients.h:21
@@ -204,6 +209,26 @@ void orc_box_vertices(int dim, const double *box, double *verts)
   for (int v = 0; v < (1 << dim); ++v)
     for (int k = 0; k < dim; ++k)
       verts[v * dim + k] = ((v >> k) & 1) ? box[2 * k + 1] : box[2 * k];
+}
+
+/* Bilinear map of a quadrilateral with vertices quad[4][2] (deal.II order) at
+ * the parametric point (xi, eta) in [0,1]^2. */
+static void quad_map(const double *quad, double xi, double eta, double *x)
+{
+  for (int k = 0; k < 2; ++k)
+    x[k] = (1 - xi) * (1 - eta) * quad[k] + xi * (1 - eta) * quad[2 + k] + (1 - xi) * eta * quad[4 + k] +
+           xi * eta * quad[6 + k];
+}
+
+/* Fine node (X, Y) of a general quadrilateral refined nf times per side.
+ * GridGenerator::general_cell + refine_global (diffusion_problem_basis.tpp:
+ * 92-99) put each new vertex at an edge midpoint or at the mean of a cell's four
+ * vertices; on a bilinear cell both are the bilinear map's values at the
+ * parametric midpoints, so the refined vertices are the map's values on the
+ * uniform parametric grid. */
+void orc_quad_node(const double *quad, int nf, int X, int Y, double *x)
+{
+  quad_map(quad, (double)X / nf, (double)Y / nf, x);
 }
 
 /* ------------------------------------------------------------------ */
@@ -395,6 +420,87 @@ void orc_local_assemble(int dim, int nf, const double *box, int coef_kind, const
     }
 }
 
+/* Element matrix and load of one fine quadrilateral with vertices fv[4][2]
+ * (deal.II order): FE_Q(1) with the bilinear mapping (deal.II's default
+ * MappingQ1), QGauss(2) (diffusion_problem_basis.tpp:163-222).  At each
+ * quadrature point J_kl = sum_a fv[a][k] d psi_a / d xi_l, grad psi_a =
+ * J^{-T} grad_xi psi_a, JxW = det J / 4; coefficient and f at the mapped point. */
+void orc_fine_element_quad(const double *fv, int coef_kind, const double *cdata, int rhs_kind, double rhs_value,
+                           double *Ke /* [4][4] */, double *be /* [4] */)
+{
+  memset(Ke, 0, sizeof(double) * 16);
+  memset(be, 0, sizeof(double) * 4);
+  for (int q = 0; q < 4; ++q)
+    {
+      const double xi = gauss_pt(q & 1), eta = gauss_pt((q >> 1) & 1);
+      double val[4], gr[4][2];
+      for (int a = 0; a < 4; ++a)
+        {
+          const double fx = (a & 1) ? xi : 1 - xi, fy = (a & 2) ? eta : 1 - eta;
+          val[a] = fx * fy;
+          gr[a][0] = ((a & 1) ? 1.0 : -1.0) * fy;
+          gr[a][1] = ((a & 2) ? 1.0 : -1.0) * fx;
+        }
+      double J[2][2] = {{0, 0}, {0, 0}}, xq[2] = {0, 0};
+      for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 2; ++k)
+          {
+            xq[k] += val[a] * fv[2 * a + k];
+            for (int l = 0; l < 2; ++l)
+              J[k][l] += fv[2 * a + k] * gr[a][l];
+          }
+      const double det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+      const double Ji[2][2] = {{J[1][1] / det, -J[0][1] / det}, {-J[1][0] / det, J[0][0] / det}};
+      double g[4][2];
+      for (int a = 0; a < 4; ++a)
+        for (int k = 0; k < 2; ++k)
+          g[a][k] = Ji[0][k] * gr[a][0] + Ji[1][k] * gr[a][1];
+      double A[3][3];
+      qp_tensor(2, coef_kind, cdata, q, A);
+      const double f = rhs_eval(2, rhs_kind, rhs_value, xq), jxw = det * 0.25;
+      for (int a = 0; a < 4; ++a)
+        {
+          for (int b = 0; b < 4; ++b)
+            {
+              double t = 0;
+              for (int i = 0; i < 2; ++i)
+                for (int j = 0; j < 2; ++j)
+                  t += g[a][i] * A[i][j] * g[b][j];
+              Ke[a * 4 + b] += t * jxw;
+            }
+          be[a] += val[a] * f * jxw;
+        }
+    }
+}
+
+/* Local system of a general quadrilateral (2D), as orc_local_assemble. */
+void orc_local_assemble_quad(int nf, const double *quad, int coef_kind, const double *tile, int rhs_kind,
+                             double rhs_value, double *Kst, double *b)
+{
+  const int np = nf + 1, nn = np * np;
+  const int stride = orc_coef_stride(2, coef_kind);
+  memset(Kst, 0, sizeof(double) * (size_t)nn * 9);
+  memset(b, 0, sizeof(double) * (size_t)nn);
+  for (int c = 0; c < nf * nf; ++c)
+    {
+      const int cx = c % nf, cy = c / nf;
+      double fv[8], Ke[16], be[4];
+      for (int a = 0; a < 4; ++a)
+        orc_quad_node(quad, nf, cx + (a & 1), cy + ((a >> 1) & 1), fv + 2 * a);
+      orc_fine_element_quad(fv, coef_kind, tile + (size_t)c * stride, rhs_kind, rhs_value, Ke, be);
+      for (int a = 0; a < 4; ++a)
+        {
+          const int na = (cx + (a & 1)) + np * (cy + ((a >> 1) & 1));
+          for (int bb = 0; bb < 4; ++bb)
+            {
+              const int ox = (bb & 1) - (a & 1), oy = ((bb >> 1) & 1) - ((a >> 1) & 1);
+              Kst[(size_t)na * 9 + (ox + 1) + 3 * (oy + 1)] += Ke[a * 4 + bb];
+            }
+          b[na] += be[a];
+        }
+    }
+}
+
 /* y = K x on all nodes */
 void orc_stencil_apply(int dim, int nf, const double *Kst, const double *x, double *y)
 {
@@ -429,10 +535,13 @@ static int is_boundary_node(int dim, int nf, int n)
 
 /* Dirichlet data g_i at all nodes (interior entries = 0): BasisQ1 by monomial
  * inversion evaluated at the fine node coordinates (diffusion_problem_basis.tpp:119-135). */
-void orc_dirichlet_data(int dim, int nf, const double *box, int index, double *g)
+static void dirichlet_data(int dim, int nf, const double *box, const double *quad, int index, double *g)
 {
   double verts[24], coeff[64];
-  orc_box_vertices(dim, box, verts);
+  if (quad)
+    memcpy(verts, quad, sizeof(double) * 8);
+  else
+    orc_box_vertices(dim, box, verts);
   orc_basis_q1_coeffs(dim, verts, coeff);
   const int np = nf + 1, nn = ipow(np, dim);
   for (int n = 0; n < nn; ++n)
@@ -444,10 +553,24 @@ void orc_dirichlet_data(int dim, int nf, const double *box, int index, double *g
         }
       int I[3] = {n % np, (n / np) % np, dim == 3 ? n / (np * np) : 0};
       double p[3];
-      for (int k = 0; k < dim; ++k)
-        p[k] = box[2 * k] + (box[2 * k + 1] - box[2 * k]) * ((double)I[k] / nf);
+      if (quad)
+        orc_quad_node(quad, nf, I[0], I[1], p);
+      else
+        for (int k = 0; k < dim; ++k)
+          p[k] = box[2 * k] + (box[2 * k + 1] - box[2 * k]) * ((double)I[k] / nf);
       g[n] = orc_basis_q1_value(dim, coeff, index, p);
     }
+}
+
+void orc_dirichlet_data(int dim, int nf, const double *box, int index, double *g)
+{
+  dirichlet_data(dim, nf, box, NULL, index, g);
+}
+
+/* The same at the nodes of a general quadrilateral's fine grid (2D). */
+void orc_dirichlet_data_quad(int nf, const double *quad, int index, double *g)
+{
+  dirichlet_data(2, nf, NULL, quad, index, g);
 }
 
 /* ------------------------------------------------------------------ */
@@ -508,7 +631,8 @@ static void band_solve(const band_t *B, double *x)
 }
 
 /* Basis problems by direct solve: phi[i][n], i < 2^d.  Returns 0 or 1 (not SPD). */
-int orc_local_bases_direct(int dim, int nf, const double *box, const double *Kst, double *phi)
+static int local_bases_direct(int dim, int nf, const double *box, const double *quad, const double *Kst,
+                              double *phi)
 {
   const int np = nf + 1, m = nf - 1, ns3 = ipow(3, dim), nb = 1 << dim;
   const int nn = ipow(np, dim), ni = ipow(m, dim);
@@ -549,7 +673,7 @@ int orc_local_bases_direct(int dim, int nf, const double *box, const double *Kst
   double *x = (double *)malloc(sizeof(double) * (ni > 0 ? ni : 1));
   for (int ib = 0; ib < nb && rc == 0; ++ib)
     {
-      orc_dirichlet_data(dim, nf, box, ib, g);
+      dirichlet_data(dim, nf, box, quad, ib, g);
       for (int i = 0; i < ni; ++i)
         {
           int n = inode[i];
@@ -577,13 +701,23 @@ int orc_local_bases_direct(int dim, int nf, const double *box, const double *Kst
   return rc;
 }
 
+int orc_local_bases_direct(int dim, int nf, const double *box, const double *Kst, double *phi)
+{
+  return local_bases_direct(dim, nf, box, NULL, Kst, phi);
+}
+
+int orc_local_bases_direct_quad(int nf, const double *quad, const double *Kst, double *phi)
+{
+  return local_bases_direct(2, nf, NULL, quad, Kst, phi);
+}
+
 /* ------------------------------------------------------------------ */
 /* The reference's own recipe: condense, CG + SSOR(omega), absolute    */
 /* tolerance, iteration cap (diffusion_problem_basis.tpp:295-317,      */
 /* :450-465).  Returns the number of bases that did not converge.      */
 /* ------------------------------------------------------------------ */
-int orc_local_bases_ssor_pcg(int dim, int nf, const double *box, const double *Kst, double omega,
-                             double abs_tol, int max_iter, double *phi, int *iters)
+static int local_bases_ssor_pcg(int dim, int nf, const double *box, const double *quad, const double *Kst,
+                                double omega, double abs_tol, int max_iter, double *phi, int *iters)
 {
   const int np = nf + 1, ns3 = ipow(3, dim), nb = 1 << dim, nn = ipow(np, dim);
   const int center = dim == 2 ? 4 : 13;
@@ -617,7 +751,7 @@ int orc_local_bases_ssor_pcg(int dim, int nf, const double *box, const double *K
   for (int ib = 0; ib < nb; ++ib)
     {
       double *x = phi + (size_t)ib * nn;
-      orc_dirichlet_data(dim, nf, box, ib, g);
+      dirichlet_data(dim, nf, box, quad, ib, g);
       /* condensed rhs (diffusion_problem_basis.tpp:455,461): -K_IB g on free
        * rows, diag*g on constrained rows */
       for (int n = 0; n < nn; ++n)
@@ -744,6 +878,12 @@ int orc_local_bases_ssor_pcg(int dim, int nf, const double *box, const double *K
   return failures;
 }
 
+int orc_local_bases_ssor_pcg(int dim, int nf, const double *box, const double *Kst, double omega,
+                             double abs_tol, int max_iter, double *phi, int *iters)
+{
+  return local_bases_ssor_pcg(dim, nf, box, NULL, Kst, omega, abs_tol, max_iter, phi, iters);
+}
+
 /* A_ij = phi_i^T K phi_j, F_i = phi_i^T b  (diffusion_problem_basis.tpp:247-285) */
 void orc_element(int dim, int nf, const double *Kst, const double *b, const double *phi, double *Ae,
                  double *Fe)
@@ -803,6 +943,34 @@ int orc_cell_build(int dim, int nf, const double *box, int coef_kind, const doub
   return rc;
 }
 
+/* One general quadrilateral coarse cell (2D), as orc_cell_build. */
+int orc_cell_build_quad(int nf, const double *quad, int coef_kind, const double *tile, int rhs_kind,
+                        double rhs_value, int solver, double omega, double abs_tol, int max_iter, double *phi,
+                        double *Ae, double *Fe, int *iters)
+{
+  const int nn = (nf + 1) * (nf + 1);
+  double *Kst = (double *)malloc(sizeof(double) * (size_t)nn * 9);
+  double *b   = (double *)malloc(sizeof(double) * nn);
+  double *ph  = phi ? phi : (double *)malloc(sizeof(double) * (size_t)4 * nn);
+  orc_local_assemble_quad(nf, quad, coef_kind, tile, rhs_kind, rhs_value, Kst, b);
+  int rc;
+  if (solver == ORC_SOLVER_DIRECT)
+    {
+      rc = local_bases_direct(2, nf, NULL, quad, Kst, ph);
+      if (iters)
+        for (int i = 0; i < 4; ++i)
+          iters[i] = 0;
+    }
+  else
+    rc = local_bases_ssor_pcg(2, nf, NULL, quad, Kst, omega, abs_tol, max_iter, ph, iters);
+  orc_element(2, nf, Kst, b, ph, Ae, Fe);
+  free(Kst);
+  free(b);
+  if (!phi)
+    free(ph);
+  return rc;
+}
+
 /* Many cells, OpenMP over cells (each cell is independent, as the reference's
  * per-rank cell loop diffusion_problem_ms.tpp:84-87).
  * box [n][d][2]; nf [n]; coef: concatenated tiles in cell order;
@@ -833,6 +1001,39 @@ int orc_batch_build(int dim, int n_cells, const double *box, const int *nf, int 
       int rc = orc_cell_build(dim, nf[c], box + (size_t)c * 2 * dim, coef_kind, coef + coff[c], rhs_kind,
                               rhs_value, solver, omega, abs_tol, max_iter, bases ? bases + boff[c] : NULL,
                               Ae + (size_t)c * nb * nb, Fe + (size_t)c * nb, iters ? iters + (size_t)c * nb : NULL);
+      if (rc)
+        fails += 1;
+    }
+  free(coff);
+  free(boff);
+  return fails;
+}
+
+/* The same for general quadrilaterals (2D): quad [n][4][2]. */
+int orc_batch_build_quad(int n_cells, const double *quad, const int *nf, int coef_kind, const double *coef,
+                         int rhs_kind, double rhs_value, int solver, double omega, double abs_tol, int max_iter,
+                         int n_threads, double *bases, double *Ae, double *Fe, int *iters)
+{
+  const int stride = orc_coef_stride(2, coef_kind);
+  size_t *coff = (size_t *)malloc(sizeof(size_t) * (n_cells + 1));
+  size_t *boff = (size_t *)malloc(sizeof(size_t) * (n_cells + 1));
+  coff[0] = boff[0] = 0;
+  for (int c = 0; c < n_cells; ++c)
+    {
+      coff[c + 1] = coff[c] + (size_t)nf[c] * nf[c] * stride;
+      boff[c + 1] = boff[c] + (size_t)4 * (nf[c] + 1) * (nf[c] + 1);
+    }
+  int fails = 0;
+#ifdef _OPENMP
+  if (n_threads > 0)
+    omp_set_num_threads(n_threads);
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : fails)
+#endif
+  for (int c = 0; c < n_cells; ++c)
+    {
+      int rc = orc_cell_build_quad(nf[c], quad + (size_t)c * 8, coef_kind, coef + coff[c], rhs_kind, rhs_value,
+                                   solver, omega, abs_tol, max_iter, bases ? bases + boff[c] : NULL,
+                                   Ae + (size_t)c * 16, Fe + (size_t)c * 4, iters ? iters + (size_t)c * 4 : NULL);
       if (rc)
         fails += 1;
     }

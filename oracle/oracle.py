@@ -16,6 +16,8 @@ msfem_oracle.c):
                                      diffusion_problem_ms.tpp:42-88
   coarse_assemble / coarse_solve     diffusion_problem_ms.tpp:108-296
   reconstruct                        diffusion_problem_basis.tpp:354-377
+  *_quad (general quadrilaterals)    diffusion_problem_basis.tpp:92-99, :161-242 with the
+                                     bilinear mapping; basis_q1.tpp:26-46 at mapped nodes
 """
 import ctypes
 import os
@@ -75,6 +77,13 @@ def _declare(L):
         "orc_coarse_assemble": (None, [_i, _i, _dp, _dp, _i, _dp, _dp, _up, _dp]),
         "orc_coarse_solve": (_i, [_i, _i, _dp, _dp, _up, _dp, _dp]),
         "orc_reconstruct": (None, [_i, _i, _dp, _dp, _dp]),
+        "orc_quad_node": (None, [_dp, _i, _i, _i, _dp]),
+        "orc_fine_element_quad": (None, [_dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_local_assemble_quad": (None, [_i, _dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_dirichlet_data_quad": (None, [_i, _dp, _i, _dp]),
+        "orc_local_bases_direct_quad": (_i, [_i, _dp, _dp, _dp]),
+        "orc_cell_build_quad": (_i, [_i, _dp, _i, _dp, _i, _d, _i, _d, _d, _i, _dp, _dp, _dp, _ip]),
+        "orc_batch_build_quad": (_i, [_i, _dp, _ip, _i, _dp, _i, _d, _i, _d, _d, _i, _i, _dp, _dp, _dp, _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -248,3 +257,63 @@ def reconstruct(dim, nf, phi, w):
     u = np.zeros(n_nodes(dim, nf))
     lib().orc_reconstruct(dim, nf, _ptr(_d64(phi).reshape(-1)), _ptr(_d64(w)), _ptr(u))
     return u
+
+
+# ---- general quadrilateral cells (2D): vertices quad[4][2] in deal.II order ----
+def quad_nodes(quad, nf):
+    """Fine node coordinates [(nf+1)^2][2] of a general quadrilateral refined nf times."""
+    q = _d64(quad).reshape(-1)
+    out = np.zeros(((nf + 1) ** 2, 2))
+    x = np.zeros(2)
+    for n in range((nf + 1) ** 2):
+        lib().orc_quad_node(_ptr(q), nf, n % (nf + 1), n // (nf + 1), _ptr(x))
+        out[n] = x
+    return out
+
+
+def fine_element_quad(fv, coef_kind, cdata, rhs_kind=RHS_CONST, rhs_value=1.0):
+    Ke = np.zeros(16)
+    be = np.zeros(4)
+    lib().orc_fine_element_quad(_ptr(_d64(fv).reshape(-1)), coef_kind, _ptr(_d64(cdata)), rhs_kind, rhs_value,
+                                _ptr(Ke), _ptr(be))
+    return Ke.reshape(4, 4), be
+
+
+def local_assemble_quad(nf, quad, coef_kind, tile, rhs_kind=RHS_CONST, rhs_value=1.0):
+    nn = n_nodes(2, nf)
+    K = np.zeros(nn * 9)
+    b = np.zeros(nn)
+    lib().orc_local_assemble_quad(nf, _ptr(_d64(quad).reshape(-1)), coef_kind, _ptr(_d64(tile).reshape(-1)),
+                                  rhs_kind, rhs_value, _ptr(K), _ptr(b))
+    return K.reshape(nn, 9), b
+
+
+def dirichlet_data_quad(nf, quad, index):
+    g = np.zeros(n_nodes(2, nf))
+    lib().orc_dirichlet_data_quad(nf, _ptr(_d64(quad).reshape(-1)), index, _ptr(g))
+    return g
+
+
+def local_bases_direct_quad(nf, quad, K):
+    phi = np.zeros(4 * n_nodes(2, nf))
+    rc = lib().orc_local_bases_direct_quad(nf, _ptr(_d64(quad).reshape(-1)), _ptr(_d64(K).reshape(-1)), _ptr(phi))
+    if rc:
+        raise ValueError("local matrix not SPD")
+    return phi.reshape(4, -1)
+
+
+def batch_build_quad(quad, nf, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, solver=SOLVER_DIRECT,
+                     omega=1.6, abs_tol=1e-12, max_iter=1000, n_threads=0, want_bases=False):
+    """batch_build for general quadrilaterals: quad [n][4][2]."""
+    q = _d64(quad).reshape(-1)
+    nf = np.ascontiguousarray(nf, dtype=np.int32)
+    n = nf.shape[0]
+    Ae = np.zeros(n * 16)
+    Fe = np.zeros(n * 4)
+    its = np.zeros(n * 4, dtype=np.int32)
+    bases = np.zeros(int(sum(4 * (int(k) + 1) ** 2 for k in nf))) if want_bases else None
+    fails = lib().orc_batch_build_quad(n, _ptr(q), _ptr(nf, _ip), coef_kind, _ptr(_d64(coef).reshape(-1)),
+                                       rhs_kind, rhs_value, solver, omega, abs_tol, max_iter, n_threads,
+                                       _ptr(bases), _ptr(Ae), _ptr(Fe), _ptr(its, _ip))
+    return {"Ae": Ae.reshape(n, 4, 4), "Fe": Fe.reshape(n, 4), "iters": its.reshape(n, 4), "bases": bases,
+            "fails": fails}

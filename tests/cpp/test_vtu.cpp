@@ -30,6 +30,18 @@ int main(int argc, char **argv)
       v[k] = (k / (np * np)) * 0.5;
     msfem::vtu::write_cell(dir + "/cell3d.vtu", 3, box, nf, {{"solution", v.data()}});
   }
+  // 2D general quadrilateral (deal.II vertex order), 4 fine cells per side, field x
+  {
+    const double quad[8] = {0.1, 0.2, 0.9, 0.1, 0.25, 0.8, 0.7, 1.05};
+    const int nf = 4, np = nf + 1;
+    std::vector<double> v(np * np);
+    for (int k = 0; k < np * np; ++k)
+      {
+        const double xi = (double)(k % np) / nf, eta = (double)(k / np) / nf;
+        v[k] = (1 - xi) * (1 - eta) * quad[0] + xi * (1 - eta) * quad[2] + (1 - xi) * eta * quad[4] + xi * eta * quad[6];
+      }
+    msfem::vtu::write_cell(dir + "/quad2d.vtu", 2, nullptr, nf, {{"solution", v.data()}}, quad);
+  }
   msfem::vtu::write_pvtu(dir + "/master.pvtu", {"solution"}, {"cell2d.vtu", "cell3d.vtu"});
   std::printf("vtu ok\n");
   return 0;

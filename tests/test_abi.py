@@ -43,7 +43,7 @@ int main(void) {
   printf("%zu %zu %zu\\n", sizeof(msfem_solver_opts), sizeof(msfem_batch), sizeof(msfem_batch_out));
   printf("%zu %zu %zu\\n", offsetof(msfem_solver_opts, rel_tol), offsetof(msfem_solver_opts, variant),
          offsetof(msfem_batch, rhs_value));
-  printf("%zu %zu\\n", offsetof(msfem_batch, x0), offsetof(msfem_batch_out, resid));
+  printf("%zu %zu %zu\\n", offsetof(msfem_batch, x0), offsetof(msfem_batch, vertices), offsetof(msfem_batch_out, resid));
   return 0;
 }
 """)
@@ -53,13 +53,13 @@ int main(void) {
     C = ctypes
     assert vals == [C.sizeof(msfem.SolverOpts), C.sizeof(msfem.Batch), C.sizeof(msfem.BatchOut),
                     msfem.SolverOpts.rel_tol.offset, msfem.SolverOpts.variant.offset, msfem.Batch.rhs_value.offset,
-                    msfem.Batch.x0.offset, msfem.BatchOut.resid.offset]
+                    msfem.Batch.x0.offset, msfem.Batch.vertices.offset, msfem.BatchOut.resid.offset]
 
 
 def test_default_opts_and_version():
     o = msfem.default_opts()
     assert o.precision == msfem.PREC_FP64 and o.rel_tol == 1e-13 and o.max_iter > 1000
-    assert msfem.lib().msfem_version() == 1
+    assert msfem.lib().msfem_version() == 2
 
 
 def test_no_device_is_a_clean_error():

@@ -63,3 +63,40 @@ def test_facade_fine_output(tmp_path):
         for k in (0, 16, 272, 288):  # the cell's corners
             X, Y = int(round(pts[k, 0] * 2)), int(round(pts[k, 1] * 2))
             assert abs(sol[k] - u[Y, X]) <= 1e-12 * max(1.0, abs(u[Y, X]))
+
+
+def test_facade_general_cells(tmp_path):
+    """set_mesh_distortion(0.2, seed 1): every coarse cell a general quadrilateral
+    (DiffusionProblemBasis from its corner points); element matrices, loads and
+    the coarse solution against the oracle's general-cell path, and the fine
+    output's corner points at the moved vertices."""
+    import xml.etree.ElementTree as ET
+    print("[gpu-case] facade general cells n_refine=2 n_refine_local=5 distortion=0.2", flush=True)
+    out, vdir = tmp_path / "facade.bin", tmp_path / "vtu"
+    vdir.mkdir()
+    p = subprocess.run([EXE, "2", "5", str(out), str(vdir), "0.2"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    raw = out.read_bytes()
+    nu = int(np.frombuffer(raw[:8], np.int64)[0])
+    u = np.frombuffer(raw[8:8 + 8 * nu], np.float64)
+    nc_cells = int(np.frombuffer(raw[8 + 8 * nu:16 + 8 * nu], np.int64)[0])
+    rest = np.frombuffer(raw[16 + 8 * nu:], np.float64).reshape(nc_cells, 20)
+    Ae, Fe = rest[:, :16].reshape(-1, 4, 4), rest[:, 16:]
+    nc, nf = 4, 32
+    quad = msfem.perturbed_quads(nc, 0.2, 1)
+    n_fine = np.full(nc * nc, nf, np.int32)
+    coef = msfem.coef_reference_quad(n_fine, quad)
+    r = O.batch_build_quad(quad, n_fine, O.COEF_TENSOR_QP, coef, O.RHS_CONST, 2.0)
+    assert np.abs(Ae - r["Ae"]).max() / np.abs(r["Ae"]).max() < 1e-10
+    assert np.abs(Fe - r["Fe"]).max() / np.abs(r["Fe"]).max() < 1e-10
+    Kc, Fc, cm, cv = O.coarse_assemble(2, nc, r["Ae"], r["Fe"], 1)
+    ur = O.coarse_solve(2, nc, Kc, Fc, cm, cv)
+    assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-10
+    master = ET.parse(vdir / "solution-ms_fine-2d.pvtu").getroot()
+    pieces = [pc.get("Source") for pc in master.iter("Piece")]
+    assert len(pieces) == nc * nc
+    for c, src in enumerate(pieces):  # cells in id order
+        pts = np.array(ET.parse(vdir / src).getroot().find("UnstructuredGrid/Piece/Points/DataArray").text.split(),
+                       float).reshape(-1, 3)
+        np1 = nf + 1
+        assert np.allclose(pts[[0, nf, nf * np1, np1 * np1 - 1], :2], quad[c], atol=1e-15)

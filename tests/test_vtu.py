@@ -59,6 +59,27 @@ def test_vtu_3d(outdir):
         assert np.linalg.det(np.stack([v[1] - v[0], v[3] - v[0], v[4] - v[0]])) == pytest.approx(0.125)
 
 
+def test_vtu_quad(outdir):
+    """General quadrilateral: points on the bilinear image of the grid, positive
+    fine cells tiling the cell (areas sum to its shoelace area), field = x."""
+    a = arrays(os.path.join(outdir, "quad2d.vtu"))
+    assert a["npts"] == 25 and a["ncells"] == 16
+    p = a["points"]
+    quad = np.array([[0.1, 0.2], [0.9, 0.1], [0.25, 0.8], [0.7, 1.05]])
+    assert np.allclose(p[0, :2], quad[0]) and np.allclose(p[4, :2], quad[1])
+    assert np.allclose(p[20, :2], quad[2]) and np.allclose(p[24, :2], quad[3])
+    assert np.allclose(p[12, :2], quad.mean(axis=0))
+    assert np.allclose(a["field:solution"], p[:, 0])
+    total = 0.0
+    for q in a["connectivity"].reshape(-1, 4):
+        x, y = p[q, 0], p[q, 1]
+        area = 0.5 * np.sum(x * np.roll(y, -1) - np.roll(x, -1) * y)
+        assert area > 0
+        total += area
+    c = quad[[0, 1, 3, 2]]
+    assert total == pytest.approx(0.5 * np.sum(c[:, 0] * np.roll(c[:, 1], -1) - np.roll(c[:, 0], -1) * c[:, 1]))
+
+
 def test_pvtu_master(outdir):
     root = ET.parse(os.path.join(outdir, "master.pvtu")).getroot()
     assert root.get("type") == "PUnstructuredGrid"
