@@ -116,10 +116,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr int PSN = coupling_plane_stride(NF); // node-coupling plane stride (kModeStream)
 
   __shared__ T sgrid[PPW * GSZ];
-  __shared__ T sred[SW * PPW * 4];        // inner-CG partial sums (type T)
-  __shared__ double sred64[SW * PPW * 2]; // FP64 residual norms
+  __shared__ T sred[problem_sums_slots<SW, PPW, 3>()];       // inner-CG partial sums (type T)
+  __shared__ double sred64[problem_sums_slots<SW, PPW, 1>()]; // FP64 residual norms
   __shared__ T sq[QLDS ? R * CPL * NT : 1];
-  __shared__ T sred2[QLDS ? SW * PPW * 2 : 1]; // second reduction of the two-reduction PCG
+  __shared__ T sred2[QLDS ? problem_sums_slots<SW, PPW, 1>() : 1]; // second reduction of the two-reduction PCG
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -560,12 +560,17 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       else
       while (true)
         {
+          // one problem per workgroup: irun is uniform, so the loop ends with
+          // it (no idle pass) and the writes below need no select
+          if constexpr (PPW == 1)
+            if (!irun)
+              break;
           // z = M r into the LDS grid (zero on padding/boundary slots)
 #pragma unroll
           for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
-              gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) : (T)0;
+              gown[i * GP + c] = (PPW == 1 || irun) ? rr[i][c] * (T)dget(i, c) : (T)0;
           __syncthreads();
           // w = K z, rows slid through a 3-row window; fused partial sums
           T wv[R][CPL];
@@ -636,7 +641,9 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
                   {
-                    const T z = rr[i][c] * (T)dget(i, c);
+                    // z = M r of this iteration: still in the own grid slot (FP64:
+                    // one LDS read instead of a conversion and a multiply)
+                    const T z = sizeof(T) == 8 ? gown[i * GP + c] : rr[i][c] * (T)dget(i, c);
                     pr[i][c] = z + tb * pr[i][c];
                     sr[i][c] = wv[i][c] + tb * sr[i][c];
                     xr[i][c] += ta * pr[i][c];

@@ -61,8 +61,8 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
   constexpr int GZ = (SZ * R + 2 > NP) ? SZ * R + 2 : NP;
 
   __shared__ T sgrid[GXY * GZ];
-  __shared__ T sred[NW * 4];        // inner-CG partial sums
-  __shared__ double sred64[NW * 2]; // FP64 residual norms
+  __shared__ T sred[problem_sums_slots<NW, 1, 3>()];       // inner-CG partial sums
+  __shared__ double sred64[problem_sums_slots<NW, 1, 1>()]; // FP64 residual norms
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int prob = xcd_block(blockIdx.x, gridDim.x);
@@ -236,10 +236,11 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       T gamma_prev = 1, alpha_prev = 1;
       while (true)
         {
-          // z = M r into the LDS grid (zero on padding/boundary slots)
+          // z = M r into the LDS grid (zero on padding/boundary slots); irun is
+          // uniform (one problem per workgroup) and true here
 #pragma unroll
           for (int i = 0; i < R; ++i)
-            gown[i * GXY] = irun ? rr[i] * dinv[i] : (T)0;
+            gown[i * GXY] = rr[i] * dinv[i];
           __syncthreads();
           // w = K z with the z-levels slid through a 3x3x3 window; fused partial sums
           T wv[R];
@@ -304,8 +305,8 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
             stencil(std::true_type());
           else
             stencil(std::false_type());
-          // one problem per workgroup: irun and the returned flag are uniform
-          const bool any = problem_sums<T, kWave, NW, 1, 3>(sums, irun, sred, wave, 0, lane);
+          // one problem per workgroup: irun is uniform, the loop ends with it
+          problem_sums<T, kWave, NW, 1, 3>(sums, irun, sred, wave, 0, lane);
           if (irun && (sums[2] <= inner_tol2 || it_total >= a.max_iter))
             irun = false;
           if (irun)
@@ -333,7 +334,7 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
               first = false;
               ++it_total;
             }
-          if (!any)
+          if (!irun)
             break;
         }
 #pragma unroll

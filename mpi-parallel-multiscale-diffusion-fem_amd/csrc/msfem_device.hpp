@@ -74,11 +74,15 @@ __device__ __forceinline__ double group_sum(double v)
 }
 
 // DPP move of a double (two 32-bit v_mov_dpp); CTRL is a GFX9 dpp_ctrl code.
+// Every control used here (quad_perm, row_ror) reads a valid lane of the same
+// row and all rows and banks are enabled, so no lane keeps an old value: the
+// move needs no initialised destination (bound_ctrl set, old undefined), which
+// saves a v_mov per DPP step.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
 {
-  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, true);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, true);
   return __hiloint2double(hi, lo);
 }
 
@@ -92,7 +96,7 @@ __device__ __forceinline__ double readlane_f64(double v, int lane)
 template <int CTRL>
 __device__ __forceinline__ float dpp_f32(float v)
 {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, true));
 }
 
 __device__ __forceinline__ float readlane_f32(float v, int lane)
@@ -218,12 +222,59 @@ __device__ __forceinline__ Cond4 cond_tensor(double a, double b, double c, doubl
           d - 0.5 * b};
 }
 
+// Sum over the 16 lanes of each DPP row, in every lane of the row.
+template <typename S>
+__device__ __forceinline__ S row_sum_any(S v)
+{
+  v += dpp_any<S, 0xB1>(v);
+  v += dpp_any<S, 0x4E>(v);
+  v += dpp_any<S, 0x124>(v);
+  v += dpp_any<S, 0x128>(v);
+  return v;
+}
+
+// Shared-memory slots (type S) that problem_sums<S, LW, SW, PPW, NV> uses.
+template <int SW, int PPW, int NV>
+__host__ __device__ constexpr int problem_sums_slots()
+{
+  return (PPW == 1 && SW > 1) ? 4 * SW * NV : SW * PPW * (NV + 1);
+}
+
 // Per-problem sums of NV values (type S) over LW lanes x SW waves, plus the OR
 // of a per-problem flag over the whole workgroup (the loop-continuation test).
-// red: [SW][PPW][NV + 1] values of type S.  Identical results in every lane.
+// red: problem_sums_slots<SW, PPW, NV>() values of type S.  Identical results
+// in every lane.
 template <typename S, int LW, int SW, int PPW, int NV>
 __device__ __forceinline__ bool problem_sums(S (&v)[NV], bool flag, S *red, int wave, int slot, int pl)
 {
+  if constexpr (PPW == 1 && SW > 1)
+    {
+      // One problem per workgroup: the flag is the same in every lane and is
+      // returned as it is.  The 4 x SW DPP-row totals go to LDS straight from
+      // the rows (no cross-row step inside the wave); lanes 0..15 then add the
+      // totals of rows l, l + 16, ... and a DPP sum over row 0 plus a
+      // readlane gives every lane the same value, in a fixed order.
+      static_assert(LW == kWave && SW <= 16, "whole waves, at most 16");
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        v[k] = row_sum_any<S>(v[k]);
+      if ((pl & 15) == 0)
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          red[(wave * 4 + (pl >> 4)) * NV + k] = v[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        {
+          S part = 0;
+#pragma unroll
+          for (int j = 0; j < 4 * SW; j += 16)
+            if (pl < 16 && j + pl < 4 * SW)
+              part += red[(j + pl) * NV + k];
+          v[k] = first_lanes_sum<S, 16>(part);
+        }
+      return flag;
+    }
 #pragma unroll
   for (int k = 0; k < NV; ++k)
     v[k] = group_sum_any<S, LW>(v[k]);
