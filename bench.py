@@ -184,6 +184,11 @@ def main():
 
     s = 3 ** dim
     f_iter = (2 * s - 1 + 10) * (nf_arr - 1) ** dim                      # per cell
+    # two-level classes: restriction and prolongation (2 FMA each per node) and
+    # the 49 x 49 coarse product per iteration
+    cs = np.array([ctx.uses_coarse_space(dim, int(k), opts[main_prec], case["coef_kind"], quads is not None)
+                   for k in nf_arr])
+    f_iter = f_iter + cs * (8 * (nf_arr - 1) ** dim + 2 * 49 * 49)
     w = 8 if main_prec == msfem.PREC_FP64 else 4
     stride = msfem.coef_stride(dim, case["coef_kind"])
     b_iter = 11 * (nf_arr + 1) ** dim * w + nf_arr ** dim * stride * w   # per cell
@@ -210,7 +215,8 @@ def main():
         "data": "synthetic (Philox-4x32-10 fields generated on the host, resident in HBM)",
         "config": {"workload": f"{args.workload}: {desc}", "coarse_cells": n, "dim": dim,
                    "fine_per_cell": classes, "local_solves": n * nb, "path": pname[main_prec],
-                   "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class"},
+                   "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class",
+                   "two_level_cells": int(cs.sum())},
         "roofline": {"bound": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get((args.workload, main_prec)),

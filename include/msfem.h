@@ -110,7 +110,13 @@ typedef struct msfem_solver_opts {
                                  by later calls with the same batch layout, pointers and
                                  options (msfem_ctx_profile is not updated; time it with
                                  msfem_ctx_mark) */
-  int32_t reserved[6];
+  int32_t coarse_space;       /* 1 (default): two-level preconditioner (Jacobi plus the
+                                 bilinear coarse space of an 8 x 8 coarse grid per local
+                                 problem) for 64^2 classes of box cells with per-fine-cell
+                                 scalar or tensor coefficients at the default launch
+                                 shapes; 0: Jacobi only.  Same converged values, fewer
+                                 iterations. */
+  int32_t reserved[5];
 } msfem_solver_opts;
 
 typedef struct msfem_batch {
@@ -149,6 +155,12 @@ int msfem_ctx_create(int device, msfem_ctx **out);
 int msfem_ctx_destroy(msfem_ctx *ctx);
 const char *msfem_last_error(const msfem_ctx *ctx); /* ctx may be NULL (creation errors) */
 void msfem_default_opts(msfem_solver_opts *opts);
+/* 1 if a class of nf fine cells per side with these options, coefficient kind
+ * and geometry (general: vertex cells) runs the two-level local solve on this
+ * context (its launch-shape table applied), else 0.  No counterpart in the
+ * reference; for callers that account for the solver's work. */
+int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_opts *opts, int coef_kind,
+                            int general);
 int msfem_version(void);
 
 /* Kernel timing of the batch builds (HIP events on the build's stream around

@@ -313,6 +313,26 @@ int build_mode(const msfem_batch *in, int nf)
   return class_mode(in->dim, in->coef_kind, nf, in->vertices != nullptr);
 }
 
+// Launch-shape id of size class k (the explicit option, else the table's entry).
+int class_variant(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, int nf)
+{
+  int variant = o->variant;
+  if (variant == 0)
+    {
+      auto it = ctx->variants.find(variant_key(in->dim, nf, o->precision, in->coef_kind));
+      if (it != ctx->variants.end())
+        variant = it->second;
+    }
+  return variant;
+}
+
+// Whether size class nf runs the two-level local solve.
+bool class_coarse(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, int nf)
+{
+  return o->coarse_space && in->dim == 2 &&
+         local2d_coarse_space(nf, build_mode(in, nf), class_variant(ctx, in, o, nf));
+}
+
 // Doubles of the FP64 streamed classes' q slots before size class k_end (each
 // such class owns a slice of one work array, so concurrent graph branches never
 // share slots).
@@ -372,12 +392,15 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.inner_rel_tol = o->inner_rel_tol;
   a.max_iter = o->max_iter;
   a.max_refine = o->max_refine;
-  int variant = o->variant;
-  if (variant == 0)
+  const int variant = class_variant(ctx, in, o, nf);
+  a.cinv = nullptr;
+  if (class_coarse(ctx, in, o, nf))
     {
-      auto it = ctx->variants.find(variant_key(in->dim, nf, o->precision, in->coef_kind));
-      if (it != ctx->variants.end())
-        variant = it->second;
+      a.cinv = ctx->bufs["coarse_inv"].p ? (const double *)ctx->bufs["coarse_inv"].p : nullptr;
+      if (!a.cinv)
+        return fail(ctx, MSFEM_ERR_HIP, "two-level scratch missing");
+      HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, in->coef_kind, in->box, in->coef, d_off(ctx, 0),
+                                          (double *)a.cinv, st));
     }
   if (in->dim == 2)
     HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
@@ -543,6 +566,17 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       if (rc)
         return rc;
     }
+  // inverse coarse matrices of the two-level class (allocated before any capture;
+  // only the 64^2 class can use them)
+  for (size_t k = 0; k < p.class_nf.size(); ++k)
+    if (class_coarse(ctx, in, o, p.class_nf[k]))
+      {
+        void *d;
+        rc = scratch(ctx, "coarse_inv", p.class_cells[k].size() * (size_t)kCoarseDofs * kCoarseDofs * sizeof(double),
+                     &d);
+        if (rc)
+          return rc;
+      }
   // global q slots of the FP64 streamed classes (allocated before any capture)
   double *work = nullptr;
   const size_t work_doubles = stream_work_doubles(ctx, in, o, p.class_nf.size());
@@ -635,6 +669,19 @@ int msfem_version(void) { return MSFEM_API_VERSION; }
 
 const char *msfem_last_error(const msfem_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
 
+int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_opts *opts, int coef_kind,
+                            int general)
+{
+  if (!ctx || !opts)
+    return 0;
+  msfem_batch b = {};
+  b.dim = dim;
+  b.coef_kind = coef_kind;
+  static const double one_vertex_set[8] = {0, 0, 1, 0, 0, 1, 1, 1};
+  b.vertices = general ? one_vertex_set : nullptr;
+  return class_coarse(ctx, &b, opts, nf) ? 1 : 0;
+}
+
 void msfem_default_opts(msfem_solver_opts *o)
 {
   if (!o)
@@ -648,6 +695,7 @@ void msfem_default_opts(msfem_solver_opts *o)
   o->abs_tol = 0.0;
   o->inner_rel_tol = 1e-4;
   o->variant = 0;
+  o->coarse_space = 1;
 }
 
 int msfem_ctx_create(int device, msfem_ctx **out)

@@ -18,6 +18,16 @@
 // workgroup barriers per iteration (z exchange, reduction).  s = K p is
 // carried by recurrence.
 //
+// Two-level variant (CS, 64^2 tiles, 8 waves x 8 rows): the preconditioner is
+// M^-1 = D^-1 + P A_c^-1 P^T with P the bilinear hats of the tile's 8 x 8
+// coarse grid (49 interior coarse nodes) and A_c = P^T K P, inverted once per
+// cell by coarse_setup_2d and kept in LDS.  Restriction: 8-lane DPP sums into
+// fixed LDS slots (each wave covers coarse rows w and w+1, each 8-lane group
+// coarse columns g and g+1); coarse product: 8 lanes per coarse row;
+// prolongation: bilinear interpolation of the 4 coarse values around the
+// lane.  Two more barriers per iteration; about 2.4x fewer iterations on
+// lognormal tiles.  The converged values are those of any preconditioner.
+//
 // Precision: an FP64 outer loop computes the true residual r = -(K phi)_I from
 // phi in global memory (FP64 coefficients) and checks
 //     ||r|| <= max(abs_tol, rel_tol ||b||);
@@ -96,9 +106,12 @@ __device__ __forceinline__ double node_diag64(const double *cg, int X, int Y)
 
 } // namespace
 
-template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE>
+template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE, bool CS>
 __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 {
+  static_assert(!CS || (NF == kCoarseH * (kCoarseSide + 1) && CPL == 1 && SW == 8 && R == kCoarseH &&
+                        (MODE == kModeScalar || MODE == kModeTensor)),
+                "the two-level form is laid out for 64^2 tiles, 8 waves of 8 rows, registers-cached coefficients");
   constexpr int NP = NF + 1;
   constexpr int LW = NF / CPL;
   static_assert(LW == 16 || LW == 32 || LW == 64, "lane group must be 16, 32 or 64 lanes");
@@ -120,6 +133,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   __shared__ double sred64[problem_sums_slots<SW, PPW, 1>()]; // FP64 residual norms
   __shared__ T sq[QLDS ? R * CPL * NT : 1];
   __shared__ T sred2[QLDS ? problem_sums_slots<SW, PPW, 1>() : 1]; // second reduction of the two-reduction PCG
+  // two-level: inverse coarse matrix, restriction partials [wave][2][group][2], coarse correction
+  __shared__ T sci[CS ? kCoarseDofs * kCoarseDofs : 1];
+  __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];
+  __shared__ T suc[CS ? kCoarseDofs : 1];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -269,6 +286,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // ---- initial state: LDS grid zero, phi = g on the boundary, x0 inside ----
   for (int k = tid; k < PPW * GSZ; k += blockDim.x)
     sgrid[k] = 0;
+  if constexpr (CS)
+    if (pvalid)
+      {
+        const double *ci = a.cinv + (size_t)(prob / a.nb_solve) * (kCoarseDofs * kCoarseDofs);
+        for (int k = tid; k < kCoarseDofs * kCoarseDofs; k += blockDim.x)
+          sci[k] = (T)ci[k];
+      }
   if (pvalid)
     for (int k = pl + LW * wave; k < NP * NP; k += LW * SW)
       {
@@ -335,6 +359,66 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
       __syncthreads();
     }
+
+  // ---- two-level pieces (lane = fine column X = pl + 1 = 8 g + xi) ----------------
+  const int cg8 = pl >> 3;                                         // 8-lane group: coarse columns g, g+1
+  const T wxa = (T)((double)(kCoarseH - 1 - (pl & 7)) / kCoarseH); // weight of coarse column g
+  const T wxb = (T)((double)(1 + (pl & 7)) / kCoarseH);            // weight of coarse column g + 1
+  auto spart_at = [](int w, int rs, int g, int ab) { return ((w * 2 + rs) * 8 + g) * 2 + ab; };
+  // restriction partials of r: rows of wave w feed coarse rows w (weight (7-i)/8)
+  // and w+1 (weight (i+1)/8)
+  auto restrict_r = [&](T (&rv)[R][CPL]) {
+    if constexpr (CS)
+      {
+        T s0 = 0, s1 = 0;
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+          {
+            s0 += rv[i][0] * (T)((double)(kCoarseH - 1 - i) / kCoarseH);
+            s1 += rv[i][0] * (T)((double)(i + 1) / kCoarseH);
+          }
+        T v4[4] = {s0 * wxa, s0 * wxb, s1 * wxa, s1 * wxb};
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+          v4[k] = oct_sum_any<T>(v4[k]);
+        if ((pl & 7) == 0)
+          {
+            spart[spart_at(wave, 0, cg8, 0)] = v4[0];
+            spart[spart_at(wave, 0, cg8, 1)] = v4[1];
+            spart[spart_at(wave, 1, cg8, 0)] = v4[2];
+            spart[spart_at(wave, 1, cg8, 1)] = v4[3];
+          }
+      }
+  };
+  // u_c = A_c^-1 (P^T r): coarse row i on lanes 8 i .. 8 i + 7, fixed order
+  auto coarse_solve = [&]() {
+    if constexpr (CS)
+      if (tid < kCoarseDofs * 8)
+        {
+          const int i = tid >> 3, c = tid & 7;
+          T acc = 0;
+#pragma unroll
+          for (int k = 0; k < 7; ++k)
+            {
+              const int j = c + 8 * k;
+              if (j < kCoarseDofs)
+                {
+                  const int jx = j % kCoarseSide + 1, jy = j / kCoarseSide + 1;
+                  const T rc = (spart[spart_at(jy, 0, jx, 0)] + spart[spart_at(jy, 0, jx - 1, 1)]) +
+                               (spart[spart_at(jy - 1, 1, jx, 0)] + spart[spart_at(jy - 1, 1, jx - 1, 1)]);
+                  acc += sci[i * kCoarseDofs + j] * rc;
+                }
+            }
+          acc = oct_sum_any<T>(acc);
+          if (c == 0)
+            suc[i] = acc;
+        }
+  };
+  // coarse value at coarse node (jx, jy); 0 on the tile boundary
+  auto ucoarse = [&](int jx, int jy) -> T {
+    return (jx >= 1 && jx <= kCoarseSide && jy >= 1 && jy <= kCoarseSide) ? suc[(jy - 1) * kCoarseSide + jx - 1]
+                                                                          : (T)0;
+  };
 
   T xr[R][CPL], rr[R][CPL], pr[R][CPL], sr[R][CPL];
   int it_total = 0;
@@ -558,6 +642,9 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
             }
         }
       else
+      {
+      if constexpr (CS)
+        restrict_r(rr);
       while (true)
         {
           // one problem per workgroup: irun is uniform, so the loop ends with
@@ -565,12 +652,28 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if constexpr (PPW == 1)
             if (!irun)
               break;
-          // z = M r into the LDS grid (zero on padding/boundary slots)
+          if constexpr (CS)
+            {
+              __syncthreads(); // restriction partials of r complete
+              coarse_solve();
+              __syncthreads(); // coarse correction complete
+              // z = D^-1 r + P u_c (the interpolation vanishes on the tile boundary)
+              const T c0 = ucoarse(cg8, wave) * wxa + ucoarse(cg8 + 1, wave) * wxb;
+              const T c1 = ucoarse(cg8, wave + 1) * wxa + ucoarse(cg8 + 1, wave + 1) * wxb;
 #pragma unroll
-          for (int i = 0; i < R; ++i)
+              for (int i = 0; i < R; ++i)
+                gown[i * GP] = rr[i][0] * (T)dget(i, 0) + (c0 * (T)((double)(kCoarseH - 1 - i) / kCoarseH) +
+                                                            c1 * (T)((double)(i + 1) / kCoarseH));
+            }
+          else
+            {
+              // z = M r into the LDS grid (zero on padding/boundary slots)
 #pragma unroll
-            for (int c = 0; c < CPL; ++c)
-              gown[i * GP + c] = (PPW == 1 || irun) ? rr[i][c] * (T)dget(i, c) : (T)0;
+              for (int i = 0; i < R; ++i)
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                  gown[i * GP + c] = (PPW == 1 || irun) ? rr[i][c] * (T)dget(i, c) : (T)0;
+            }
           __syncthreads();
           // w = K z, rows slid through a 3-row window; fused partial sums
           T wv[R][CPL];
@@ -643,7 +746,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   {
                     // z = M r of this iteration: still in the own grid slot (FP64:
                     // one LDS read instead of a conversion and a multiply)
-                    const T z = sizeof(T) == 8 ? gown[i * GP + c] : rr[i][c] * (T)dget(i, c);
+                    const T z = (CS || sizeof(T) == 8) ? gown[i * GP + c] : rr[i][c] * (T)dget(i, c);
                     pr[i][c] = z + tb * pr[i][c];
                     sr[i][c] = wv[i][c] + tb * sr[i][c];
                     xr[i][c] += ta * pr[i][c];
@@ -653,10 +756,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               alpha_prev = alpha;
               first = false;
               ++it_total;
+              if constexpr (CS)
+                restrict_r(rr);
             }
           if (!any)
             break;
         }
+      }
       // ---- phi += d in FP64 (the FP64 streamed form has already added it) ---------
       if (run && !(MODE == kModeStream && sizeof(T) == 8))
         {
@@ -842,6 +948,113 @@ __global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, c
     }
 }
 
+// Two-level setup: A_c = P^T K P over the 49 interior nodes of the 8 x 8
+// coarse grid of a 64^2 tile, then A_c^-1 (Gauss-Jordan; A_c is SPD, no
+// pivoting), one workgroup of kSetupThreads per cell.  K_c of every fine cell
+// is its QGauss(2) element matrix; each fine cell lies in one coarse block,
+// whose 4 x 4 block matrix sum_c W^T K_c W (W: bilinear weights of the fine
+// cell's vertices to the block corners) is summed by 4 threads (two fine rows
+// each) and combined in a fixed order.  cinv: [cells in class][49][49].
+constexpr int kSetupThreads = 256;
+
+__global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind,
+                                                                const double *box, const double *coef,
+                                                                const long long *coef_off, double *cinv)
+{
+  constexpr int NF = kCoarseH * (kCoarseSide + 1), H = kCoarseH, NB = kCoarseSide + 1, N = kCoarseDofs;
+  constexpr int W2 = 2 * N; // augmented row [A | I]
+  __shared__ double sh[N * W2];        // block partials [64][4][16], then the augmented matrix
+  __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
+  __shared__ double fac[N], rowk[W2];  // Gauss-Jordan: column k, row k
+  static_assert(NB * NB * 4 * 16 <= N * W2, "block partials fit the matrix storage");
+  const int cc = blockIdx.x, t = threadIdx.x;
+  if (cc >= n_cells_class)
+    return;
+  const int cell = cells[cc];
+  const double *bx = box + (size_t)cell * 4;
+  const double hx = (bx[1] - bx[0]) / NF, hy = (bx[3] - bx[2]) / NF;
+  const double *cf = coef + coef_off[cell];
+  {
+    const int b = t >> 2, part = t & 3, bxi = b % NB, byi = b / NB;
+    double acc[4][4] = {};
+    for (int ly = 2 * part; ly < 2 * part + 2; ++ly)
+      for (int lx = 0; lx < H; ++lx)
+        {
+          double K[4][4];
+          qg_element_2d(coef_kind, cf, (byi * H + ly) * NF + bxi * H + lx, hx, hy, K);
+          double Wt[4][4]; // [vertex a][block corner m]
+          for (int v = 0; v < 4; ++v)
+            {
+              const double px = (double)(lx + (v & 1)) / H, py = (double)(ly + (v >> 1)) / H;
+              for (int m = 0; m < 4; ++m)
+                Wt[v][m] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
+            }
+          for (int m = 0; m < 4; ++m)
+            for (int n = 0; n < 4; ++n)
+              {
+                double sum = 0;
+                for (int u = 0; u < 4; ++u)
+                  for (int v = 0; v < 4; ++v)
+                    sum += Wt[u][m] * K[u][v] * Wt[v][n];
+                acc[m][n] += sum;
+              }
+        }
+    for (int m = 0; m < 16; ++m)
+      sh[(b * 4 + part) * 16 + m] = acc[m >> 2][m & 3];
+  }
+  __syncthreads();
+  for (int e = t; e < NB * NB * 16; e += kSetupThreads)
+    {
+      const int b = e >> 4, m = e & 15;
+      kb[e] = ((sh[(b * 4 + 0) * 16 + m] + sh[(b * 4 + 1) * 16 + m]) + sh[(b * 4 + 2) * 16 + m]) +
+              sh[(b * 4 + 3) * 16 + m];
+    }
+  __syncthreads();
+  // augmented [A_c | I]; A_c(I, J) = sum over the blocks holding both coarse nodes
+  for (int e = t; e < N * W2; e += kSetupThreads)
+    {
+      const int i = e / W2, col = e % W2;
+      double v = 0;
+      if (col < N)
+        {
+          const int ix = i % kCoarseSide + 1, iy = i / kCoarseSide + 1;
+          const int jx = col % kCoarseSide + 1, jy = col / kCoarseSide + 1;
+          for (int dy = 0; dy < 2; ++dy)
+            for (int dx = 0; dx < 2; ++dx)
+              {
+                const int bxi = ix - 1 + dx, byi = iy - 1 + dy; // block with corner I
+                const int mi = (ix - bxi) + 2 * (iy - byi), ox = jx - bxi, oy = jy - byi;
+                if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1)
+                  v += kb[(byi * NB + bxi) * 16 + mi * 4 + ox + 2 * oy];
+              }
+        }
+      else
+        v = (col - N == i) ? 1.0 : 0.0;
+      sh[e] = v;
+    }
+  __syncthreads();
+  for (int k = 0; k < N; ++k)
+    {
+      if (t < N)
+        fac[t] = sh[t * W2 + k];
+      else if (t < N + W2)
+        rowk[t - N] = sh[k * W2 + t - N];
+      __syncthreads();
+      // thread (column c, half h) updates rows of its half: no two threads write one entry
+      if (t < 2 * W2)
+        {
+          const int c = t % W2, h = t / W2, i0 = h ? (N + 1) / 2 : 0, i1 = h ? N : (N + 1) / 2;
+          const double rk = rowk[c], piv = fac[k];
+          for (int i = i0; i < i1; ++i)
+            sh[i * W2 + c] = (i == k) ? rk / piv : sh[i * W2 + c] - (fac[i] / piv) * rk;
+        }
+      __syncthreads();
+    }
+  double *out = cinv + (size_t)cc * N * N;
+  for (int e = t; e < N * N; e += kSetupThreads)
+    out[e] = sh[(e / N) * W2 + N + e % N];
+}
+
 // ---- launch table -----------------------------------------------------------------
 // Variant ids per fine resolution (the autotuner times every id and keeps the
 // fastest per class; id 0 is the built-in default):
@@ -855,7 +1068,7 @@ __global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, c
 //   128: 1 = (SW 16, R 8, CPL 2, 4)
 namespace {
 
-template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE>
+template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE, bool CS = false>
 hipError_t launch_one(const LocalArgs &a, hipStream_t st)
 {
   constexpr int PPW = kWave / (NF / CPL);
@@ -864,7 +1077,7 @@ hipError_t launch_one(const LocalArgs &a, hipStream_t st)
   const int grid = ((a.n_problems + PPW - 1) / PPW + 7) / 8 * 8;
   if (a.n_problems == 0)
     return hipSuccess;
-  hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE, WPE>), dim3(grid), dim3(kWave * SW), 0, st, a);
+  hipLaunchKernelGGL((local_solve_2d<T, NF, CPL, SW, R, MODE, WPE, CS>), dim3(grid), dim3(kWave * SW), 0, st, a);
   return hipGetLastError();
 }
 
@@ -888,6 +1101,11 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
             default: return launch_one<T, 32, 1, 4, 8, MODE, 2>(a, st);
           }
       case 64:
+        // the (8 waves, 8 rows) shapes also come in the two-level form (a.cinv set)
+        if constexpr (MODE == kModeScalar || MODE == kModeTensor)
+          if (a.cinv)
+            return variant == 4 ? launch_one<T, 64, 1, 8, 8, MODE, 4, true>(a, st)
+                                : launch_one<T, 64, 1, 8, 8, MODE, 2, true>(a, st);
         switch (variant)
           {
             case 2: return launch_one<T, 64, 1, 4, 16, MODE, 1>(a, st);
@@ -922,6 +1140,11 @@ hipError_t launch_stream(int nf, int variant, const LocalArgs &a, hipStream_t st
       default:
         return hipErrorInvalidValue;
     }
+}
+
+bool local2d_coarse_space(int nf, int mode, int variant)
+{
+  return nf == 64 && (mode == kModeScalar || mode == kModeTensor) && (variant == 0 || variant == 1 || variant == 4);
 }
 
 int local2d_variants(int nf)
@@ -989,6 +1212,16 @@ hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int 
     grid = 65536;
   hipLaunchKernelGGL(prep_cond_2d, dim3((unsigned)grid), dim3(block), 0, st, n_cells_class, cells, nf, coef_kind,
                      box, verts, coef, coef_off, cond, cond_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind, const double *box,
+                                  const double *coef, const long long *coef_off, double *cinv, hipStream_t st)
+{
+  if (n_cells_class == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(coarse_setup_2d, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells, coef_kind,
+                     box, coef, coef_off, cinv);
   return hipGetLastError();
 }
 

@@ -29,6 +29,11 @@ __host__ __device__ constexpr int coupling_plane_stride(int nf) { return (nf + 1
 // nf^2 for every launch shape with SW * R = nf and nf = 64 CPL lanes.
 __host__ __device__ constexpr int stream_q_slots(int nf) { return nf * nf; }
 
+// Two-level preconditioner of the 64^2 local problems: bilinear coarse space
+// of the 8 x 8 coarse grid (spacing kCoarseH fine cells) of the tile, its
+// kCoarseSide^2 interior nodes (zero Dirichlet on the tile boundary).
+constexpr int kCoarseH = 8, kCoarseSide = 7, kCoarseDofs = kCoarseSide * kCoarseSide;
+
 // Workgroup -> block index such that the blocks one XCD receives (hardware
 // dispatch is round-robin over the 8 XCDs) hold consecutive problems: the
 // bases of one coarse cell then run on one XCD at the same time and share its
@@ -55,6 +60,7 @@ struct LocalArgs {
   double *bases;              // concatenated basis blocks (working FP64 solution + output)
   const long long *bas_off;   // [n_cells]
   const double *x0;           // optional initial guesses (bases layout) or nullptr
+  const double *cinv;         // two-level: [cells in class][kCoarseDofs^2] inverse coarse matrices, or nullptr
   int *iters;                 // [n_cells][nb] or nullptr
   int *status;                // [n_cells][nb] or nullptr
   double *resid;              // [n_cells][nb] or nullptr
@@ -220,6 +226,18 @@ __device__ __forceinline__ Cond4 cond_tensor(double a, double b, double c, doubl
   const double d = a * rho * (1.0 / 6.0) + c * ir * (1.0 / 6.0);
   return {a * rho * (1.0 / 3.0) - c * ir * (1.0 / 6.0), c * ir * (1.0 / 3.0) - a * rho * (1.0 / 6.0), d + 0.5 * b,
           d - 0.5 * b};
+}
+
+// Sum over each group of 8 consecutive lanes, in every lane of the group
+// (quad_perm [1,0,3,2], [2,3,0,1], row_half_mirror); bitwise identical in the 8
+// lanes since every step adds the same two values in either order.
+template <typename S>
+__device__ __forceinline__ S oct_sum_any(S v)
+{
+  v += dpp_any<S, 0xB1>(v);
+  v += dpp_any<S, 0x4E>(v);
+  v += dpp_any<S, 0x141>(v);
+  return v;
 }
 
 // Sum over the 16 lanes of each DPP row, in every lane of the row.

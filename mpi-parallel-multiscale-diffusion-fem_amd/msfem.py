@@ -25,6 +25,7 @@ BC_ZERO_DIRICHLET, BC_REFERENCE = 0, 1
 # Every symbol include/msfem.h declares.
 EXPORTS = [
     "msfem_ctx_create", "msfem_ctx_destroy", "msfem_last_error", "msfem_default_opts", "msfem_version",
+    "msfem_uses_coarse_space",
     "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
     "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
@@ -48,7 +49,7 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("max_refine", ctypes.c_int32),
                 ("partition_of_unity", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double),
                 ("inner_rel_tol", ctypes.c_double), ("variant", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 6)]
+                ("coarse_space", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 class Batch(ctypes.Structure):
@@ -92,6 +93,7 @@ def lib():
         L.msfem_set_variant.argtypes = [vp, i32, i32, i32, i32, i32]
         L.msfem_default_opts.argtypes = [ctypes.POINTER(SolverOpts)]
         L.msfem_default_opts.restype = None
+        L.msfem_uses_coarse_space.argtypes = [vp, i32, i32, ctypes.POINTER(SolverOpts), i32, i32]
         L.msfem_batch_build.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchOut)]
         L.msfem_batch_build_device.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts),
                                                ctypes.POINTER(BatchOut), vp]
@@ -183,6 +185,10 @@ class Context:
         for e in table["entries"]:
             self.set_variant(e["dim"], e["nf"], e["precision"], e["coef_kind"], e["variant"])
         return len(table["entries"])
+
+    def uses_coarse_space(self, dim, nf, opts, coef_kind, general=False):
+        """Whether a class of nf fine cells per side runs the two-level local solve here."""
+        return bool(lib().msfem_uses_coarse_space(self.h, dim, nf, ctypes.byref(opts), coef_kind, 1 if general else 0))
 
     def set_profiling(self, on=True):
         _check(lib().msfem_ctx_set_profiling(self.h, 1 if on else 0), self.h)

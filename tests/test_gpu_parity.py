@@ -249,3 +249,30 @@ def test_every_launch_shape(ctx, prec):
         say(f"launch shape 3d nf=16 variant={v} prec={prec}")
         c3 = C.lognormal_cells(dim=3, nc=16, nf=16, seed=5, which=np.array([7, 100]))
         check_against_oracle(c3, C.gpu_build(ctx, c3, precision=prec, variant=v), C.oracle_build(c3))
+
+
+@pytest.mark.parametrize("kind", ["lognormal", "tensor"])
+def test_two_level_preconditioner(ctx, kind):
+    """64^2 cells with the two-level local solve (coarse_space=1, the default) and
+    with Jacobi alone (coarse_space=0): the same values (oracle, 1e-10; FP32 +
+    refinement at 1e-8 within 1e-6 energy norm) and at most 0.6x the iterations."""
+    say(f"two-level preconditioner, {kind} 64^2")
+    if kind == "lognormal":
+        case = C.lognormal_cells(which=np.array([3, 700, 2222, 4000]))
+    else:
+        case = C.tensor_cells(nc=128, which=np.arange(8), n_fine=[64] * 8)
+    ref = C.oracle_build(case)
+    its = {}
+    for cs in (0, 1):
+        for variant in (1, 4):  # both launch shapes of the two-level form
+            out = C.gpu_build(ctx, case, coarse_space=cs, variant=variant)
+            check_against_oracle(case, out, ref)
+            its[cs, variant] = out["iters"].mean()
+        assert ctx.uses_coarse_space(2, 64, msfem.default_opts(coarse_space=cs, variant=1), case["coef_kind"]) == bool(cs)
+        out = C.gpu_build(ctx, case, coarse_space=cs, precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+        assert (out["status"] == 0).all() and C.energy_error(case, out["bases"], ref["bases"]) < 1e-6
+    say(f"  mean iterations: Jacobi {its[0, 1]:.1f}, two-level {its[1, 1]:.1f} / {its[1, 4]:.1f}")
+    assert its[1, 1] < 0.6 * its[0, 1] and its[1, 4] < 0.6 * its[0, 4]
+    # shapes outside the two-level table keep Jacobi
+    assert not ctx.uses_coarse_space(2, 64, msfem.default_opts(variant=2), case["coef_kind"])
+    assert not ctx.uses_coarse_space(2, 32, msfem.default_opts(), case["coef_kind"])
