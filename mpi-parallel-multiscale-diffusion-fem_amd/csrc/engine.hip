@@ -333,6 +333,18 @@ bool class_coarse(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
          local2d_coarse_space(nf, build_mode(in, nf), class_variant(ctx, in, o, nf));
 }
 
+// Doubles of the two-level classes' inverse coarse matrices before size class
+// k_end (each such class owns a slice: concurrent graph branches share none).
+size_t coarse_inv_doubles(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k_end)
+{
+  const Plan &p = ctx->plan;
+  size_t n = 0;
+  for (size_t k = 0; k < k_end && k < p.class_nf.size(); ++k)
+    if (class_coarse(ctx, in, o, p.class_nf[k]))
+      n += p.class_cells[k].size() * (size_t)kCoarseDofs * kCoarseDofs;
+  return n;
+}
+
 // Doubles of the FP64 streamed classes' q slots before size class k_end (each
 // such class owns a slice of one work array, so concurrent graph branches never
 // share slots).
@@ -396,11 +408,12 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.cinv = nullptr;
   if (class_coarse(ctx, in, o, nf))
     {
-      a.cinv = ctx->bufs["coarse_inv"].p ? (const double *)ctx->bufs["coarse_inv"].p : nullptr;
-      if (!a.cinv)
+      double *base = (double *)ctx->bufs["coarse_inv"].p;
+      if (!base)
         return fail(ctx, MSFEM_ERR_HIP, "two-level scratch missing");
-      HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, in->coef_kind, in->box, in->coef, d_off(ctx, 0),
-                                          (double *)a.cinv, st));
+      a.cinv = base + coarse_inv_doubles(ctx, in, o, k);
+      HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
+                                          d_off(ctx, 0), (double *)a.cinv, st));
     }
   if (in->dim == 2)
     HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
@@ -566,17 +579,15 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       if (rc)
         return rc;
     }
-  // inverse coarse matrices of the two-level class (allocated before any capture;
-  // only the 64^2 class can use them)
-  for (size_t k = 0; k < p.class_nf.size(); ++k)
-    if (class_coarse(ctx, in, o, p.class_nf[k]))
-      {
-        void *d;
-        rc = scratch(ctx, "coarse_inv", p.class_cells[k].size() * (size_t)kCoarseDofs * kCoarseDofs * sizeof(double),
-                     &d);
-        if (rc)
-          return rc;
-      }
+  // inverse coarse matrices of the two-level classes (allocated before any capture)
+  const size_t cinv_doubles = coarse_inv_doubles(ctx, in, o, p.class_nf.size());
+  if (cinv_doubles > 0)
+    {
+      void *d;
+      rc = scratch(ctx, "coarse_inv", cinv_doubles * sizeof(double), &d);
+      if (rc)
+        return rc;
+    }
   // global q slots of the FP64 streamed classes (allocated before any capture)
   double *work = nullptr;
   const size_t work_doubles = stream_work_doubles(ctx, in, o, p.class_nf.size());

@@ -109,13 +109,12 @@ __device__ __forceinline__ double node_diag64(const double *cg, int X, int Y)
 template <typename T, int NF, int CPL, int SW, int R, int MODE, int WPE, bool CS>
 __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 {
-  static_assert(!CS || (NF == kCoarseH * (kCoarseSide + 1) && CPL == 1 && SW == 8 && R == kCoarseH &&
-                        (MODE == kModeScalar || MODE == kModeTensor)),
-                "the two-level form is laid out for 64^2 tiles, 8 waves of 8 rows, registers-cached coefficients");
   constexpr int NP = NF + 1;
   constexpr int LW = NF / CPL;
   static_assert(LW == 16 || LW == 32 || LW == 64, "lane group must be 16, 32 or 64 lanes");
   constexpr int PPW = kWave / LW;
+  static_assert(!CS || ((NF == 64 || NF == 128) && PPW == 1 && MODE != kModeGlobal),
+                "the two-level form: 64^2 and 128^2 tiles, one problem per workgroup");
   static_assert(SW * R >= NF - 1, "rows not covered");
   // LDS grid of one problem: pitch GP, GR rows; the padding column NF+1 and
   // rows beyond NF stay zero so that every window read is in bounds.
@@ -135,7 +134,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   __shared__ T sred2[QLDS ? problem_sums_slots<SW, PPW, 1>() : 1]; // second reduction of the two-reduction PCG
   // two-level: inverse coarse matrix, restriction partials [wave][2][group][2], coarse correction
   __shared__ T sci[CS ? kCoarseDofs * kCoarseDofs : 1];
-  __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];
+  __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];  // [wave][coarse row 0/1][group][column 0/1]
   __shared__ T suc[CS ? kCoarseDofs : 1];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -360,24 +359,47 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       __syncthreads();
     }
 
-  // ---- two-level pieces (lane = fine column X = pl + 1 = 8 g + xi) ----------------
-  const int cg8 = pl >> 3;                                         // 8-lane group: coarse columns g, g+1
-  const T wxa = (T)((double)(kCoarseH - 1 - (pl & 7)) / kCoarseH); // weight of coarse column g
-  const T wxb = (T)((double)(1 + (pl & 7)) / kCoarseH);            // weight of coarse column g + 1
+  // ---- two-level pieces -------------------------------------------------------------
+  // Coarse spacing CH = NF / 8 fine cells.  An 8-lane group g holds the fine
+  // columns CH g + 1 .. CH g + CH (8 lanes x CPL columns), i.e. it feeds coarse
+  // columns g and g + 1; the rows of wave w lie in coarse band w / WB (WB = CH / R
+  // waves per band) and feed coarse rows w / WB and w / WB + 1.
+  constexpr int CH = NF / 8, WB = CH / R > 0 ? CH / R : 1;
+  static_assert(!CS || (CH * 8 == NF && 8 * CPL == CH && WB * R == CH && SW == 8 * WB),
+                "two-level layout: 8-lane groups of CH columns, whole waves per coarse band");
+  const int cg8 = pl >> 3, cband = wave / WB;
+  const int yy0 = 1 + (wave % WB) * R; // row i: yy = yy0 + i in 1..CH above the band's lower coarse row
+  T wxa[CPL], wxb[CPL];                 // weights of coarse columns g, g + 1
+#pragma unroll
+  for (int c = 0; c < CPL; ++c)
+    {
+      const int xx = (pl & 7) * CPL + c + 1;
+      wxa[c] = (T)((double)(CH - xx) / CH);
+      wxb[c] = (T)((double)xx / CH);
+    }
+  auto wya = [&](int i) -> T { return (T)((double)(CH - yy0 - i) / CH); };
+  auto wyb = [&](int i) -> T { return (T)((double)(yy0 + i) / CH); };
   auto spart_at = [](int w, int rs, int g, int ab) { return ((w * 2 + rs) * 8 + g) * 2 + ab; };
-  // restriction partials of r: rows of wave w feed coarse rows w (weight (7-i)/8)
-  // and w+1 (weight (i+1)/8)
+  // restriction partials of r (8-lane DPP sums; lane 0 of each group writes)
   auto restrict_r = [&](T (&rv)[R][CPL]) {
     if constexpr (CS)
       {
-        T s0 = 0, s1 = 0;
+        T v4[4] = {0, 0, 0, 0};
 #pragma unroll
-        for (int i = 0; i < R; ++i)
+        for (int c = 0; c < CPL; ++c)
           {
-            s0 += rv[i][0] * (T)((double)(kCoarseH - 1 - i) / kCoarseH);
-            s1 += rv[i][0] * (T)((double)(i + 1) / kCoarseH);
+            T s0 = 0, s1 = 0;
+#pragma unroll
+            for (int i = 0; i < R; ++i)
+              {
+                s0 += rv[i][c] * wya(i);
+                s1 += rv[i][c] * wyb(i);
+              }
+            v4[0] += s0 * wxa[c];
+            v4[1] += s0 * wxb[c];
+            v4[2] += s1 * wxa[c];
+            v4[3] += s1 * wxb[c];
           }
-        T v4[4] = {s0 * wxa, s0 * wxb, s1 * wxa, s1 * wxb};
 #pragma unroll
         for (int k = 0; k < 4; ++k)
           v4[k] = oct_sum_any<T>(v4[k]);
@@ -404,8 +426,14 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               if (j < kCoarseDofs)
                 {
                   const int jx = j % kCoarseSide + 1, jy = j / kCoarseSide + 1;
-                  const T rc = (spart[spart_at(jy, 0, jx, 0)] + spart[spart_at(jy, 0, jx - 1, 1)]) +
-                               (spart[spart_at(jy - 1, 1, jx, 0)] + spart[spart_at(jy - 1, 1, jx - 1, 1)]);
+                  T rc = 0;
+#pragma unroll
+                  for (int b = 0; b < WB; ++b)
+                    {
+                      const int wl = jy * WB + b, wu = (jy - 1) * WB + b; // band jy (lower row), band jy-1 (upper)
+                      rc += (spart[spart_at(wl, 0, jx, 0)] + spart[spart_at(wl, 0, jx - 1, 1)]) +
+                            (spart[spart_at(wu, 1, jx, 0)] + spart[spart_at(wu, 1, jx - 1, 1)]);
+                    }
                   acc += sci[i * kCoarseDofs + j] * rc;
                 }
             }
@@ -418,6 +446,17 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   auto ucoarse = [&](int jx, int jy) -> T {
     return (jx >= 1 && jx <= kCoarseSide && jy >= 1 && jy <= kCoarseSide) ? suc[(jy - 1) * kCoarseSide + jx - 1]
                                                                           : (T)0;
+  };
+  // the lane's interpolation rows: z_c(i, c) = ci0[c] wya(i) + ci1[c] wyb(i)
+  auto interp_cols = [&](T (&ci0)[CPL], T (&ci1)[CPL]) {
+    const T u00 = ucoarse(cg8, cband), u10 = ucoarse(cg8 + 1, cband);
+    const T u01 = ucoarse(cg8, cband + 1), u11 = ucoarse(cg8 + 1, cband + 1);
+#pragma unroll
+    for (int c = 0; c < CPL; ++c)
+      {
+        ci0[c] = u00 * wxa[c] + u10 * wxb[c];
+        ci1[c] = u01 * wxa[c] + u11 * wxb[c];
+      }
   };
 
   T xr[R][CPL], rr[R][CPL], pr[R][CPL], sr[R][CPL];
@@ -497,13 +536,28 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           // problem's slots of a global work array (thread-private, coalesced)
           // and x is added straight into the FP64 solution phi.
           static_assert(PPW == 1, "kModeStream runs one problem per workgroup");
+          // two-level: coarse interpolation rows of the current z (zero without it)
+          T ci0[CPL], ci1[CPL];
+#pragma unroll
+          for (int c = 0; c < CPL; ++c)
+            ci0[c] = ci1[c] = 0;
+          if constexpr (CS) // irun is uniform (one problem per workgroup): true here
+            {
+              restrict_r(rr);
+              __syncthreads(); // restriction partials complete
+              coarse_solve();
+              __syncthreads(); // coarse correction complete
+              interp_cols(ci0, ci1);
+            }
           T v2[2] = {0, 0}; // (r, z), (r, r)
 #pragma unroll
           for (int i = 0; i < R; ++i)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
               {
-                const T z = rr[i][c] * (T)dget(i, c);
+                T z = rr[i][c] * (T)dget(i, c);
+                if constexpr (CS)
+                  z += ci0[c] * wya(i) + ci1[c] * wyb(i);
                 gown[i * GP + c] = irun ? z : (T)0;
                 v2[0] += rr[i][c] * z;
                 v2[1] += rr[i][c] * rr[i][c];
@@ -621,10 +675,30 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                               ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
                           }
                         rr[i][c] -= alpha * q;
-                        const T z = rr[i][c] * (T)dget(i, c);
-                        v2[0] += rr[i][c] * z;
-                        v2[1] += rr[i][c] * rr[i][c];
+                        if constexpr (!CS)
+                          {
+                            const T z = rr[i][c] * (T)dget(i, c);
+                            v2[0] += rr[i][c] * z;
+                            v2[1] += rr[i][c] * rr[i][c];
+                          }
                       }
+                  if constexpr (CS) // z = D^-1 r + P A_c^-1 P^T r of the new r (uniform branch)
+                    {
+                      restrict_r(rr);
+                      __syncthreads(); // restriction partials complete
+                      coarse_solve();
+                      __syncthreads(); // coarse correction complete
+                      interp_cols(ci0, ci1);
+#pragma unroll
+                      for (int i = 0; i < R; ++i)
+#pragma unroll
+                        for (int c = 0; c < CPL; ++c)
+                          {
+                            const T z = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
+                            v2[0] += rr[i][c] * z;
+                            v2[1] += rr[i][c] * rr[i][c];
+                          }
+                    }
                   ++it_total;
                 }
               any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
@@ -638,7 +712,12 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               for (int i = 0; i < R; ++i)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                  gown[i * GP + c] = irun ? rr[i][c] * (T)dget(i, c) + beta * gown[i * GP + c] : (T)0;
+                  {
+                    T z = rr[i][c] * (T)dget(i, c);
+                    if constexpr (CS)
+                      z += ci0[c] * wya(i) + ci1[c] * wyb(i);
+                    gown[i * GP + c] = irun ? z + beta * gown[i * GP + c] : (T)0;
+                  }
             }
         }
       else
@@ -658,12 +737,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               coarse_solve();
               __syncthreads(); // coarse correction complete
               // z = D^-1 r + P u_c (the interpolation vanishes on the tile boundary)
-              const T c0 = ucoarse(cg8, wave) * wxa + ucoarse(cg8 + 1, wave) * wxb;
-              const T c1 = ucoarse(cg8, wave + 1) * wxa + ucoarse(cg8 + 1, wave + 1) * wxb;
+              T ci0[CPL], ci1[CPL];
+              interp_cols(ci0, ci1);
 #pragma unroll
               for (int i = 0; i < R; ++i)
-                gown[i * GP] = rr[i][0] * (T)dget(i, 0) + (c0 * (T)((double)(kCoarseH - 1 - i) / kCoarseH) +
-                                                            c1 * (T)((double)(i + 1) / kCoarseH));
+#pragma unroll
+                for (int c = 0; c < CPL; ++c)
+                  gown[i * GP + c] = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
             }
           else
             {
@@ -949,19 +1029,24 @@ __global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, c
 }
 
 // Two-level setup: A_c = P^T K P over the 49 interior nodes of the 8 x 8
-// coarse grid of a 64^2 tile, then A_c^-1 (Gauss-Jordan; A_c is SPD, no
-// pivoting), one workgroup of kSetupThreads per cell.  K_c of every fine cell
-// is its QGauss(2) element matrix; each fine cell lies in one coarse block,
-// whose 4 x 4 block matrix sum_c W^T K_c W (W: bilinear weights of the fine
-// cell's vertices to the block corners) is summed by 4 threads (two fine rows
-// each) and combined in a fixed order.  cinv: [cells in class][49][49].
+// coarse grid of an NF^2 tile (coarse spacing NF / 8), then A_c^-1
+// (Gauss-Jordan; A_c is SPD, no pivoting), one workgroup of kSetupThreads per
+// cell.  K_c of every fine cell is its QGauss(2) element matrix (box cells or
+// general quadrilaterals); each fine cell lies in one coarse block, whose 4 x 4
+// block matrix sum_c W^T K_c W (W: bilinear weights of the fine cell's
+// vertices to the block corners, in the tile's index space) is summed by 4
+// threads (a quarter of the block's fine rows each) and combined in a fixed
+// order.  cinv: [cells in class][49][49].
 constexpr int kSetupThreads = 256;
 
+template <int NF>
 __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind,
-                                                                const double *box, const double *coef,
-                                                                const long long *coef_off, double *cinv)
+                                                                const double *box, const double *verts,
+                                                                const double *coef, const long long *coef_off,
+                                                                double *cinv)
 {
-  constexpr int NF = kCoarseH * (kCoarseSide + 1), H = kCoarseH, NB = kCoarseSide + 1, N = kCoarseDofs;
+  constexpr int H = NF / 8, NB = kCoarseSide + 1, N = kCoarseDofs;
+  static_assert(H % 4 == 0 && NB * H == NF, "four threads per coarse block");
   constexpr int W2 = 2 * N; // augmented row [A | I]
   __shared__ double sh[N * W2];        // block partials [64][4][16], then the augmented matrix
   __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
@@ -971,17 +1056,17 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
   if (cc >= n_cells_class)
     return;
   const int cell = cells[cc];
-  const double *bx = box + (size_t)cell * 4;
-  const double hx = (bx[1] - bx[0]) / NF, hy = (bx[3] - bx[2]) / NF;
+  const double *bx = verts ? nullptr : box + (size_t)cell * 4;
+  const double *vq = verts ? verts + (size_t)cell * 8 : nullptr;
   const double *cf = coef + coef_off[cell];
   {
     const int b = t >> 2, part = t & 3, bxi = b % NB, byi = b / NB;
     double acc[4][4] = {};
-    for (int ly = 2 * part; ly < 2 * part + 2; ++ly)
+    for (int ly = part * (H / 4); ly < (part + 1) * (H / 4); ++ly)
       for (int lx = 0; lx < H; ++lx)
         {
           double K[4][4];
-          qg_element_2d(coef_kind, cf, (byi * H + ly) * NF + bxi * H + lx, hx, hy, K);
+          fine_element_2d(coef_kind, cf, (byi * H + ly) * NF + bxi * H + lx, NF, bx, vq, K);
           double Wt[4][4]; // [vertex a][block corner m]
           for (int v = 0; v < 4; ++v)
             {
@@ -1126,6 +1211,14 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
 template <typename T>
 hipError_t launch_stream(int nf, int variant, const LocalArgs &a, hipStream_t st)
 {
+  if (a.cinv) // two-level form of the default shapes
+    {
+      if (nf == 64)
+        return launch_one<T, 64, 1, 8, 8, kModeStream, 2, true>(a, st);
+      if (nf == 128)
+        return launch_one<T, 128, 2, 16, 8, kModeStream, 4, true>(a, st);
+      return hipErrorInvalidValue;
+    }
   switch (nf)
     {
       case 64:
@@ -1144,6 +1237,8 @@ hipError_t launch_stream(int nf, int variant, const LocalArgs &a, hipStream_t st
 
 bool local2d_coarse_space(int nf, int mode, int variant)
 {
+  if (mode == kModeStream)
+    return nf == 128 || (nf == 64 && (variant == 0 || variant == 1 || variant > 3));
   return nf == 64 && (mode == kModeScalar || mode == kModeTensor) && (variant == 0 || variant == 1 || variant == 4);
 }
 
@@ -1215,13 +1310,20 @@ hipError_t launch_prep_cond_2d(int n_cells_class, const int *cells, int nf, int 
   return hipGetLastError();
 }
 
-hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind, const double *box,
-                                  const double *coef, const long long *coef_off, double *cinv, hipStream_t st)
+hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                                  const double *verts, const double *coef, const long long *coef_off, double *cinv,
+                                  hipStream_t st)
 {
   if (n_cells_class == 0)
     return hipSuccess;
-  hipLaunchKernelGGL(coarse_setup_2d, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells, coef_kind,
-                     box, coef, coef_off, cinv);
+  if (nf == 64)
+    hipLaunchKernelGGL(coarse_setup_2d<64>, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells,
+                       coef_kind, box, verts, coef, coef_off, cinv);
+  else if (nf == 128)
+    hipLaunchKernelGGL(coarse_setup_2d<128>, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells,
+                       coef_kind, box, verts, coef, coef_off, cinv);
+  else
+    return hipErrorInvalidValue;
   return hipGetLastError();
 }
 

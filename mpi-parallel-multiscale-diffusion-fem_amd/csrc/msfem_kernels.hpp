@@ -11,9 +11,11 @@ namespace msfem {
 int local2d_variants(int nf);
 // whether a class (resolution, storage mode, launch-shape id) runs the two-level form
 bool local2d_coarse_space(int nf, int mode, int variant);
-// inverse coarse matrices of the two-level form (64^2 box cells): cinv [cells in class][49 * 49]
-hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind, const double *box,
-                                  const double *coef, const long long *coef_off, double *cinv, hipStream_t st);
+// inverse coarse matrices of the two-level form (nf = 64, 128; boxes or general
+// quadrilaterals): cinv [cells in class][49 * 49]
+hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                                  const double *verts, const double *coef, const long long *coef_off, double *cinv,
+                                  hipStream_t st);
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st);
 // verts: general quadrilaterals [n_cells][4][2] or nullptr (boxes)
 hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
