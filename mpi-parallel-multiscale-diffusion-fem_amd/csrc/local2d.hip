@@ -369,14 +369,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 "two-level layout: 8-lane groups of CH columns, whole waves per coarse band");
   const int cg8 = pl >> 3, cband = wave / WB;
   const int yy0 = 1 + (wave % WB) * R; // row i: yy = yy0 + i in 1..CH above the band's lower coarse row
-  T wxa[CPL], wxb[CPL];                 // weights of coarse columns g, g + 1
-#pragma unroll
-  for (int c = 0; c < CPL; ++c)
-    {
-      const int xx = (pl & 7) * CPL + c + 1;
-      wxa[c] = (T)((double)(CH - xx) / CH);
-      wxb[c] = (T)((double)xx / CH);
-    }
+  // weights of coarse columns g, g + 1 for the lane's column c (recomputed where
+  // used: cheaper than keeping them live across the iteration)
+  auto wxa = [&](int c) -> T { return (T)(CH - ((pl & 7) * CPL + c + 1)) * (T)(1.0 / CH); };
+  auto wxb = [&](int c) -> T { return (T)((pl & 7) * CPL + c + 1) * (T)(1.0 / CH); };
   auto wya = [&](int i) -> T { return (T)((double)(CH - yy0 - i) / CH); };
   auto wyb = [&](int i) -> T { return (T)((double)(yy0 + i) / CH); };
   auto spart_at = [](int w, int rs, int g, int ab) { return ((w * 2 + rs) * 8 + g) * 2 + ab; };
@@ -395,10 +391,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 s0 += rv[i][c] * wya(i);
                 s1 += rv[i][c] * wyb(i);
               }
-            v4[0] += s0 * wxa[c];
-            v4[1] += s0 * wxb[c];
-            v4[2] += s1 * wxa[c];
-            v4[3] += s1 * wxb[c];
+            v4[0] += s0 * wxa(c);
+            v4[1] += s0 * wxb(c);
+            v4[2] += s1 * wxa(c);
+            v4[3] += s1 * wxb(c);
           }
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -454,8 +450,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
     for (int c = 0; c < CPL; ++c)
       {
-        ci0[c] = u00 * wxa[c] + u10 * wxb[c];
-        ci1[c] = u01 * wxa[c] + u11 * wxb[c];
+        ci0[c] = u00 * wxa(c) + u10 * wxb(c);
+        ci1[c] = u01 * wxa(c) + u11 * wxb(c);
       }
   };
 
@@ -708,6 +704,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               // the own slots of the grid take the next direction in place
               const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
               rz = v2[0];
+              if constexpr (CS) // the coarse values are still in LDS (next write: next iteration)
+                interp_cols(ci0, ci1);
 #pragma unroll
               for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -1066,7 +1064,28 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
       for (int lx = 0; lx < H; ++lx)
         {
           double K[4][4];
-          fine_element_2d(coef_kind, cf, (byi * H + ly) * NF + bxi * H + lx, NF, bx, vq, K);
+          const int f = (byi * H + ly) * NF + bxi * H + lx;
+          if (!vq && coef_kind <= 1)
+            {
+              // per-fine-cell coefficient on a box: the exact Q1 element matrix
+              // a rho Kxx + c / rho Kyy + b Kxy (rho = hy / hx; unit-square Kxx,
+              // Kyy and Kxy = (s t^T + t s^T) / 4, s_u = +-1 by x side, t_u by y side)
+              const double a = coef_kind == 0 ? cf[f] : cf[3 * f], b = coef_kind == 0 ? 0.0 : cf[3 * f + 1];
+              const double c = coef_kind == 0 ? cf[f] : cf[3 * f + 2];
+              const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+              const double ka = a * rho * (1.0 / 6.0), kc = c / rho * (1.0 / 6.0), kb = 0.25 * b;
+              for (int u = 0; u < 4; ++u)
+                for (int v = 0; v < 4; ++v)
+                  {
+                    const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
+                    const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
+                    const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
+                    const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
+                    K[u][v] = ka * sx * ny + kc * sy * nx + kb * (su * tv + tu * sv);
+                  }
+            }
+          else
+            fine_element_2d(coef_kind, cf, f, NF, bx, vq, K);
           double Wt[4][4]; // [vertex a][block corner m]
           for (int v = 0; v < 4; ++v)
             {
@@ -1074,15 +1093,13 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
               for (int m = 0; m < 4; ++m)
                 Wt[v][m] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
             }
+          double Q[4][4]; // K W
+          for (int u = 0; u < 4; ++u)
+            for (int n = 0; n < 4; ++n)
+              Q[u][n] = ((K[u][0] * Wt[0][n] + K[u][1] * Wt[1][n]) + K[u][2] * Wt[2][n]) + K[u][3] * Wt[3][n];
           for (int m = 0; m < 4; ++m)
             for (int n = 0; n < 4; ++n)
-              {
-                double sum = 0;
-                for (int u = 0; u < 4; ++u)
-                  for (int v = 0; v < 4; ++v)
-                    sum += Wt[u][m] * K[u][v] * Wt[v][n];
-                acc[m][n] += sum;
-              }
+              acc[m][n] += ((Wt[0][m] * Q[0][n] + Wt[1][m] * Q[1][n]) + Wt[2][m] * Q[2][n]) + Wt[3][m] * Q[3][n];
         }
     for (int m = 0; m < 16; ++m)
       sh[(b * 4 + part) * 16 + m] = acc[m >> 2][m & 3];
@@ -1129,9 +1146,9 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
       if (t < 2 * W2)
         {
           const int c = t % W2, h = t / W2, i0 = h ? (N + 1) / 2 : 0, i1 = h ? N : (N + 1) / 2;
-          const double rk = rowk[c], piv = fac[k];
+          const double rk = rowk[c], rinv = 1.0 / fac[k]; // one division per pivot, not per entry
           for (int i = i0; i < i1; ++i)
-            sh[i * W2 + c] = (i == k) ? rk / piv : sh[i * W2 + c] - (fac[i] / piv) * rk;
+            sh[i * W2 + c] = (i == k) ? rk * rinv : sh[i * W2 + c] - (fac[i] * rinv) * rk;
         }
       __syncthreads();
     }
