@@ -164,7 +164,9 @@ int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_
 int msfem_version(void);
 
 /* Kernel timing of the batch builds (HIP events on the build's stream around
- * the local-solve and element launches).  After msfem_ctx_set_profiling(ctx, 1),
+ * the local-solve launches -- after each class's preparation: conductances,
+ * general cells' boundary data, the two-level coarse setup -- and around the
+ * element launches).  After msfem_ctx_set_profiling(ctx, 1),
  * msfem_ctx_profile returns the device milliseconds of the last completed
  * build (it waits for that build's events).  No counterpart in the reference,
  * whose only timer is the TimerOutput scope "basis initialization and

@@ -361,12 +361,12 @@ size_t stream_work_doubles(msfem_ctx *ctx, const msfem_batch *in, const msfem_so
   return n;
 }
 
-int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
-                       size_t k, double *bases, double *cond, double *work, hipStream_t st)
+// Per-class preparation before the solve: conductances / node couplings,
+// general cells' boundary data, the two-level inverse coarse matrices.
+int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k, double *bases,
+                         double *cond, hipStream_t st)
 {
   Plan &p = ctx->plan;
-  const int nb = p.nb;
-  const bool pou = o->partition_of_unity != 0;
   const int nf = p.class_nf[k];
   const int ncls = (int)p.class_cells[k].size();
   const int *cells = d_cells(ctx) + p.cells_dev_off[k];
@@ -380,6 +380,30 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
                                           d_off(ctx, 0), cond, d_off(ctx, 2), fp32, st));
   if (in->vertices) // the solve kernels read the boundary data from the bases
     HIP_TRY(ctx, launch_quad_dirichlet_2d(ncls, cells, nf, in->vertices, bases, d_off(ctx, 1), st));
+  if (class_coarse(ctx, in, o, nf))
+    {
+      double *base = (double *)ctx->bufs["coarse_inv"].p;
+      if (!base)
+        return fail(ctx, MSFEM_ERR_HIP, "two-level scratch missing");
+      HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
+                                          d_off(ctx, 0), base + coarse_inv_doubles(ctx, in, o, k), st));
+    }
+  return MSFEM_OK;
+}
+
+// The local solves of class k (after launch_class_prepare), then the
+// partition-of-unity fix-up.
+int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+                       size_t k, double *bases, double *cond, double *work, hipStream_t st)
+{
+  Plan &p = ctx->plan;
+  const int nb = p.nb;
+  const bool pou = o->partition_of_unity != 0;
+  const int nf = p.class_nf[k];
+  const int ncls = (int)p.class_cells[k].size();
+  const int *cells = d_cells(ctx) + p.cells_dev_off[k];
+  const bool fp32 = o->precision == MSFEM_PREC_FP32_REFINE;
+  const int mode = build_mode(in, nf);
   LocalArgs a;
   a.nb_solve = pou ? nb - 1 : nb;
   a.n_problems = ncls * a.nb_solve;
@@ -407,14 +431,7 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   const int variant = class_variant(ctx, in, o, nf);
   a.cinv = nullptr;
   if (class_coarse(ctx, in, o, nf))
-    {
-      double *base = (double *)ctx->bufs["coarse_inv"].p;
-      if (!base)
-        return fail(ctx, MSFEM_ERR_HIP, "two-level scratch missing");
-      a.cinv = base + coarse_inv_doubles(ctx, in, o, k);
-      HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
-                                          d_off(ctx, 0), (double *)a.cinv, st));
-    }
+    a.cinv = (const double *)ctx->bufs["coarse_inv"].p + coarse_inv_doubles(ctx, in, o, k);
   if (in->dim == 2)
     HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
   else
@@ -513,7 +530,9 @@ int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o,
           e = hipStreamWaitEvent(s, ctx->fork, 0);
           if (e != hipSuccess)
             break;
-          rc = launch_class_solve(ctx, in, o, out, k, bases, cond, work, s);
+          rc = launch_class_prepare(ctx, in, o, k, bases, cond, s);
+          if (rc == MSFEM_OK)
+            rc = launch_class_solve(ctx, in, o, out, k, bases, cond, work, s);
           if (rc == MSFEM_OK)
             rc = launch_class_element(ctx, in, out, k, bases, cond, s);
           if (rc == MSFEM_OK)
@@ -607,9 +626,16 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
       HIP_TRY(ctx, fill_zero(out->status, nstat * sizeof(int32_t), st));
       HIP_TRY(ctx, fill_zero(out->resid, nstat * sizeof(double), st));
     }
+  // all classes' preparation, then their local solves (the profiled span), then
+  // all element matrices
+  for (size_t k = 0; k < p.class_nf.size(); ++k)
+    {
+      rc = launch_class_prepare(ctx, in, o, k, bases, cond, st);
+      if (rc)
+        return rc;
+    }
   if (ctx->profiling)
     HIP_TRY(ctx, hipEventRecord(ctx->ev[0], st));
-  // all classes' local solves first, then all element matrices
   for (size_t k = 0; k < p.class_nf.size(); ++k)
     {
       rc = launch_class_solve(ctx, in, o, out, k, bases, cond, work, st);
