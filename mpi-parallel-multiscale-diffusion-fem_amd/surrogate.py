@@ -112,7 +112,12 @@ class Conv:
         col = buf.get("col", self.K * N, np.uint16)
         y = buf.get(self.name + ".y", self.cout * N, np.float32)
         _ok(L.msfem_nn_im2col(self.cin, B, H, W, self.k, self.k, self.pad, x.ptr, col.ptr, st))
-        _ok(L.msfem_nn_gemm_bf16(0, 0, self.cout, N, self.K, self.wbf.ptr, self.K, col.ptr, N, y.ptr, N, 0, st))
+        # deep layers have few output tiles: split K (fixed-order reduction) to fill the GPU
+        tiles = -(-self.cout // (32 if self.cout <= 32 else 64 if self.cout <= 64 else 128)) * -(-N // 128)
+        splits = int(max(1, min(256 // tiles, self.K // 256))) if tiles < 256 else 1
+        work = buf.get("splitk", splits * self.cout * N, np.float32) if splits > 1 else None
+        _ok(L.msfem_nn_gemm_bf16_splitk(0, 0, self.cout, N, self.K, self.wbf.ptr, self.K, col.ptr, N, y.ptr, N, 0,
+                                        splits, work.ptr if work is not None else None, st))
         _ok(L.msfem_nn_bias_act(self.cout, N, self.b.ptr, y.ptr, 1 if self.relu else 0, st))
         self.saved = (x, B, H, W, y)
         return y, Ho, Wo
