@@ -327,6 +327,16 @@ int msfem_nn_gemm_bf16_splitk(int transA, int transB, int M, int N, int K, const
                               int ldb, float *C, int ldc, int accumulate, int splits, float *work, void *stream);
 /* col[(c*kh+ky)*kw+kx][(b*Ho+y)*Wo+x] = bf16(in[c][b][y+ky-pad][x+kx-pad]) (0 outside) */
 int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const float *in, void *col, void *stream);
+/* Implicit-GEMM convolution: out[Cout][B Ho Wo] = w[Cout][Cin kh kw] x
+ * im2col(in) with the column matrix gathered inside the GEMM's tile loads
+ * (never stored); the same values as msfem_nn_im2col + msfem_nn_gemm_bf16.
+ * Requires Wo % 8 == 0 (MSFEM_ERR_ARG otherwise).  splits > 1: split K with
+ * a fixed-order reduction through work[splits][Cout][B Ho Wo] (FP32).
+ * bias (or NULL): out = act(out + bias[row]) as msfem_nn_bias_act, fused
+ * into the GEMM's epilogue (relu: 1 = ReLU, 0 = identity). */
+int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,
+                       const float *in, float *out, int splits, float *work, const float *bias, int relu,
+                       void *stream);
 /* adjoint of im2col (FP32, gather form) */
 int msfem_nn_col2im(int C, int B, int H, int W, int kh, int kw, int pad, const float *dcol, float *dx, void *stream);
 int msfem_nn_bias_act(int C, uint64_t N, const float *bias, float *y, int relu, void *stream);

@@ -40,11 +40,22 @@ constexpr int kGBN = 128, kGBK = 32, kGPad = 8;
 
 typedef uint16_t bf16x8_t __attribute__((ext_vector_type(8)));
 
-template <int BM, bool TA, bool TB, bool VEC>
+// Geometry of an implicit convolution operand: B[k][n] = bf16(in[ci][b][y + ky
+// - pad][x + kx - pad]) with k = (ci kh + ky) kw + kx, n = (b Ho + y) Wo + x
+// (the im2col matrix, never stored).  Wo must be a multiple of 8.
+struct ConvGeo {
+  const float *in;
+  int B, H, W, kh, kw, pad, Ho, Wo;
+  const float *bias; // fused epilogue without split K: out = act(acc + bias[row]); nullptr: none
+  int relu;
+};
+
+template <int BM, bool TA, bool TB, bool VEC, bool CONV = false>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, const uint16_t *A, int lda,
                                                         const uint16_t *B, int ldb, float *C, int ldc, int accumulate,
-                                                        int kchunk, float *work)
+                                                        int kchunk, float *work, ConvGeo cg = ConvGeo())
 {
+  static_assert(!CONV || (!TB && kGBN * kGBK / 8 % 256 == 0), "implicit convolution: B is [k][n], 16 chunks per row");
   constexpr int LDR = kGBK + kGPad; // LDS row (bf16): 80 bytes
   __shared__ __align__(16) uint16_t sA[BM][LDR];
   __shared__ __align__(16) uint16_t sB[kGBN][LDR];
@@ -72,6 +83,33 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
     return v;
   };
   bf16x8_t ra[APT], rb[BPT];
+  // CONV: every chunk of this thread covers the same 8 columns n (c % 16 is
+  // tid % 16 for all t), so their (b, y, x0) are decoded once
+  int cb = 0, cy = 0, cx0 = 0;
+  bool cvalid = false;
+  if constexpr (CONV)
+    {
+      const int nn = n0 + (tid % (kGBN / 8)) * 8;
+      cvalid = nn < N;
+      cx0 = nn % cg.Wo;
+      const int rest = nn / cg.Wo;
+      cy = rest % cg.Ho;
+      cb = rest / cg.Ho;
+    }
+  auto conv8 = [&](int k) -> bf16x8_t {
+    bf16x8_t v;
+    const int taps = cg.kh * cg.kw, ci = k / taps, r = k - ci * taps, ky = r / cg.kw, kx = r - ky * cg.kw;
+    const int yy = cy + ky - cg.pad;
+    const bool rowok = cvalid && k < ke && yy >= 0 && yy < cg.H;
+    const float *src = cg.in + ((size_t)(ci * cg.B + cb) * cg.H + (rowok ? yy : 0)) * cg.W;
+#pragma unroll
+    for (int e = 0; e < 8; ++e)
+      {
+        const int xx = cx0 + e + kx - cg.pad;
+        v[e] = (rowok && xx >= 0 && xx < cg.W) ? f32_to_bf16(src[xx]) : (uint16_t)0;
+      }
+    return v;
+  };
   auto fetch = [&](int k0) {
 #pragma unroll
     for (int t = 0; t < APT; ++t)
@@ -89,7 +127,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
     for (int t = 0; t < BPT; ++t)
       {
         const int c = tid + 256 * t;
-        if (TB) // B stored [n][k]: chunk along k
+        if constexpr (CONV) // gathered straight from the activations
+          rb[t] = conv8(k0 + c / (kGBN / 8));
+        else if (TB) // B stored [n][k]: chunk along k
           rb[t] = load8(B, ldb, n0 + c / 4, N, k0 + (c % 4) * 8, ke);
         else // B[k][n]: chunk along n
           rb[t] = load8(B, ldb, k0 + c / (kGBN / 8), ke, n0 + (c % (kGBN / 8)) * 8, N);
@@ -194,7 +234,15 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
           if (row < M && col < N)
             {
               float *c = out + (size_t)row * ldo + col;
-              *c = acc_in ? *c + acc[i][j][e] : acc[i][j][e];
+              float v = acc_in ? *c + acc[i][j][e] : acc[i][j][e];
+              if constexpr (CONV)
+                if (cg.bias && gridDim.z == 1)
+                  {
+                    v += cg.bias[row];
+                    if (cg.relu && v < 0.0f) // as bias_act_kernel (NaN passes)
+                      v = 0.0f;
+                  }
+              *c = v;
             }
         }
 }
@@ -573,6 +621,64 @@ int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const f
   else
     hipLaunchKernelGGL(im2col_kernel, dim3(grid_for(rows * N)), dim3(256), 0, (hipStream_t)stream, C, B, H, W, kh, kw,
                        pad, in, (uint16_t *)col);
+  return check_launch();
+}
+
+int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout, const float *in,
+                       float *out, int splits, float *work, const float *bias, int relu, void *stream)
+{
+  if (Cin <= 0 || B <= 0 || H <= 0 || W <= 0 || kh <= 0 || kw <= 0 || pad < 0 || Cout <= 0 || !w || !in || !out ||
+      splits < 1 || (splits > 1 && !work))
+    return MSFEM_ERR_ARG;
+  const int Ho = H + 2 * pad - kh + 1, Wo = W + 2 * pad - kw + 1;
+  const long long Nl = (long long)B * Ho * Wo, Kl = (long long)Cin * kh * kw;
+  if (Ho <= 0 || Wo <= 0 || Wo % 8 != 0 || Nl >= (1ll << 31) || Kl >= (1ll << 31) ||
+      (long long)Cin * B * H * W >= (1ll << 31))
+    return MSFEM_ERR_ARG;
+  const int N = (int)Nl, K = (int)Kl;
+  int kchunk = K;
+  if (splits > 1)
+    {
+      kchunk = ((K + splits - 1) / splits + kGBK - 1) / kGBK * kGBK;
+      splits = (K + kchunk - 1) / kchunk;
+    }
+  const int BM = Cout <= 32 ? 32 : (Cout <= 64 ? 64 : 128);
+  dim3 grid((N + kGBN - 1) / kGBN, (Cout + BM - 1) / BM, splits), block(256);
+  if (grid.y > 65535 || splits > 65535)
+    return MSFEM_ERR_ARG;
+  hipStream_t st = (hipStream_t)stream;
+  const uint16_t *a = (const uint16_t *)w;
+  const bool vec = K % 8 == 0 && ((uintptr_t)a % 16) == 0;
+  float *wk = splits > 1 ? work : nullptr;
+  const ConvGeo cg{in, B, H, W, kh, kw, pad, Ho, Wo, bias, relu};
+#define MSFEM_CONV_CASE(BMv)                                                                                           \
+  if (vec)                                                                                                           \
+    hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, false, true, true>), grid, block, 0, st, Cout, N, K, a, K,       \
+                       (const uint16_t *)nullptr, N, out, N, 0, kchunk, wk, cg);                                      \
+  else                                                                                                               \
+    hipLaunchKernelGGL((gemm_bf16_kernel<BMv, false, false, false, true>), grid, block, 0, st, Cout, N, K, a, K,      \
+                       (const uint16_t *)nullptr, N, out, N, 0, kchunk, wk, cg);
+  if (BM == 32)
+    {
+      MSFEM_CONV_CASE(32)
+    }
+  else if (BM == 64)
+    {
+      MSFEM_CONV_CASE(64)
+    }
+  else
+    {
+      MSFEM_CONV_CASE(128)
+    }
+#undef MSFEM_CONV_CASE
+  if (splits > 1)
+    {
+      hipLaunchKernelGGL(splitk_reduce_kernel, dim3(grid_for((size_t)Cout * N)), dim3(256), 0, st, Cout, N, splits,
+                         work, out, N, 0);
+      if (bias)
+        hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for((size_t)Cout * N)), dim3(256), 0, st, Cout, (uint64_t)N, bias,
+                           out, relu);
+    }
   return check_launch();
 }
 

@@ -32,6 +32,7 @@ def _nn():
         L.msfem_nn_gemm_bf16.argtypes = [i32, i32, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, vp]
         L.msfem_nn_gemm_bf16_splitk.argtypes = [i32, i32, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp]
         L.msfem_nn_im2col.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
+        L.msfem_nn_conv_bf16.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
         L.msfem_nn_act_bwd.argtypes = [u64, vp, vp, vp, i32, vp]
@@ -109,16 +110,21 @@ class Conv:
         L, st = _nn(), self.ctx.stream
         Ho, Wo = self.out_hw(H, W)
         N = B * Ho * Wo
-        col = buf.get("col", self.K * N, np.uint16)
         y = buf.get(self.name + ".y", self.cout * N, np.float32)
-        _ok(L.msfem_nn_im2col(self.cin, B, H, W, self.k, self.k, self.pad, x.ptr, col.ptr, st))
         # deep layers have few output tiles: split K (fixed-order reduction) to fill the GPU
         tiles = -(-self.cout // (32 if self.cout <= 32 else 64 if self.cout <= 64 else 128)) * -(-N // 128)
         splits = int(max(1, min(256 // tiles, self.K // 256))) if tiles < 256 else 1
         work = buf.get("splitk", splits * self.cout * N, np.float32) if splits > 1 else None
-        _ok(L.msfem_nn_gemm_bf16_splitk(0, 0, self.cout, N, self.K, self.wbf.ptr, self.K, col.ptr, N, y.ptr, N, 0,
-                                        splits, work.ptr if work is not None else None, st))
-        _ok(L.msfem_nn_bias_act(self.cout, N, self.b.ptr, y.ptr, 1 if self.relu else 0, st))
+        wptr = work.ptr if work is not None else None
+        if Wo % 8 == 0:  # implicit GEMM, bias and activation in its epilogue
+            _ok(L.msfem_nn_conv_bf16(self.cin, B, H, W, self.k, self.k, self.pad, self.wbf.ptr, self.cout, x.ptr,
+                                     y.ptr, splits, wptr, self.b.ptr, 1 if self.relu else 0, st))
+        else:
+            col = buf.get("col", self.K * N, np.uint16)
+            _ok(L.msfem_nn_im2col(self.cin, B, H, W, self.k, self.k, self.pad, x.ptr, col.ptr, st))
+            _ok(L.msfem_nn_gemm_bf16_splitk(0, 0, self.cout, N, self.K, self.wbf.ptr, self.K, col.ptr, N, y.ptr, N, 0,
+                                            splits, wptr, st))
+            _ok(L.msfem_nn_bias_act(self.cout, N, self.b.ptr, y.ptr, 1 if self.relu else 0, st))
         self.saved = (x, B, H, W, y)
         return y, Ho, Wo
 

@@ -166,6 +166,47 @@ def test_im2col_geometries(ctx, C, B, H, W, k, pad):
     assert np.array_equal(dcol.numpy().reshape(-1), bf16_bits(R.bf16(col)).reshape(-1))
 
 
+@pytest.mark.parametrize("Cin,Cout,B,H,W,k,pad,splits", [
+    (1, 32, 3, 16, 16, 3, 1, 1), (32, 32, 2, 64, 64, 3, 1, 1), (64, 128, 4, 16, 8, 3, 1, 1),
+    (256, 256, 16, 8, 8, 3, 1, 4), (128, 64, 2, 32, 32, 3, 1, 3), (5, 7, 3, 9, 8, 3, 1, 1)])
+def test_implicit_conv(ctx, Cin, Cout, B, H, W, k, pad, splits):
+    """msfem_nn_conv_bf16 (column matrix gathered in the GEMM's tile loads) is
+    bitwise equal to msfem_nn_im2col + the GEMM, split K included, and matches
+    the oracle's convolution in FP64 within the BF16 operand rounding."""
+    say(f"implicit conv Cin={Cin} Cout={Cout} B={B} H={H} W={W} splits={splits}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(Cin + Cout + B + H)
+    x = rng.standard_normal((Cin, B, H, W)).astype(np.float32)
+    K = Cin * k * k
+    w = (rng.standard_normal((Cout, K)) / np.sqrt(K)).astype(np.float32)
+    Ho, Wo = H + 2 * pad - k + 1, W + 2 * pad - k + 1
+    N = B * Ho * Wo
+    dx, dw = ctx.to_device(x), ctx.to_device(bf16_bits(R.bf16(w)))
+    y1, y2 = ctx.alloc(Cout * N, np.float32), ctx.alloc(Cout * N, np.float32)
+    work = ctx.alloc(max(1, splits) * Cout * N, np.float32)
+    wp = work.ptr if splits > 1 else None
+    bias = rng.standard_normal(Cout).astype(np.float32)
+    db = ctx.to_device(bias)
+    for relu in (1, 0):  # fused bias + activation against the separate kernel
+        assert L.msfem_nn_conv_bf16(Cin, B, H, W, k, k, pad, dw.ptr, Cout, dx.ptr, y1.ptr, splits, wp, db.ptr, relu,
+                                    st) == 0
+        col = ctx.alloc(K * N, np.uint16)
+        assert L.msfem_nn_im2col(Cin, B, H, W, k, k, pad, dx.ptr, col.ptr, st) == 0
+        assert L.msfem_nn_gemm_bf16_splitk(0, 0, Cout, N, K, dw.ptr, K, col.ptr, N, y2.ptr, N, 0, splits, wp, st) == 0
+        assert L.msfem_nn_bias_act(Cout, N, db.ptr, y2.ptr, relu, st) == 0
+        a, b = y1.numpy().reshape(-1), y2.numpy().reshape(-1)
+        assert np.array_equal(a, b)
+    # without bias: the plain product, against the oracle's convolution in FP64
+    assert L.msfem_nn_conv_bf16(Cin, B, H, W, k, k, pad, dw.ptr, Cout, dx.ptr, y1.ptr, splits, wp, None, 0, st) == 0
+    a = y1.numpy().reshape(-1)
+    cref, _ = R.im2col(x.astype(np.float64), k, pad)
+    ref = R.bf16(w).astype(np.float64) @ R.bf16(cref)
+    assert np.abs(a - ref.reshape(-1)).max() < 1e-4 * np.abs(ref).max()
+    # a width that is not a multiple of 8 is refused (the caller uses the explicit path)
+    assert L.msfem_nn_conv_bf16(Cin, B, H, 6, k, k, pad, dw.ptr, Cout, dx.ptr, y1.ptr, 1, None, None, 0,
+                                st) == msfem.ERR_ARG
+
+
 def test_mse_and_adam(ctx):
     say("nn mse + adam")
     L, st = S._nn(), ctx.stream
