@@ -369,12 +369,32 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 "two-level layout: 8-lane groups of CH columns, whole waves per coarse band");
   const int cg8 = pl >> 3, cband = wave / WB;
   const int yy0 = 1 + (wave % WB) * R; // row i: yy = yy0 + i in 1..CH above the band's lower coarse row
-  // weights of coarse columns g, g + 1 for the lane's column c (recomputed where
-  // used: cheaper than keeping them live across the iteration)
-  auto wxa = [&](int c) -> T { return (T)(CH - ((pl & 7) * CPL + c + 1)) * (T)(1.0 / CH); };
-  auto wxb = [&](int c) -> T { return (T)((pl & 7) * CPL + c + 1) * (T)(1.0 / CH); };
-  auto wya = [&](int i) -> T { return (T)((double)(CH - yy0 - i) / CH); };
-  auto wyb = [&](int i) -> T { return (T)((double)(yy0 + i) / CH); };
+  // Interpolation weights: coarse columns g, g + 1 for the lane's column c,
+  // coarse rows of the band for row i.  128^2 tiles (16 nodes per lane, 128
+  // registers) recompute them where used from opaque copies of the lane and
+  // row offsets (two ALU operations each): hoisted out of the iteration loop
+  // they would be kept live and spilled.  64^2 tiles have the registers and
+  // let the compiler keep them.
+  constexpr bool OPAQUE_W = NF == 128;
+  auto lane_off = [&]() {
+    int v = (pl & 7) * CPL + 1;
+    if constexpr (OPAQUE_W)
+      asm volatile("" : "+v"(v));
+    return v;
+  };
+  auto row_off = [&]() {
+    int v = yy0;
+    if constexpr (OPAQUE_W)
+      {
+        v = __builtin_amdgcn_readfirstlane(v);
+        asm volatile("" : "+s"(v));
+      }
+    return v;
+  };
+  auto wxa = [&](int c) -> T { return (T)(CH - lane_off() - c) * (T)(1.0 / CH); };
+  auto wxb = [&](int c) -> T { return (T)(lane_off() + c) * (T)(1.0 / CH); };
+  auto wya = [&](int i) -> T { return (T)(CH - row_off() - i) * (T)(1.0 / CH); };
+  auto wyb = [&](int i) -> T { return (T)(row_off() + i) * (T)(1.0 / CH); };
   auto spart_at = [](int w, int rs, int g, int ab) { return ((w * 2 + rs) * 8 + g) * 2 + ab; };
   // restriction partials of r (8-lane DPP sums; lane 0 of each group writes)
   auto restrict_r = [&](T (&rv)[R][CPL]) {
