@@ -337,6 +337,28 @@ int msfem_nn_im2col(int C, int B, int H, int W, int kh, int kw, int pad, const f
 int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,
                        const float *in, float *out, int splits, float *work, const float *bias, int relu,
                        void *stream);
+/* Direct 3x3 convolution (padding 1): the input rows of a 128-pixel output
+ * tile are staged once per 32-channel block in LDS and shared by the 9 taps.
+ * Requires Cin % 32 == 0, Cout % 32 == 0, W in {16, 32, 64} with
+ * H % (128 / W) == 0 or W = H = 8 with B even, and 16-byte aligned wp and
+ * in (MSFEM_ERR_ARG otherwise).
+ * wp = the BF16 weights [Cout][Cin 3 3] regrouped by msfem_nn_conv3x3_pack
+ * (Cout * Cin * 9 values).  Same result as msfem_nn_conv_bf16 with kh = kw = 3,
+ * pad = 1 up to FP32 rounding of the sums (the k order differs).
+ * upsample = 1: in is [Cin][B][H/2][W/2] (8-byte aligned), read as its
+ * nearest upsampling to H x W (the values of msfem_nn_up2, never stored).
+ * add (or NULL): out = act(out + bias[row]) + add, one FP32 addition per
+ * value (a skip connection; bitwise what msfem_nn_axpby(1, y, 1, add) gives). */
+int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stream);
+int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
+                          float *out, const float *bias, int relu, const float *add, void *stream);
+/* Direct convolution for few output channels (Cout <= 8: a network's head)
+ * on the VALU: exact BF16 x BF16 products of the rounded input and weights w
+ * [Cout][Cin kh kw] summed in FP32; bias/relu as msfem_nn_conv_bf16.  The same
+ * values as the im2col + GEMM form up to FP32 rounding of the sums.  Requires
+ * kh = kw in {1, 2, 3} and Cin kh kw C <= 16384 with C = 4 (Cout <= 4) or 8. */
+int msfem_nn_conv_small(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,
+                        const float *in, float *out, const float *bias, int relu, void *stream);
 /* adjoint of im2col (FP32, gather form) */
 int msfem_nn_col2im(int C, int B, int H, int W, int kh, int kw, int pad, const float *dcol, float *dx, void *stream);
 int msfem_nn_bias_act(int C, uint64_t N, const float *bias, float *y, int relu, void *stream);

@@ -36,9 +36,17 @@ __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_floa
 // aligned bases); otherwise element loads.  Zero fill beyond the edges.
 // Split K (gridDim.z > 1): block z covers K range [z kchunk, (z+1) kchunk) and
 // writes its partial tile to work[z][M][N] for a fixed-order reduction.
-constexpr int kGBN = 128, kGBK = 32, kGPad = 8;
+constexpr int kGBN = 128, kGBK = 32, kGPad = 4;
 
 typedef uint16_t bf16x8_t __attribute__((ext_vector_type(8)));
+typedef uint16_t bf16x4_t __attribute__((ext_vector_type(4)));
+
+// 8 bf16 into an LDS row at an 8-byte aligned offset (rows are 72 bytes)
+__device__ __forceinline__ void store8(uint16_t *dst, bf16x8_t v)
+{
+  *(bf16x4_t *)dst = v.lo;
+  *(bf16x4_t *)(dst + 4) = v.hi;
+}
 
 // Geometry of an implicit convolution operand: B[k][n] = bf16(in[ci][b][y + ky
 // - pad][x + kx - pad]) with k = (ci kh + ky) kw + kx, n = (b Ho + y) Wo + x
@@ -55,8 +63,12 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
                                                         const uint16_t *B, int ldb, float *C, int ldc, int accumulate,
                                                         int kchunk, float *work, ConvGeo cg = ConvGeo())
 {
-  static_assert(!CONV || (!TB && kGBN * kGBK / 8 % 256 == 0), "implicit convolution: B is [k][n], 16 chunks per row");
-  constexpr int LDR = kGBK + kGPad; // LDS row (bf16): 80 bytes
+  static_assert(!CONV || !TB, "implicit convolution: B is [k][n]");
+  static_assert(TB || (kGBN / 8 == 16 && kGBK / 2 == 16), "B[k][n] tile: 16 column octets x 16 k pairs");
+  // LDS row (bf16): 72 bytes = 18 banks.  The MFMA operand reads (32 rows, 8
+  // bytes each) then cover the 64 banks once, and so do the k-pair writes of
+  // a B[k][n] tile (rows 8 apart, 144 = 16 mod 64 banks, plus the pair index).
+  constexpr int LDR = kGBK + kGPad;
   __shared__ __align__(16) uint16_t sA[BM][LDR];
   __shared__ __align__(16) uint16_t sB[kGBN][LDR];
   constexpr int WM = BM == 32 ? 1 : 2, WN = 4 / WM;          // waves along M, N
@@ -83,13 +95,15 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
     return v;
   };
   bf16x8_t ra[APT], rb[BPT];
-  // CONV: every chunk of this thread covers the same 8 columns n (c % 16 is
-  // tid % 16 for all t), so their (b, y, x0) are decoded once
+  // B[k][n] tile: thread tid holds column octet tid / 16 at rows k = 2 (tid %
+  // 16) + t, t = 0, 1, so it writes LDS as 32-bit k pairs.  CONV: the octet is
+  // fixed per thread, so its (b, y, x0) are decoded once
+  const int boct = tid / 16, bk2 = 2 * (tid % 16);
   int cb = 0, cy = 0, cx0 = 0;
   bool cvalid = false;
   if constexpr (CONV)
     {
-      const int nn = n0 + (tid % (kGBN / 8)) * 8;
+      const int nn = n0 + boct * 8;
       cvalid = nn < N;
       cx0 = nn % cg.Wo;
       const int rest = nn / cg.Wo;
@@ -128,11 +142,11 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
       {
         const int c = tid + 256 * t;
         if constexpr (CONV) // gathered straight from the activations
-          rb[t] = conv8(k0 + c / (kGBN / 8));
+          rb[t] = conv8(k0 + bk2 + t);
         else if (TB) // B stored [n][k]: chunk along k
           rb[t] = load8(B, ldb, n0 + c / 4, N, k0 + (c % 4) * 8, ke);
         else // B[k][n]: chunk along n
-          rb[t] = load8(B, ldb, k0 + c / (kGBN / 8), ke, n0 + (c % (kGBN / 8)) * 8, N);
+          rb[t] = load8(B, ldb, k0 + bk2 + t, ke, n0 + boct * 8, N);
       }
   };
   auto stash = [&]() {
@@ -143,7 +157,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
         if (c < ACH)
           {
             if (!TA)
-              *(bf16x8_t *)&sA[c / 4][(c % 4) * 8] = ra[t];
+              store8(&sA[c / 4][(c % 4) * 8], ra[t]);
             else
               {
                 const int k = c / (BM / 8), mq = (c % (BM / 8)) * 8;
@@ -153,19 +167,20 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
               }
           }
       }
-#pragma unroll
-    for (int t = 0; t < BPT; ++t)
+    if constexpr (TB)
       {
-        const int c = tid + 256 * t;
-        if (TB)
-          *(bf16x8_t *)&sB[c / 4][(c % 4) * 8] = rb[t];
-        else
-          {
-            const int k = c / (kGBN / 8), nq = (c % (kGBN / 8)) * 8;
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-              sB[nq + e][k] = rb[t][e];
+        for (int t = 0; t < BPT; ++t)
+          {
+            const int c = tid + 256 * t;
+            store8(&sB[c / 4][(c % 4) * 8], rb[t]);
           }
+      }
+    else
+      {
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          *(uint32_t *)&sB[boct * 8 + e][bk2] = (uint32_t)rb[0][e] | ((uint32_t)rb[1][e] << 16);
       }
   };
 
@@ -245,6 +260,278 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
               *c = v;
             }
         }
+}
+
+typedef __bf16 bf16x8m_t __attribute__((ext_vector_type(8))); // v_mfma_f32_32x32x16_bf16 operand
+
+// Direct 3x3 convolution, padding 1, stride 1, on the matrix cores:
+//   out[co][b][y][x] = act(bias[co] + sum_{ci,ky,kx} w[co][ci][ky][kx] in[ci][b][y + ky - 1][x + kx - 1])
+// with BF16 operands and FP32 accumulation.  A workgroup owns BM output
+// channels x 128 output pixels = TR = 128 / W whole rows of one image (W = 8:
+// two whole 8 x 8 images, TI = 2).  Per block of 32 input channels it stages
+// the TR + 2 input rows around them (zero halo) of each image in LDS with the
+// channels innermost (80-byte pixels) and the
+// block's weights as [tap][co][ci]; the 9 taps then take their
+// v_mfma_f32_32x32x16_bf16 operands from LDS, so each input value is read from
+// memory once per channel block (the implicit GEMM reads it once per tap).
+// The next block's input and weights are loaded into registers during the
+// current block's MFMAs.  wp: weights packed by conv3x3_pack_kernel,
+// wp[co][blk][tap][c] = w[co][32 blk + c][tap].  The sum over k runs tap by
+// tap within a channel block, so results differ from the implicit GEMM's by
+// FP32 rounding only.
+template <int BM, int W, bool UP>
+__global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H, const float *in, const uint16_t *wp,
+                                                           float *out, const float *bias, int relu, const float *add)
+{
+  constexpr int TI = W == 8 ? 2 : 1, TR = kGBN / W / TI; // images and rows per tile (W = 8: H = 8)
+  constexpr int IR = TR + 2, IC = W + 2, NPIX = TI * IR * IC;
+  constexpr int CP = 40;                                  // LDS pitch (bf16) of a pixel and of a weight row
+  constexpr int NG = W / 4, IIT = TI * IR * NG * 4;      // input items (staged row, 4-column group, channel octet)
+  static_assert(IIT <= 256, "one input item per thread");
+  constexpr int WIT = BM * 36, WPT = (WIT + 255) / 256;   // weight items (co, tap, channel octet)
+  constexpr int WM = BM == 32 ? 1 : 2, WN = 4 / WM;       // waves along co, pixels
+  constexpr int TM = BM / WM / 32, TN = kGBN / WN / 32;   // 32x32 blocks per wave
+  static_assert(kGBN % W == 0 && TM == 1, "tile of whole rows, one 32-row block per wave");
+  __shared__ __align__(16) uint16_t sIn[NPIX * CP];
+  __shared__ __align__(16) uint16_t sW[9 * BM * CP];
+
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int wm = (wave / WN) * 32, wn = (wave % WN) * TN * 32;
+  const int HW = H * W;
+  const size_t plane = (size_t)B * HW; // channel stride of in and out
+  const int p0 = blockIdx.x * kGBN;    // first output pixel, flattened (b, y, x)
+  const int b = p0 / HW, y0 = (p0 - b * HW) / W;
+  const int m0 = blockIdx.y * BM;
+  const int K = Cin * 9, nblk = Cin / 32;
+
+  // Input staging: item (channel octet, 4-column group, staged row), octet
+  // fastest (the 16-byte LDS stores of a quarter wave then hit distinct
+  // banks); each loads 8 channel planes x 4 columns as float4 and is kept raw
+  // in registers during the MFMAs, converted to BF16 at the LDS store.  The
+  // halo columns x = -1 and x = W are outside every image (tiles are whole
+  // rows) and are zeroed once.
+  // UP: in is the half-resolution tensor [Cin][B][H/2][W/2], read at (y/2,
+  // x/2) (nearest upsampling): 2 source columns (float2) per 4-column group.
+  float4 ri[UP ? 1 : 8];
+  float2 rh[UP ? 8 : 1];
+  bf16x8_t rw[WPT];
+  const int ioct = tid % 4, ig = tid / 4 % NG, irr = tid / 4 / NG; // irr = img IR + row
+  const int iy = y0 - 1 + irr % IR;
+  const bool iact = tid < IIT, iok = iact && iy >= 0 && iy < H;
+  const size_t iplane = UP ? plane / 4 : plane; // channel stride of in
+  const float *isrc =
+      in + (iok ? (UP ? (size_t)(b + irr / IR) * (HW / 4) + (size_t)(iy / 2) * (W / 2) + 2 * ig
+                      : (size_t)(b + irr / IR) * HW + (size_t)iy * W + 4 * ig)
+                : 0);
+  auto fetch = [&](int blk) {
+    if (iok)
+      {
+        const float *src = isrc + (size_t)(blk * 32 + ioct * 8) * iplane;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if constexpr (UP)
+            rh[e] = *(const float2 *)(src + e * iplane);
+          else
+            ri[e] = *(const float4 *)(src + e * iplane);
+      }
+#pragma unroll
+    for (int t = 0; t < WPT; ++t)
+      {
+        const int u = tid + 256 * t;
+        if (u < WIT)
+          rw[t] = *(const bf16x8_t *)(wp + (size_t)(m0 + u / 36) * K + blk * 288 + (u % 36) * 8);
+      }
+  };
+  auto stash = [&]() {
+    if (iact)
+      {
+        uint16_t *dst = &sIn[(irr * IC + 1 + 4 * ig) * CP + ioct * 8];
+#pragma unroll
+        for (int px = 0; px < 4; ++px)
+          {
+            bf16x8_t v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              {
+                float f;
+                if constexpr (UP)
+                  f = px < 2 ? rh[e].x : rh[e].y;
+                else
+                  f = px == 0 ? ri[e].x : px == 1 ? ri[e].y : px == 2 ? ri[e].z : ri[e].w;
+                v[e] = iok ? f32_to_bf16(f) : (uint16_t)0;
+              }
+            *(bf16x8_t *)(dst + px * CP) = v;
+          }
+      }
+#pragma unroll
+    for (int t = 0; t < WPT; ++t)
+      {
+        const int u = tid + 256 * t;
+        if (u < WIT)
+          {
+            const int m = u / 36, j = u % 36; // j = tap * 4 + channel octet
+            *(bf16x8_t *)&sW[((j / 4) * BM + m) * CP + (j % 4) * 8] = rw[t];
+          }
+      }
+  };
+  for (int h = tid; h < TI * IR * 2 * 4; h += 256) // halo columns: (staged row, side, octet)
+    {
+      const int rr = h / 8, side = h / 4 % 2, oct = h % 4;
+      *(bf16x8_t *)&sIn[(rr * IC + side * (IC - 1)) * CP + oct * 8] = bf16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+
+  f32x16 acc[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      acc[j][e] = 0.0f;
+  // this lane's output pixels n = wn + 32 j + lane % 32 (image n / (TR W), row, column)
+  // sit at staged (img IR + row, column) for tap (0, 0); its 8 channels of a
+  // 16-channel step start at kh
+  int pb[TN];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+    {
+      const int n = wn + j * 32 + lane % 32, img = n / (TR * W), r = n % (TR * W) / W;
+      pb[j] = ((img * IR + r) * IC + n % W) * CP;
+    }
+  const int kh = 8 * (lane / 32);
+  const int arow = (wm + lane % 32) * CP + kh;
+
+  fetch(0);
+  stash();
+  __syncthreads();
+  for (int blk = 0; blk < nblk; ++blk)
+    {
+      const bool more = blk + 1 < nblk;
+      if (more)
+        fetch(blk + 1); // in flight during the MFMAs below
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+        {
+          const int toff = ((tap / 3) * IC + tap % 3) * CP;
+#pragma unroll
+          for (int ks = 0; ks < 32; ks += 16)
+            {
+              const bf16x8m_t av = *(const bf16x8m_t *)&sW[tap * BM * CP + arow + ks];
+#pragma unroll
+              for (int j = 0; j < TN; ++j)
+                {
+                  const bf16x8m_t bv = *(const bf16x8m_t *)&sIn[pb[j] + toff + kh + ks];
+                  acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc[j], 0, 0, 0);
+                }
+            }
+        }
+      __syncthreads();
+      if (more)
+        {
+          stash();
+          __syncthreads();
+        }
+    }
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      {
+        const int row = m0 + wm + 8 * (e / 4) + 4 * (lane / 32) + (e % 4);
+        float v = acc[j][e];
+        if (bias)
+          {
+            v += bias[row];
+            if (relu && v < 0.0f) // as bias_act_kernel (NaN passes)
+              v = 0.0f;
+          }
+        const size_t o = (size_t)row * plane + p0 + wn + j * 32 + lane % 32;
+        if (add) // skip connection: act(acc + bias) + add, one FP32 addition
+          v += add[o];
+        out[o] = v;
+      }
+}
+
+// Direct convolution with few output channels (a network's head) on the
+// VALU: one thread per output pixel computes CO channels (CO >= Cout; weights
+// of the channels beyond Cout are zero, their sums are not stored) from the
+// BF16-rounded input and weights, the KS x KS window of 8 input channels
+// loaded at once.  Weights
+// as FP32 in LDS, [k][CO].  Each BF16 x BF16 product is exact in FP32, as in
+// the MFMA path; the sums run in the order k = (ci KS + ky) KS + kx.
+template <int CO, int KS>
+__global__ void __launch_bounds__(256) conv_small_kernel(int Cin, int B, int H, int W, int pad, int Cout,
+                                                         const float *in, const uint16_t *w, float *out,
+                                                         const float *bias, int relu)
+{
+  constexpr int CU = 8, T = KS * KS; // input channels loaded at once
+  extern __shared__ float swk[];
+  const int K = Cin * T;
+  for (int t = threadIdx.x; t < K * CO; t += blockDim.x)
+    swk[t] = t % CO < Cout ? bf16_to_f32(w[(size_t)(t % CO) * K + t / CO]) : 0.0f;
+  __syncthreads();
+  const int Ho = H + 2 * pad - KS + 1, Wo = W + 2 * pad - KS + 1;
+  const size_t N = (size_t)B * Ho * Wo, plane = (size_t)B * H * W;
+  for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x)
+    {
+      const int x = (int)(n % Wo), y = (int)(n / Wo % Ho), b = (int)(n / ((size_t)Wo * Ho));
+      float acc[CO];
+#pragma unroll
+      for (int c = 0; c < CO; ++c)
+        acc[c] = 0.0f;
+      const float *src = in + (size_t)b * H * W;
+      for (int ci0 = 0; ci0 < Cin; ci0 += CU)
+        {
+          float v[CU][T];
+#pragma unroll
+          for (int u = 0; u < CU; ++u)
+#pragma unroll
+            for (int t = 0; t < T; ++t)
+              {
+                const int yy = y + t / KS - pad, xx = x + t % KS - pad;
+                v[u][t] = (ci0 + u < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
+                              ? src[(ci0 + u) * plane + yy * W + xx]
+                              : 0.0f;
+              }
+#pragma unroll
+          for (int u = 0; u < CU; ++u)
+            if (ci0 + u < Cin)
+#pragma unroll
+              for (int t = 0; t < T; ++t)
+                {
+                  const float f = bf16_to_f32(f32_to_bf16(v[u][t]));
+                  const float *wk = swk + ((ci0 + u) * T + t) * CO;
+#pragma unroll
+                  for (int c = 0; c < CO; ++c)
+                    acc[c] = fmaf(f, wk[c], acc[c]);
+                }
+        }
+#pragma unroll
+      for (int c = 0; c < CO; ++c)
+        if (c < Cout)
+          {
+            float r = acc[c];
+            if (bias)
+              {
+                r += bias[c];
+                if (relu && r < 0.0f) // as bias_act_kernel (NaN passes)
+                  r = 0.0f;
+              }
+            out[c * N + n] = r;
+          }
+    }
+}
+
+// wp[co][blk][tap][c] = w[co][(32 blk + c) 9 + tap]: the weights of a 3x3
+// convolution regrouped per 32-channel block and tap (conv3x3_bf16_kernel).
+__global__ void conv3x3_pack_kernel(int Cout, int Cin, const uint16_t *w, uint16_t *wp)
+{
+  const int K = Cin * 9;
+  const size_t total = (size_t)Cout * K;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    {
+      const int co = (int)(t / K), k = (int)(t % K);
+      const int blk = k / 288, r = k % 288, tap = r / 32, c = r % 32;
+      wp[t] = w[(size_t)co * K + (blk * 32 + c) * 9 + tap];
+    }
 }
 
 // C (+)= sum over z of work[z] in fixed order (the split-K partials).
@@ -679,6 +966,82 @@ int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, co
         hipLaunchKernelGGL(bias_act_kernel, dim3(grid_for((size_t)Cout * N)), dim3(256), 0, st, Cout, (uint64_t)N, bias,
                            out, relu);
     }
+  return check_launch();
+}
+
+int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stream)
+{
+  if (Cin <= 0 || Cin % 32 != 0 || Cout <= 0 || !w || !wp || (long long)Cout * Cin * 9 >= (1ll << 31))
+    return MSFEM_ERR_ARG;
+  const size_t n = (size_t)Cout * Cin * 9;
+  hipLaunchKernelGGL(conv3x3_pack_kernel, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, Cout, Cin,
+                     (const uint16_t *)w, (uint16_t *)wp);
+  return check_launch();
+}
+
+int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
+                          float *out, const float *bias, int relu, const float *add, void *stream)
+{
+  if (Cin <= 0 || Cin % 32 != 0 || B <= 0 || H <= 0 || Cout <= 0 || Cout % 32 != 0 || !wp || !in || !out ||
+      (W != 8 && W != 16 && W != 32 && W != 64) || (W == 8 ? H != 8 : H % (kGBN / W) != 0) ||
+      ((uintptr_t)wp % 16) != 0 || ((uintptr_t)in % (upsample ? 8 : 16)) != 0 || (upsample && H % 2 != 0))
+    return MSFEM_ERR_ARG;
+  const long long plane = (long long)B * H * W;
+  if (plane % kGBN != 0 || (long long)Cin * plane >= (1ll << 31) || (long long)Cout * plane >= (1ll << 31) ||
+      (long long)Cout * Cin * 9 >= (1ll << 31))
+    return MSFEM_ERR_ARG;
+  const int BM = Cout % 64 == 0 ? 64 : 32;
+  const dim3 grid((unsigned)(plane / kGBN), (unsigned)(Cout / BM)), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  const uint16_t *w = (const uint16_t *)wp;
+#define MSFEM_C3_CASE(BMv, Wv)                                                                                         \
+  if (BM == BMv && W == Wv)                                                                                          \
+    {                                                                                                                \
+      if (upsample)                                                                                                  \
+        hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, true>), grid, block, 0, st, Cin, B, H, in, w, out, bias,    \
+                           relu, add);                                                                               \
+      else                                                                                                           \
+        hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, false>), grid, block, 0, st, Cin, B, H, in, w, out, bias,   \
+                           relu, add);                                                                               \
+    }
+  MSFEM_C3_CASE(32, 8)
+  MSFEM_C3_CASE(32, 16)
+  MSFEM_C3_CASE(32, 32)
+  MSFEM_C3_CASE(32, 64)
+  MSFEM_C3_CASE(64, 8)
+  MSFEM_C3_CASE(64, 16)
+  MSFEM_C3_CASE(64, 32)
+  MSFEM_C3_CASE(64, 64)
+#undef MSFEM_C3_CASE
+  return check_launch();
+}
+
+int msfem_nn_conv_small(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,
+                        const float *in, float *out, const float *bias, int relu, void *stream)
+{
+  if (Cin <= 0 || B <= 0 || H <= 0 || W <= 0 || kh != kw || kh < 1 || kh > 3 || pad < 0 || Cout <= 0 ||
+      Cout > 8 || !w || !in || !out)
+    return MSFEM_ERR_ARG;
+  const int Ho = H + 2 * pad - kh + 1, Wo = W + 2 * pad - kw + 1;
+  const int CO = Cout <= 4 ? 4 : 8;
+  const long long K = (long long)Cin * kh * kw;
+  if (Ho <= 0 || Wo <= 0 || K * CO * 4 > 65536 || (long long)Cin * B * H * W >= (1ll << 31) ||
+      (long long)Cout * B * Ho * Wo >= (1ll << 31))
+    return MSFEM_ERR_ARG;
+  const size_t N = (size_t)B * Ho * Wo, lds = (size_t)K * CO * sizeof(float);
+  hipStream_t st = (hipStream_t)stream;
+  const uint16_t *wb = (const uint16_t *)w;
+#define MSFEM_CS_CASE(c, k)                                                                                           \
+  if (CO == c && kh == k)                                                                                            \
+    hipLaunchKernelGGL((conv_small_kernel<c, k>), dim3(grid_for(N)), dim3(256), lds, st, Cin, B, H, W, pad, Cout, in,  \
+                       wb, out, bias, relu);
+  MSFEM_CS_CASE(4, 1)
+  MSFEM_CS_CASE(4, 2)
+  MSFEM_CS_CASE(4, 3)
+  MSFEM_CS_CASE(8, 1)
+  MSFEM_CS_CASE(8, 2)
+  MSFEM_CS_CASE(8, 3)
+#undef MSFEM_CS_CASE
   return check_launch();
 }
 

@@ -33,6 +33,9 @@ def _nn():
         L.msfem_nn_gemm_bf16_splitk.argtypes = [i32, i32, i32, i32, i32, vp, i32, vp, i32, vp, i32, i32, i32, vp, vp]
         L.msfem_nn_im2col.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_conv_bf16.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]
+        L.msfem_nn_conv3x3_pack.argtypes = [i32, i32, vp, vp, vp]
+        L.msfem_nn_conv_small.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, i32, vp]
+        L.msfem_nn_conv3x3_bf16.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
         L.msfem_nn_act_bwd.argtypes = [u64, vp, vp, vp, i32, vp]
@@ -95,6 +98,8 @@ class Conv:
         self.gb = ctx.alloc(cout, np.float32)
         self.mw, self.vw = ctx.to_device(np.zeros_like(w)), ctx.to_device(np.zeros_like(w))
         self.mb, self.vb = ctx.to_device(np.zeros(cout, np.float32)), ctx.to_device(np.zeros(cout, np.float32))
+        # BF16 weights regrouped for the direct 3x3 kernel, rebuilt when wbf changes
+        self.wver, self.wpk, self.wpk_ver = 0, None, -1
         self.sync_bf16()
 
     def n_params(self):
@@ -102,6 +107,34 @@ class Conv:
 
     def sync_bf16(self):
         _ok(_nn().msfem_nn_to_bf16(self.cout * self.K, self.w.ptr, self.wbf.ptr, self.ctx.stream))
+        self.wver += 1
+
+    def packed(self):
+        """Device address of the weights regrouped for msfem_nn_conv3x3_bf16 (repacked after a change)."""
+        if self.wpk_ver != self.wver:
+            if self.wpk is None:
+                self.wpk = self.ctx.alloc(self.cout * self.K, np.uint16)
+            _ok(_nn().msfem_nn_conv3x3_pack(self.cin, self.cout, self.wbf.ptr, self.wpk.ptr, self.ctx.stream))
+            self.wpk_ver = self.wver
+        return self.wpk.ptr
+
+    def forward_up_skip(self, x, B, H, W, skip, out):
+        """Serving form of a decoder layer: out = act(conv(up2(x)) + b) + skip with x
+        at H/2 x W/2, upsampled inside the convolution's staging (no saved state)."""
+        _ok(_nn().msfem_nn_conv3x3_bf16(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 1, out.ptr, self.b.ptr,
+                                        1 if self.relu else 0, skip.ptr, self.ctx.stream))
+
+    def small_direct(self):
+        """Whether msfem_nn_conv_small takes this layer (the head: at most 8 output channels)."""
+        return self.cout <= 8 and self.k <= 3 and self.K * (4 if self.cout <= 4 else 8) <= 16384
+
+    def direct3x3(self, B, H, W):
+        """Whether msfem_nn_conv3x3_bf16 takes this layer on B images of H x W."""
+        if not (self.k == 3 and self.pad == 1 and self.cin % 32 == 0 and self.cout % 32 == 0):
+            return False
+        if W == 8:  # tiles of two whole 8 x 8 images
+            return H == 8 and B % 2 == 0
+        return W in (16, 32, 64) and H % (128 // W) == 0
 
     def out_hw(self, H, W):
         return H + 2 * self.pad - self.k + 1, W + 2 * self.pad - self.k + 1
@@ -116,7 +149,13 @@ class Conv:
         splits = int(max(1, min(256 // tiles, self.K // 256))) if tiles < 256 else 1
         work = buf.get("splitk", splits * self.cout * N, np.float32) if splits > 1 else None
         wptr = work.ptr if work is not None else None
-        if Wo % 8 == 0:  # implicit GEMM, bias and activation in its epilogue
+        if self.direct3x3(B, H, W):  # input rows staged once per channel block, 9 taps from LDS
+            _ok(L.msfem_nn_conv3x3_bf16(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 0, y.ptr, self.b.ptr,
+                                        1 if self.relu else 0, None, st))
+        elif self.small_direct():  # the head: too few output channels for the matrix cores
+            _ok(L.msfem_nn_conv_small(self.cin, B, H, W, self.k, self.k, self.pad, self.wbf.ptr, self.cout, x.ptr,
+                                      y.ptr, self.b.ptr, 1 if self.relu else 0, st))
+        elif Wo % 8 == 0:  # implicit GEMM, bias and activation in its epilogue
             _ok(L.msfem_nn_conv_bf16(self.cin, B, H, W, self.k, self.k, self.pad, self.wbf.ptr, self.cout, x.ptr,
                                      y.ptr, splits, wptr, self.b.ptr, 1 if self.relu else 0, st))
         else:
@@ -157,6 +196,7 @@ class Conv:
         L, st = _nn(), self.ctx.stream
         _ok(L.msfem_nn_adam(self.cout * self.K, self.w.ptr, self.gw.ptr, self.mw.ptr, self.vw.ptr, self.wbf.ptr, lr, b1,
                             b2, eps, step, wd, st))
+        self.wver += 1
         _ok(L.msfem_nn_adam(self.cout, self.b.ptr, self.gb.ptr, self.mb.ptr, self.vb.ptr, None, lr, b1, b2, eps, step,
                             0.0, st))
 
@@ -203,9 +243,13 @@ class Surrogate:
         return f
 
     # ---- forward / backward ---------------------------------------------------------
-    def forward(self, x, B):
+    def forward(self, x, B, keep=True):
         """x: DeviceArray [1][B][tile][tile] FP32 (normalized log-coefficient).
-        Returns the head output [4][B][tile-1][tile-1] (DeviceArray view)."""
+        Returns the head output [4][B][tile-1][tile-1] (DeviceArray view).
+        keep=False (inference): decoder layers that the direct 3x3 kernel takes
+        read the upsampled input and add the skip inside the convolution; the
+        values are bitwise those of the training form, which keeps the
+        intermediates the backward pass needs."""
         L, st, buf = _nn(), self.ctx.stream, self.buf
         H = self.tile
         h, _, _ = self.enc[0].forward(x, B, H, H, buf)
@@ -222,6 +266,12 @@ class Surrogate:
         self.skips = skips
         for l in range(self.levels - 2, -1, -1):
             C = self.chans[l + 1]
+            if not keep and self.dec[l].direct3x3(B, 2 * H, 2 * H):
+                H *= 2
+                out = buf.get(f"sum{l}", self.chans[l] * B * H * H, np.float32)
+                self.dec[l].forward_up_skip(h, B, H, H, skips[l][0], out)
+                h = out
+                continue
             u = buf.get(f"up{l}", C * B * (2 * H) ** 2, np.float32)
             _ok(L.msfem_nn_up2(C * B, H, H, h.ptr, u.ptr, 0, st))
             H *= 2
@@ -330,7 +380,7 @@ class Surrogate:
         nbase = 4 * (t + 1) ** 2
         for s in range(0, n, batch):
             b = min(batch, n - s)
-            out = self.forward(Ptr(x.ptr + s * t * t * 4), b)
+            out = self.forward(Ptr(x.ptr + s * t * t * 4), b, keep=False)
             m = 4 * b * (t - 1) ** 2
             _ok(L.msfem_nn_axpby(m, self.dscale, out.ptr, 0.0, None, out.ptr, st))
             _ok(L.msfem_nn_delta_to_bases(b, t, out.ptr, bases_ptr + s * nbase * 8, st))
@@ -398,7 +448,7 @@ def evaluate(net, data, start, count, batch=64):
     for s in range(start, start + count, batch):
         b = min(batch, start + count - s)
         targets(net, data, s, b, tgt)
-        out = net.forward(Ptr(data.x.ptr + s * data.nf ** 2 * 4), b)
+        out = net.forward(Ptr(data.x.ptr + s * data.nf ** 2 * 4), b, keep=False)
         loss, _ = net.loss_and_grad(out, tgt, b * n_per)
         tot += loss * b
         m += b

@@ -207,6 +207,106 @@ def test_implicit_conv(ctx, Cin, Cout, B, H, W, k, pad, splits):
                                 st) == msfem.ERR_ARG
 
 
+@pytest.mark.parametrize("Cin,Cout,B,H,W", [
+    (32, 32, 2, 64, 64), (64, 32, 1, 64, 64), (32, 64, 3, 32, 32), (128, 64, 1, 32, 32), (64, 128, 2, 16, 16),
+    (256, 128, 1, 16, 16), (32, 96, 2, 8, 16), (64, 64, 1, 6, 64), (128, 256, 4, 8, 8), (256, 256, 2, 8, 8),
+    (32, 32, 6, 8, 8)])
+def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
+    """msfem_nn_conv3x3_bf16 (input rows staged once per channel block, 9 taps
+    from LDS) against the oracle's convolution in FP64 on the same BF16
+    operands: every output within the recursive FP32 summation bound
+    K 2^-24 sum|w x| (K = 9 Cin terms), bias + ReLU applied as bias_act; and
+    against the implicit GEMM within twice that bound."""
+    say(f"direct conv3x3 Cin={Cin} Cout={Cout} B={B} H={H} W={W}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(Cin + 3 * Cout + B + H + W)
+    x = rng.standard_normal((Cin, B, H, W)).astype(np.float32)
+    K = Cin * 9
+    w = (rng.standard_normal((Cout, K)) / np.sqrt(K)).astype(np.float32)
+    bias = rng.standard_normal(Cout).astype(np.float32)
+    N = B * H * W
+    dx, dw, db = ctx.to_device(x), ctx.to_device(bf16_bits(R.bf16(w))), ctx.to_device(bias)
+    dwp = ctx.alloc(Cout * K, np.uint16)
+    y1, y2 = ctx.alloc(Cout * N, np.float32), ctx.alloc(Cout * N, np.float32)
+    assert L.msfem_nn_conv3x3_pack(Cin, Cout, dw.ptr, dwp.ptr, st) == 0
+    cref, _ = R.im2col(R.bf16(x.astype(np.float64)), 3, 1)
+    wb = R.bf16(w).astype(np.float64)
+    ref, aref = wb @ cref, np.abs(wb) @ np.abs(cref)
+    bound = K * 2.0 ** -24 * aref * 1.0001 + 1e-30
+    for relu in (0, 1):
+        assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, dx.ptr, 0, y1.ptr, db.ptr, relu, None, st) == 0
+        assert L.msfem_nn_conv_bf16(Cin, B, H, W, 3, 3, 1, dw.ptr, Cout, dx.ptr, y2.ptr, 1, None, db.ptr, relu,
+                                    st) == 0
+        a, b = y1.numpy().reshape(Cout, N).astype(np.float64), y2.numpy().reshape(Cout, N).astype(np.float64)
+        r = ref + bias[:, None]
+        if relu:
+            r = np.maximum(r, 0.0)  # ReLU is 1-Lipschitz: the bound carries over
+        assert (np.abs(a - r) <= bound).all(), np.abs(a - r).max()
+        assert (np.abs(a - b) <= 2 * bound).all()
+    # without bias: the plain sums
+    assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, dx.ptr, 0, y1.ptr, None, 0, None, st) == 0
+    assert (np.abs(y1.numpy().reshape(Cout, N) - ref) <= bound).all()
+    # decoder form: nearest upsampling of a half-resolution input in the staging
+    # and the skip sum in the epilogue, bitwise what up2 + conv + axpby give
+    xs = ctx.to_device(rng.standard_normal((Cin, B, H // 2, W // 2)).astype(np.float32))
+    skip = ctx.to_device(rng.standard_normal((Cout, N)).astype(np.float32))
+    assert L.msfem_nn_up2(Cin * B, H // 2, W // 2, xs.ptr, dx.ptr, 0, st) == 0
+    assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, dx.ptr, 0, y2.ptr, db.ptr, 1, None, st) == 0
+    assert L.msfem_nn_axpby(Cout * N, 1.0, y2.ptr, 1.0, skip.ptr, y2.ptr, st) == 0
+    assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, xs.ptr, 1, y1.ptr, db.ptr, 1, skip.ptr, st) == 0
+    assert np.array_equal(y1.numpy().reshape(-1), y2.numpy().reshape(-1))
+    # shapes outside the kernel's tiling are refused (the caller takes the implicit GEMM)
+    bad = [(Cin, B, H + 1, W), (Cin + 16, B, H, W), (Cin, B, H, 4)]
+    bad += [(Cin, B + 1, 8, 8), (Cin, B, 16, 8)] if W == 8 else [(Cin, 2, 4, 8)]
+    for c, bb, h, ww in bad:  # every shape here is refused before any launch
+        assert L.msfem_nn_conv3x3_bf16(c, bb, h, ww, dwp.ptr, Cout, dx.ptr, 0, y1.ptr, None, 0, None, st) == msfem.ERR_ARG
+    assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout + 16, dx.ptr, 0, y1.ptr, None, 0, None, st) == msfem.ERR_ARG
+    for a in (dx, dw, db, dwp, y1, y2):
+        a.free()
+
+
+@pytest.mark.parametrize("Cin,Cout,B,H,W,k,pad", [
+    (32, 4, 3, 64, 64, 2, 0), (5, 3, 2, 9, 7, 3, 1), (1, 8, 2, 16, 16, 3, 1), (64, 1, 1, 12, 10, 2, 0),
+    (11, 6, 2, 8, 9, 1, 0), (3, 7, 2, 10, 12, 3, 1)])
+def test_conv_small(ctx, Cin, Cout, B, H, W, k, pad):
+    """msfem_nn_conv_small (VALU, few output channels) against the oracle's
+    convolution in FP64 on the same BF16 operands within the recursive FP32
+    summation bound K 2^-24 sum|w x|, and against im2col + GEMM + bias_act
+    within twice that bound."""
+    say(f"small conv Cin={Cin} Cout={Cout} B={B} H={H} W={W} k={k}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(Cin + Cout + H + W + k)
+    x = rng.standard_normal((Cin, B, H, W)).astype(np.float32)
+    K = Cin * k * k
+    w = (rng.standard_normal((Cout, K)) / np.sqrt(K)).astype(np.float32)
+    bias = rng.standard_normal(Cout).astype(np.float32)
+    dx, dw, db = ctx.to_device(x), ctx.to_device(bf16_bits(R.bf16(w))), ctx.to_device(bias)
+    cref, (Ho, Wo) = R.im2col(R.bf16(x.astype(np.float64)), k, pad)
+    N = B * Ho * Wo
+    wb = R.bf16(w).astype(np.float64)
+    ref, bound = wb @ cref, K * 2.0 ** -24 * (np.abs(wb) @ np.abs(cref)) * 1.0001 + 1e-30
+    y1, y2, col = ctx.alloc(Cout * N, np.float32), ctx.alloc(Cout * N, np.float32), ctx.alloc(K * N, np.uint16)
+    for relu in (0, 1):
+        assert L.msfem_nn_conv_small(Cin, B, H, W, k, k, pad, dw.ptr, Cout, dx.ptr, y1.ptr, db.ptr, relu, st) == 0
+        assert L.msfem_nn_im2col(Cin, B, H, W, k, k, pad, dx.ptr, col.ptr, st) == 0
+        assert L.msfem_nn_gemm_bf16(0, 0, Cout, N, K, dw.ptr, K, col.ptr, N, y2.ptr, N, 0, st) == 0
+        assert L.msfem_nn_bias_act(Cout, N, db.ptr, y2.ptr, relu, st) == 0
+        a, b = y1.numpy().reshape(Cout, N).astype(np.float64), y2.numpy().reshape(Cout, N).astype(np.float64)
+        r = ref + bias[:, None]
+        if relu:
+            r = np.maximum(r, 0.0)
+        assert (np.abs(a - r) <= bound).all(), np.abs(a - r).max()
+        assert (np.abs(a - b) <= 2 * bound).all()
+    assert L.msfem_nn_conv_small(Cin, B, H, W, k, k, pad, dw.ptr, Cout, dx.ptr, y1.ptr, None, 0, st) == 0
+    assert (np.abs(y1.numpy().reshape(Cout, N) - ref) <= bound).all()
+    # more than 8 output channels, other kernel sizes, weights beyond the LDS budget: refused
+    assert L.msfem_nn_conv_small(Cin, B, H, W, k, k, pad, dw.ptr, 9, dx.ptr, y1.ptr, None, 0, st) == msfem.ERR_ARG
+    assert L.msfem_nn_conv_small(Cin, B, H, W, 4, 4, pad, dw.ptr, Cout, dx.ptr, y1.ptr, None, 0, st) == msfem.ERR_ARG
+    assert L.msfem_nn_conv_small(4097, 1, 1, 1, 1, 1, 0, dw.ptr, 4, dx.ptr, y1.ptr, None, 0, st) == msfem.ERR_ARG
+    for a in (dx, dw, db, y1, y2, col):
+        a.free()
+
+
 def test_mse_and_adam(ctx):
     say("nn mse + adam")
     L, st = S._nn(), ctx.stream
