@@ -340,7 +340,8 @@ int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, co
 /* Direct 3x3 convolution (padding 1): the input rows of a 128-pixel output
  * tile are staged once per 32-channel block in LDS and shared by the 9 taps.
  * Requires Cin % 32 == 0, Cout % 32 == 0, W in {16, 32, 64} with
- * H % (128 / W) == 0 or W = H = 8 with B even, and 16-byte aligned wp and
+ * H % (128 / W) == 0, or W = H in {4, 8} with B H W % 128 == 0 (whole images
+ * per tile), and 16-byte aligned wp and
  * in (MSFEM_ERR_ARG otherwise).
  * wp = the BF16 weights [Cout][Cin 3 3] regrouped by msfem_nn_conv3x3_pack
  * (Cout * Cin * 9 values).  Same result as msfem_nn_conv_bf16 with kh = kw = 3,

@@ -267,8 +267,8 @@ typedef __bf16 bf16x8m_t __attribute__((ext_vector_type(8))); // v_mfma_f32_32x3
 // Direct 3x3 convolution, padding 1, stride 1, on the matrix cores:
 //   out[co][b][y][x] = act(bias[co] + sum_{ci,ky,kx} w[co][ci][ky][kx] in[ci][b][y + ky - 1][x + kx - 1])
 // with BF16 operands and FP32 accumulation.  A workgroup owns BM output
-// channels x 128 output pixels = TR = 128 / W whole rows of one image (W = 8:
-// two whole 8 x 8 images, TI = 2).  Per block of 32 input channels it stages
+// channels x 128 output pixels = TR = 128 / W whole rows of one image (W <= 8:
+// TI = 128 / W^2 whole W x W images).  Per block of 32 input channels it stages
 // the TR + 2 input rows around them (zero halo) of each image in LDS with the
 // channels innermost (80-byte pixels) and the
 // block's weights as [tap][co][ci]; the 9 taps then take their
@@ -283,7 +283,7 @@ template <int BM, int W, bool UP>
 __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H, const float *in, const uint16_t *wp,
                                                            float *out, const float *bias, int relu, const float *add)
 {
-  constexpr int TI = W == 8 ? 2 : 1, TR = kGBN / W / TI; // images and rows per tile (W = 8: H = 8)
+  constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI; // images and rows per tile (W <= 8: H = W)
   constexpr int IR = TR + 2, IC = W + 2, NPIX = TI * IR * IC;
   constexpr int CP = 40;                                  // LDS pitch (bf16) of a pixel and of a weight row
   constexpr int NG = W / 4, IIT = TI * IR * NG * 4;      // input items (staged row, 4-column group, channel octet)
@@ -983,7 +983,7 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
                           float *out, const float *bias, int relu, const float *add, void *stream)
 {
   if (Cin <= 0 || Cin % 32 != 0 || B <= 0 || H <= 0 || Cout <= 0 || Cout % 32 != 0 || !wp || !in || !out ||
-      (W != 8 && W != 16 && W != 32 && W != 64) || (W == 8 ? H != 8 : H % (kGBN / W) != 0) ||
+      (W != 4 && W != 8 && W != 16 && W != 32 && W != 64) || (W <= 8 ? H != W : H % (kGBN / W) != 0) ||
       ((uintptr_t)wp % 16) != 0 || ((uintptr_t)in % (upsample ? 8 : 16)) != 0 || (upsample && H % 2 != 0))
     return MSFEM_ERR_ARG;
   const long long plane = (long long)B * H * W;
@@ -1004,10 +1004,12 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
         hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, false>), grid, block, 0, st, Cin, B, H, in, w, out, bias,   \
                            relu, add);                                                                               \
     }
+  MSFEM_C3_CASE(32, 4)
   MSFEM_C3_CASE(32, 8)
   MSFEM_C3_CASE(32, 16)
   MSFEM_C3_CASE(32, 32)
   MSFEM_C3_CASE(32, 64)
+  MSFEM_C3_CASE(64, 4)
   MSFEM_C3_CASE(64, 8)
   MSFEM_C3_CASE(64, 16)
   MSFEM_C3_CASE(64, 32)

@@ -132,8 +132,8 @@ class Conv:
         """Whether msfem_nn_conv3x3_bf16 takes this layer on B images of H x W."""
         if not (self.k == 3 and self.pad == 1 and self.cin % 32 == 0 and self.cout % 32 == 0):
             return False
-        if W == 8:  # tiles of two whole 8 x 8 images
-            return H == 8 and B % 2 == 0
+        if W in (4, 8):  # tiles of 128 / W^2 whole W x W images
+            return H == W and (B * H * W) % 128 == 0
         return W in (16, 32, 64) and H % (128 // W) == 0
 
     def out_hw(self, H, W):
@@ -504,7 +504,7 @@ def main(argv=None):
     import sys
     import time
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", required=True, help="checkpoint (.npz), rewritten after every epoch")
+    ap.add_argument("--out", default=None, help="checkpoint (.npz), rewritten after every epoch")
     ap.add_argument("--cells", type=int, default=5000, help="training cells (4 local solves each)")
     ap.add_argument("--val", type=int, default=256, help="validation cells (drawn after the training cells)")
     ap.add_argument("--epochs", type=int, default=30)
@@ -517,7 +517,13 @@ def main(argv=None):
     ap.add_argument("--serve", type=int, default=0, help="after training, report serving on an N x N coarse mesh")
     ap.add_argument("--serve-seed", type=int, default=41)
     ap.add_argument("--report", default=None, help="JSON file for the serving report")
+    ap.add_argument("--time-forward", type=int, default=0,
+                    help="time N forward passes over 128 random tiles (after one untimed pass)")
+    ap.add_argument("--forward-form", choices=("serving", "training", "both"), default="both",
+                    help="--time-forward: the serving form (fused decoder) and/or the training form")
     a = ap.parse_args(argv)
+    if not a.load and not a.out:
+        ap.error("--out (train) or --load (serve) is required")
 
     def log(msg):
         print(msg, flush=True)
@@ -528,6 +534,20 @@ def main(argv=None):
         net = Surrogate.load(ctx, a.load)
     else:
         net = train_new(ctx, a, log)
+    if a.time_forward:
+        x = ctx.to_device(np.random.default_rng(0).standard_normal(128 * net.tile ** 2).astype(np.float32))
+        forms = {"serving": (False,), "training": (True,), "both": (False, True)}[a.forward_form]
+        for keep in forms:
+            net.forward(x, 128, keep=keep)
+            ctx.synchronize()
+            t = time.perf_counter()
+            for _ in range(a.time_forward):
+                net.forward(x, 128, keep=keep)
+            ctx.synchronize()
+            dt = (time.perf_counter() - t) / a.time_forward
+            log(f"{'training' if keep else 'serving'} forward, 128 tiles: {dt * 1e3:.3f} ms "
+                f"({net.flops_per_sample() * 128 / dt / 1e12:.1f} TFLOP/s)")
+        x.free()
     if a.serve:
         rep = serve_report(ctx, net, n_side=a.serve, seed=a.serve_seed, log=log)
         log(json.dumps(rep))

@@ -210,7 +210,7 @@ def test_implicit_conv(ctx, Cin, Cout, B, H, W, k, pad, splits):
 @pytest.mark.parametrize("Cin,Cout,B,H,W", [
     (32, 32, 2, 64, 64), (64, 32, 1, 64, 64), (32, 64, 3, 32, 32), (128, 64, 1, 32, 32), (64, 128, 2, 16, 16),
     (256, 128, 1, 16, 16), (32, 96, 2, 8, 16), (64, 64, 1, 6, 64), (128, 256, 4, 8, 8), (256, 256, 2, 8, 8),
-    (32, 32, 6, 8, 8)])
+    (32, 32, 6, 8, 8), (256, 256, 16, 4, 4), (64, 32, 8, 4, 4)])
 def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
     """msfem_nn_conv3x3_bf16 (input rows staged once per channel block, 9 taps
     from LDS) against the oracle's convolution in FP64 on the same BF16
@@ -235,8 +235,15 @@ def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
     bound = K * 2.0 ** -24 * aref * 1.0001 + 1e-30
     for relu in (0, 1):
         assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, dx.ptr, 0, y1.ptr, db.ptr, relu, None, st) == 0
-        assert L.msfem_nn_conv_bf16(Cin, B, H, W, 3, 3, 1, dw.ptr, Cout, dx.ptr, y2.ptr, 1, None, db.ptr, relu,
-                                    st) == 0
+        if W % 8 == 0:  # the implicit GEMM
+            assert L.msfem_nn_conv_bf16(Cin, B, H, W, 3, 3, 1, dw.ptr, Cout, dx.ptr, y2.ptr, 1, None, db.ptr, relu,
+                                        st) == 0
+        else:  # the explicit form
+            col = ctx.alloc(K * N, np.uint16)
+            assert L.msfem_nn_im2col(Cin, B, H, W, 3, 3, 1, dx.ptr, col.ptr, st) == 0
+            assert L.msfem_nn_gemm_bf16(0, 0, Cout, N, K, dw.ptr, K, col.ptr, N, y2.ptr, N, 0, st) == 0
+            assert L.msfem_nn_bias_act(Cout, N, db.ptr, y2.ptr, relu, st) == 0
+            col.free()
         a, b = y1.numpy().reshape(Cout, N).astype(np.float64), y2.numpy().reshape(Cout, N).astype(np.float64)
         r = ref + bias[:, None]
         if relu:
@@ -256,8 +263,8 @@ def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
     assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout, xs.ptr, 1, y1.ptr, db.ptr, 1, skip.ptr, st) == 0
     assert np.array_equal(y1.numpy().reshape(-1), y2.numpy().reshape(-1))
     # shapes outside the kernel's tiling are refused (the caller takes the implicit GEMM)
-    bad = [(Cin, B, H + 1, W), (Cin + 16, B, H, W), (Cin, B, H, 4)]
-    bad += [(Cin, B + 1, 8, 8), (Cin, B, 16, 8)] if W == 8 else [(Cin, 2, 4, 8)]
+    bad = [(Cin, B, H + 1, W), (Cin + 16, B, H, W), (Cin, B, H, 2)]
+    bad += [(Cin, B + 1, W, W), (Cin, B, 2 * W, W)] if W <= 8 else [(Cin, 2, 4, 8), (Cin, 8, 2, 4)]
     for c, bb, h, ww in bad:  # every shape here is refused before any launch
         assert L.msfem_nn_conv3x3_bf16(c, bb, h, ww, dwp.ptr, Cout, dx.ptr, 0, y1.ptr, None, 0, None, st) == msfem.ERR_ARG
     assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, dwp.ptr, Cout + 16, dx.ptr, 0, y1.ptr, None, 0, None, st) == msfem.ERR_ARG
