@@ -181,6 +181,17 @@ int msfem_ctx_profile(msfem_ctx *ctx, double *ms_local_solve, double *ms_element
 #define MSFEM_N_MARKS 8
 int msfem_ctx_mark(msfem_ctx *ctx, int slot);
 int msfem_ctx_elapsed(msfem_ctx *ctx, int from, int to, double *ms);
+/* Two-stream ordering (no counterpart in the reference): msfem_ctx_aux_stream
+ * returns the context's second stream (created at default priority on first
+ * use; msfem_ctx_synchronize also waits for it); msfem_ctx_mark_on records
+ * mark `slot` on `stream` (NULL = the context stream); msfem_ctx_wait_mark
+ * makes `stream` (NULL = the context stream) wait for the last recording of
+ * mark `slot`.  Lets a caller overlap work on the two streams, e.g. the
+ * surrogate's inference of one chunk of cells with the builds of the previous
+ * chunk. */
+int msfem_ctx_aux_stream(msfem_ctx *ctx, void **stream);
+int msfem_ctx_mark_on(msfem_ctx *ctx, int slot, void *stream);
+int msfem_ctx_wait_mark(msfem_ctx *ctx, int slot, void *stream);
 
 /* Device-memory plumbing for callers without an allocator of their own
  * (no counterpart in the reference, whose data never leaves the host).

@@ -58,7 +58,8 @@ struct msfem_ctx {
   bool profiling = false;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr}; // solve begin/end, element begin/end
   bool ev_recorded = false;
-  hipEvent_t marks[MSFEM_N_MARKS] = {}; // user timing marks on the context stream
+  hipEvent_t marks[MSFEM_N_MARKS] = {}; // user marks (timing, cross-stream order)
+  hipStream_t aux = nullptr;            // the caller-visible second stream
   // graph builds: side streams (one per size class), fork/join events, and the
   // instantiated graphs; `generation` changes whenever a buffer a graph may
   // hold is reallocated or the plan changes, which retires every cached graph
@@ -811,6 +812,41 @@ int msfem_ctx_synchronize(msfem_ctx *ctx)
     return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->aux)
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->aux));
+  return MSFEM_OK;
+}
+
+int msfem_ctx_aux_stream(msfem_ctx *ctx, void **stream)
+{
+  if (!ctx || !stream)
+    return fail(ctx, MSFEM_ERR_ARG, "null argument");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!ctx->aux)
+    HIP_TRY(ctx, hipStreamCreateWithFlags(&ctx->aux, hipStreamNonBlocking)); // default priority
+  *stream = ctx->aux;
+  return MSFEM_OK;
+}
+
+int msfem_ctx_mark_on(msfem_ctx *ctx, int slot, void *stream)
+{
+  if (!ctx || slot < 0 || slot >= MSFEM_N_MARKS)
+    return fail(ctx, MSFEM_ERR_ARG, "mark slot %d outside [0, %d)", slot, MSFEM_N_MARKS);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (!ctx->marks[slot])
+    HIP_TRY(ctx, hipEventCreate(&ctx->marks[slot]));
+  HIP_TRY(ctx, hipEventRecord(ctx->marks[slot], stream ? (hipStream_t)stream : ctx->stream));
+  return MSFEM_OK;
+}
+
+int msfem_ctx_wait_mark(msfem_ctx *ctx, int slot, void *stream)
+{
+  if (!ctx || slot < 0 || slot >= MSFEM_N_MARKS)
+    return fail(ctx, MSFEM_ERR_ARG, "mark slot %d outside [0, %d)", slot, MSFEM_N_MARKS);
+  if (!ctx->marks[slot])
+    return fail(ctx, MSFEM_ERR_ARG, "mark slot %d never recorded", slot);
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamWaitEvent(stream ? (hipStream_t)stream : ctx->stream, ctx->marks[slot], 0));
   return MSFEM_OK;
 }
 
@@ -902,6 +938,11 @@ int msfem_ctx_destroy(msfem_ctx *ctx)
   for (auto &e : ctx->marks)
     if (e)
       hipEventDestroy(e);
+  if (ctx->aux)
+    {
+      hipStreamSynchronize(ctx->aux);
+      hipStreamDestroy(ctx->aux);
+    }
   drop_graphs(ctx);
   for (auto &sd : ctx->side)
     if (sd)

@@ -21,12 +21,13 @@ COEF_SCALAR_CELL, COEF_TENSOR_CELL, COEF_SCALAR_QP, COEF_TENSOR_QP = 0, 1, 2, 3
 RHS_CONST, RHS_SINSIN = 0, 1
 PREC_FP64, PREC_FP32_REFINE = 0, 1
 BC_ZERO_DIRICHLET, BC_REFERENCE = 0, 1
+N_MARKS = 8  # MSFEM_N_MARKS: mark slots per context
 
 # Every symbol include/msfem.h declares.
 EXPORTS = [
     "msfem_ctx_create", "msfem_ctx_destroy", "msfem_last_error", "msfem_default_opts", "msfem_version",
     "msfem_uses_coarse_space",
-    "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
+    "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_ctx_aux_stream", "msfem_ctx_mark_on", "msfem_ctx_wait_mark", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
     "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
@@ -82,6 +83,9 @@ def lib():
         L.msfem_ctx_profile.argtypes = [vp, ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
         L.msfem_ctx_mark.argtypes = [vp, ctypes.c_int]
         L.msfem_ctx_elapsed.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.POINTER(dbl)]
+        L.msfem_ctx_aux_stream.argtypes = [vp, ctypes.POINTER(vp)]
+        L.msfem_ctx_mark_on.argtypes = [vp, ctypes.c_int, vp]
+        L.msfem_ctx_wait_mark.argtypes = [vp, ctypes.c_int, vp]
         L.msfem_dev_alloc.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(vp)]
         L.msfem_dev_free.argtypes = [vp, vp]
         L.msfem_memcpy_h2d.argtypes = [vp, vp, vp, ctypes.c_uint64]
@@ -202,6 +206,21 @@ class Context:
     def mark(self, slot):
         """Record timing mark `slot` (a HIP event) on the context stream."""
         _check(lib().msfem_ctx_mark(self.h, slot), self.h)
+
+    def mark_on(self, slot, stream):
+        """Record mark `slot` on `stream` (None: the context stream)."""
+        _check(lib().msfem_ctx_mark_on(self.h, slot, stream), self.h)
+
+    def wait_mark(self, slot, stream):
+        """Make `stream` (None: the context stream) wait for the last recording of mark `slot`."""
+        _check(lib().msfem_ctx_wait_mark(self.h, slot, stream), self.h)
+
+    @property
+    def aux_stream(self):
+        """The context's second stream (created on first use)."""
+        s = ctypes.c_void_p()
+        _check(lib().msfem_ctx_aux_stream(self.h, ctypes.byref(s)), self.h)
+        return s.value
 
     def elapsed(self, start, end):
         """Device milliseconds between two recorded marks (waits for `end`)."""

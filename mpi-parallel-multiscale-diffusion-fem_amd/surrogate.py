@@ -100,13 +100,19 @@ class Conv:
         self.mb, self.vb = ctx.to_device(np.zeros(cout, np.float32)), ctx.to_device(np.zeros(cout, np.float32))
         # BF16 weights regrouped for the direct 3x3 kernel, rebuilt when wbf changes
         self.wver, self.wpk, self.wpk_ver = 0, None, -1
+        self.stream = None  # None: the context stream
         self.sync_bf16()
+
+    @property
+    def st(self):
+        """The stream this layer's kernels are enqueued on."""
+        return self.ctx.stream if self.stream is None else self.stream
 
     def n_params(self):
         return self.cout * self.K + self.cout
 
     def sync_bf16(self):
-        _ok(_nn().msfem_nn_to_bf16(self.cout * self.K, self.w.ptr, self.wbf.ptr, self.ctx.stream))
+        _ok(_nn().msfem_nn_to_bf16(self.cout * self.K, self.w.ptr, self.wbf.ptr, self.st))
         self.wver += 1
 
     def packed(self):
@@ -114,7 +120,7 @@ class Conv:
         if self.wpk_ver != self.wver:
             if self.wpk is None:
                 self.wpk = self.ctx.alloc(self.cout * self.K, np.uint16)
-            _ok(_nn().msfem_nn_conv3x3_pack(self.cin, self.cout, self.wbf.ptr, self.wpk.ptr, self.ctx.stream))
+            _ok(_nn().msfem_nn_conv3x3_pack(self.cin, self.cout, self.wbf.ptr, self.wpk.ptr, self.st))
             self.wpk_ver = self.wver
         return self.wpk.ptr
 
@@ -122,7 +128,7 @@ class Conv:
         """Serving form of a decoder layer: out = act(conv(up2(x)) + b) + skip with x
         at H/2 x W/2, upsampled inside the convolution's staging (no saved state)."""
         _ok(_nn().msfem_nn_conv3x3_bf16(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 1, out.ptr, self.b.ptr,
-                                        1 if self.relu else 0, skip.ptr, self.ctx.stream))
+                                        1 if self.relu else 0, skip.ptr, self.st))
 
     def small_direct(self):
         """Whether msfem_nn_conv_small takes this layer (the head: at most 8 output channels)."""
@@ -140,7 +146,7 @@ class Conv:
         return H + 2 * self.pad - self.k + 1, W + 2 * self.pad - self.k + 1
 
     def forward(self, x, B, H, W, buf):
-        L, st = _nn(), self.ctx.stream
+        L, st = _nn(), self.st
         Ho, Wo = self.out_hw(H, W)
         N = B * Ho * Wo
         y = buf.get(self.name + ".y", self.cout * N, np.float32)
@@ -169,7 +175,7 @@ class Conv:
 
     def backward(self, dy, buf, want_dx=True):
         """dy: gradient w.r.t. the layer output (modified in place).  Returns dx or None."""
-        L, st = _nn(), self.ctx.stream
+        L, st = _nn(), self.st
         x, B, H, W, y = self.saved
         Ho, Wo = self.out_hw(H, W)
         N = B * Ho * Wo
@@ -193,7 +199,7 @@ class Conv:
         return dx
 
     def adam(self, lr, step, wd=0.0, b1=0.9, b2=0.999, eps=1e-8):
-        L, st = _nn(), self.ctx.stream
+        L, st = _nn(), self.st
         _ok(L.msfem_nn_adam(self.cout * self.K, self.w.ptr, self.gw.ptr, self.mw.ptr, self.vw.ptr, self.wbf.ptr, lr, b1,
                             b2, eps, step, wd, st))
         self.wver += 1
@@ -222,10 +228,22 @@ class Surrogate:
         self.enc += [Conv(ctx, f"e{l}", c[l - 1], c[l], 3, 1, True, rng) for l in range(1, self.levels)]
         self.dec = [Conv(ctx, f"d{l}", c[l + 1], c[l], 3, 1, True, rng) for l in range(self.levels - 1)]
         self.head = Conv(ctx, "head", c[0], out_ch, 2, 0, False, rng)
+        self.stream = None  # None: the context stream
         self.layers = self.enc + self.dec + [self.head]
         self.buf = Buffers(ctx)
         self.step_count = 0
         self.dscale = 1.0  # network output = delta / dscale
+
+    @property
+    def st(self):
+        """The stream the network's kernels are enqueued on."""
+        return self.ctx.stream if self.stream is None else self.stream
+
+    def set_stream(self, stream):
+        """Enqueue the network's kernels on `stream` (None: the context stream)."""
+        self.stream = stream
+        for layer in self.layers:
+            layer.stream = stream
 
     def n_params(self):
         return sum(l.n_params() for l in self.layers)
@@ -250,7 +268,7 @@ class Surrogate:
         read the upsampled input and add the skip inside the convolution; the
         values are bitwise those of the training form, which keeps the
         intermediates the backward pass needs."""
-        L, st, buf = _nn(), self.ctx.stream, self.buf
+        L, st, buf = _nn(), self.st, self.buf
         H = self.tile
         h, _, _ = self.enc[0].forward(x, B, H, H, buf)
         skips = []
@@ -286,7 +304,7 @@ class Surrogate:
 
     def backward(self, dout):
         """dout: gradient of the loss w.r.t. the head output (consumed)."""
-        L, st, buf = _nn(), self.ctx.stream, self.buf
+        L, st, buf = _nn(), self.st, self.buf
         B = self.B
         dh = self.head.backward(dout, buf)
         H = self.tile
@@ -320,7 +338,7 @@ class Surrogate:
 
     def loss_and_grad(self, out, target, n):
         """MSE over n values; returns (loss, grad DeviceArray)."""
-        L, st = _nn(), self.ctx.stream
+        L, st = _nn(), self.st
         grad = self.buf.get("dout", n, np.float32)
         nb = 256
         part = self.buf.get("loss_part", nb, np.float32)
@@ -370,12 +388,13 @@ class Surrogate:
         return s
 
     # ---- serving ------------------------------------------------------------------------
-    def predict_bases(self, x, n, bases_ptr, batch=128):
+    def predict_bases(self, x, n, bases_ptr, batch=512):
         """x: device FP32 [n][tile*tile] normalized log-coefficients (g / CLIP);
         bases_ptr: device FP64 bases [n][4][(tile+1)^2], filled with N_i + dscale *
-        prediction (the x0 of msfem_batch_build_device).  Enqueued on the context
-        stream, not synchronized."""
-        L, st = _nn(), self.ctx.stream
+        prediction (the x0 of msfem_batch_build_device), `batch` tiles per forward
+        pass.  Enqueued on the network's stream (the context stream unless
+        set_stream chose another), not synchronized."""
+        L, st = _nn(), self.st
         t = self.tile
         nbase = 4 * (t + 1) ** 2
         for s in range(0, n, batch):
@@ -420,11 +439,18 @@ class TileSet:
         self.iters = ctx.alloc(n * 4, np.int32)
         self.status = ctx.alloc(n * 4, np.int32)
 
-    def solve(self, opts, x0_ptr=None):
-        """Batched basis build into self.bases (enqueued; returns nothing)."""
-        self.ctx.batch_build_device(2, self.n_fine, self.box.ptr, msfem.COEF_SCALAR_CELL, self.coef.ptr, self.Ae.ptr,
-                                    self.Fe.ptr, self.bases.ptr, self.iters.ptr, self.status.ptr,
-                                    rhs_kind=msfem.RHS_CONST, rhs_value=1.0, x0_ptr=x0_ptr, opts=opts,
+    def solve(self, opts, x0_ptr=None, c0=0, c1=None):
+        """Batched basis build of cells c0..c1-1 (default: all) into self.bases, on
+        the context stream (enqueued; returns nothing).  x0_ptr: initial bases of
+        all n cells (the slice of c0..c1-1 is used)."""
+        c1 = self.n if c1 is None else c1
+        nb = 4 * (self.nf + 1) ** 2
+        self.ctx.batch_build_device(2, self.n_fine[c0:c1], self.box.ptr + c0 * 4 * 8, msfem.COEF_SCALAR_CELL,
+                                    self.coef.ptr + c0 * self.nf ** 2 * 8, self.Ae.ptr + c0 * 16 * 8,
+                                    self.Fe.ptr + c0 * 4 * 8, self.bases.ptr + c0 * nb * 8,
+                                    self.iters.ptr + c0 * 4 * 4, self.status.ptr + c0 * 4 * 4,
+                                    rhs_kind=msfem.RHS_CONST, rhs_value=1.0,
+                                    x0_ptr=None if x0_ptr is None else x0_ptr + c0 * nb * 8, opts=opts,
                                     stream=self.ctx.stream)
 
     def free(self):
@@ -603,11 +629,54 @@ def energy_error(alpha, phi, ref):
     return float(np.max(np.sqrt(energy_norm2(alpha, phi - ref) / energy_norm2(alpha, ref))))
 
 
-def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, sample=64, log=None, data=None):
+def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512):
+    """One serving step -- prediction and warm-started basis build of all cells
+    of `data` -- in `chunks` chunks: the inference of every chunk runs on the
+    context's second stream (a mark after each), the build of chunk k on the
+    context stream behind chunk k's mark, enqueued right after the inference of
+    chunk k + 1, so that inference (matrix cores, memory) and builds (VALU)
+    overlap.  The values are bitwise those of predict_bases + one build of all
+    cells.  Uses marks mark0 .. mark0 + chunks; the inference first waits for
+    the context stream, so x0 is not rewritten under a previous step's builds."""
+    ctx, t = net.ctx, net.tile
+    n, nb = data.n, 4 * (data.nf + 1) ** 2
+    if mark0 < 0 or mark0 + chunks >= msfem.N_MARKS:
+        raise ValueError("not enough marks for the chunks")
+    # chunk bounds on multiples of the inference batch
+    per = -(-n // chunks)
+    step = -(-per // batch) * batch if n > batch else n
+    bounds = [(c, min(n, c + step)) for c in range(0, n, step)]
+    aux = ctx.aux_stream
+    ctx.mark_on(mark0, None)
+    ctx.wait_mark(mark0, aux)
+
+    def infer(k):
+        c0, c1 = bounds[k]
+        net.set_stream(aux)
+        try:
+            net.predict_bases(Ptr(data.x.ptr + c0 * t * t * 4), c1 - c0, x0.ptr + c0 * nb * 8, batch=batch)
+        finally:
+            net.set_stream(None)
+        ctx.mark_on(mark0 + 1 + k, aux)
+
+    def build(k):
+        ctx.wait_mark(mark0 + 1 + k, None)
+        data.solve(opts, x0.ptr, *bounds[k])
+
+    infer(0)
+    for k in range(len(bounds)):
+        if k + 1 < len(bounds):
+            infer(k + 1)
+        build(k)
+
+
+def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, sample=64, log=None, data=None,
+                 pipe_batch=512):
     """Times one step -- the basis build of n_side^2 fresh GRF cells plus the
-    coarse assembly -- four ways: plain FP64, FP32+FP64 refinement, and each of
+    coarse assembly -- five ways: plain FP64, FP32+FP64 refinement, and each of
     them warm-started from the network's prediction (inference included in the
-    step).  Reports cells/s (wall clock over `reps` steps), device times of the
+    step), FP32 also with the inference of one chunk of cells overlapping the
+    builds of the previous one (serve_pipelined).  Reports cells/s (wall clock over `reps` steps), device times of the
     inference and of the local solves (HIP events on the context stream),
     iterations, the fallback rate (share of local problems whose prediction
     failed the residual gate and needed CG iterations) and the energy-norm
@@ -622,10 +691,12 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, 
     Kc, Fc = ctx.alloc(ncn * 9, np.float64), ctx.alloc(ncn, np.float64)
     cm, cv = ctx.alloc(ncn, np.uint8), ctx.alloc(ncn, np.float64)
     pick = np.unique(np.linspace(0, n - 1, min(sample, n)).astype(np.int64))
-    legs = [("fp64", msfem.default_opts(precision=msfem.PREC_FP64), False),
-            ("fp32_refine", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), False),
-            ("fp32_refine_surrogate", msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol), True),
-            ("fp64_surrogate", msfem.default_opts(precision=msfem.PREC_FP64), True)]
+    fp32 = msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=rel_tol)
+    legs = [("fp64", msfem.default_opts(precision=msfem.PREC_FP64), None),
+            ("fp32_refine", fp32, None),
+            ("fp32_refine_surrogate", fp32, "serial"),
+            ("fp32_refine_surrogate_pipelined", fp32, "pipelined"),
+            ("fp64_surrogate", msfem.default_opts(precision=msfem.PREC_FP64), "serial")]
     ctx.set_profiling(True)
     # inference alone (device time between two marks, per pass over all cells)
     for _ in range(warmup):
@@ -642,9 +713,12 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, 
         solve_ms = []
 
         def step():
-            if use:
-                net.predict_bases(data.x, n, x0.ptr)
-            data.solve(opts, x0.ptr if use else None)
+            if use == "pipelined":  # inference of chunk k+1 beside the builds of chunk k
+                serve_pipelined(net, data, x0, opts, batch=pipe_batch)
+            else:
+                if use:
+                    net.predict_bases(data.x, n, x0.ptr)
+                data.solve(opts, x0.ptr if use else None)
             ctx.coarse_assemble_device(2, n_side, data.Ae.ptr, data.Fe.ptr, msfem.BC_ZERO_DIRICHLET, Kc.ptr, Fc.ptr,
                                        cm.ptr, cv.ptr, stream=ctx.stream)
         for _ in range(warmup):
@@ -653,7 +727,8 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, 
         t = time.perf_counter()
         for _ in range(reps):
             step()
-            solve_ms.append(ctx.profile()[0])
+            if use != "pipelined":  # (reading the build timers waits for the build)
+                solve_ms.append(ctx.profile()[0])
         ctx.synchronize()
         ms = (time.perf_counter() - t) * 1e3 / reps
         st = data.status.numpy()
@@ -664,7 +739,8 @@ def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, 
         if ref is None:
             ref = bases
         r = {"ms": round(ms, 3), "cells_per_s": round(n / (ms * 1e-3), 1),
-             "solve_kernel_ms": round(float(np.mean(solve_ms)), 3), "iters_mean": round(float(its.mean()), 2),
+             "solve_kernel_ms": round(float(np.mean(solve_ms)), 3) if solve_ms else None,
+             "iters_mean": round(float(its.mean()), 2),
              "iters_p99": int(np.percentile(its, 99)),
              "energy_err_max": float(max(energy_error(alpha[c], bases[c], ref[c]) for c in pick))}
         if use:

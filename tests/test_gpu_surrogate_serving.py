@@ -100,3 +100,36 @@ def test_surrogate_gain_and_gate(ctx):
     assert worst_energy_error(data, got, ref, np.where(hit)[0][:16]) <= 1e-6
     data.free()
     x0.free()
+
+
+def test_pipelined_serving_is_bitwise(ctx):
+    """serve_pipelined (inference of chunk k+1 on the context's second stream
+    beside the builds of chunk k) gives bitwise the bases, A^K, F^K and
+    iteration counts of predict_bases + one build of all cells with the same
+    inference batch, over two steps in a row (the second step's inference waits
+    for the first step's builds)."""
+    net = S.Surrogate.load(ctx, WEIGHTS)
+    data = S.TileSet(ctx, 640, 43)
+    n, per = data.n, 4 * (data.nf + 1) ** 2
+    o32 = msfem.default_opts(precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+    x0 = ctx.alloc(n * per, np.float64)
+    outs = (data.bases, data.Ae, data.Fe, data.iters)
+    for batch, chunks in ((512, 4), (128, 3)):
+        say(f"pipelined serving: 640 GRF cells, inference batch {batch}, {chunks} chunks")
+        net.predict_bases(data.x, n, x0.ptr, batch=batch)
+        data.solve(o32, x0.ptr)
+        ctx.synchronize()
+        want = [a.numpy().reshape(-1).copy() for a in outs]
+        pred = x0.numpy().reshape(-1).copy()
+        for steps in (1, 2):  # one step; two steps back to back
+            for a in (data.bases, x0):
+                a.upload(np.zeros(n * per))
+            for _ in range(steps):
+                S.serve_pipelined(net, data, x0, o32, chunks=chunks, batch=batch)
+            ctx.synchronize()
+            assert not data.status.numpy().any()
+            assert np.array_equal(x0.numpy().reshape(-1), pred)
+            for w, a in zip(want, outs):
+                assert np.array_equal(a.numpy().reshape(-1), w)
+    data.free()
+    x0.free()
