@@ -450,6 +450,164 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
       }
 }
 
+// Persistent form of conv3x3_bf16_kernel for 32 output channels and NB <= 2
+// blocks of 32 input channels (the full-resolution layers of an
+// encoder-decoder, whose workgroups would otherwise restage the same weights
+// for every tile and find no work to overlap with their first loads; the
+// host takes it for NB = 1, where it measured faster than the tile form): a
+// workgroup stages the weights of every block once, then walks the steps
+// (tile, block), tiles blockIdx.x + k gridDim.x; the input of the next step is
+// loaded into registers during the MFMAs and the epilogue of the current one.
+// The sums run in the same order as in conv3x3_bf16_kernel (blocks, taps,
+// 16-channel steps), so the results are bitwise those of the tile form.
+template <int W, bool UP, int NB>
+__global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int ntiles, const float *in,
+                                                            const uint16_t *wp, float *out, const float *bias,
+                                                            int relu, const float *add)
+{
+  constexpr int BM = 32, K = NB * 32 * 9;
+  constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI;
+  constexpr int IR = TR + 2, IC = W + 2, NPIX = TI * IR * IC;
+  constexpr int CP = 40;
+  constexpr int NG = W / 4, IIT = TI * IR * NG * 4;
+  static_assert(IIT <= 256 && kGBN % W == 0 && NB >= 1 && NB <= 2, "shape outside the persistent form");
+  __shared__ __align__(16) uint16_t sIn[NPIX * CP];
+  __shared__ __align__(16) uint16_t sW[NB * 9 * BM * CP];
+
+  const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
+  const int wn = wave * 32; // four waves along the pixels, one 32 x 32 block each
+  const int HW = H * W;
+  const size_t plane = (size_t)B * HW;
+  const size_t iplane = UP ? plane / 4 : plane;
+
+  // the weights of every block, [blk][tap][co][ci], once
+  for (int u = tid; u < NB * BM * 36; u += 256)
+    {
+      const int blk = u / (BM * 36), r = u % (BM * 36), m = r / 36, j = r % 36;
+      *(bf16x8_t *)&sW[((blk * 9 + j / 4) * BM + m) * CP + (j % 4) * 8] =
+          *(const bf16x8_t *)(wp + (size_t)m * K + blk * 288 + j * 8);
+    }
+  for (int h = tid; h < TI * IR * 2 * 4; h += 256) // halo columns: (staged row, side, octet)
+    {
+      const int rr = h / 8, side = h / 4 % 2, oct = h % 4;
+      *(bf16x8_t *)&sIn[(rr * IC + side * (IC - 1)) * CP + oct * 8] = bf16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+
+  // input staging as in conv3x3_bf16_kernel; iok describes the step whose
+  // values are in the registers
+  float4 ri[UP ? 1 : 8];
+  float2 rh[UP ? 8 : 1];
+  const int ioct = tid % 4, ig = tid / 4 % NG, irr = tid / 4 / NG;
+  const bool iact = tid < IIT;
+  bool iok = false;
+  auto fetch = [&](int tile, int blk) {
+    const int p0 = tile * kGBN, b = p0 / HW, y0 = (p0 - b * HW) / W;
+    const int iy = y0 - 1 + irr % IR;
+    iok = iact && iy >= 0 && iy < H;
+    if (iok)
+      {
+        const float *src = in + (UP ? (size_t)(b + irr / IR) * (HW / 4) + (size_t)(iy / 2) * (W / 2) + 2 * ig
+                                    : (size_t)(b + irr / IR) * HW + (size_t)iy * W + 4 * ig) +
+                           (size_t)(blk * 32 + ioct * 8) * iplane;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+          if constexpr (UP)
+            rh[e] = *(const float2 *)(src + e * iplane);
+          else
+            ri[e] = *(const float4 *)(src + e * iplane);
+      }
+  };
+  auto stash = [&]() {
+    if (iact)
+      {
+        uint16_t *dst = &sIn[(irr * IC + 1 + 4 * ig) * CP + ioct * 8];
+#pragma unroll
+        for (int px = 0; px < 4; ++px)
+          {
+            bf16x8_t v;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              {
+                float f;
+                if constexpr (UP)
+                  f = px < 2 ? rh[e].x : rh[e].y;
+                else
+                  f = px == 0 ? ri[e].x : px == 1 ? ri[e].y : px == 2 ? ri[e].z : ri[e].w;
+                v[e] = iok ? f32_to_bf16(f) : (uint16_t)0;
+              }
+            *(bf16x8_t *)(dst + px * CP) = v;
+          }
+      }
+  };
+
+  const int n = wn + lane % 32, img = n / (TR * W), r = n % (TR * W) / W;
+  const int pb = ((img * IR + r) * IC + n % W) * CP;
+  const int kh = 8 * (lane / 32);
+  const int arow = (lane % 32) * CP + kh;
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    acc[e] = 0.0f;
+
+  int tile = blockIdx.x, blk = 0;
+  if (tile >= ntiles) // (the host sizes the grid to at most ntiles)
+    return;
+  fetch(tile, 0);
+  stash();
+  __syncthreads();
+  for (;;)
+    {
+      int ntile = tile, nblk = blk + 1;
+      if (nblk == NB)
+        {
+          nblk = 0;
+          ntile = tile + gridDim.x;
+        }
+      const bool more = ntile < ntiles;
+      if (more)
+        fetch(ntile, nblk); // in flight during the MFMAs and the epilogue below
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+        {
+          const int toff = ((tap / 3) * IC + tap % 3) * CP;
+#pragma unroll
+          for (int ks = 0; ks < 32; ks += 16)
+            {
+              const bf16x8m_t av = *(const bf16x8m_t *)&sW[(blk * 9 + tap) * BM * CP + arow + ks];
+              const bf16x8m_t bv = *(const bf16x8m_t *)&sIn[pb + toff + kh + ks];
+              acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+            }
+        }
+      if (blk == NB - 1) // the tile is complete: epilogue, then a fresh accumulator
+        {
+#pragma unroll
+          for (int e = 0; e < 16; ++e)
+            {
+              const int row = 8 * (e / 4) + 4 * (lane / 32) + (e % 4);
+              float v = acc[e];
+              if (bias)
+                {
+                  v += bias[row];
+                  if (relu && v < 0.0f) // as bias_act_kernel (NaN passes)
+                    v = 0.0f;
+                }
+              const size_t o = (size_t)row * plane + (size_t)tile * kGBN + n;
+              if (add)
+                v += add[o];
+              out[o] = v;
+              acc[e] = 0.0f;
+            }
+        }
+      __syncthreads(); // every wave is done with sIn
+      if (!more)
+        break;
+      stash();
+      __syncthreads();
+      tile = ntile;
+      blk = nblk;
+    }
+}
+
 // Direct convolution with few output channels (a network's head) on the
 // VALU: one thread per output pixel computes CO channels (CO >= Cout; weights
 // of the channels beyond Cout are zero, their sums are not stored) from the
@@ -820,6 +978,25 @@ int check_launch()
   return hipGetLastError() == hipSuccess ? MSFEM_OK : MSFEM_ERR_HIP;
 }
 
+// Launches a persistent kernel of 256 threads on min(ntiles, resident
+// workgroups of the whole device) workgroups.
+template <typename... KArgs, typename... Args>
+hipError_t launch_tiles(void (*kernel)(KArgs...), int ntiles, hipStream_t st, Args... args)
+{
+  int dev = 0, ncu = 0, per_cu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess)
+    e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e == hipSuccess)
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0);
+  if (e != hipSuccess)
+    return e;
+  const long long slots = (long long)(ncu > 0 ? ncu : 1) * (per_cu > 0 ? per_cu : 1);
+  const int grid = (int)(ntiles < slots ? ntiles : slots);
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(256), 0, st, args...);
+  return hipGetLastError();
+}
+
 } // namespace
 
 extern "C" {
@@ -990,10 +1167,32 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
   if (plane % kGBN != 0 || (long long)Cin * plane >= (1ll << 31) || (long long)Cout * plane >= (1ll << 31) ||
       (long long)Cout * Cin * 9 >= (1ll << 31))
     return MSFEM_ERR_ARG;
-  const int BM = Cout % 64 == 0 ? 64 : 32;
-  const dim3 grid((unsigned)(plane / kGBN), (unsigned)(Cout / BM)), block(256);
   hipStream_t st = (hipStream_t)stream;
   const uint16_t *w = (const uint16_t *)wp;
+  const int ntiles = (int)(plane / kGBN);
+  if (Cout == 32 && Cin == 32) // persistent form: weights staged once per workgroup
+    {
+      // (with two input blocks the resident weights cost a third of the
+      // occupancy, and the tile form is faster)
+      const int nb = 1;
+      hipError_t e = hipSuccess;
+#define MSFEM_C3P_CASE(Wv, UPv, NBv)                                                                                  \
+  if (W == Wv && (upsample != 0) == UPv && nb == NBv)                                                                \
+    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv>, ntiles, st, B, H, ntiles, in, w, out, bias, relu, add);
+#define MSFEM_C3P_W(Wv)                                                                                               \
+  MSFEM_C3P_CASE(Wv, false, 1)                                                                                       \
+  MSFEM_C3P_CASE(Wv, true, 1)
+      MSFEM_C3P_W(4)
+      MSFEM_C3P_W(8)
+      MSFEM_C3P_W(16)
+      MSFEM_C3P_W(32)
+      MSFEM_C3P_W(64)
+#undef MSFEM_C3P_W
+#undef MSFEM_C3P_CASE
+      return e == hipSuccess ? check_launch() : MSFEM_ERR_HIP;
+    }
+  const int BM = Cout % 64 == 0 ? 64 : 32;
+  const dim3 grid((unsigned)ntiles, (unsigned)(Cout / BM)), block(256);
 #define MSFEM_C3_CASE(BMv, Wv)                                                                                         \
   if (BM == BMv && W == Wv)                                                                                          \
     {                                                                                                                \

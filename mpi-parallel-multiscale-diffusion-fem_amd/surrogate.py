@@ -544,7 +544,8 @@ def main(argv=None):
     ap.add_argument("--serve-seed", type=int, default=41)
     ap.add_argument("--report", default=None, help="JSON file for the serving report")
     ap.add_argument("--time-forward", type=int, default=0,
-                    help="time N forward passes over 128 random tiles (after one untimed pass)")
+                    help="time N forward passes over random tiles (after one untimed pass)")
+    ap.add_argument("--forward-batch", type=int, default=128, help="--time-forward: tiles per pass")
     ap.add_argument("--forward-form", choices=("serving", "training", "both"), default="both",
                     help="--time-forward: the serving form (fused decoder) and/or the training form")
     a = ap.parse_args(argv)
@@ -561,18 +562,19 @@ def main(argv=None):
     else:
         net = train_new(ctx, a, log)
     if a.time_forward:
-        x = ctx.to_device(np.random.default_rng(0).standard_normal(128 * net.tile ** 2).astype(np.float32))
+        fb = a.forward_batch
+        x = ctx.to_device(np.random.default_rng(0).standard_normal(fb * net.tile ** 2).astype(np.float32))
         forms = {"serving": (False,), "training": (True,), "both": (False, True)}[a.forward_form]
         for keep in forms:
-            net.forward(x, 128, keep=keep)
+            net.forward(x, fb, keep=keep)
             ctx.synchronize()
             t = time.perf_counter()
             for _ in range(a.time_forward):
-                net.forward(x, 128, keep=keep)
+                net.forward(x, fb, keep=keep)
             ctx.synchronize()
             dt = (time.perf_counter() - t) / a.time_forward
-            log(f"{'training' if keep else 'serving'} forward, 128 tiles: {dt * 1e3:.3f} ms "
-                f"({net.flops_per_sample() * 128 / dt / 1e12:.1f} TFLOP/s)")
+            log(f"{'training' if keep else 'serving'} forward, {fb} tiles: {dt * 1e3:.3f} ms "
+                f"({net.flops_per_sample() * fb / dt / 1e12:.1f} TFLOP/s)")
         x.free()
     if a.serve:
         rep = serve_report(ctx, net, n_side=a.serve, seed=a.serve_seed, log=log)

@@ -272,6 +272,44 @@ def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
         a.free()
 
 
+@pytest.mark.parametrize("Cin,B,H,W,up", [
+    (32, 2, 64, 64, 0), (32, 1, 64, 64, 1), (64, 3, 32, 32, 0), (32, 2, 32, 16, 1), (32, 6, 8, 8, 1),
+    (32, 8, 4, 4, 0), (64, 1, 64, 64, 1)])
+def test_conv3x3_persistent_matches_tiles(ctx, Cin, B, H, W, up):
+    """32 output and 32 input channels take the persistent kernel (weights
+    staged once per workgroup, the next tile's loads in flight); its sums run
+    in the tile kernel's order, so a 64-channel convolution with weights
+    [w; w] (the tile kernel) gives bitwise the same first 32 channels, bias,
+    ReLU, upsampling and skip sum included.  (Cin = 64: both sides are tile
+    kernels, BM = 32 and 64.)"""
+    say(f"persistent conv3x3 Cin={Cin} B={B} H={H} W={W} up={up}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(Cin + B + H + W + up)
+    hi, wi = (H // 2, W // 2) if up else (H, W)
+    x = ctx.to_device(rng.standard_normal((Cin, B, hi, wi)).astype(np.float32))
+    K, N = Cin * 9, B * H * W
+    w = bf16_bits(R.bf16((rng.standard_normal((32, K)) / np.sqrt(K)).astype(np.float32)))
+    bias = rng.standard_normal(32).astype(np.float32)
+    skip = rng.standard_normal((32, N)).astype(np.float32)
+    d32 = [ctx.to_device(a) for a in (w, bias, skip)]
+    d64 = [ctx.to_device(np.concatenate([a, a])) for a in (w, bias, skip)]
+    outs = []
+    for (dw, db, dsk), cout in ((d32, 32), (d64, 64)):
+        wpk, y = ctx.alloc(cout * K, np.uint16), ctx.alloc(cout * N, np.float32)
+        assert L.msfem_nn_conv3x3_pack(Cin, cout, dw.ptr, wpk.ptr, st) == 0
+        assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, wpk.ptr, cout, x.ptr, up, y.ptr, db.ptr, 1, dsk.ptr, st) == 0
+        outs.append(y.numpy().reshape(cout, N)[:32].copy())
+        # without bias and skip: the plain sums agree as well
+        assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, wpk.ptr, cout, x.ptr, up, y.ptr, None, 0, None, st) == 0
+        outs.append(y.numpy().reshape(cout, N)[:32].copy())
+        wpk.free()
+        y.free()
+    assert np.array_equal(outs[0], outs[2]) and np.array_equal(outs[1], outs[3])
+    assert np.abs(outs[1]).max() > 0
+    for a in [x] + d32 + d64:
+        a.free()
+
+
 @pytest.mark.parametrize("Cin,Cout,B,H,W,k,pad", [
     (32, 4, 3, 64, 64, 2, 0), (5, 3, 2, 9, 7, 3, 1), (1, 8, 2, 16, 16, 3, 1), (64, 1, 1, 12, 10, 2, 0),
     (11, 6, 2, 8, 9, 1, 0), (3, 7, 2, 10, 12, 3, 1)])
