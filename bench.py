@@ -221,6 +221,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 4),
+        "local_solves_per_s": round(cells_per_s * nb, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -327,6 +328,7 @@ def bench_surrogate(args):
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": main_leg["ms"],
+        "local_solves_per_s": round(main_leg["cells_per_s"] * 4, 1),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
