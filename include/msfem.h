@@ -221,7 +221,8 @@ int msfem_set_variant(msfem_ctx *ctx, int dim, int nf, int precision, int coef_k
  * get_global_element_matrix / get_global_element_rhs (:320-334).
  * Host pointers; the call is synchronous.  Returns MSFEM_ERR_NOCONV (after
  * writing every output) when a basis solve hit its iteration cap, where the
- * reference throws SolverControl::NoConvergence. */
+ * reference throws SolverControl::NoConvergence.  An empty batch (n_cells =
+ * 0) is a no-op returning MSFEM_OK; its arrays and outputs may be NULL. */
 int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts,
                       msfem_batch_out *out);
 

@@ -576,10 +576,10 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
   int rc = validate(ctx, in, o);
   if (rc)
     return rc;
+  if (in->n_cells == 0) // nothing to build or write (the outputs may be NULL)
+    return MSFEM_OK;
   if (!out || !out->Ae || !out->Fe)
     return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
-  if (in->n_cells == 0)
-    return MSFEM_OK;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   rc = make_plan(ctx, in);
   if (rc)
@@ -979,10 +979,10 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   int rc = validate(ctx, in, opts);
   if (rc)
     return rc;
+  if (in->n_cells == 0) // nothing to build or write (the outputs may be NULL)
+    return MSFEM_OK;
   if (!out || !out->Ae || !out->Fe)
     return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
-  if (in->n_cells == 0)
-    return MSFEM_OK;
   if (in->vertices)
     for (int c = 0; c < in->n_cells; ++c)
       if (!quad_ok(in->vertices + (size_t)c * 8))

@@ -193,6 +193,30 @@ def test_invalid_arguments_are_rejected(ctx):
         ctx.batch_build(3, [16], C.unit_cells(3, 2)[:1], msfem.COEF_TENSOR_CELL, np.ones(16 ** 3 * 6))
 
 
+def test_empty_and_ragged_batches(ctx):
+    """An empty batch is a no-op (host and device entry points, graph form too);
+    a ragged batch -- one cell of each resolution in an order that interleaves
+    the size classes -- gives every cell exactly what a batch of that cell
+    alone gives (class grouping and batch composition do not change a value)."""
+    say("empty and ragged batches")
+    for graph in (0, 1):
+        out = ctx.batch_build(2, np.zeros(0, np.int32), np.zeros((0, 4)), msfem.COEF_SCALAR_CELL, np.zeros(0),
+                              opts=msfem.default_opts(graph=graph))
+        assert out["Ae"].shape == (0, 4, 4) and out["bases"].size == 0
+    ctx.batch_build_device(2, np.zeros(0, np.int32), None, msfem.COEF_SCALAR_CELL, None, None, None,
+                           stream=ctx.stream)
+    ctx.synchronize()
+    case = C.tensor_cells(nc=128, which=np.array([5, 900, 77, 3000, 12, 16000]), n_fine=[128, 16, 64, 32, 16, 64])
+    whole = C.gpu_build(ctx, case)
+    boff = np.concatenate([[0], np.cumsum(4 * (case["n_fine"] + 1) ** 2)])
+    coff = np.concatenate([[0], np.cumsum(3 * case["n_fine"] ** 2)])
+    for i, nf in enumerate(case["n_fine"]):
+        one = dict(case, n_fine=case["n_fine"][i:i + 1], box=case["box"][i:i + 1], coef=case["coef"][coff[i]:coff[i + 1]])
+        alone = C.gpu_build(ctx, one)
+        assert np.array_equal(whole["bases"][boff[i]:boff[i + 1]], alone["bases"])
+        assert np.array_equal(whole["Ae"][i], alone["Ae"][0]) and np.array_equal(whole["iters"][i], alone["iters"][0])
+
+
 def test_full_lognormal_batch_properties(ctx):
     """The whole 64x64-coarse batch (4096 cells, 64^2 fine): size-independent
     properties on every cell, oracle parity on a sample."""
