@@ -126,6 +126,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr bool QLDS = MODE == kModeStream;
   constexpr int NT = kWave * SW;
   constexpr int PSN = coupling_plane_stride(NF); // node-coupling plane stride (kModeStream)
+  // kModeStream, 64^2 (8 nodes per lane, 256 registers): the lane's window
+  // of couplings of the first NHOIST planes (FP32: all four; FP64: E and N)
+  // stays in registers across the whole solve, the others are streamed
+  constexpr bool HOISTC = MODE == kModeStream && NF == 64 && WPE <= 2;
+  constexpr int NHOIST = HOISTC ? (sizeof(T) == 4 ? 4 : 2) : 0;
 
   __shared__ T sgrid[PPW * GSZ];
   __shared__ T sred[problem_sums_slots<SW, PPW, 3>()];       // inner-CG partial sums (type T)
@@ -582,6 +587,23 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
             irun = false;
           T rz = v2[0];
+          // HOISTC: the couplings of the lane's rows Y0 - 1 .. Y0 + R - 1 and
+          // columns X0 - 1 .. X0 + CPL, loaded once (every index lies inside the
+          // four padded planes; the unused entries are dead)
+          T creg[HOISTC ? R + 1 : 1][HOISTC ? NHOIST : 1][CPL + 2];
+          if constexpr (HOISTC)
+            {
+#pragma unroll
+              for (int r = 0; r <= R; ++r)
+#pragma unroll
+                for (int pn = 0; pn < NHOIST; ++pn)
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    {
+                      const int q = pn * PSN + (Y0 - 1 + r) * NP + X0 - 1 + k;
+                      creg[r][pn][k] = !pvalid ? (T)0 : sizeof(T) == 8 ? (T)cg[q] : (T)c32[q];
+                    }
+            }
           typedef T __attribute__((address_space(1))) * gtptr;
           typedef const T __attribute__((address_space(1))) * gcptr;
           while (any)
@@ -607,7 +629,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               typedef double __attribute__((address_space(1))) * gdptr;
               const gdptr ph = (gdptr)pb;
               // couplings at window column k (node X0 - 1 + k) of node row Y
-              auto cpl = [&](int plane, int Y, int k) -> T { return cp[(unsigned)(plane * PSN + Y * NP + X0 - 1 + k)]; };
+              auto cpl = [&](int plane, int Y, int k) -> T {
+                if (plane < NHOIST) // (plane and Y - Y0 are compile-time constants here)
+                  return creg[Y - Y0 + 1][HOISTC ? plane : 0][k];
+                return cp[(unsigned)(plane * PSN + Y * NP + X0 - 1 + k)];
+              };
               // couplings of the node row (E, N, NE, NW) and of the row below
               // (pN, pNE, pNW) in a sliding register window (in FP64 this spills
               // a few values, still faster than reloading each node's 8 couplings)
