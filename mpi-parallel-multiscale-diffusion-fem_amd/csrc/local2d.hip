@@ -544,7 +544,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
             sr[i][c] = 0;
           }
       bool irun = run, first = true;
-      T gamma_prev = 1, alpha_prev = 1;
+      T inv_gamma_prev = 1, inv_alpha_prev = 1; // 1 / gamma, 1 / alpha of the previous iteration
       if constexpr (MODE == kModeStream)
         {
           // Standard Jacobi PCG with two reductions per iteration.  p lives in
@@ -855,13 +855,17 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
             {
               const T gamma = v3[0], delta = v3[1];
               T beta = 0, alpha;
-              if (first)
-                alpha = quotient<T>(gamma, delta);
-              else
+              // Chronopoulos-Gear: beta = gamma / gamma_prev, alpha = gamma /
+              // (delta - beta gamma / alpha_prev), with 1 / gamma and 1 / alpha
+              // carried to the next iteration (two reciprocals, no divisions)
+              const T inv_gamma = reciprocal<T>(gamma);
+              T denom = delta;
+              if (!first)
                 {
-                  beta = quotient<T>(gamma, gamma_prev);
-                  alpha = quotient<T>(gamma, delta - beta * quotient<T>(gamma, alpha_prev));
+                  beta = gamma * inv_gamma_prev;
+                  denom = delta - beta * gamma * inv_alpha_prev;
                 }
+              alpha = gamma * reciprocal<T>(denom);
               const T tb = beta, ta = alpha;
 #pragma unroll
               for (int i = 0; i < R; ++i)
@@ -876,8 +880,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     xr[i][c] += ta * pr[i][c];
                     rr[i][c] -= ta * sr[i][c];
                   }
-              gamma_prev = gamma;
-              alpha_prev = alpha;
+              inv_gamma_prev = inv_gamma;
+              inv_alpha_prev = denom * inv_gamma;
               first = false;
               ++it_total;
               if constexpr (CS)
