@@ -13,6 +13,7 @@ namespace msfem {
 namespace {
 
 constexpr int kElemThreads = 256;
+constexpr int kElemMaxNF = 128; // largest fine resolution of a cell (2D)
 
 __device__ __forceinline__ int sym_index(int dim, int i, int j)
 {
@@ -32,13 +33,20 @@ __device__ __forceinline__ int sym_index(int dim, int i, int j)
 }
 
 // QUAD: general quadrilaterals (2D, a.verts), geometry per fine cell
-template <int DIM, bool QUAD>
+// QPK: the coefficient kind may be one given at the quadrature points; false
+// compiles only the per-fine-cell kinds (their Gauss sums are folded into unit
+// matrices), which keeps the common instance's registers (and so its occupancy)
+// free of the quadrature-point path.
+template <int DIM, bool QUAD, bool QPK = true>
 __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
 {
   constexpr int NV = 1 << DIM, NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
   constexpr int NVAL = NV * NV + NV;
   __shared__ double sM[NS][NV * NV]; // unit element matrices of each tensor component
   __shared__ double sred[kElemThreads / kWave][NVAL];
+  // f = prod_k sin(pi x_k) is separable: sin(pi x_k) at every fine cell's two
+  // Gauss coordinates along each axis, tabulated once per cell (box cells)
+  __shared__ double ssin[QUAD ? 1 : DIM][QUAD ? 1 : 2 * kElemMaxNF];
 
   const int cell = a.cells[blockIdx.x];
   const int nf = a.nf, np = nf + 1;
@@ -103,6 +111,14 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
         }
       sM[s][ab] = v;
     }
+  if constexpr (!QUAD)
+    if (a.rhs_kind == 1)
+      for (int t = threadIdx.x; t < DIM * 2 * nf; t += blockDim.x)
+        {
+          const int k = t / (2 * nf), fi = t % (2 * nf) / 2;
+          const double xi = (t & 1) ? 0.5 + gs : 0.5 - gs;
+          ssin[k][t % (2 * nf)] = sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi / nf) + h[k] * xi));
+        }
   __syncthreads();
 
   const double *cf = a.coef + a.coef_off[cell];
@@ -129,7 +145,7 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
           for (int ab = 0; ab < 16; ++ab)
             Ke[ab] = K4[ab / 4][ab % 4];
         }
-      else if (a.coef_kind == 0 || a.coef_kind == 1)
+      else if (!QPK || a.coef_kind == 0 || a.coef_kind == 1)
         {
           double cs[NS];
           if (a.coef_kind == 0)
@@ -157,7 +173,7 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
               Ke[ab] = v;
             }
         }
-      else
+      else if constexpr (QPK)
         {
           // coefficient at the quadrature points: Gauss sum per fine cell
 #pragma unroll
@@ -203,10 +219,7 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
                   fv = 1.0;
                   const int fi[3] = {fx, fy, fz};
                   for (int k = 0; k < DIM; ++k)
-                    {
-                      const double xi = ((q >> k) & 1) ? 0.5 + gs : 0.5 - gs;
-                      fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * xi));
-                    }
+                    fv *= ssin[k][2 * fi[k] + ((q >> k) & 1)];
                 }
 #pragma unroll
               for (int aa = 0; aa < NV; ++aa)
@@ -452,6 +465,8 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
     return hipSuccess;
   if (a.dim == 2 && a.verts)
     hipLaunchKernelGGL((element_kernel<2, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else if (a.dim == 2 && a.coef_kind <= 1) // per-fine-cell coefficient
+    hipLaunchKernelGGL((element_kernel<2, false, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.dim == 2)
     hipLaunchKernelGGL((element_kernel<2, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.coef_kind == 0) // scalar per fine cell
