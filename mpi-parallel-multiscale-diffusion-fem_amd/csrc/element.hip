@@ -14,6 +14,7 @@ namespace {
 
 constexpr int kElemThreads = 256;
 constexpr int kElemMaxNF = 128; // largest fine resolution of a cell (2D)
+constexpr int kElem3MaxNF = 16; // and in 3D
 
 __device__ __forceinline__ int sym_index(int dim, int i, int j)
 {
@@ -304,6 +305,9 @@ __global__ void __launch_bounds__(kElemThreads) element3d_kernel(ElementArgs a)
   constexpr int NV = 8, NQ = 8, NU = NV * (NV + 1) / 2, NACC = NU + NV;
   __shared__ double sL[NV * NV];
   __shared__ double sred[kElemThreads / kWave][NACC];
+  // the load's factors: sin(pi x_k) at the two Gauss coordinates of every fine
+  // cell along each axis (sin sin sin load), or 1 (constant load: f = rhs_value)
+  __shared__ double sfac[3][2 * kElem3MaxNF];
 
   const int cell = a.cells[blockIdx.x];
   const int nf = a.nf, np = nf + 1;
@@ -335,7 +339,15 @@ __global__ void __launch_bounds__(kElemThreads) element3d_kernel(ElementArgs a)
           v += grad(aa, k, q) * grad(bb, k, q) * jxw;
       sL[t] = v;
     }
+  for (int t = threadIdx.x; t < 3 * 2 * nf; t += blockDim.x)
+    {
+      const int k = t / (2 * nf), fi = t % (2 * nf) / 2;
+      sfac[k][t % (2 * nf)] =
+          a.rhs_kind == 1 ? sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi / nf) + h[k] * gauss(t & 1, 0)))
+                          : 1.0;
+    }
   __syncthreads();
+  const double fconst = a.rhs_kind == 1 ? 1.0 : a.rhs_value;
 
   const double *cf = a.coef + a.coef_off[cell];
   const double *phi = a.bases + a.bas_off[cell];
@@ -366,14 +378,8 @@ __global__ void __launch_bounds__(kElemThreads) element3d_kernel(ElementArgs a)
         be[aa] = 0;
       for (int q = 0; q < NQ; ++q)
         {
-          double fv = a.rhs_value;
-          if (a.rhs_kind == 1)
-            {
-              const int fi[3] = {fx, fy, fz};
-              fv = 1.0;
-              for (int k = 0; k < 3; ++k)
-                fv *= sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) + h[k] * gauss(q, k)));
-            }
+          const double fv =
+              fconst * sfac[0][2 * fx + (q & 1)] * sfac[1][2 * fy + ((q >> 1) & 1)] * sfac[2][2 * fz + ((q >> 2) & 1)];
 #pragma unroll
           for (int aa = 0; aa < NV; ++aa)
             {
