@@ -1095,11 +1095,10 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
 {
   constexpr int H = NF / 8, NB = kCoarseSide + 1, N = kCoarseDofs;
   static_assert(H % 4 == 0 && NB * H == NF, "four threads per coarse block");
-  constexpr int W2 = 2 * N; // augmented row [A | I]
-  __shared__ double sh[N * W2];        // block partials [64][4][16], then the augmented matrix
+  constexpr int SHN = NB * NB * 4 * 16 > N * N ? NB * NB * 4 * 16 : N * N;
+  __shared__ double sh[SHN];           // block partials [64][4][16], then A_c, inverted in place
   __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
-  __shared__ double fac[N], rowk[W2];  // Gauss-Jordan: column k, row k
-  static_assert(NB * NB * 4 * 16 <= N * W2, "block partials fit the matrix storage");
+  __shared__ double fac[N], rowk[N];   // Gauss-Jordan: column k, row k
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
     return;
@@ -1162,49 +1161,51 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
               sh[(b * 4 + 3) * 16 + m];
     }
   __syncthreads();
-  // augmented [A_c | I]; A_c(I, J) = sum over the blocks holding both coarse nodes
-  for (int e = t; e < N * W2; e += kSetupThreads)
+  // A_c(I, J) = sum over the blocks holding both coarse nodes
+  for (int e = t; e < N * N; e += kSetupThreads)
     {
-      const int i = e / W2, col = e % W2;
+      const int i = e / N, col = e % N;
+      const int ix = i % kCoarseSide + 1, iy = i / kCoarseSide + 1;
+      const int jx = col % kCoarseSide + 1, jy = col / kCoarseSide + 1;
       double v = 0;
-      if (col < N)
-        {
-          const int ix = i % kCoarseSide + 1, iy = i / kCoarseSide + 1;
-          const int jx = col % kCoarseSide + 1, jy = col / kCoarseSide + 1;
-          for (int dy = 0; dy < 2; ++dy)
-            for (int dx = 0; dx < 2; ++dx)
-              {
-                const int bxi = ix - 1 + dx, byi = iy - 1 + dy; // block with corner I
-                const int mi = (ix - bxi) + 2 * (iy - byi), ox = jx - bxi, oy = jy - byi;
-                if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1)
-                  v += kb[(byi * NB + bxi) * 16 + mi * 4 + ox + 2 * oy];
-              }
-        }
-      else
-        v = (col - N == i) ? 1.0 : 0.0;
+      for (int dy = 0; dy < 2; ++dy)
+        for (int dx = 0; dx < 2; ++dx)
+          {
+            const int bxi = ix - 1 + dx, byi = iy - 1 + dy; // block with corner I
+            const int mi = (ix - bxi) + 2 * (iy - byi), ox = jx - bxi, oy = jy - byi;
+            if (ox >= 0 && ox <= 1 && oy >= 0 && oy <= 1)
+              v += kb[(byi * NB + bxi) * 16 + mi * 4 + ox + 2 * oy];
+          }
       sh[e] = v;
     }
   __syncthreads();
+  // in-place Gauss-Jordan (A_c is SPD: no pivoting): with p = 1 / a_kk,
+  // a_ij -= a_ik p a_kj (i, j != k), a_kj *= p, a_ik *= -p, a_kk = p
   for (int k = 0; k < N; ++k)
     {
       if (t < N)
-        fac[t] = sh[t * W2 + k];
-      else if (t < N + W2)
-        rowk[t - N] = sh[k * W2 + t - N];
+        fac[t] = sh[t * N + k];
+      else if (t < 2 * N)
+        rowk[t - N] = sh[k * N + t - N];
       __syncthreads();
-      // thread (column c, half h) updates rows of its half: no two threads write one entry
-      if (t < 2 * W2)
+      const double p = 1.0 / fac[k]; // one division per pivot, not per entry
+      for (int e = t; e < N * N; e += kSetupThreads)
         {
-          const int c = t % W2, h = t / W2, i0 = h ? (N + 1) / 2 : 0, i1 = h ? N : (N + 1) / 2;
-          const double rk = rowk[c], rinv = 1.0 / fac[k]; // one division per pivot, not per entry
-          for (int i = i0; i < i1; ++i)
-            sh[i * W2 + c] = (i == k) ? rk * rinv : sh[i * W2 + c] - (fac[i] * rinv) * rk;
+          const int i = e / N, j = e % N;
+          double v;
+          if (i == k)
+            v = j == k ? p : rowk[j] * p;
+          else if (j == k)
+            v = -(fac[i] * p);
+          else
+            v = sh[e] - (fac[i] * p) * rowk[j];
+          sh[e] = v;
         }
       __syncthreads();
     }
   double *out = cinv + (size_t)cc * N * N;
   for (int e = t; e < N * N; e += kSetupThreads)
-    out[e] = sh[(e / N) * W2 + N + e % N];
+    out[e] = sh[e];
 }
 
 // ---- launch table -----------------------------------------------------------------
