@@ -116,7 +116,14 @@ typedef struct msfem_solver_opts {
                                  scalar or tensor coefficients at the default launch
                                  shapes; 0: Jacobi only.  Same converged values, fewer
                                  iterations. */
-  int32_t reserved[5];
+  int32_t q1_start;           /* without x0: 1 (default) = the interior starts at the Q1
+                                 interpolant of the Dirichlet data (box cells: the nodal
+                                 Q1 function itself; general cells: the transfinite
+                                 interpolant of the boundary values over the parametric
+                                 node grid), 0 = zero interior.  The stopping test stays
+                                 relative to ||b|| = ||K_IB g_i|| either way; the start
+                                 only shortens the solve. */
+  int32_t reserved[4];
 } msfem_solver_opts;
 
 typedef struct msfem_batch {

@@ -421,6 +421,7 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.bases = bases;
   a.bas_off = d_off(ctx, 1);
   a.x0 = in->x0;
+  a.q1_start = o->q1_start;
   a.iters = out->iters;
   a.status = out->status;
   a.resid = out->resid;
@@ -734,6 +735,7 @@ void msfem_default_opts(msfem_solver_opts *o)
   o->inner_rel_tol = 1e-4;
   o->variant = 0;
   o->coarse_space = 1;
+  o->q1_start = 1;
 }
 
 int msfem_ctx_create(int device, msfem_ctx **out)

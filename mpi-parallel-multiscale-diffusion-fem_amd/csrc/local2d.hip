@@ -304,15 +304,27 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         const bool bnd = X == 0 || X == NF || Y == 0 || Y == NF;
         if (bnd && a.verts) // general quadrilateral: boundary data already in place
           continue;
-        double v;
+        double v = 0.0;
         if (bnd)
           v = nodal_q1(basis, 2, X, Y, 0, NF);
-        else
+        else if (a.x0)
           {
             // a non-finite initial value (e.g. a failed prediction) starts from 0
-            v = a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP + k] : 0.0;
+            v = a.x0[a.bas_off[cell] + (size_t)basis * NP * NP + k];
             if (!isfinite(v))
               v = 0.0;
+          }
+        else if (a.q1_start && !a.verts)
+          v = nodal_q1(basis, 2, X, Y, 0, NF); // the Q1 interpolant of the boundary data
+        else if (a.q1_start)
+          {
+            // general cell: transfinite (Coons) interpolant of the boundary values
+            // over the parametric node grid (boundary written by quad_dirichlet_2d)
+            const double sx = (double)X / NF, ty = (double)Y / NF;
+            const double *p = phi;
+            v = (1.0 - sx) * p[Y * NP] + sx * p[Y * NP + NF] + (1.0 - ty) * p[X] + ty * p[NF * NP + X] -
+                ((1.0 - sx) * (1.0 - ty) * p[0] + sx * (1.0 - ty) * p[NF] + (1.0 - sx) * ty * p[NF * NP] +
+                 sx * ty * p[NF * NP + NF]);
           }
         phi[k] = v;
       }
@@ -344,8 +356,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       }
   };
 
+  // with a given start the tolerance is still relative to ||b|| = ||K_IB g||
+  // (the residual of the zero start), computed here from the boundary values
+  const bool given_start = a.x0 || a.q1_start;
   double tol_eff = 0.0, bnorm = 0.0;
-  if (a.x0)
+  if (given_start)
     {
       double s[1] = {0};
 #pragma unroll 1
@@ -510,7 +525,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           rr[i][c] = dget(i, c) != 0.0f ? gown[i * GP + c] : (T)0;
       problem_sums<double, LW, SW, PPW, 1>(s, false, sred64, wave, slot, pl);
       rnorm = sqrt(s[0]);
-      if (cyc == 0 && !a.x0)
+      if (cyc == 0 && !given_start)
         {
           bnorm = rnorm;
           tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);

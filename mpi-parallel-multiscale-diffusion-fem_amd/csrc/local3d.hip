@@ -135,8 +135,10 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       {
         const int XX = k % NP, YY = (k / NP) % NP, ZZ = k / (NP * NP);
         const bool bnd = XX == 0 || XX == NF || YY == 0 || YY == NF || ZZ == 0 || ZZ == NF;
-        const double v = bnd ? nodal_q1(basis, 3, XX, YY, ZZ, NF)
-                             : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
+        // interior: the given start, else the Q1 interpolant of the boundary data, else 0
+        const double v = bnd || (!a.x0 && a.q1_start)
+                           ? nodal_q1(basis, 3, XX, YY, ZZ, NF)
+                           : (a.x0 ? a.x0[a.bas_off[cell] + (size_t)basis * NP * NP * NP + k] : 0.0);
         phi[k] = isfinite(v) ? v : 0.0; // a non-finite initial value starts from 0
       }
   __syncthreads();
@@ -167,8 +169,10 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
     return y;
   };
 
+  // with a given start the tolerance is still relative to ||b|| = ||K_IB g||
+  const bool given_start = a.x0 || a.q1_start;
   double tol_eff = 0.0, bnorm = 0.0;
-  if (a.x0)
+  if (given_start)
     {
       double s[1] = {0};
 #pragma unroll 1
@@ -204,7 +208,7 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
         rr[i] = dinv[i] != (T)0 ? gown[i * GXY] : (T)0;
       problem_sums<double, kWave, NW, 1, 1>(s, false, sred64, wave, 0, lane);
       rnorm = sqrt(s[0]);
-      if (cyc == 0 && !a.x0)
+      if (cyc == 0 && !given_start)
         {
           bnorm = rnorm;
           tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);

@@ -60,6 +60,7 @@ struct LocalArgs {
   double *bases;              // concatenated basis blocks (working FP64 solution + output)
   const long long *bas_off;   // [n_cells]
   const double *x0;           // optional initial guesses (bases layout) or nullptr
+  int q1_start;               // without x0: 1 = interior starts at the Q1 interpolant of g
   const double *cinv;         // two-level: [cells in class][kCoarseDofs^2] inverse coarse matrices, or nullptr
   int *iters;                 // [n_cells][nb] or nullptr
   int *status;                // [n_cells][nb] or nullptr

@@ -50,7 +50,8 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("max_refine", ctypes.c_int32),
                 ("partition_of_unity", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double),
                 ("inner_rel_tol", ctypes.c_double), ("variant", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("coarse_space", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
+                ("coarse_space", ctypes.c_int32), ("q1_start", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 4)]
 
 
 class Batch(ctypes.Structure):

@@ -172,6 +172,42 @@ def test_initial_guess_shortens_solve(ctx):
     assert warm["iters"].sum() < cold["iters"].sum()
 
 
+@pytest.mark.parametrize("prec", [msfem.PREC_FP64, msfem.PREC_FP32_REFINE])
+def test_q1_start(ctx, prec):
+    """The default start (interior = the Q1 interpolant of g) and the zero start
+    meet the same test ||r|| <= rel_tol ||K_IB g||, so both match the oracle; the
+    Q1 start takes fewer iterations."""
+    say(f"q1 start prec={prec}")
+    case = C.lognormal_cells(which=np.array([20, 21, 22, 23]))
+    ref = C.oracle_build(case)
+    q1 = C.gpu_build(ctx, case, precision=prec)
+    zero = C.gpu_build(ctx, case, precision=prec, q1_start=0)
+    check_against_oracle(case, q1, ref)
+    check_against_oracle(case, zero, ref)
+    assert q1["iters"].sum() < zero["iters"].sum(), (q1["iters"], zero["iters"])
+
+
+def test_q1_start_constant_coefficient_is_exact(ctx):
+    """Constant scalar coefficient on box cells: the bilinear nodal function is the
+    discrete solution, so the Q1 start meets the tolerance in 0 iterations and the
+    bases are the closed-form hats (32^2 cells: Jacobi; 64^2: two-level)."""
+    say("q1 start constant coefficient")
+    box = C.unit_cells(2, 4)
+    nf = np.array([32] * 8 + [64] * 8, np.int32)
+    coef = np.full(int((nf.astype(np.int64) ** 2).sum()), 3.0)
+    out = ctx.batch_build(2, nf, box, msfem.COEF_SCALAR_CELL, coef)
+    zero = ctx.batch_build(2, nf, box, msfem.COEF_SCALAR_CELL, coef, opts=msfem.default_opts(q1_start=0))
+    assert (out["iters"] == 0).all() and (zero["iters"] > 0).all()
+    assert rel(out["bases"], zero["bases"]) < TOL and rel(out["Ae"], zero["Ae"]) < TOL
+    off = 0
+    for n in nf:
+        t = np.arange(n + 1) / n
+        hats = [np.outer(ty, tx).ravel() for ty in (1 - t, t) for tx in (1 - t, t)]
+        for i in range(4):
+            assert np.abs(out["bases"][off:off + (n + 1) ** 2] - hats[i]).max() < 1e-14
+            off += (n + 1) ** 2
+
+
 def test_iteration_cap_reports_noconvergence(ctx):
     say("iteration cap")
     case = C.lognormal_cells(which=np.array([3]))
