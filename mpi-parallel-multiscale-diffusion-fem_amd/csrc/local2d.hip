@@ -138,7 +138,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   __shared__ T sq[QLDS ? R * CPL * NT : 1];
   __shared__ T sred2[QLDS ? problem_sums_slots<SW, PPW, 1>() : 1]; // second reduction of the two-reduction PCG
   // two-level: inverse coarse matrix, restriction partials [wave][2][group][2], coarse correction
-  __shared__ T sci[CS ? kCoarseDofs * kCoarseDofs : 1];
+  // inverse coarse matrix, rows padded to kSciStride entries: the coarse product's
+  // wave reads 8 rows x 8 consecutive entries, and a stride of 56 (= 24 mod 32
+  // words for FP32, 16 mod 32 for FP64) puts those rows on distinct LDS banks
+  // (the FP64 128^2 instance has no LDS left for the padding)
+  constexpr int kSciStride = (sizeof(T) == 8 && NF == 128) ? kCoarseDofs : 56;
+  static_assert(kSciStride >= kCoarseDofs, "padded coarse rows");
+  __shared__ T sci[CS ? kCoarseDofs * kSciStride : 1];
   __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];  // [wave][coarse row 0/1][group][column 0/1]
   __shared__ T suc[CS ? kCoarseDofs : 1];
 
@@ -295,7 +301,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       {
         const double *ci = a.cinv + (size_t)(prob / a.nb_solve) * (kCoarseDofs * kCoarseDofs);
         for (int k = tid; k < kCoarseDofs * kCoarseDofs; k += blockDim.x)
-          sci[k] = (T)ci[k];
+          sci[(k / kCoarseDofs) * kSciStride + k % kCoarseDofs] = (T)ci[k];
       }
   if (pvalid)
     for (int k = pl + LW * wave; k < NP * NP; k += LW * SW)
@@ -470,7 +476,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                       rc += (spart[spart_at(wl, 0, jx, 0)] + spart[spart_at(wl, 0, jx - 1, 1)]) +
                             (spart[spart_at(wu, 1, jx, 0)] + spart[spart_at(wu, 1, jx - 1, 1)]);
                     }
-                  acc += sci[i * kCoarseDofs + j] * rc;
+                  acc += sci[i * kSciStride + j] * rc;
                 }
             }
           acc = oct_sum_any<T>(acc);
