@@ -1119,7 +1119,7 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
   constexpr int SHN = NB * NB * 4 * 16 > N * N ? NB * NB * 4 * 16 : N * N;
   __shared__ double sh[SHN];           // block partials [64][4][16], then A_c, inverted in place
   __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
-  __shared__ double fac[N], rowk[N];   // Gauss-Jordan: column k, row k
+  __shared__ double fac[2][N], rowk[2][N]; // Gauss-Jordan: column k, row k (double-buffered)
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
     return;
@@ -1200,33 +1200,57 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
       sh[e] = v;
     }
   __syncthreads();
-  // in-place Gauss-Jordan (A_c is SPD: no pivoting): with p = 1 / a_kk,
-  // a_ij -= a_ik p a_kj (i, j != k), a_kj *= p, a_ik *= -p, a_kk = p
+  // Gauss-Jordan in place (A_c is SPD: no pivoting): with p = 1 / a_kk,
+  // a_ij -= a_ik p a_kj (i, j != k), a_kj *= p, a_ik *= -p, a_kk = p.  Thread t
+  // keeps entries e = t + kSetupThreads m in registers; per pivot the owners of
+  // column k and row k publish them into a double-buffered pair (one barrier
+  // per pivot: buffer k & 1 is rewritten at pivot k + 2, after every thread
+  // has passed barrier k + 1, i.e. finished reading it at pivot k).
+  constexpr int EM = (N * N + kSetupThreads - 1) / kSetupThreads;
+  double v[EM];
+#pragma unroll
+  for (int m = 0; m < EM; ++m)
+    {
+      const int e = t + kSetupThreads * m;
+      v[m] = e < N * N ? sh[e] : 0.0;
+    }
   for (int k = 0; k < N; ++k)
     {
-      if (t < N)
-        fac[t] = sh[t * N + k];
-      else if (t < 2 * N)
-        rowk[t - N] = sh[k * N + t - N];
-      __syncthreads();
-      const double p = 1.0 / fac[k]; // one division per pivot, not per entry
-      for (int e = t; e < N * N; e += kSetupThreads)
+      double *fk = fac[k & 1], *rk = rowk[k & 1];
+#pragma unroll
+      for (int m = 0; m < EM; ++m)
         {
-          const int i = e / N, j = e % N;
-          double v;
-          if (i == k)
-            v = j == k ? p : rowk[j] * p;
-          else if (j == k)
-            v = -(fac[i] * p);
-          else
-            v = sh[e] - (fac[i] * p) * rowk[j];
-          sh[e] = v;
+          const int e = t + kSetupThreads * m, i = e / N, j = e % N;
+          if (e < N * N && j == k)
+            fk[i] = v[m];
+          if (e < N * N && i == k)
+            rk[j] = v[m];
         }
       __syncthreads();
+      const double p = 1.0 / fk[k]; // one division per pivot, not per entry
+#pragma unroll
+      for (int m = 0; m < EM; ++m)
+        {
+          const int e = t + kSetupThreads * m, i = e / N, j = e % N;
+          if (e < N * N)
+            {
+              if (i == k)
+                v[m] = j == k ? p : rk[j] * p;
+              else if (j == k)
+                v[m] = -(fk[i] * p);
+              else
+                v[m] = v[m] - (fk[i] * p) * rk[j];
+            }
+        }
     }
   double *out = cinv + (size_t)cc * N * N;
-  for (int e = t; e < N * N; e += kSetupThreads)
-    out[e] = sh[e];
+#pragma unroll
+  for (int m = 0; m < EM; ++m)
+    {
+      const int e = t + kSetupThreads * m;
+      if (e < N * N)
+        out[e] = v[m];
+    }
 }
 
 // ---- launch table -----------------------------------------------------------------
