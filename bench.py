@@ -231,7 +231,9 @@ def main():
                    "fine_per_cell": classes, "local_solves": n * nb, "path": pname[main_prec],
                    "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class",
                    "two_level_cells": int(cs.sum())},
-        "roofline": {"bound": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
+        # compute-bound (the flops ceiling): FP64 vector instructions, whose MI355X
+        # peak equals the dense FP64 MFMA figure, 78.6 TFLOP/s
+        "roofline": {"bound": "mfma", "engine": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get((args.workload, main_prec)),
                      "kernel_ms": round(solve_ms, 4),
