@@ -147,6 +147,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   __shared__ T sci[CS ? kCoarseDofs * kSciStride : 1];
   __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];  // [wave][coarse row 0/1][group][column 0/1]
   __shared__ T suc[CS ? kCoarseDofs : 1];
+  constexpr bool WAVE_CS = CS && MODE != kModeStream; // per-wave coarse correction
+  __shared__ T swrc[WAVE_CS ? SW * kWave : 1];         // [wave][coarse dof]: P^T r, wave-private
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -489,6 +491,59 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     return (jx >= 1 && jx <= kCoarseSide && jy >= 1 && jy <= kCoarseSide) ? suc[(jy - 1) * kCoarseSide + jx - 1]
                                                                           : (T)0;
   };
+  // Per-wave coarse correction (single-reduction form): wave w interpolates
+  // from coarse rows cband and cband + 1 only, so it forms those <= 14 values
+  // of u_c = A_c^-1 (P^T r) itself and needs no barrier between the coarse
+  // product and the interpolation.  Lanes j < 49 form (P^T r)_j into the
+  // wave's private slots; lane 4 q + h sums row q (coarse row cband + q / 7,
+  // column q % 7 + 1) over the entries 13 h .. 13 h + 12, a 4-lane butterfly
+  // completes it.  Every wave evaluates a shared coarse row in the same order,
+  // so the waves agree bitwise.
+  auto coarse_wave = [&](T (&ci0)[CPL], T (&ci1)[CPL]) {
+    if constexpr (WAVE_CS)
+      {
+        static_assert(WB == 1 && SW == 8, "per-wave coarse rows: one wave per coarse band");
+        T *wrc = swrc + wave * kWave;
+        if (pl < kCoarseDofs)
+          {
+            const int jx = pl % kCoarseSide + 1, jy = pl / kCoarseSide + 1;
+            wrc[pl] = (spart[spart_at(jy, 0, jx, 0)] + spart[spart_at(jy, 0, jx - 1, 1)]) +
+                      (spart[spart_at(jy - 1, 1, jx, 0)] + spart[spart_at(jy - 1, 1, jx - 1, 1)]);
+          }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int q = pl >> 2, h = pl & 3;
+        const int jyq = cband + (q >= kCoarseSide ? 1 : 0), jxq = q % kCoarseSide + 1;
+        T acc = 0;
+        if (q < 2 * kCoarseSide && jyq >= 1 && jyq <= kCoarseSide)
+          {
+            const T *srow = sci + ((jyq - 1) * kCoarseSide + jxq - 1) * kSciStride;
+#pragma unroll
+            for (int k = 0; k < 13; ++k)
+              {
+                const int j = 13 * h + k;
+                if (j < kCoarseDofs)
+                  acc += srow[j] * wrc[j];
+              }
+          }
+        acc += dpp_any<T, 0xB1>(acc);
+        acc += dpp_any<T, 0x4E>(acc);
+        // u at coarse column jx (0..8) of coarse row cband + r: lane 4 (7 r + jx - 1)
+        auto uq = [&](int jx, int r) -> T {
+          const int src = 4 * (r * kCoarseSide + (jx >= 1 && jx <= kCoarseSide ? jx - 1 : 0));
+          const T v = __shfl(acc, src, kWave);
+          return (jx >= 1 && jx <= kCoarseSide) ? v : (T)0;
+        };
+        const T u00 = uq(cg8, 0), u10 = uq(cg8 + 1, 0), u01 = uq(cg8, 1), u11 = uq(cg8 + 1, 1);
+#pragma unroll
+        for (int c = 0; c < CPL; ++c)
+          {
+            ci0[c] = u00 * wxa(c) + u10 * wxb(c);
+            ci1[c] = u01 * wxa(c) + u11 * wxb(c);
+          }
+      }
+  };
   // the lane's interpolation rows: z_c(i, c) = ci0[c] wya(i) + ci1[c] wyb(i)
   auto interp_cols = [&](T (&ci0)[CPL], T (&ci1)[CPL]) {
     const T u00 = ucoarse(cg8, cband), u10 = ucoarse(cg8 + 1, cband);
@@ -799,11 +854,9 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           if constexpr (CS)
             {
               __syncthreads(); // restriction partials of r complete
-              coarse_solve();
-              __syncthreads(); // coarse correction complete
               // z = D^-1 r + P u_c (the interpolation vanishes on the tile boundary)
               T ci0[CPL], ci1[CPL];
-              interp_cols(ci0, ci1);
+              coarse_wave(ci0, ci1);
 #pragma unroll
               for (int i = 0; i < R; ++i)
 #pragma unroll
