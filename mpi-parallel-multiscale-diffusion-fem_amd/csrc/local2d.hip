@@ -147,7 +147,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   __shared__ T sci[CS ? kCoarseDofs * kSciStride : 1];
   __shared__ T spart[CS ? SW * 2 * 8 * 2 : 1];  // [wave][coarse row 0/1][group][column 0/1]
   __shared__ T suc[CS ? kCoarseDofs : 1];
-  constexpr bool WAVE_CS = CS && MODE != kModeStream; // per-wave coarse correction
+  // per-wave coarse correction (the FP64 128^2 instance has no LDS left for it)
+  constexpr bool WAVE_CS = CS && !(sizeof(T) == 8 && NF == 128);
   __shared__ T swrc[WAVE_CS ? SW * kWave : 1];         // [wave][coarse dof]: P^T r, wave-private
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
@@ -491,31 +492,37 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     return (jx >= 1 && jx <= kCoarseSide && jy >= 1 && jy <= kCoarseSide) ? suc[(jy - 1) * kCoarseSide + jx - 1]
                                                                           : (T)0;
   };
-  // Per-wave coarse correction (single-reduction form): wave w interpolates
-  // from coarse rows cband and cband + 1 only, so it forms those <= 14 values
-  // of u_c = A_c^-1 (P^T r) itself and needs no barrier between the coarse
-  // product and the interpolation.  Lanes j < 49 form (P^T r)_j into the
-  // wave's private slots; lane 4 q + h sums row q (coarse row cband + q / 7,
-  // column q % 7 + 1) over the entries 13 h .. 13 h + 12, a 4-lane butterfly
-  // completes it.  Every wave evaluates a shared coarse row in the same order,
-  // so the waves agree bitwise.
-  auto coarse_wave = [&](T (&ci0)[CPL], T (&ci1)[CPL]) {
+  // Per-wave coarse correction: wave w interpolates from coarse rows cband and
+  // cband + 1 only, so it forms those <= 14 values of u_c = A_c^-1 (P^T r)
+  // itself and needs no barrier between the coarse product and the
+  // interpolation.  Lanes j < 49 form (P^T r)_j into the wave's private slots;
+  // lane 4 q + h sums row q (coarse row cband + q / 7, column q % 7 + 1) over
+  // the entries 13 h .. 13 h + 12, a 4-lane butterfly completes it (every lane
+  // of the quad holds the row's value).  Every wave evaluates a shared coarse
+  // row in the same order, so the waves agree bitwise.
+  auto coarse_wave_acc = [&]() -> T {
+    T acc = 0;
     if constexpr (WAVE_CS)
       {
-        static_assert(WB == 1 && SW == 8, "per-wave coarse rows: one wave per coarse band");
         T *wrc = swrc + wave * kWave;
         if (pl < kCoarseDofs)
           {
             const int jx = pl % kCoarseSide + 1, jy = pl / kCoarseSide + 1;
-            wrc[pl] = (spart[spart_at(jy, 0, jx, 0)] + spart[spart_at(jy, 0, jx - 1, 1)]) +
-                      (spart[spart_at(jy - 1, 1, jx, 0)] + spart[spart_at(jy - 1, 1, jx - 1, 1)]);
+            T rc = 0;
+#pragma unroll
+            for (int b = 0; b < WB; ++b)
+              {
+                const int wl = jy * WB + b, wu = (jy - 1) * WB + b; // band jy (lower row), band jy-1 (upper)
+                rc += (spart[spart_at(wl, 0, jx, 0)] + spart[spart_at(wl, 0, jx - 1, 1)]) +
+                      (spart[spart_at(wu, 1, jx, 0)] + spart[spart_at(wu, 1, jx - 1, 1)]);
+              }
+            wrc[pl] = rc;
           }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const int q = pl >> 2, h = pl & 3;
         const int jyq = cband + (q >= kCoarseSide ? 1 : 0), jxq = q % kCoarseSide + 1;
-        T acc = 0;
         if (q < 2 * kCoarseSide && jyq >= 1 && jyq <= kCoarseSide)
           {
             const T *srow = sci + ((jyq - 1) * kCoarseSide + jxq - 1) * kSciStride;
@@ -529,6 +536,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           }
         acc += dpp_any<T, 0xB1>(acc);
         acc += dpp_any<T, 0x4E>(acc);
+      }
+    return acc;
+  };
+  // the lane's interpolation columns from the wave's coarse values (acc as above)
+  auto coarse_wave_interp = [&](T acc, T (&ci0)[CPL], T (&ci1)[CPL]) {
+    if constexpr (WAVE_CS)
+      {
         // u at coarse column jx (0..8) of coarse row cband + r: lane 4 (7 r + jx - 1)
         auto uq = [&](int jx, int r) -> T {
           const int src = 4 * (r * kCoarseSide + (jx >= 1 && jx <= kCoarseSide ? jx - 1 : 0));
@@ -544,6 +558,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           }
       }
   };
+  auto coarse_wave = [&](T (&ci0)[CPL], T (&ci1)[CPL]) { coarse_wave_interp(coarse_wave_acc(), ci0, ci1); };
   // the lane's interpolation rows: z_c(i, c) = ci0[c] wya(i) + ci1[c] wyb(i)
   auto interp_cols = [&](T (&ci0)[CPL], T (&ci1)[CPL]) {
     const T u00 = ucoarse(cg8, cband), u10 = ucoarse(cg8 + 1, cband);
@@ -638,13 +653,22 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
           for (int c = 0; c < CPL; ++c)
             ci0[c] = ci1[c] = 0;
+          T cacc = 0; // per-wave coarse sums of the current z (kept for the p update)
           if constexpr (CS) // irun is uniform (one problem per workgroup): true here
             {
               restrict_r(rr);
               __syncthreads(); // restriction partials complete
-              coarse_solve();
-              __syncthreads(); // coarse correction complete
-              interp_cols(ci0, ci1);
+              if constexpr (WAVE_CS)
+                {
+                  cacc = coarse_wave_acc();
+                  coarse_wave_interp(cacc, ci0, ci1);
+                }
+              else
+                {
+                  coarse_solve();
+                  __syncthreads(); // coarse correction complete
+                  interp_cols(ci0, ci1);
+                }
             }
           T v2[2] = {0, 0}; // (r, z), (r, r)
 #pragma unroll
@@ -804,9 +828,17 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     {
                       restrict_r(rr);
                       __syncthreads(); // restriction partials complete
-                      coarse_solve();
-                      __syncthreads(); // coarse correction complete
-                      interp_cols(ci0, ci1);
+                      if constexpr (WAVE_CS)
+                        {
+                          cacc = coarse_wave_acc();
+                          coarse_wave_interp(cacc, ci0, ci1);
+                        }
+                      else
+                        {
+                          coarse_solve();
+                          __syncthreads(); // coarse correction complete
+                          interp_cols(ci0, ci1);
+                        }
 #pragma unroll
                       for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -826,7 +858,9 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               // the own slots of the grid take the next direction in place
               const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
               rz = v2[0];
-              if constexpr (CS) // the coarse values are still in LDS (next write: next iteration)
+              if constexpr (WAVE_CS) // from the kept per-wave coarse sums
+                coarse_wave_interp(cacc, ci0, ci1);
+              else if constexpr (CS) // the coarse values are still in LDS (next write: next iteration)
                 interp_cols(ci0, ci1);
 #pragma unroll
               for (int i = 0; i < R; ++i)
