@@ -46,7 +46,7 @@ PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a plain copy of known size (--copy-calibration):
 # profiles/local_solve_2d_lognormal64_hbm_traffic.md
-TRAFFIC_BYTES = {("lognormal64", 0): 2.170539e9}
+TRAFFIC_BYTES = {("lognormal64", 0): 2.168743e9}
 METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 
 
