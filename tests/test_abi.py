@@ -44,6 +44,7 @@ int main(void) {
   printf("%zu %zu %zu\\n", offsetof(msfem_solver_opts, rel_tol), offsetof(msfem_solver_opts, variant),
          offsetof(msfem_batch, rhs_value));
   printf("%zu %zu %zu\\n", offsetof(msfem_batch, x0), offsetof(msfem_batch, vertices), offsetof(msfem_batch_out, resid));
+  printf("%zu %zu\\n", offsetof(msfem_solver_opts, coarse_space), offsetof(msfem_solver_opts, q1_start));
   return 0;
 }
 """)
@@ -53,12 +54,14 @@ int main(void) {
     C = ctypes
     assert vals == [C.sizeof(msfem.SolverOpts), C.sizeof(msfem.Batch), C.sizeof(msfem.BatchOut),
                     msfem.SolverOpts.rel_tol.offset, msfem.SolverOpts.variant.offset, msfem.Batch.rhs_value.offset,
-                    msfem.Batch.x0.offset, msfem.Batch.vertices.offset, msfem.BatchOut.resid.offset]
+                    msfem.Batch.x0.offset, msfem.Batch.vertices.offset, msfem.BatchOut.resid.offset,
+                    msfem.SolverOpts.coarse_space.offset, msfem.SolverOpts.q1_start.offset]
 
 
 def test_default_opts_and_version():
     o = msfem.default_opts()
     assert o.precision == msfem.PREC_FP64 and o.rel_tol == 1e-13 and o.max_iter > 1000
+    assert o.coarse_space == 1 and o.q1_start == 1
     assert msfem.lib().msfem_version() == 2
 
 
