@@ -19,7 +19,7 @@ One step = every cell's basis solves + A^K, F^K + the coarse assembly, with all
 inputs already resident in HBM.  The dominant kernel (the local solves) is timed
 with HIP events on the stream it runs on.  Algorithmic work per CG iteration of
 a problem with N_I interior nodes: F_iter = (2*3^d - 1 + 10) N_I flops (stencil +
-CG vector updates), B_iter = 11 N w + n^d w_c bytes (survey model).
+CG vector updates), B_iter = 11 N w + n^d w_c bytes (LDS byte model).
 
 The CPU baseline is the oracle's restatement of the reference's own recipe
 (CG + SSOR(1.6), absolute tolerance 1e-12, 1000 iterations;
@@ -287,7 +287,7 @@ def copy_calibration(mib):
 def cpu_baseline(dim, box, nf_arr, coef_kind, coef, rhs_kind, rhs_value, n, sample_cells, quads=None):
     """The oracle's restatement of the reference recipe (CG + SSOR(1.6), abs tol
     1e-12, 1000 iterations) on an evenly spaced sample of the cells."""
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the checker, for the CPU-baseline leg only
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))  # the oracle, for the CPU-baseline leg only
     import msfem
     import oracle as O
     sample = np.unique(np.linspace(0, n - 1, min(sample_cells, n)).astype(np.int64))

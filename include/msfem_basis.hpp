@@ -355,7 +355,7 @@ private:
   std::string filename_;
 };
 
-// The driver around the hot path (include/base/diffusion_problem_ms.tpp:447-491):
+// The multiscale problem around the hot path (include/base/diffusion_problem_ms.tpp:447-491):
 // coarse mesh of the unit square/cube, one basis object
 // per cell, coarse assembly with the reference boundary data, FP64 coarse
 // solve, weights back to the cells.

@@ -308,7 +308,7 @@ hipError_t fill_zero(void *p, size_t bytes, hipStream_t st)
 }
 
 // Local solves (coefficient preparation, solve, partition-of-unity fix-up) of size class k.
-// Storage mode of size class nf in this build.
+// Storage mode of size class nf in the current batch build.
 int build_mode(const msfem_batch *in, int nf)
 {
   return class_mode(in->dim, in->coef_kind, nf, in->vertices != nullptr);

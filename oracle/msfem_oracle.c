@@ -2,7 +2,7 @@
  * msfem_oracle.c -- CPU FP64 restatement of the MsFEM basis pipeline of
  * konsim83/MPI-Parallel-Multiscale-Diffusion-FEM.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke
+ * TEST INFRASTRUCTURE ONLY.  This file is the oracle: only tests/, the smoke
  * check and the CPU-baseline leg of bench.py load it.  The product library
  * never links, loads or calls it.
  *

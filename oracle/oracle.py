@@ -1,7 +1,7 @@
 """ctypes front end of the CPU FP64 oracle (msfem_oracle.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
-CPU-baseline leg of bench.py, always as the checker.  The product library never
+CPU-baseline leg of bench.py, always as the oracle.  The product library never
 imports this module.
 
 Every function restates one piece of the reference (file:line citations are in

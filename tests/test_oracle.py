@@ -1,4 +1,4 @@
-"""Pins the CPU oracle (oracle/msfem_oracle.c) before it is trusted as the checker.
+"""Pins the CPU oracle (oracle/msfem_oracle.c) before the GPU tests compare against it.
 
 Each test fixes a piece of the reference's local pipeline by something other than
 the oracle's own formula: golden vectors from an independent numpy/scipy
