@@ -36,9 +36,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG = os.path.join(ROOT, "mpi-parallel-multiscale-diffusion-fem_amd")
-for p in (PKG, os.path.join(ROOT, "tests")):
-    if p not in sys.path:
-        sys.path.insert(0, p)
+if PKG not in sys.path:
+    sys.path.insert(0, PKG)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_VALU_TFLOPS = 78.6  # non-packed FP32 FMA issue rate equals FP64 (packed FP32 is disabled)
@@ -53,7 +52,7 @@ METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-
 def workload(name):
     """-> (case, nc, main precision, bc kind, description)."""
     import msfem
-    import msfem_cases as C
+    import workloads as C
     if name == "lognormal64":
         return (C.lognormal_cells(dim=2, nc=64, nf=64, seed=2), 64, msfem.PREC_FP64, 0,
                 "64^2 coarse Q1 cells x 64^2 fine Q1 per cell, lognormal alpha (contrast 1e3), f=1, 16384 local solves")

@@ -91,8 +91,7 @@ def save(entries, path=TABLE):
 
 
 def main():
-    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
-    import msfem_cases as C
+    import workloads as C
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="lognormal64", choices=["lognormal64", "tensor128", "lognormal3d"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])

@@ -67,6 +67,13 @@ struct msfem_ctx {
   hipEvent_t fork = nullptr, join[kSideStreams] = {};
   std::map<std::string, hipGraphExec_t> graphs;
   unsigned long long generation = 0, graph_generation = 0;
+  // The plan metadata and the scratch buffers are shared by every build of the
+  // context: `scratch_done` is recorded on a build's stream after its last
+  // kernel, so a later build on another stream waits for it, and a rewrite of
+  // the metadata or a reallocation waits for it on the host first.
+  hipEvent_t scratch_done = nullptr;
+  hipStream_t scratch_stream = nullptr;
+  bool scratch_recorded = false;
 };
 
 namespace {
@@ -95,6 +102,42 @@ int fail(msfem_ctx *ctx, int code, const char *fmt, ...)
     }                                                                                                       \
   while (0)
 
+// Host wait until the last build that used the context's shared scratch and
+// metadata has finished (before they are freed or rewritten).
+int wait_scratch_idle(msfem_ctx *ctx)
+{
+  if (ctx->scratch_recorded)
+    HIP_TRY(ctx, hipEventSynchronize(ctx->scratch_done));
+  return MSFEM_OK;
+}
+
+// Orders work about to be enqueued on `st` after the last user of the shared
+// scratch when that ran on another stream (not under the caller's graph capture,
+// where an outside event cannot be waited on; a capturing caller keeps to one
+// stream).
+int order_after_scratch(msfem_ctx *ctx, hipStream_t st, bool *capturing)
+{
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIP_TRY(ctx, hipStreamIsCapturing(st, &cap));
+  *capturing = cap != hipStreamCaptureStatusNone;
+  if (ctx->scratch_recorded && ctx->scratch_stream != st && !*capturing)
+    HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->scratch_done, 0));
+  return MSFEM_OK;
+}
+
+// Marks the end of the shared scratch's use by the work just enqueued on `st`.
+int retire_scratch(msfem_ctx *ctx, hipStream_t st, bool capturing)
+{
+  if (capturing)
+    return MSFEM_OK;
+  if (!ctx->scratch_done)
+    HIP_TRY(ctx, hipEventCreateWithFlags(&ctx->scratch_done, hipEventDisableTiming));
+  HIP_TRY(ctx, hipEventRecord(ctx->scratch_done, st));
+  ctx->scratch_stream = st;
+  ctx->scratch_recorded = true;
+  return MSFEM_OK;
+}
+
 // Grow-only device scratch owned by the context.
 int scratch(msfem_ctx *ctx, const char *name, size_t bytes, void **out)
 {
@@ -103,7 +146,12 @@ int scratch(msfem_ctx *ctx, const char *name, size_t bytes, void **out)
     {
       ++ctx->generation;
       if (b.p)
-        HIP_TRY(ctx, hipFree(b.p));
+        {
+          int rc = wait_scratch_idle(ctx);
+          if (rc)
+            return rc;
+          HIP_TRY(ctx, hipFree(b.p));
+        }
       b.p = nullptr;
       b.bytes = 0;
       HIP_TRY(ctx, hipMalloc(&b.p, bytes > 0 ? bytes : 1));
@@ -276,6 +324,11 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
   memcpy(host.data() + off, p.coef_off.data(), n * sizeof(long long));
   memcpy(host.data() + off + n * sizeof(long long), p.bas_off.data(), n * sizeof(long long));
   memcpy(host.data() + off + 2 * n * sizeof(long long), p.cond_off.data(), n * sizeof(long long));
+  // an earlier build (possibly on another stream) may still read the old
+  // metadata: wait for it, then copy synchronously
+  rc = wait_scratch_idle(ctx);
+  if (rc)
+    return rc;
   HIP_TRY(ctx, hipMemcpy(d, host.data(), bytes, hipMemcpyHostToDevice));
   p.uploaded = true;
   return MSFEM_OK;
@@ -490,14 +543,18 @@ int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o,
       drop_graphs(ctx);
       ctx->graph_generation = ctx->generation;
     }
+  // the key is built from the named fields (never from struct bytes, whose
+  // padding a C caller may leave uninitialised); the n_fine contents enter
+  // through the generation, which changes with the plan
   std::string key;
-  auto add = [&key](const void *v, size_t n) { key.append((const char *)v, n); };
-  add(&ctx->generation, sizeof ctx->generation);
-  add(in, sizeof *in);
-  add(o, sizeof *o);
-  add(out, sizeof *out);
-  add(&bases, sizeof bases);
-  add(&st, sizeof st);
+  auto add = [&key](const auto &v) { key.append((const char *)&v, sizeof v); };
+  add(ctx->generation);
+  add(in->dim), add(in->n_cells), add(in->box), add(in->coef_kind), add(in->coef), add(in->rhs_kind);
+  add(in->rhs_value), add(in->x0), add(in->vertices);
+  add(o->precision), add(o->max_iter), add(o->max_refine), add(o->partition_of_unity), add(o->rel_tol);
+  add(o->abs_tol), add(o->inner_rel_tol), add(o->variant), add(o->coarse_space), add(o->q1_start);
+  add(out->Ae), add(out->Fe), add(out->bases), add(out->iters), add(out->status), add(out->resid);
+  add(bases), add(st);
   auto it = ctx->graphs.find(key);
   if (it == ctx->graphs.end())
     {
@@ -571,6 +628,9 @@ int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o,
   return MSFEM_OK;
 }
 
+int enqueue_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+                  hipStream_t st);
+
 int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
                  hipStream_t st)
 {
@@ -582,7 +642,21 @@ int build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts 
   if (!out || !out->Ae || !out->Fe)
     return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  rc = make_plan(ctx, in);
+  bool capturing = false;
+  rc = order_after_scratch(ctx, st, &capturing);
+  if (rc)
+    return rc;
+  rc = enqueue_build(ctx, in, o, out, st);
+  if (rc)
+    return rc;
+  return retire_scratch(ctx, st, capturing);
+}
+
+// The build's kernels on `st` (arguments validated, device current).
+int enqueue_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, msfem_batch_out *out,
+                  hipStream_t st)
+{
+  int rc = make_plan(ctx, in);
   if (rc)
     return rc;
   Plan &p = ctx->plan;
@@ -957,6 +1031,11 @@ int msfem_ctx_destroy(msfem_ctx *ctx)
   for (auto &e : ctx->join)
     if (e)
       hipEventDestroy(e);
+  if (ctx->scratch_done)
+    {
+      hipEventSynchronize(ctx->scratch_done);
+      hipEventDestroy(ctx->scratch_done);
+    }
   for (auto &kv : ctx->bufs)
     if (kv.second.p)
       hipFree(kv.second.p);
@@ -1094,9 +1173,13 @@ int msfem_coarse_solve_device(msfem_ctx *ctx, int dim, int nc, const double *Kc,
   int rc = scratch(ctx, "coarse_work", coarse_solve_work_doubles(dim, nc) * sizeof(double), (void **)&work);
   if (rc)
     return rc;
+  bool capturing = false;
+  rc = order_after_scratch(ctx, (hipStream_t)stream, &capturing);
+  if (rc)
+    return rc;
   HIP_TRY(ctx, launch_coarse_solve(dim, nc, Kc, Fc, cmask, cval, rel_tol, max_iter, u, iters, work,
                                    (hipStream_t)stream));
-  return MSFEM_OK;
+  return retire_scratch(ctx, (hipStream_t)stream, capturing);
 }
 
 int msfem_coarse_solve(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc, const uint8_t *cmask,
@@ -1169,6 +1252,10 @@ int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t
       b += (long long)nb * nn[c];
       o += nn[c];
     }
+  // an earlier reconstruction on another stream may still read the metadata
+  int rc = wait_scratch_idle(ctx);
+  if (rc)
+    return rc;
   HostStage s{ctx};
   const long long *db = s.up("r_boff", boff.data(), n_cells);
   const long long *dq = s.up("r_ooff", ooff.data(), n_cells);
@@ -1177,7 +1264,9 @@ int msfem_reconstruct_device(msfem_ctx *ctx, int dim, int n_cells, const int32_t
     return s.rc;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream)); // metadata in place before the caller's stream reads it
   HIP_TRY(ctx, launch_reconstruct(n_cells, nb, db, dq, dn, bases, w, u, (hipStream_t)stream));
-  return MSFEM_OK;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  HIP_TRY(ctx, hipStreamIsCapturing((hipStream_t)stream, &cap));
+  return retire_scratch(ctx, (hipStream_t)stream, cap != hipStreamCaptureStatusNone);
 }
 
 int msfem_fine_solve(msfem_ctx *ctx, int dim, int n, int coef_kind, const double *coef, int rhs_kind,

@@ -1,89 +1,10 @@
-"""Workloads of the MsFEM basis build, at any size: inputs for the GPU engine and
-the oracle alike (generated by the product library's host generators)."""
+"""Test-side helpers over the package's workload definitions: the oracle's build
+of a case and the energy-norm error against it."""
 import numpy as np
 
 import msfem
-
-CLIP = 0.5 * np.log(1e3)  # lognormal contrast clipped to 1e3
-
-
-def unit_cells(dim, nc, which=None):
-    box = msfem.unit_boxes(dim, nc)
-    if which is not None:
-        box = box[which]
-    return box
-
-
-def periodic_cells(nc=8, nf=32, eps=1 / 64, which=None):
-    """alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps) at the quadrature points, f = 1."""
-    box = unit_cells(2, nc, which)
-    nfa = np.full(len(box), nf, np.int32)
-    return dict(dim=2, n_fine=nfa, box=box, coef_kind=msfem.COEF_SCALAR_QP, coef=msfem.coef_periodic(nfa, box, eps),
-                rhs_kind=msfem.RHS_CONST, rhs_value=1.0)
-
-
-def lognormal_cells(dim=2, nc=64, nf=64, seed=2, which=None):
-    """alpha = exp(g), g ~ N(0,1) i.i.d. per fine cell clipped to contrast 1e3, f = 1.
-    `which` selects cells of the full nc^dim batch (same tiles as in the full batch)."""
-    box = unit_cells(dim, nc)
-    nfa = np.full(len(box), nf, np.int32)
-    coef = msfem.coef_lognormal(dim, nfa, seed, CLIP)
-    if which is not None:
-        which = np.asarray(which)
-        box, nfa = box[which], nfa[which]
-        coef = coef.reshape(-1, nf ** dim)[which].reshape(-1)
-    return dict(dim=dim, n_fine=nfa, box=box, coef_kind=msfem.COEF_SCALAR_CELL, coef=coef,
-                rhs_kind=msfem.RHS_CONST, rhs_value=1.0)
-
-
-def tensor_cells(nc=128, seed=3, sigma=1.5, which=None, n_fine=None):
-    """SPD tensor per fine cell, resolution class by probe contrast, f = sin(pi x) sin(pi y)."""
-    box = unit_cells(2, nc)
-    nfa, _ = msfem.resolution_classes(nc, seed, sigma)
-    if which is not None:
-        box, nfa = box[which], nfa[which]
-    if n_fine is not None:
-        nfa = np.asarray(n_fine, np.int32)
-    return dict(dim=2, n_fine=nfa, box=box, coef_kind=msfem.COEF_TENSOR_CELL, coef=msfem.coef_tensor(nfa, seed, sigma),
-                rhs_kind=msfem.RHS_SINSIN, rhs_value=0.0)
-
-
-def reference_cells(nc=8, nf=128, which=None):
-    """The reference default: MatrixCoeff with PI_D at the quadrature points, f = 2."""
-    box = unit_cells(2, nc, which)
-    nfa = np.full(len(box), nf, np.int32)
-    return dict(dim=2, n_fine=nfa, box=box, coef_kind=msfem.COEF_TENSOR_QP, coef=msfem.coef_reference(2, nfa, box),
-                rhs_kind=msfem.RHS_CONST, rhs_value=2.0)
-
-
-def quad_cells(nc=8, nf=32, amp=0.24, seed=9, kind="lognormal", which=None, n_fine=None):
-    """General quadrilaterals: the unit square's nc x nc mesh with its interior
-    vertices moved by up to amp/nc per axis.  kind: "lognormal" (scalar per
-    fine cell, f = 1), "reference" (MatrixCoeff at the mapped quadrature
-    points, f = 2) or "tensor" (SPD tensor per fine cell, f = sin sin)."""
-    q = msfem.perturbed_quads(nc, amp, seed)
-    if which is not None:
-        q = q[np.asarray(which)]
-    nfa = np.asarray(n_fine, np.int32) if n_fine is not None else np.full(len(q), nf, np.int32)
-    if kind == "lognormal":
-        coef, ck, rk, rv = msfem.coef_lognormal(2, nfa, seed, CLIP), msfem.COEF_SCALAR_CELL, msfem.RHS_CONST, 1.0
-    elif kind == "reference":
-        coef, ck, rk, rv = msfem.coef_reference_quad(nfa, q), msfem.COEF_TENSOR_QP, msfem.RHS_CONST, 2.0
-    else:
-        coef, ck, rk, rv = msfem.coef_tensor(nfa, seed, 1.5), msfem.COEF_TENSOR_CELL, msfem.RHS_SINSIN, 0.0
-    return dict(dim=2, n_fine=nfa, box=None, vertices=q, coef_kind=ck, coef=coef, rhs_kind=rk, rhs_value=rv)
-
-
-def split_bases(case, flat):
-    """flat bases -> list of [nb][(nf+1)^d] arrays per cell."""
-    dim = case["dim"]
-    nb = 1 << dim
-    out, off = [], 0
-    for nf in case["n_fine"]:
-        nn = (int(nf) + 1) ** dim
-        out.append(flat[off:off + nb * nn].reshape(nb, nn))
-        off += nb * nn
-    return out
+from workloads import *  # noqa: F401,F403  (the workload generators)
+from workloads import split_bases
 
 
 def oracle_build(case, want_bases=True, solver=0, n_threads=0):
@@ -93,12 +14,6 @@ def oracle_build(case, want_bases=True, solver=0, n_threads=0):
                                   case["rhs_value"], solver=solver, n_threads=n_threads, want_bases=want_bases)
     return O.batch_build(case["dim"], case["box"], case["n_fine"], case["coef_kind"], case["coef"], case["rhs_kind"],
                          case["rhs_value"], solver=solver, n_threads=n_threads, want_bases=want_bases)
-
-
-def gpu_build(ctx, case, **opts):
-    o = msfem.default_opts(**opts)
-    return ctx.batch_build(case["dim"], case["n_fine"], case["box"], case["coef_kind"], case["coef"], case["rhs_kind"],
-                           case["rhs_value"], opts=o, vertices=case.get("vertices"))
 
 
 def energy_error(case, gpu_flat, ref_flat):

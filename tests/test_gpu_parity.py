@@ -336,3 +336,62 @@ def test_two_level_preconditioner(ctx, kind):
     # shapes outside the two-level table keep Jacobi
     assert not ctx.uses_coarse_space(2, 64, msfem.default_opts(variant=2), case["coef_kind"])
     assert not ctx.uses_coarse_space(2, 32, msfem.default_opts(), case["coef_kind"])
+
+
+@pytest.mark.parametrize("bc", [msfem.BC_ZERO_DIRICHLET, msfem.BC_REFERENCE])
+def test_coarse_3d_lognormal_full_batch(ctx, bc):
+    """The 3D coarse system of the 16^3 x 16^3 lognormal batch
+    (diffusion_problem_ms.tpp:161-296 at dim 3): every cell built on the GPU (FP32
+    CG + FP64 refinement to rel. residual 1e-13), then the 27-point coarse
+    assembly with the 3D boundary data and the FP64 coarse CG on the GPU, against
+    the oracle's assembly and direct solve of the same A^K, F^K (1e-10); a sample
+    of cells against the oracle's own build (1e-10)."""
+    say(f"coarse 3d lognormal 4096 cells bc={bc}")
+    case = C.lognormal_cells(dim=3, nc=16, nf=16, seed=5)
+    out = C.gpu_build(ctx, case, precision=msfem.PREC_FP32_REFINE)
+    assert (out["status"] == 0).all()
+    Ae, Fe = out["Ae"], out["Fe"]
+    assert np.abs(Ae - Ae.transpose(0, 2, 1)).max() < 1e-12 * np.abs(Ae).max()
+    assert np.abs(Ae.sum(2)).max() < 1e-10 * np.abs(Ae).max()
+    assert np.abs(Fe.sum(1) - (1 / 16) ** 3).max() < 1e-15
+    sample = np.array([0, 1365, 4095])
+    ref = C.oracle_build(C.lognormal_cells(dim=3, nc=16, nf=16, seed=5, which=sample))
+    assert rel(Ae[sample], ref["Ae"]) < TOL and rel(Fe[sample], ref["Fe"]) < TOL
+    Kc, Fc, cm, cv = ctx.coarse_assemble(3, 16, Ae, Fe, bc)
+    Kr, Fr, cmr, cvr = O.coarse_assemble(3, 16, Ae, Fe, bc)
+    assert rel(Kc, Kr) < 1e-13 and rel(Fc, Fr) < 1e-13 and (cm == cmr).all() and rel(cv, cvr) < 1e-15
+    u, it = ctx.coarse_solve(3, 16, Kc, Fc, cm, cv, rel_tol=1e-14)
+    ur = O.coarse_solve(3, 16, Kr, Fr, cmr, cvr)
+    say(f"  coarse CG iterations {it}")
+    assert rel(u, ur) < TOL, it
+
+
+def test_async_builds_on_two_streams_with_layout_change(ctx):
+    """Two device builds enqueued back to back with no host synchronisation: the
+    first on the context stream, the second -- another batch layout, so the
+    context's shared plan metadata and scratch are rewritten -- on the
+    context's non-blocking second stream.  Each gives bitwise what the same
+    batch gives built alone."""
+    say("async builds on two streams, layout change")
+    a = C.lognormal_cells(which=np.arange(0, 4096, 64))
+    b = C.tensor_cells(nc=128, which=np.array([3, 99, 1000, 7000, 12000, 16000]), n_fine=[64, 128, 16, 32, 64, 128])
+    want = [C.gpu_build(ctx, a), C.gpu_build(ctx, b)]
+    aux = ctx.aux_stream
+    bufs = []
+    for case, stream in ((a, ctx.stream), (b, aux)):
+        n = len(case["n_fine"])
+        d = dict(box=ctx.to_device(np.asarray(case["box"]).reshape(-1)), coef=ctx.to_device(case["coef"]),
+                 Ae=ctx.alloc(n * 16, np.float64), Fe=ctx.alloc(n * 4, np.float64),
+                 bases=ctx.alloc(msfem.bases_size(2, case["n_fine"]), np.float64),
+                 it=ctx.alloc(n * 4, np.int32), st=ctx.alloc(n * 4, np.int32))
+        bufs.append(d)
+        ctx.batch_build_device(2, case["n_fine"], d["box"].ptr, case["coef_kind"], d["coef"].ptr, d["Ae"].ptr,
+                               d["Fe"].ptr, d["bases"].ptr, d["it"].ptr, d["st"].ptr, rhs_kind=case["rhs_kind"],
+                               rhs_value=case["rhs_value"], stream=stream)
+    ctx.synchronize()
+    for d, w in zip(bufs, want):
+        assert np.array_equal(d["Ae"].numpy().reshape(w["Ae"].shape), w["Ae"])
+        assert np.array_equal(d["bases"].numpy(), w["bases"])
+        assert np.array_equal(d["it"].numpy().reshape(w["iters"].shape), w["iters"])
+        for arr in d.values():
+            arr.free()

@@ -52,9 +52,57 @@ def test_reference_coefficient_matches_long_double(dim):
     assert off == c.size
 
 
-def test_reference_coefficient_pi_d_and_contrast():
-    """PI_D (3.14592..., not pi) enters the oscillation: at x = y = 0 the
-    diagonal value is 1; its eigenvalues stay in [1e-4, 1.9999]."""
+PI_D = 3.14592653509793218403  # include/coefficients/coefficients.h:21 (not pi)
+GAUSS0 = 0.5 - 0.5 / np.sqrt(3)  # first QGauss(2) abscissa on [0, 1]
+
+
+def special_point(kind, m):
+    """x with 2 PI_D 57 x = theta exactly (up to one rounding of x), where
+    sin(theta) is known: kind 'one' -> theta = pi/2 + 2 pi m (sin = 1),
+    'minus' -> 3 pi/2 + 2 pi m (sin = -1), 'zero' -> pi m (sin = 0)."""
+    theta = {"one": np.pi / 2 + 2 * np.pi * m, "minus": 1.5 * np.pi + 2 * np.pi * m, "zero": np.pi * m}[kind]
+    return theta / (2 * PI_D * 57), {"one": 1.0, "minus": -1.0, "zero": 0.0}[kind]
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_reference_coefficient_value_pin(dim):
+    """MatrixCoeff (include/coefficients/matrix_coeff.tpp:44-91, k = 57,
+    scale 0.9999, PI_D = 3.14592..., matrix_coeff.hpp:45-49) at points where
+    the oscillation's sines have exact values: the tensor is d I with
+    d = 1 - 0.9999 (sx + sy) / 2 (the rotation R d I R^T only rounds), z
+    ignored in 3D.  The expected values need no restatement of the formula:
+    pi in place of PI_D moves these points' sines by 7e-4 to 5e-2, k = 56 by
+    more than 0.1, scale 0.999 moves d by at least 1e-4; the bar is 1e-12.  Both the
+    product generator and the oracle's long-double restatement are checked."""
+    ns = 3 if dim == 2 else 6
+    h = 1e-3
+    diag = [0, 2] if dim == 2 else [0, 3, 5]
+    off = [1] if dim == 2 else [1, 2, 4]
+    cases = [("one", 10, "one", 3), ("minus", 7, "minus", 12), ("zero", 9, "one", 10), ("one", 4, "minus", 5),
+             ("zero", 12, "zero", 3), ("minus", 11, "zero", 1)]
+    for kx, mx, ky, my in cases:
+        x, sx = special_point(kx, mx)
+        y, sy = special_point(ky, my)
+        want = 1.0 - 0.9999 * (0.5 * sx + 0.5 * sy)
+        lo = [x - h * GAUSS0, y - h * GAUSS0, 0.3][:dim]
+        box = np.array([[[l, l + h] for l in lo]])
+        c = msfem.coef_reference(dim, [1], box)[:ns]  # first quadrature point of the one fine cell
+        p = [lo[k] + h * GAUSS0 for k in range(dim)]
+        for got in (c, O.matrix_coeff_ref(dim, p)):
+            assert np.abs(np.asarray(got)[diag] - want).max() < 1e-12, (kx, mx, ky, my, got, want)
+            assert np.abs(np.asarray(got)[off]).max() < 1e-15
+    # the sines' extremes bound the spectrum to [1e-4, 1.9999]: contrast ~2e4
+    x1, _ = special_point("one", 3)
+    xm, _ = special_point("minus", 3)
+    for (px, py), want in (((x1, x1), 1e-4), ((xm, xm), 1.9999)):
+        box = np.array([[[px - h * GAUSS0, px - h * GAUSS0 + h], [py - h * GAUSS0, py - h * GAUSS0 + h]]])
+        assert abs(msfem.coef_reference(2, [1], box)[0] - want) < 1e-12
+
+
+def test_reference_coefficient_spectrum_bounds():
+    """Every quadrature value of MatrixCoeff has its eigenvalues in
+    [1e-4, 1.9999] and is isotropic (the rotation by PI_D/3 of the equal
+    diagonal entries)."""
     c = msfem.coef_reference(2, [16], msfem.unit_boxes(2, 1))
     A = c.reshape(-1, 3)
     tr = A[:, 0] + A[:, 2]

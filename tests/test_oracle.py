@@ -219,3 +219,112 @@ def test_coarse_reference_bcs_dirichlet_and_neumann():
     assert abs(Fc[node] - want) < 1e-15
     u = O.coarse_solve(2, nc, Kc, Fc, cm, cv)
     assert np.allclose(u[cm == 1], cv[cm == 1])
+
+
+def unit_boxes3(nc):
+    return np.array([[[i / nc, (i + 1) / nc], [j / nc, (j + 1) / nc], [k / nc, (k + 1) / nc]]
+                     for k in range(nc) for j in range(nc) for i in range(nc)])
+
+
+def test_coarse_3d_constant_coefficient_is_textbook_trilinear_stencil():
+    """3D coarse assembly (diffusion_problem_ms.tpp:161-255 at dim 3) with a
+    constant coefficient: phi_i = N_i, so the coarse matrix is the textbook
+    trilinear Q1 Laplacian of the (nc+1)^3 grid -- at an interior node the
+    27-point stencil alpha H (8/3 centre, 0 face, -1/6 edge, -1/12 corner
+    neighbours) -- and the load is f H^3 / 8 per adjacent cell.  Zero
+    Dirichlet marks every boundary node."""
+    nc, nf, alpha = 3, 4, 2.5
+    box = unit_boxes3(nc)
+    r = O.batch_build(3, box, np.full(nc ** 3, nf), O.COEF_SCALAR_CELL, np.full(nc ** 3 * nf ** 3, alpha),
+                      O.RHS_CONST, 1.0)
+    Kc, Fc, cm, cv = O.coarse_assemble(3, nc, r["Ae"], r["Fe"], 0)
+    H = 1.0 / nc
+    np1 = nc + 1
+    for node in range(np1 ** 3):
+        I = np.array([node % np1, (node // np1) % np1, node // (np1 * np1)])
+        for t in range(27):
+            o = np.array([t % 3 - 1, (t // 3) % 3 - 1, t // 9 - 1])
+            J = I + o
+            if (J < 0).any() or (J > nc).any():
+                assert Kc[node, t] == 0
+                continue
+            # cells shared by node I and node J: per axis, offset 0 -> the cells
+            # on either side that exist, offset +-1 -> the one between them
+            n_shared = np.prod([(int(I[m] > 0) + int(I[m] < nc)) if o[m] == 0 else 1 for m in range(3)])
+            per_cell = {0: H / 3, 1: 0.0, 2: -H / 12, 3: -H / 12}[int(np.abs(o).sum())]
+            assert abs(Kc[node, t] - alpha * n_shared * per_cell) < 1e-14, (node, t)
+        n_adj = np.prod([int(I[m] > 0) + int(I[m] < nc) for m in range(3)])
+        assert abs(Fc[node] - n_adj * H ** 3 / 8) < 1e-15
+        assert cm[node] == int((I == 0).any() or (I == nc).any())
+    interior = 1 + np1 + np1 * np1
+    assert np.allclose(Kc[interior].reshape(3, 3, 3)[1], alpha * H * np.array([[-1 / 6, 0, -1 / 6], [0, 8 / 3, 0],
+                                                                                  [-1 / 6, 0, -1 / 6]]), atol=1e-15)
+    assert abs(Kc[interior].sum()) < 1e-14
+
+
+def test_coarse_3d_manufactured_convergence():
+    """alpha = 1, f = sin(pi x) sin(pi y) sin(pi z), zero Dirichlet: u = f / (3 pi^2);
+    the 3D MsFEM coarse solution's nodal error falls at O(H^2)."""
+    errs = []
+    for nc in (2, 4, 8):
+        nf = 4
+        r = O.batch_build(3, unit_boxes3(nc), np.full(nc ** 3, nf), O.COEF_SCALAR_CELL, np.ones(nc ** 3 * nf ** 3),
+                          O.RHS_SINSIN, 0.0)
+        Kc, Fc, cm, cv = O.coarse_assemble(3, nc, r["Ae"], r["Fe"], 0)
+        u = O.coarse_solve(3, nc, Kc, Fc, cm, cv)
+        np1 = nc + 1
+        idx = np.arange(np1 ** 3)
+        X, Y, Z = idx % np1 / nc, (idx // np1) % np1 / nc, idx // (np1 * np1) / nc
+        exact = np.sin(np.pi * X) * np.sin(np.pi * Y) * np.sin(np.pi * Z) / (3 * np.pi ** 2)
+        errs.append(np.abs(u - exact).max())
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert (rates > 1.8).all(), (errs, rates)
+
+
+def test_coarse_3d_reference_bcs():
+    """Reference boundary data in 3D (diffusion_problem_ms.tpp:128-137, :206-235;
+    dirichlet_bc.tpp:17-43, neumann_bc.tpp:17-41 ignore z): Dirichlet g =
+    (x-1/2)^2 + (y-1/2)^2 on x = 0, y = 0, z = 0; Neumann h = cos(2 PI_D x)
+    cos(2 PI_D y) on x = 1, y = 1, z = 1, integrated with QGauss(2) on the face
+    against the standard trilinear face values (hand sum below)."""
+    nc, nf = 2, 4
+    r = O.batch_build(3, unit_boxes3(nc), np.full(nc ** 3, nf), O.COEF_SCALAR_CELL, np.ones(nc ** 3 * nf ** 3),
+                      O.RHS_CONST, 0.0)
+    Kc, Fc, cm, cv = O.coarse_assemble(3, nc, r["Ae"], r["Fe"], 1)
+    np1 = nc + 1
+    idx = np.arange(np1 ** 3)
+    X, Y, Z = idx % np1 / nc, (idx // np1) % np1 / nc, idx // (np1 * np1) / nc
+    assert (cm == ((X == 0) | (Y == 0) | (Z == 0))).all()
+    assert np.allclose(cv[cm == 1], ((X - 0.5) ** 2 + (Y - 0.5) ** 2)[cm == 1], atol=1e-15)
+    pid = 3.14592653509793218403
+    gs = np.array([0.5 - 0.5 / np.sqrt(3), 0.5 + 0.5 / np.sqrt(3)])
+    h = lambda x, y: np.cos(2 * pid * x) * np.cos(2 * pid * y)
+    # node (1, 1, 1/2) (x = y = 1, z = 1/2): on face x = 1 of cells (1,1,0) and
+    # (1,1,1) and on face y = 1 of the same cells; the node is the face vertex
+    # at (y or x) = 1 and z = 1/2
+    node = 2 + np1 * 2 + np1 * np1 * 1
+    H = 0.5
+    want = 0.0
+    for cz in (0, 1):
+        for qa in gs:          # along the free in-face axis (y on x = 1, x on y = 1)
+            for qz in gs:
+                zq = (cz + qz) * H
+                pz = qz if cz == 0 else 1 - qz      # trilinear factor along z (node at z = 1/2)
+                w = (0.5 * H) ** 2
+                a = (1 + qa) * H                      # coordinate along the free axis in cell 1
+                want += h(1.0, a) * qa * pz * w       # face x = 1: factor along y is qa (node at y = 1)
+                want += h(a, 1.0) * qa * pz * w       # face y = 1: factor along x is qa (node at x = 1)
+    assert abs(Fc[node] - want) < 1e-15, (Fc[node], want)
+    # a node on z = 1 only: (1/2, 1/2, 1) sees face z = 1 of the four cells around it
+    node = 1 + np1 * 1 + np1 * np1 * 2
+    want = 0.0
+    for cx in (0, 1):
+        for cy in (0, 1):
+            for qx in gs:
+                for qy in gs:
+                    px = qx if cx == 0 else 1 - qx
+                    py = qy if cy == 0 else 1 - qy
+                    want += h((cx + qx) * H, (cy + qy) * H) * px * py * (0.5 * H) ** 2
+    assert abs(Fc[node] - want) < 1e-15, (Fc[node], want)
+    u = O.coarse_solve(3, nc, Kc, Fc, cm, cv)
+    assert np.allclose(u[cm == 1], cv[cm == 1])
