@@ -58,7 +58,7 @@
 extern "C" {
 #endif
 
-#define MSFEM_API_VERSION 2
+#define MSFEM_API_VERSION 3
 
 /* status codes */
 #define MSFEM_OK 0
@@ -123,7 +123,17 @@ typedef struct msfem_solver_opts {
                                  node grid), 0 = zero interior.  The stopping test stays
                                  relative to ||b|| = ||K_IB g_i|| either way; the start
                                  only shortens the solve. */
-  int32_t reserved[4];
+  double energy_tol;          /* > 0 (box cells): replaces the residual test above by one
+                                 that guarantees ||phi - phi*||_K <= energy_tol ||phi*||_K
+                                 for every basis (phi* the exact discrete solution, K the
+                                 fine stiffness): stop when ||r||_2 <= energy_tol
+                                 sqrt(lam_lb e_lb), with lam_lb = (min over the tile of
+                                 lambda_min(A)) x (smallest eigenvalue of the
+                                 unit-coefficient stiffness) <= lambda_min(K_II) and
+                                 e_lb = (int |grad N_i|^2)^2 / int grad N_i^T A^-1 grad N_i
+                                 <= ||phi*||_K^2 (complementary energy with the
+                                 divergence-free flux grad N_i).  0 (default): off. */
+  int32_t reserved[2];
 } msfem_solver_opts;
 
 typedef struct msfem_batch {
@@ -154,6 +164,12 @@ typedef struct msfem_batch_out {
   int32_t *iters;   /* optional: [n_cells][nb] inner CG iterations */
   int32_t *status;  /* optional: [n_cells][nb] 0 converged, 1 not converged */
   double *resid;    /* optional: [n_cells][nb] final true residual norm */
+  double *energy_bound; /* optional: [n_cells][nb] a posteriori bound of the relative
+                           energy-norm error ||phi - phi*||_K / ||phi*||_K of each
+                           basis: sqrt(q / (A^K_ii - q)), q = ||r||_2^2 / lam_lb
+                           (||phi||_K^2 = ||phi*||_K^2 + ||phi - phi*||_K^2 since
+                           K phi* = 0 on the interior); NaN for general
+                           quadrilaterals.  No counterpart in the reference. */
 } msfem_batch_out;
 
 /* ---- context --------------------------------------------------------- */
@@ -239,6 +255,17 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
  * Convergence is reported only through out->status. */
 int msfem_batch_build_device(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *opts,
                              msfem_batch_out *out, void *stream);
+
+/* Relative energy-norm error of every basis of a batch against reference bases
+ * (e.g. a mixed-precision or surrogate-started build against the plain FP64
+ * build): err[c][i] = ||phi_i - phi_ref,i||_K / ||phi_ref,i||_K with the cell's
+ * fine stiffness K (the element kernel's phi^T K phi on the difference and on
+ * the reference).  in gives the layout and the coefficients (its x0, rhs and
+ * outputs are unused); every array except in->n_fine in device memory, work
+ * enqueued on `stream`.  No counterpart in the reference: the measure of the
+ * error the mixed-precision and surrogate forms state against the FP64 path. */
+int msfem_energy_error_device(msfem_ctx *ctx, const msfem_batch *in, const double *bases, const double *bases_ref,
+                              double *err, void *stream);
 
 /* ---- coarse system ------------------------------------------------------ */
 /* Unit square/cube split into nc^dim coarse cells, lexicographic, x fastest

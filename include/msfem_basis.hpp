@@ -185,7 +185,7 @@ public:
     iters_.assign((size_t)n * nb, 0);
     msfem_batch in{dim, n, n_fine_.data(), box.data(), data_.coef_kind, coef.data(), data_.rhs_kind, data_.rhs_value,
                    nullptr, general_ ? vert.data() : nullptr};
-    msfem_batch_out out{Ae_.data(), Fe_.data(), bases_.data(), iters_.data(), nullptr, nullptr};
+    msfem_batch_out out{Ae_.data(), Fe_.data(), bases_.data(), iters_.data(), nullptr, nullptr, nullptr};
     check(msfem_batch_build(engine_.get(), &in, &opts_, &out), engine_.get());
     built_ = true;
   }

@@ -207,6 +207,425 @@ __global__ void __launch_bounds__(kCoarseThreads) coarse_cg_kernel(int nc, const
     *iters = it;
 }
 
+// ---- multigrid-preconditioned CG ---------------------------------------------------
+// The coarse system plays the role of the reference's AMG-preconditioned CG
+// (diffusion_problem_ms.tpp:267-289).  Here the grid is structured, so the
+// preconditioner is one geometric V(2,2) cycle: levels n, n/2, ... (while the
+// cells per side are even and >= 4), bilinear/trilinear prolongation P, full
+// weighting P^T, Galerkin operators A_{l+1} = P^T A_l P on the free nodes (a
+// coarse node is constrained when the fine node at its position is), damped
+// Jacobi smoothing (omega = 0.7), and a dense Cholesky solve on the coarsest
+// level.  The cycle is symmetric, so CG applies.  One workgroup runs the whole
+// solve; every vector lives in the scratch `w` (L2-resident at these sizes) and
+// every pass ends in a workgroup barrier.  Layout: MgLayout (host-computed).
+constexpr double kMgOmega = 0.7;
+
+template <int DIM>
+struct MgGrid {
+  int n; // cells per side; nodes (n+1)^DIM, x fastest
+  __device__ __forceinline__ int np() const { return n + 1; }
+  __device__ __forceinline__ void coords(int node, int (&I)[3]) const
+  {
+    const int p = n + 1;
+    I[0] = node % p;
+    I[1] = (node / p) % p;
+    I[2] = DIM == 3 ? node / (p * p) : 0;
+  }
+  __device__ __forceinline__ int index(const int (&I)[3]) const
+  {
+    const int p = n + 1;
+    return I[0] + p * (I[1] + (DIM == 3 ? p * I[2] : 0));
+  }
+  __device__ __forceinline__ bool inside(const int (&I)[3]) const
+  {
+    for (int k = 0; k < DIM; ++k)
+      if (I[k] < 0 || I[k] > n)
+        return false;
+    return true;
+  }
+};
+
+template <int DIM>
+__global__ void __launch_bounds__(kCoarseThreads)
+  coarse_mgcg_kernel(MgLayout g, int nc, const double *Kc, const double *Fc, const unsigned char *cmask,
+                     const double *cval, double rel_tol, int max_iter, double *u, int *iters, double *w)
+{
+  constexpr int S = DIM == 2 ? 9 : 27;
+  __shared__ double red[kCoarseThreads / kWave];
+  const int tid = threadIdx.x;
+  const int L = g.levels;
+  MgGrid<DIM> G0{nc};
+  const int nn0 = (int)g.nn[0];
+  double *cx = w + g.off_cx, *cr = w + g.off_cr, *cp = w + g.off_cp, *cq = w + g.off_cq;
+
+  // constrained node of level l: the level-0 node at its position is constrained
+  auto masked = [&](int l, int node) -> bool {
+    if (l == 0)
+      return cmask[node] != 0;
+    MgGrid<DIM> Gl{g.n[l]};
+    int I[3];
+    Gl.coords(node, I);
+    for (int k = 0; k < DIM; ++k)
+      I[k] <<= l;
+    return cmask[G0.index(I)] != 0;
+  };
+  auto Aof = [&](int l) -> const double * { return l == 0 ? Kc : w + g.off_A[l]; };
+  // (A_l v)_node over the neighbours inside the grid (rows of constrained nodes: 0)
+  auto apply = [&](int l, const double *v, int node) -> double {
+    if (masked(l, node))
+      return 0.0;
+    MgGrid<DIM> Gl{g.n[l]};
+    int I[3];
+    Gl.coords(node, I);
+    const double *A = Aof(l) + (size_t)node * S;
+    double s = 0;
+    for (int t = 0; t < S; ++t)
+      {
+        int J[3] = {I[0] + t % 3 - 1, I[1] + (t / 3) % 3 - 1, DIM == 3 ? I[2] + t / 9 - 1 : 0};
+        if (Gl.inside(J))
+          s += A[t] * v[Gl.index(J)];
+      }
+    return s;
+  };
+
+  // ---- set-up: Galerkin operators, Jacobi diagonals, coarsest Cholesky ------------
+  for (int l = 1; l < L; ++l)
+    {
+      MgGrid<DIM> Gf{g.n[l - 1]}, Gc{g.n[l]};
+      const double *Af = Aof(l - 1);
+      double *Ac = w + g.off_A[l];
+      for (int c = tid; c < (int)g.nn[l]; c += kCoarseThreads)
+        {
+          double acc[S];
+#pragma unroll
+          for (int t = 0; t < S; ++t)
+            acc[t] = 0;
+          if (!masked(l, c))
+            {
+              int I[3];
+              Gc.coords(c, I);
+              for (int o = 0; o < S; ++o) // fine node i = 2I + o in the support of coarse node I
+                {
+                  const int od[3] = {o % 3 - 1, (o / 3) % 3 - 1, DIM == 3 ? o / 9 - 1 : 0};
+                  int i[3] = {2 * I[0] + od[0], 2 * I[1] + od[1], 2 * I[2] + od[2]};
+                  if (!Gf.inside(i))
+                    continue;
+                  const int fi = Gf.index(i);
+                  if (masked(l - 1, fi))
+                    continue;
+                  double wi = 1;
+                  for (int k = 0; k < DIM; ++k)
+                    wi *= od[k] ? 0.5 : 1.0;
+                  for (int t = 0; t < S; ++t) // fine neighbour j = i + s
+                    {
+                      int j[3] = {i[0] + t % 3 - 1, i[1] + (t / 3) % 3 - 1, DIM == 3 ? i[2] + t / 9 - 1 : 0};
+                      if (!Gf.inside(j) || masked(l - 1, Gf.index(j)))
+                        continue;
+                      const double a = wi * Af[(size_t)fi * S + t];
+                      // coarse parents J of j: per axis one (even) or two (odd) with weight 1/2
+                      int nodd = 0;
+                      for (int k = 0; k < DIM; ++k)
+                        nodd += j[k] & 1;
+                      for (int comb = 0; comb < (1 << nodd); ++comb)
+                        {
+                          int J[3] = {0, 0, 0}, bitpos = 0;
+                          double wj = 1;
+                          for (int k = 0; k < DIM; ++k)
+                            if (j[k] & 1)
+                              {
+                                J[k] = (j[k] >> 1) + ((comb >> bitpos) & 1);
+                                ++bitpos;
+                                wj *= 0.5;
+                              }
+                            else
+                              J[k] = j[k] >> 1;
+                          if (masked(l, Gc.index(J)))
+                            continue;
+                          int tt = 0, mul = 1;
+                          for (int k = 0; k < DIM; ++k)
+                            {
+                              tt += (J[k] - I[k] + 1) * mul;
+                              mul *= 3;
+                            }
+                          acc[tt] += a * wj;
+                        }
+                    }
+                }
+            }
+          for (int t = 0; t < S; ++t)
+            Ac[(size_t)c * S + t] = acc[t];
+        }
+      __syncthreads();
+    }
+  for (int l = 0; l < L; ++l)
+    {
+      double *d = w + g.off_d[l];
+      const double *A = Aof(l);
+      for (int n = tid; n < (int)g.nn[l]; n += kCoarseThreads)
+        d[n] = masked(l, n) ? 0.0 : 1.0 / A[(size_t)n * S + S / 2];
+    }
+  // coarsest level: free-node numbering (thread 0), dense matrix, Cholesky in place
+  const int lc = L - 1, Nc = (int)g.nn[lc];
+  double *M = w + g.off_dense, *dmap = w + g.off_dmap;
+  __shared__ int s_nfree;
+  if (tid == 0)
+    {
+      int f = 0;
+      for (int n = 0; n < Nc; ++n)
+        dmap[n] = masked(lc, n) ? -1.0 : (double)f++;
+      s_nfree = f;
+    }
+  __syncthreads();
+  const int F = s_nfree;
+  for (int e = tid; e < F * F; e += kCoarseThreads)
+    M[e] = 0.0;
+  __syncthreads();
+  {
+    MgGrid<DIM> Gl{g.n[lc]};
+    const double *A = Aof(lc);
+    for (int n = tid; n < Nc; n += kCoarseThreads)
+      {
+        const int fa = (int)dmap[n];
+        if (fa < 0)
+          continue;
+        int I[3];
+        Gl.coords(n, I);
+        for (int t = 0; t < S; ++t)
+          {
+            int J[3] = {I[0] + t % 3 - 1, I[1] + (t / 3) % 3 - 1, DIM == 3 ? I[2] + t / 9 - 1 : 0};
+            if (!Gl.inside(J))
+              continue;
+            const int fb = (int)dmap[Gl.index(J)];
+            if (fb >= 0)
+              M[(size_t)fa * F + fb] = A[(size_t)n * S + t];
+          }
+      }
+  }
+  __syncthreads();
+  for (int k = 0; k < F; ++k)
+    {
+      if (tid == 0)
+        M[(size_t)k * F + k] = sqrt(M[(size_t)k * F + k]);
+      __syncthreads();
+      for (int i = k + 1 + tid; i < F; i += kCoarseThreads)
+        M[(size_t)i * F + k] /= M[(size_t)k * F + k];
+      __syncthreads();
+      const int m = F - k - 1; // update the trailing lower triangle, rows/cols k+1..F-1
+      for (int e = tid; e < m * m; e += kCoarseThreads)
+        {
+          const int i = k + 1 + e / m, j = k + 1 + e % m;
+          if (j <= i)
+            M[(size_t)i * F + j] -= M[(size_t)i * F + k] * M[(size_t)j * F + k];
+        }
+      __syncthreads();
+    }
+
+  // ---- one V(2,2) cycle: z = V(b0) into level 0's x ------------------------------
+  auto vcycle = [&](const double *b0) {
+    for (int l = 0; l + 1 < L; ++l)
+      {
+        const double *b = l == 0 ? b0 : w + g.off_b[l];
+        double *x = w + g.off_x[l], *t = w + g.off_t[l];
+        const double *d = w + g.off_d[l];
+        const int nn = (int)g.nn[l];
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          x[n] = kMgOmega * d[n] * b[n];
+        __syncthreads();
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+        __syncthreads();
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          x[n] += kMgOmega * d[n] * t[n];
+        __syncthreads();
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+        __syncthreads();
+        MgGrid<DIM> Gf{g.n[l]}, Gc{g.n[l + 1]};
+        double *bc = w + g.off_b[l + 1];
+        for (int c = tid; c < (int)g.nn[l + 1]; c += kCoarseThreads)
+          {
+            double sres = 0;
+            if (!masked(l + 1, c))
+              {
+                int I[3];
+                Gc.coords(c, I);
+                for (int o = 0; o < S; ++o)
+                  {
+                    const int od[3] = {o % 3 - 1, (o / 3) % 3 - 1, DIM == 3 ? o / 9 - 1 : 0};
+                    int i[3] = {2 * I[0] + od[0], 2 * I[1] + od[1], 2 * I[2] + od[2]};
+                    if (!Gf.inside(i))
+                      continue;
+                    double wi = 1;
+                    for (int k = 0; k < DIM; ++k)
+                      wi *= od[k] ? 0.5 : 1.0;
+                    sres += wi * t[Gf.index(i)];
+                  }
+              }
+            bc[c] = sres;
+          }
+        __syncthreads();
+      }
+    // coarsest: dense triangular solves by one thread
+    {
+      const double *b = lc == 0 ? b0 : w + g.off_b[lc];
+      double *x = w + g.off_x[lc], *y = w + g.off_t[lc];
+      if (tid == 0)
+        {
+          for (int n = 0; n < Nc; ++n)
+            if (dmap[n] >= 0)
+              y[(int)dmap[n]] = b[n];
+          for (int i = 0; i < F; ++i)
+            {
+              double v = y[i];
+              for (int j = 0; j < i; ++j)
+                v -= M[(size_t)i * F + j] * y[j];
+              y[i] = v / M[(size_t)i * F + i];
+            }
+          for (int i = F - 1; i >= 0; --i)
+            {
+              double v = y[i];
+              for (int j = i + 1; j < F; ++j)
+                v -= M[(size_t)j * F + i] * y[j];
+              y[i] = v / M[(size_t)i * F + i];
+            }
+        }
+      __syncthreads();
+      for (int n = tid; n < Nc; n += kCoarseThreads)
+        x[n] = dmap[n] >= 0 ? y[(int)dmap[n]] : 0.0;
+      __syncthreads();
+    }
+    for (int l = L - 2; l >= 0; --l)
+      {
+        const double *b = l == 0 ? b0 : w + g.off_b[l];
+        double *x = w + g.off_x[l], *t = w + g.off_t[l];
+        const double *d = w + g.off_d[l];
+        const double *xc = w + g.off_x[l + 1];
+        const int nn = (int)g.nn[l];
+        MgGrid<DIM> Gf{g.n[l]}, Gc{g.n[l + 1]};
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          {
+            if (masked(l, n))
+              continue;
+            int j[3];
+            Gf.coords(n, j);
+            int nodd = 0;
+            for (int k = 0; k < DIM; ++k)
+              nodd += j[k] & 1;
+            double v = 0;
+            for (int comb = 0; comb < (1 << nodd); ++comb)
+              {
+                int J[3] = {0, 0, 0}, bitpos = 0;
+                double wj = 1;
+                for (int k = 0; k < DIM; ++k)
+                  if (j[k] & 1)
+                    {
+                      J[k] = (j[k] >> 1) + ((comb >> bitpos) & 1);
+                      ++bitpos;
+                      wj *= 0.5;
+                    }
+                  else
+                    J[k] = j[k] >> 1;
+                v += wj * xc[Gc.index(J)];
+              }
+            x[n] += v;
+          }
+        __syncthreads();
+        for (int sweep = 0; sweep < 2; ++sweep)
+          {
+            for (int n = tid; n < nn; n += kCoarseThreads)
+              t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+            __syncthreads();
+            for (int n = tid; n < nn; n += kCoarseThreads)
+              x[n] += kMgOmega * d[n] * t[n];
+            __syncthreads();
+          }
+      }
+  };
+
+  // ---- CG on the constrained system (x = cval on the constrained nodes) -----------
+  for (int n = tid; n < nn0; n += kCoarseThreads)
+    {
+      cx[n] = cmask[n] ? cval[n] : 0.0;
+      cp[n] = 0.0;
+    }
+  __syncthreads();
+  double bb = 0;
+  for (int n = tid; n < nn0; n += kCoarseThreads)
+    {
+      double rv = 0.0;
+      if (!cmask[n]) // F - K x with the Dirichlet values in x (every neighbour counted)
+        {
+          int I[3];
+          G0.coords(n, I);
+          double s = 0;
+          for (int t = 0; t < S; ++t)
+            {
+              int J[3] = {I[0] + t % 3 - 1, I[1] + (t / 3) % 3 - 1, DIM == 3 ? I[2] + t / 9 - 1 : 0};
+              if (G0.inside(J))
+                s += Kc[(size_t)n * S + t] * cx[G0.index(J)];
+            }
+          rv = Fc[n] - s;
+        }
+      cr[n] = rv;
+      bb += rv * rv;
+    }
+  bb = block_sum(bb, red);
+  const double tol = rel_tol * sqrt(bb);
+  double rn = sqrt(bb), rz = 0;
+  int it = 0;
+  double *z = w + g.off_x[0];
+  if (rn > tol)
+    {
+      vcycle(cr);
+      for (int n = tid; n < nn0; n += kCoarseThreads)
+        {
+          cp[n] = z[n];
+          rz += cr[n] * z[n];
+        }
+      rz = block_sum(rz, red);
+    }
+  while (rn > tol && it < max_iter)
+    {
+      double pq = 0;
+      for (int n = tid; n < nn0; n += kCoarseThreads)
+        {
+          const double qv = apply(0, cp, n);
+          cq[n] = qv;
+          pq += cp[n] * qv;
+        }
+      pq = block_sum(pq, red);
+      const double alpha = rz / pq;
+      double rr = 0;
+      for (int n = tid; n < nn0; n += kCoarseThreads)
+        {
+          cx[n] += alpha * cp[n];
+          const double rv = cr[n] - alpha * cq[n];
+          cr[n] = rv;
+          rr += rv * rv;
+        }
+      rr = block_sum(rr, red);
+      rn = sqrt(rr);
+      ++it;
+      if (rn <= tol || it >= max_iter)
+        break;
+      vcycle(cr);
+      double rz_new = 0;
+      for (int n = tid; n < nn0; n += kCoarseThreads)
+        rz_new += cr[n] * z[n];
+      rz_new = block_sum(rz_new, red);
+      const double beta = rz_new / rz;
+      rz = rz_new;
+      for (int n = tid; n < nn0; n += kCoarseThreads)
+        cp[n] = z[n] + beta * cp[n];
+      __syncthreads();
+    }
+  __syncthreads();
+  for (int n = tid; n < nn0; n += kCoarseThreads)
+    u[n] = cx[n];
+  if (tid == 0)
+    *iters = it;
+}
+
 __global__ void reconstruct_kernel(int n_cells, int nb, const long long *bas_off, const long long *out_off,
                                    const int *n_nodes, const double *bases, const double *w, double *u)
 {
@@ -243,8 +662,48 @@ hipError_t launch_coarse_assemble(int dim, int nc, const double *Ae, const doubl
   return hipGetLastError();
 }
 
+MgLayout mg_layout(int dim, int nc)
+{
+  MgLayout g = {};
+  auto nodes = [dim](long long n) { return dim == 2 ? (n + 1) * (n + 1) : (n + 1) * (n + 1) * (n + 1); };
+  int n = nc, L = 0;
+  g.n[L++] = n;
+  while (n % 2 == 0 && n >= 4 && L < kMgMaxLevels)
+    {
+      n /= 2;
+      g.n[L++] = n;
+    }
+  if (nodes(g.n[L - 1]) > kMgMaxDense)
+    return MgLayout{}; // the grid does not coarsen to a small dense problem: no multigrid
+  g.levels = L;
+  const long long S = dim == 2 ? 9 : 27;
+  long long off = 0;
+  for (int l = 0; l < L; ++l)
+    g.nn[l] = nodes(g.n[l]);
+  g.off_cx = off, off += g.nn[0];
+  g.off_cr = off, off += g.nn[0];
+  g.off_cp = off, off += g.nn[0];
+  g.off_cq = off, off += g.nn[0];
+  for (int l = 0; l < L; ++l)
+    {
+      g.off_x[l] = off, off += g.nn[l];
+      g.off_t[l] = off, off += g.nn[l];
+      g.off_d[l] = off, off += g.nn[l];
+      g.off_b[l] = off, off += l > 0 ? g.nn[l] : 0; // level 0's right-hand side is the CG residual
+      g.off_A[l] = off, off += l > 0 ? g.nn[l] * S : 0; // level 0's operator is the coarse matrix itself
+    }
+  const long long nd = g.nn[L - 1];
+  g.off_dense = off, off += nd * nd;
+  g.off_dmap = off, off += nd;
+  g.total = off;
+  return g;
+}
+
 size_t coarse_solve_work_doubles(int dim, int nc)
 {
+  const MgLayout g = mg_layout(dim, nc);
+  if (g.levels > 0)
+    return (size_t)g.total;
   const size_t np = nc + 1;
   return 5 * (dim == 2 ? np * np : np * np * np);
 }
@@ -253,6 +712,17 @@ hipError_t launch_coarse_solve(int dim, int nc, const double *Kc, const double *
                                const double *cval, double rel_tol, int max_iter, double *u, int *iters,
                                double *work, hipStream_t st)
 {
+  const MgLayout g = mg_layout(dim, nc);
+  if (g.levels > 0)
+    {
+      if (dim == 2)
+        hipLaunchKernelGGL(coarse_mgcg_kernel<2>, dim3(1), dim3(kCoarseThreads), 0, st, g, nc, Kc, Fc, cmask, cval,
+                           rel_tol, max_iter, u, iters, work);
+      else
+        hipLaunchKernelGGL(coarse_mgcg_kernel<3>, dim3(1), dim3(kCoarseThreads), 0, st, g, nc, Kc, Fc, cmask, cval,
+                           rel_tol, max_iter, u, iters, work);
+      return hipGetLastError();
+    }
   if (dim == 2)
     hipLaunchKernelGGL(coarse_cg_kernel<2>, dim3(1), dim3(kCoarseThreads), 0, st, nc, Kc, Fc, cmask, cval, rel_tol,
                        max_iter, u, iters, work);

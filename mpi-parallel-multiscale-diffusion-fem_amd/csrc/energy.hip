@@ -1,0 +1,304 @@
+// Energy-norm control of the local solves (no counterpart in the reference,
+// whose local solves stop at an absolute residual of 1e-12,
+// include/base/diffusion_problem_basis.tpp:297).
+//
+// For basis i of a box cell with interior residual r of the computed phi:
+//   ||phi - phi*||_K^2 = r^T K_II^-1 r <= ||r||^2 / lam_lb,
+//   lam_lb = (min over the tile of lambda_min(A)) * mu_min   (K(A) >= a K(I)),
+//   mu_min = smallest eigenvalue of the unit-coefficient Q1 stiffness on the
+//            interior nodes: sum_a k_a prod_{b != a} m_b over the separable
+//            modes sin(j pi x), k = (2 - 2 cos t)/h, m = h (4 + 2 cos t)/6,
+//   ||phi*||_K^2 >= e_lb = e_unit^2 / D_i   (complementary energy with the
+//            divergence-free flux t grad N_i: N_i is multilinear, so harmonic),
+//   e_unit = int |grad N_i|^2,  D_i = int grad N_i^T A^-1 grad N_i,
+// where the discrete energy of a Q1 function equals the exact integral for
+// coefficients constant per fine cell (QGauss(2) is exact there) and bounds the
+// integral with A replaced by its smallest eigenvalue per cell for the
+// quadrature-point kinds.  energy_terms_kernel computes lam_lb per cell and,
+// for the stopping rule, ptol = tau sqrt(lam_lb e_lb) per problem;
+// energy_bound_kernel turns the final residual norms and A^K_ii = ||phi||_K^2
+// into the a posteriori bound sqrt(q / (A^K_ii - q)), q = ||r||^2 / lam_lb.
+#include "msfem_device.hpp"
+#include "msfem_kernels.hpp"
+
+namespace msfem {
+
+namespace {
+
+constexpr int kEnergyThreads = 256;
+
+// lambda_min of fine cell f and the entries of A^-1 (2D: b11, b12, b22; 3D:
+// the scalar 1/lambda on the diagonal) used in D_i.
+template <int DIM>
+__device__ __forceinline__ double cell_lambda(int kind, const double *cf, int f, double (&B)[3])
+{
+  constexpr int NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
+  double lam = 0;
+  if (kind == 0) // scalar per fine cell
+    lam = cf[f];
+  else if (kind == 2) // scalar at the quadrature points: their minimum
+    {
+      lam = cf[(size_t)f * NQ];
+      for (int q = 1; q < NQ; ++q)
+        lam = fmin(lam, cf[(size_t)f * NQ + q]);
+    }
+  else if (DIM == 2 && kind == 1) // SPD tensor per fine cell: exact inverse
+    {
+      const double a = cf[3 * f], b = cf[3 * f + 1], c = cf[3 * f + 2];
+      const double det = a * c - b * b, m = 0.5 * (a + c), d = sqrt(0.25 * (a - c) * (a - c) + b * b);
+      lam = det / (m + d); // the smaller root, without cancellation
+      B[0] = c / det;
+      B[1] = -b / det;
+      B[2] = a / det;
+      return lam;
+    }
+  else if (DIM == 2 && kind == 3) // tensor at the quadrature points: smallest eigenvalue
+    {
+      lam = 1e300;
+      for (int q = 0; q < NQ; ++q)
+        {
+          const double *t = cf + ((size_t)f * NQ + q) * NS;
+          const double a = t[0], b = t[1], c = t[2];
+          const double det = a * c - b * b, m = 0.5 * (a + c), d = sqrt(0.25 * (a - c) * (a - c) + b * b);
+          lam = fmin(lam, det / (m + d));
+        }
+    }
+  B[0] = 1.0 / lam;
+  B[1] = 0.0;
+  B[2] = 1.0 / lam;
+  return lam;
+}
+
+// Normalised integrals over fine interval j (of nf) of l^2 and l, l = t (bit 1)
+// or 1 - t (bit 0).
+__device__ __forceinline__ double jint2(int bit, int j, int nf)
+{
+  const int k = bit ? j : nf - 1 - j;
+  const double a = (double)k / nf, b = (double)(k + 1) / nf;
+  return (b * b * b - a * a * a) / 3.0;
+}
+__device__ __forceinline__ double jint1(int bit, int j, int nf)
+{
+  const int k = bit ? j : nf - 1 - j;
+  const double a = (double)k / nf, b = (double)(k + 1) / nf;
+  return 0.5 * (b * b - a * a);
+}
+
+// Fixed-order block reduction (sum or min) of kEnergyThreads values.
+__device__ __forceinline__ double block_reduce(double v, double *red, bool is_min)
+{
+  const int tid = threadIdx.x;
+  red[tid] = v;
+  __syncthreads();
+  for (int s = kEnergyThreads / 2; s > 0; s >>= 1)
+    {
+      if (tid < s)
+        red[tid] = is_min ? fmin(red[tid], red[tid + s]) : red[tid] + red[tid + s];
+      __syncthreads();
+    }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+// One workgroup per cell of a size class: lam_lb[cell] and, when tau > 0,
+// ptol[cell][i] = tau sqrt(lam_lb e_lb_i).
+template <int DIM>
+__global__ void __launch_bounds__(kEnergyThreads)
+  energy_terms_kernel(int n_cells_class, const int *cells, int nf, int kind, const double *box, const double *coef,
+                      const long long *coef_off, double tau, double *lam_lb, double *ptol)
+{
+  constexpr int NB = 1 << DIM;
+  __shared__ double red[kEnergyThreads];
+  const int k = blockIdx.x;
+  if (k >= n_cells_class)
+    return;
+  const int cell = cells[k];
+  const double *bx = box + (size_t)cell * 2 * DIM;
+  const double *cf = coef + coef_off[cell];
+  double H[3] = {1, 1, 1}, h[3] = {1, 1, 1};
+  for (int a = 0; a < DIM; ++a)
+    {
+      H[a] = bx[2 * a + 1] - bx[2 * a];
+      h[a] = H[a] / nf;
+    }
+  int nfd = nf * nf;
+  if (DIM == 3)
+    nfd *= nf;
+  double lam = 1e300, D[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    D[i] = 0;
+  for (int f = threadIdx.x; f < nfd; f += kEnergyThreads)
+    {
+      double B[3];
+      lam = fmin(lam, cell_lambda<DIM>(kind, cf, f, B));
+      const int j[3] = {f % nf, (f / nf) % nf, DIM == 3 ? f / (nf * nf) : 0};
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        {
+          int bit[3];
+          double s[3], J2[3], J1[3];
+          for (int a = 0; a < DIM; ++a)
+            {
+              bit[a] = (i >> a) & 1;
+              s[a] = bit[a] ? 1.0 : -1.0;
+              J2[a] = jint2(bit[a], j[a], nf);
+              J1[a] = jint1(bit[a], j[a], nf);
+            }
+          // G[k][m] = int over the fine cell of d_k N_i d_m N_i
+          double g = 0;
+          if (DIM == 2)
+            {
+              const double gxx = H[1] * J2[1] / (H[0] * nf), gyy = H[0] * J2[0] / (H[1] * nf);
+              const double gxy = s[0] * s[1] * J1[0] * J1[1];
+              g = B[0] * gxx + 2.0 * B[1] * gxy + B[2] * gyy;
+            }
+          else
+            {
+              // scalar coefficients only in 3D: A^-1 = I / lam
+              const double gxx = H[1] * H[2] * J2[1] * J2[2] / (H[0] * nf);
+              const double gyy = H[0] * H[2] * J2[0] * J2[2] / (H[1] * nf);
+              const double gzz = H[0] * H[1] * J2[0] * J2[1] / (H[2] * nf);
+              g = B[0] * (gxx + gyy + gzz);
+            }
+          D[i] += g;
+        }
+    }
+  // mu_min over the separable modes j = 1 .. nf - 1 per axis
+  double mu = 1e300;
+  const int nm = nf - 1;
+  const int nmodes = DIM == 2 ? nm * nm : nm * nm * nm;
+  for (int q = threadIdx.x; q < nmodes; q += kEnergyThreads)
+    {
+      const int jm[3] = {q % nm + 1, (q / nm) % nm + 1, DIM == 3 ? q / (nm * nm) + 1 : 1};
+      double kk[3], mm[3];
+      for (int a = 0; a < DIM; ++a)
+        {
+          const double c = cospi((double)jm[a] / nf);
+          kk[a] = (2.0 - 2.0 * c) / h[a];
+          mm[a] = h[a] * (4.0 + 2.0 * c) / 6.0;
+        }
+      double v;
+      if (DIM == 2)
+        v = kk[0] * mm[1] + mm[0] * kk[1];
+      else
+        v = kk[0] * mm[1] * mm[2] + mm[0] * kk[1] * mm[2] + mm[0] * mm[1] * kk[2];
+      mu = fmin(mu, v);
+    }
+  lam = block_reduce(lam, red, true);
+  mu = block_reduce(mu, red, true);
+  double Dsum[NB];
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    Dsum[i] = block_reduce(D[i], red, false);
+  const double llb = lam * mu;
+  if (threadIdx.x == 0)
+    {
+      lam_lb[cell] = llb;
+      if (ptol)
+        {
+          double vol = 1, inv = 0;
+          for (int a = 0; a < DIM; ++a)
+            {
+              vol *= H[a];
+              inv += 1.0 / (H[a] * H[a]);
+            }
+          const double e_unit = vol * inv / (DIM == 2 ? 3.0 : 9.0);
+          for (int i = 0; i < NB; ++i)
+            {
+              const double e_lb = e_unit * e_unit / Dsum[i];
+              ptol[(size_t)cell * NB + i] = tau * sqrt(llb * e_lb);
+            }
+        }
+    }
+}
+
+// One thread per (cell of the class, basis): the a posteriori bound.  With the
+// partition-of-unity option the last basis is 1 - sum of the others, whose
+// interior residual is minus the sum of theirs: its norm is bounded by the sum.
+__global__ void energy_bound_kernel(int n_cells_class, const int *cells, int nb, int pou, const double *lam_lb,
+                                    const double *resid, const double *Ae, double *bound)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_cells_class * nb)
+    return;
+  const int cell = cells[t / nb], i = t % nb;
+  const size_t o = (size_t)cell * nb;
+  double eta = resid[o + i];
+  if (pou && i == nb - 1)
+    {
+      eta = 0;
+      for (int j = 0; j < nb - 1; ++j)
+        eta += resid[o + j];
+    }
+  const double lam = lam_lb ? lam_lb[cell] : __builtin_nan("");
+  const double q = eta * eta / lam, e = Ae[o * nb + (size_t)i * nb + i];
+  bound[o + i] = e > q ? sqrt(q / (e - q)) : (lam > 0 ? __builtin_inf() : __builtin_nan(""));
+}
+
+__global__ void sub_kernel(size_t n, const double *x, const double *y, double *d)
+{
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    d[i] = x[i] - y[i];
+}
+
+// err[c][i] = sqrt(max(Ad_ii, 0) / Ar_ii) for every cell and basis.
+__global__ void energy_ratio_kernel(int n_cells, int nb, const double *Ad, const double *Ar, double *err)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_cells * nb)
+    return;
+  const size_t ii = (size_t)(t / nb) * nb * nb + (size_t)(t % nb) * (nb + 1);
+  err[t] = sqrt(fmax(Ad[ii], 0.0) / Ar[ii]);
+}
+
+} // namespace
+
+hipError_t launch_sub(size_t n, const double *x, const double *y, double *d, hipStream_t st)
+{
+  if (n == 0)
+    return hipSuccess;
+  size_t grid = (n + 255) / 256;
+  if (grid > 8192)
+    grid = 8192;
+  hipLaunchKernelGGL(sub_kernel, dim3((unsigned)grid), dim3(256), 0, st, n, x, y, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_ratio(int n_cells, int nb, const double *Ad, const double *Ar, double *err, hipStream_t st)
+{
+  const int n = n_cells * nb;
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(energy_ratio_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n_cells, nb, Ad, Ar, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_terms(int dim, int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                               const double *coef, const long long *coef_off, double tau, double *lam_lb,
+                               double *ptol, hipStream_t st)
+{
+  if (n_cells_class == 0)
+    return hipSuccess;
+  if (dim == 2)
+    hipLaunchKernelGGL(energy_terms_kernel<2>, dim3(n_cells_class), dim3(kEnergyThreads), 0, st, n_cells_class,
+                       cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
+  else
+    hipLaunchKernelGGL(energy_terms_kernel<3>, dim3(n_cells_class), dim3(kEnergyThreads), 0, st, n_cells_class,
+                       cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_bound(int n_cells_class, const int *cells, int nb, int pou, const double *lam_lb,
+                               const double *resid, const double *Ae, double *bound, hipStream_t st)
+{
+  const int n = n_cells_class * nb;
+  if (n == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(energy_bound_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n_cells_class, cells, nb, pou,
+                     lam_lb, resid, Ae, bound);
+  return hipGetLastError();
+}
+
+} // namespace msfem

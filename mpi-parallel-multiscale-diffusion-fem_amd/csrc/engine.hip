@@ -74,6 +74,10 @@ struct msfem_ctx {
   hipEvent_t scratch_done = nullptr;
   hipStream_t scratch_stream = nullptr;
   bool scratch_recorded = false;
+  // the current build computes the energy terms (stopping rule or bound), and
+  // whether its last basis comes from the partition of unity
+  bool energy_terms = false;
+  int energy_pou = 0;
 };
 
 namespace {
@@ -256,8 +260,11 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
                     in->n_fine[c]);
     }
   if (o->max_iter < 1 || o->max_refine < 1 || !(o->rel_tol >= 0) || !(o->abs_tol >= 0) ||
-      (o->rel_tol == 0 && o->abs_tol == 0))
+      (o->rel_tol == 0 && o->abs_tol == 0 && !(o->energy_tol > 0)) || !(o->energy_tol >= 0) ||
+      !(o->energy_tol < 1))
     return fail(ctx, MSFEM_ERR_ARG, "invalid tolerances or iteration caps");
+  if (o->energy_tol > 0 && in->vertices)
+    return fail(ctx, MSFEM_ERR_ARG, "the energy stopping rule needs box cells (its eigenvalue bound is for boxes)");
   for (int c = 0; c < in->n_cells; ++c)
     if (!nf_supported(in->dim, in->n_fine[c]))
       return fail(ctx, MSFEM_ERR_ARG, "cell %d: unsupported fine resolution %d for dim %d", c, in->n_fine[c],
@@ -442,6 +449,10 @@ int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solv
       HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
                                           d_off(ctx, 0), base + coarse_inv_doubles(ctx, in, o, k), st));
     }
+  if (ctx->energy_terms && !in->vertices)
+    HIP_TRY(ctx, launch_energy_terms(in->dim, ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0),
+                                     o->energy_tol, (double *)ctx->bufs["lam_lb"].p,
+                                     o->energy_tol > 0 ? (double *)ctx->bufs["ptol"].p : nullptr, st));
   return MSFEM_OK;
 }
 
@@ -483,6 +494,7 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.inner_rel_tol = o->inner_rel_tol;
   a.max_iter = o->max_iter;
   a.max_refine = o->max_refine;
+  a.ptol = o->energy_tol > 0 ? (const double *)ctx->bufs["ptol"].p : nullptr;
   const int variant = class_variant(ctx, in, o, nf);
   a.cinv = nullptr;
   if (class_coarse(ctx, in, o, nf))
@@ -521,6 +533,10 @@ int launch_class_element(msfem_ctx *ctx, const msfem_batch *in, msfem_batch_out 
   e.Ae = out->Ae;
   e.Fe = out->Fe;
   HIP_TRY(ctx, launch_element(e, st));
+  if (out->energy_bound)
+    HIP_TRY(ctx, launch_energy_bound(e.n_cells_class, e.cells, p.nb, ctx->energy_pou,
+                                     in->vertices ? nullptr : (const double *)ctx->bufs["lam_lb"].p, out->resid,
+                                     out->Ae, out->energy_bound, st));
   return MSFEM_OK;
 }
 
@@ -553,7 +569,9 @@ int run_graph(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o,
   add(in->rhs_value), add(in->x0), add(in->vertices);
   add(o->precision), add(o->max_iter), add(o->max_refine), add(o->partition_of_unity), add(o->rel_tol);
   add(o->abs_tol), add(o->inner_rel_tol), add(o->variant), add(o->coarse_space), add(o->q1_start);
+  add(o->energy_tol);
   add(out->Ae), add(out->Fe), add(out->bases), add(out->iters), add(out->status), add(out->resid);
+  add(out->energy_bound);
   add(bases), add(st);
   auto it = ctx->graphs.find(key);
   if (it == ctx->graphs.end())
@@ -680,6 +698,26 @@ int enqueue_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
     {
       void *d;
       rc = scratch(ctx, "coarse_inv", cinv_doubles * sizeof(double), &d);
+      if (rc)
+        return rc;
+    }
+  // energy terms (stopping rule and/or a posteriori bound; allocated before any capture)
+  msfem_batch_out eout = *out;
+  out = &eout;
+  ctx->energy_terms = (o->energy_tol > 0 || out->energy_bound) && !in->vertices;
+  ctx->energy_pou = o->partition_of_unity != 0;
+  if (ctx->energy_terms)
+    {
+      void *d;
+      rc = scratch(ctx, "lam_lb", (size_t)in->n_cells * sizeof(double), &d);
+      if (rc == MSFEM_OK && o->energy_tol > 0)
+        rc = scratch(ctx, "ptol", (size_t)in->n_cells * p.nb * sizeof(double), &d);
+      if (rc)
+        return rc;
+    }
+  if (out->energy_bound && !out->resid) // the bound needs the final residual norms
+    {
+      rc = scratch(ctx, "resid_e", (size_t)in->n_cells * p.nb * sizeof(double), (void **)&eout.resid);
       if (rc)
         return rc;
     }
@@ -1090,6 +1128,7 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   dout.iters = s.alloc<int32_t>("h_iters", (size_t)n * nb, true);
   dout.status = s.alloc<int32_t>("h_status", (size_t)n * nb, true);
   dout.resid = s.alloc<double>("h_resid", (size_t)n * nb, out->resid != nullptr);
+  dout.energy_bound = s.alloc<double>("h_ebound", (size_t)n * nb, out->energy_bound != nullptr);
   if (s.rc)
     return s.rc;
   rc = build_device(ctx, &din, opts, &dout, ctx->stream);
@@ -1102,6 +1141,7 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   s.down(out->iters, dout.iters, (size_t)n * nb);
   s.down(status.data(), dout.status, (size_t)n * nb);
   s.down(out->resid, dout.resid, (size_t)n * nb);
+  s.down(out->energy_bound, dout.energy_bound, (size_t)n * nb);
   if (s.rc)
     return s.rc;
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -1113,6 +1153,54 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   if (bad)
     return fail(ctx, MSFEM_ERR_NOCONV, "%d basis solve(s) did not converge", bad);
   return MSFEM_OK;
+}
+
+int msfem_energy_error_device(msfem_ctx *ctx, const msfem_batch *in, const double *bases, const double *bases_ref,
+                              double *err, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  msfem_solver_opts o;
+  msfem_default_opts(&o);
+  int rc = validate(ctx, in, &o);
+  if (rc)
+    return rc;
+  if (in->n_cells == 0)
+    return MSFEM_OK;
+  if (!bases || !bases_ref || !err)
+    return fail(ctx, MSFEM_ERR_ARG, "bases, bases_ref and err are required");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = (hipStream_t)stream;
+  bool capturing = false;
+  rc = order_after_scratch(ctx, st, &capturing);
+  if (rc == MSFEM_OK)
+    rc = make_plan(ctx, in);
+  if (rc)
+    return rc;
+  Plan &p = ctx->plan;
+  const size_t nae = (size_t)in->n_cells * p.nb * p.nb;
+  double *diff, *Ad, *Ar, *Fs;
+  rc = scratch(ctx, "ee_diff", p.bas_total * sizeof(double), (void **)&diff);
+  if (rc == MSFEM_OK)
+    rc = scratch(ctx, "ee_Ae", 2 * nae * sizeof(double), (void **)&Ad);
+  if (rc == MSFEM_OK)
+    rc = scratch(ctx, "ee_Fe", (size_t)in->n_cells * p.nb * sizeof(double), (void **)&Fs);
+  if (rc)
+    return rc;
+  Ar = Ad + nae;
+  HIP_TRY(ctx, launch_sub((size_t)p.bas_total, bases, bases_ref, diff, st));
+  for (size_t k = 0; k < p.class_nf.size(); ++k)
+    for (int pass = 0; pass < 2; ++pass)
+      {
+        msfem_batch_out eo = {};
+        eo.Ae = pass ? Ar : Ad;
+        eo.Fe = Fs;
+        rc = launch_class_element(ctx, in, &eo, k, pass ? (double *)bases_ref : diff, nullptr, st);
+        if (rc)
+          return rc;
+      }
+  HIP_TRY(ctx, launch_energy_ratio(in->n_cells, p.nb, Ad, Ar, err, st));
+  return retire_scratch(ctx, st, capturing);
 }
 
 int msfem_coarse_assemble_device(msfem_ctx *ctx, int dim, int nc, const double *Ae, const double *Fe, int bc_kind,

@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         }
       problem_sums<double, LW, SW, PPW, 1>(s, false, sred64, wave, slot, pl);
       bnorm = sqrt(s[0]);
-      tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+      tol_eff = a.ptol ? a.ptol[(size_t)cell * a.nb + basis] : fmax(a.abs_tol, a.rel_tol * bnorm);
       __syncthreads();
     }
 
@@ -604,7 +604,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       if (cyc == 0 && !given_start)
         {
           bnorm = rnorm;
-          tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+          tol_eff = a.ptol ? a.ptol[(size_t)cell * a.nb + basis] : fmax(a.abs_tol, a.rel_tol * bnorm);
         }
       if (state == 0)
         {

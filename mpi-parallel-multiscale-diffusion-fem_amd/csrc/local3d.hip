@@ -184,7 +184,7 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
           }
       problem_sums<double, kWave, NW, 1, 1>(s, false, sred64, wave, 0, lane);
       bnorm = sqrt(s[0]);
-      tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+      tol_eff = a.ptol ? a.ptol[(size_t)cell * a.nb + basis] : fmax(a.abs_tol, a.rel_tol * bnorm);
       __syncthreads();
     }
 
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(NF *NF *SZ) local_solve_3d(LocalArgs a)
       if (cyc == 0 && !given_start)
         {
           bnorm = rnorm;
-          tol_eff = fmax(a.abs_tol, a.rel_tol * bnorm);
+          tol_eff = a.ptol ? a.ptol[(size_t)cell * a.nb + basis] : fmax(a.abs_tol, a.rel_tol * bnorm);
         }
       if (state == 0)
         {

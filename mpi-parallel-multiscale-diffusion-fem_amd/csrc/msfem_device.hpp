@@ -65,6 +65,8 @@ struct LocalArgs {
   int *iters;                 // [n_cells][nb] or nullptr
   int *status;                // [n_cells][nb] or nullptr
   double *resid;              // [n_cells][nb] or nullptr
+  const double *ptol;         // [n_cells][nb] absolute residual tolerance per problem (energy
+                              // stopping rule) or nullptr: max(abs_tol, rel_tol ||b||)
   double rel_tol, abs_tol, inner_rel_tol;
   int max_iter, max_refine;
 };

@@ -55,7 +55,35 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st);
 hipError_t launch_pou_fixup(int n_cells_class, const int *cells, int dim, int nf, double *bases,
                             const long long *bas_off, hipStream_t st);
 
+// energy.hip: energy-norm control of the local solves (box cells).  lam_lb
+// [n_cells]: lower bound of lambda_min(K_II); ptol [n_cells][nb] (or nullptr):
+// tau sqrt(lam_lb e_lb), the residual norm that guarantees a relative
+// energy-norm error <= tau; bound [n_cells][nb]: the a posteriori bound from
+// the final residual norms and A^K_ii (lam_lb nullptr: NaN).
+hipError_t launch_energy_terms(int dim, int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                               const double *coef, const long long *coef_off, double tau, double *lam_lb,
+                               double *ptol, hipStream_t st);
+hipError_t launch_energy_bound(int n_cells_class, const int *cells, int nb, int pou, const double *lam_lb,
+                               const double *resid, const double *Ae, double *bound, hipStream_t st);
+// d = x - y (n doubles); err[c][i] = sqrt(max(Ad_ii, 0) / Ar_ii)
+hipError_t launch_sub(size_t n, const double *x, const double *y, double *d, hipStream_t st);
+hipError_t launch_energy_ratio(int n_cells, int nb, const double *Ad, const double *Ar, double *err, hipStream_t st);
+
 // coarse.hip
+// Scratch layout of the multigrid-preconditioned coarse CG (offsets in doubles).
+constexpr int kMgMaxLevels = 12;
+constexpr int kMgMaxDense = 343; // largest coarsest level solved densely (7^3 or 18^2 nodes)
+struct MgLayout {
+  int levels;                        // 0: no multigrid for this size (Jacobi-PCG instead)
+  int n[kMgMaxLevels];               // cells per side of level l (n[0] = nc)
+  long long nn[kMgMaxLevels];        // nodes of level l
+  long long off_x[kMgMaxLevels], off_t[kMgMaxLevels], off_d[kMgMaxLevels], off_b[kMgMaxLevels],
+    off_A[kMgMaxLevels];
+  long long off_cx, off_cr, off_cp, off_cq; // CG vectors on level 0
+  long long off_dense, off_dmap;     // coarsest Cholesky factor [F][F], free-node numbers
+  long long total;
+};
+MgLayout mg_layout(int dim, int nc);
 hipError_t launch_coarse_assemble(int dim, int nc, const double *Ae, const double *Fe, int bc_kind, double *Kc,
                                   double *Fc, unsigned char *cmask, double *cval, hipStream_t st);
 hipError_t launch_coarse_solve(int dim, int nc, const double *Kc, const double *Fc, const unsigned char *cmask,

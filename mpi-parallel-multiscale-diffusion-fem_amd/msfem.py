@@ -29,7 +29,7 @@ EXPORTS = [
     "msfem_uses_coarse_space",
     "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_ctx_aux_stream", "msfem_ctx_mark_on", "msfem_ctx_wait_mark", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
     "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
-    "msfem_batch_build", "msfem_batch_build_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
+    "msfem_batch_build", "msfem_batch_build_device", "msfem_energy_error_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
@@ -50,8 +50,8 @@ class SolverOpts(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int32), ("max_iter", ctypes.c_int32), ("max_refine", ctypes.c_int32),
                 ("partition_of_unity", ctypes.c_int32), ("rel_tol", ctypes.c_double), ("abs_tol", ctypes.c_double),
                 ("inner_rel_tol", ctypes.c_double), ("variant", ctypes.c_int32), ("graph", ctypes.c_int32),
-                ("coarse_space", ctypes.c_int32), ("q1_start", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 4)]
+                ("coarse_space", ctypes.c_int32), ("q1_start", ctypes.c_int32), ("energy_tol", ctypes.c_double),
+                ("reserved", ctypes.c_int32 * 2)]
 
 
 class Batch(ctypes.Structure):
@@ -63,7 +63,8 @@ class Batch(ctypes.Structure):
 
 class BatchOut(ctypes.Structure):
     _fields_ = [("Ae", ctypes.c_void_p), ("Fe", ctypes.c_void_p), ("bases", ctypes.c_void_p),
-                ("iters", ctypes.c_void_p), ("status", ctypes.c_void_p), ("resid", ctypes.c_void_p)]
+                ("iters", ctypes.c_void_p), ("status", ctypes.c_void_p), ("resid", ctypes.c_void_p),
+                ("energy_bound", ctypes.c_void_p)]
 
 
 _lib = None
@@ -102,6 +103,7 @@ def lib():
         L.msfem_batch_build.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts), ctypes.POINTER(BatchOut)]
         L.msfem_batch_build_device.argtypes = [vp, ctypes.POINTER(Batch), ctypes.POINTER(SolverOpts),
                                                ctypes.POINTER(BatchOut), vp]
+        L.msfem_energy_error_device.argtypes = [vp, ctypes.POINTER(Batch), vp, vp, vp, vp]
         L.msfem_coarse_assemble.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, vp, vp]
         L.msfem_coarse_assemble_device.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, vp, vp, vp]
         L.msfem_coarse_solve.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, i32, vp, vp]
@@ -249,8 +251,10 @@ class Context:
 
     # ---- the hot path, host arrays ------------------------------------------------
     def batch_build(self, dim, n_fine, box, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, x0=None,
-                    opts=None, want_bases=True, raise_noconv=True, vertices=None):
-        """vertices: [n][4][2] general quadrilaterals (2D; box is then ignored, may be None)."""
+                    opts=None, want_bases=True, raise_noconv=True, vertices=None, energy_bound=False):
+        """vertices: [n][4][2] general quadrilaterals (2D; box is then ignored, may be None).
+        energy_bound: also return the a posteriori bound of each basis's relative
+        energy-norm error ("energy_bound", [n][nb])."""
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         n = int(n_fine.shape[0])
         nb = 1 << dim
@@ -261,10 +265,11 @@ class Context:
         opts = opts or default_opts()
         out = {"Ae": np.zeros((n, nb, nb)), "Fe": np.zeros((n, nb)), "iters": np.zeros((n, nb), np.int32),
                "status": np.zeros((n, nb), np.int32), "resid": np.zeros((n, nb)),
-               "bases": np.zeros(bases_size(dim, n_fine)) if want_bases else None}
+               "bases": np.zeros(bases_size(dim, n_fine)) if want_bases else None,
+               "energy_bound": np.zeros((n, nb)) if energy_bound else None}
         b = Batch(dim, n, _p(n_fine), _p(box), coef_kind, _p(coef), rhs_kind, rhs_value, _p(x0), _p(vertices))
         o = BatchOut(_p(out["Ae"]), _p(out["Fe"]), _p(out["bases"]), _p(out["iters"]), _p(out["status"]),
-                     _p(out["resid"]))
+                     _p(out["resid"]), _p(out["energy_bound"]))
         rc = lib().msfem_batch_build(self.h, ctypes.byref(b), ctypes.byref(opts), ctypes.byref(o))
         if rc == ERR_NOCONV and not raise_noconv:
             return out
@@ -274,14 +279,22 @@ class Context:
     # ---- the hot path, device pointers (e.g. torch tensors' data_ptr()) --------------
     def batch_build_device(self, dim, n_fine, box_ptr, coef_kind, coef_ptr, Ae_ptr, Fe_ptr, bases_ptr=None,
                            iters_ptr=None, status_ptr=None, resid_ptr=None, rhs_kind=RHS_CONST, rhs_value=1.0,
-                           x0_ptr=None, opts=None, stream=None, vertices_ptr=None):
+                           x0_ptr=None, opts=None, stream=None, vertices_ptr=None, energy_bound_ptr=None):
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
         opts = opts or default_opts()
         b = Batch(dim, int(n_fine.shape[0]), _p(n_fine), box_ptr, coef_kind, coef_ptr, rhs_kind, rhs_value, x0_ptr,
                   vertices_ptr)
-        o = BatchOut(Ae_ptr, Fe_ptr, bases_ptr, iters_ptr, status_ptr, resid_ptr)
+        o = BatchOut(Ae_ptr, Fe_ptr, bases_ptr, iters_ptr, status_ptr, resid_ptr, energy_bound_ptr)
         _check(lib().msfem_batch_build_device(self.h, ctypes.byref(b), ctypes.byref(opts), ctypes.byref(o), stream),
                self.h)
+
+    def energy_error_device(self, dim, n_fine, box_ptr, coef_kind, coef_ptr, bases_ptr, ref_ptr, err_ptr,
+                            stream=None, vertices_ptr=None):
+        """err[c][i] = ||phi_i - phi_ref,i||_K / ||phi_ref,i||_K for every basis (device pointers)."""
+        n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
+        b = Batch(dim, int(n_fine.shape[0]), _p(n_fine), box_ptr, coef_kind, coef_ptr, RHS_CONST, 0.0, None,
+                  vertices_ptr)
+        _check(lib().msfem_energy_error_device(self.h, ctypes.byref(b), bases_ptr, ref_ptr, err_ptr, stream), self.h)
 
     # ---- coarse system -----------------------------------------------------------------
     def coarse_assemble(self, dim, nc, Ae, Fe, bc_kind):

@@ -317,3 +317,103 @@ def batch_build_quad(quad, nf, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.
                                        _ptr(bases), _ptr(Ae), _ptr(Fe), _ptr(its, _ip))
     return {"Ae": Ae.reshape(n, 4, 4), "Fe": Fe.reshape(n, 4), "iters": its.reshape(n, 4), "bases": bases,
             "fails": fails}
+
+
+# ---------------------------------------------------------------------------
+# A posteriori energy-norm bound of a local solve (no counterpart in the
+# reference, whose only stopping rule is an absolute residual,
+# include/base/diffusion_problem_basis.tpp:297).  For the interior equations
+# K_II x = b of basis i with computed phi (boundary values exact) and interior
+# residual r:
+#   ||phi - phi*||_K^2 = r^T K_II^-1 r <= ||r||^2 / lam_lb,
+#   lam_lb = min over the tile of lambda_min(A) * mu_min (Loewner: K(A) >= a K(I)),
+#   ||phi*||_K^2 >= e_unit^2 / D_i  (complementary energy with the divergence-free
+#   flux t grad N_i: N_i is multilinear, hence harmonic),
+#   e_unit = int |grad N_i|^2,  D_i = int grad N_i^T A^-1 grad N_i.
+# ---------------------------------------------------------------------------
+def unit_stiffness_lambda_min(dim, nf, H):
+    """Smallest eigenvalue of the unit-coefficient Q1 stiffness on the interior
+    nodes of a box with side lengths H and nf fine cells per side.  The operator
+    is a sum of tensor products of the 1D stiffness and mass, whose common
+    eigenvectors are sin(j pi x); eigenvalue sum_a k_a prod_{b != a} m_b with
+    k = (2 - 2 cos t)/h, m = h (4 + 2 cos t)/6, t = j pi / nf per axis.  The
+    minimum over all j is taken (on strongly stretched boxes it need not sit at
+    j = 1 on every axis)."""
+    c = np.cos(np.arange(1, nf) * np.pi / nf)
+    h = np.asarray(H, dtype=np.float64) / nf
+    k = [(2 - 2 * c) / h[a] for a in range(dim)]
+    m = [h[a] * (4 + 2 * c) / 6 for a in range(dim)]
+    grids = np.meshgrid(*[np.arange(nf - 1)] * dim, indexing="ij")
+    lam = sum(k[a][grids[a]] * np.prod([m[b][grids[b]] for b in range(dim) if b != a], axis=0) for a in range(dim))
+    return float(lam.min())
+
+
+def _cell_lambda_and_inverse(dim, kind, cdata):
+    """(smallest eigenvalue over the cell's values, A^-1 to use in D) for one fine cell."""
+    nq = 1 << dim
+    if kind == COEF_SCALAR_CELL:
+        a = float(cdata[0])
+        return a, np.eye(dim) / a
+    if kind == COEF_SCALAR_QP:
+        a = float(np.min(cdata[:nq]))
+        return a, np.eye(dim) / a
+    ns = 3 if dim == 2 else 6
+    idx = [(0, 0), (0, 1), (1, 1)] if dim == 2 else [(0, 0), (0, 1), (0, 2), (1, 1), (1, 2), (2, 2)]
+
+    def full(v):
+        A = np.zeros((dim, dim))
+        for (i, j), x in zip(idx, v):
+            A[i, j] = A[j, i] = x
+        return A
+    if kind == COEF_TENSOR_CELL:
+        A = full(cdata[:ns])
+        return float(np.linalg.eigvalsh(A)[0]), np.linalg.inv(A)
+    lam = min(float(np.linalg.eigvalsh(full(cdata[q * ns:(q + 1) * ns]))[0]) for q in range(nq))
+    return lam, np.eye(dim) / lam
+
+
+def energy_bound_terms(dim, nf, box, coef_kind, tile):
+    """-> (lam_lb, e_lb[nb]): the lower bounds above for one box cell."""
+    box = _d64(box).reshape(dim, 2)
+    H = box[:, 1] - box[:, 0]
+    stride = coef_stride(dim, coef_kind)
+    tile = _d64(tile).reshape(-1)
+    nb = 1 << dim
+    t = np.arange(nf + 1) / nf
+    # per axis and fine index j: int over the cell of l^2 and of l, l = t or 1 - t (normalised)
+    J0 = {1: (t[1:] ** 3 - t[:-1] ** 3) / 3, 0: ((1 - t[:-1]) ** 3 - (1 - t[1:]) ** 3) / 3}
+    J1 = {1: (t[1:] ** 2 - t[:-1] ** 2) / 2, 0: ((1 - t[:-1]) ** 2 - (1 - t[1:]) ** 2) / 2}
+    vol = float(np.prod(H))
+    e_unit = vol * sum(1.0 / (3 ** (dim - 1) * H[k] ** 2) for k in range(dim))
+    lam_min, D = np.inf, np.zeros(nb)
+    for f in range(nf ** dim):
+        j = [f % nf, (f // nf) % nf, f // (nf * nf)][:dim]
+        lam, B = _cell_lambda_and_inverse(dim, coef_kind, tile[f * stride:(f + 1) * stride])
+        lam_min = min(lam_min, lam)
+        for i in range(nb):
+            bit = [(i >> k) & 1 for k in range(dim)]
+            s = [1.0 if b else -1.0 for b in bit]
+            # G[k][m] = int_cell d_k N_i d_m N_i (exact for the multilinear N_i)
+            G = np.zeros((dim, dim))
+            for k in range(dim):
+                for m in range(dim):
+                    v = s[k] * s[m] / (H[k] * H[m])
+                    for a in range(dim):
+                        if a == k and a == m:
+                            v *= H[a] / nf
+                        elif a == k or a == m:
+                            v *= H[a] * J1[bit[a]][j[a]]
+                        else:
+                            v *= H[a] * J0[bit[a]][j[a]]
+                    G[k, m] = v
+            D[i] += float(np.sum(B * G))
+    lam_lb = lam_min * unit_stiffness_lambda_min(dim, nf, H)
+    return lam_lb, e_unit ** 2 / D
+
+
+def energy_bound(lam_lb, e_ref, eta):
+    """Bound of ||phi - phi*||_K / ||phi*||_K from the residual norm eta and
+    e_ref = ||phi||_K^2 of the computed phi (A^K_ii): ||phi*||^2 = ||phi||^2 - ||e||^2."""
+    err2 = np.asarray(eta, dtype=np.float64) ** 2 / lam_lb
+    den = np.asarray(e_ref, dtype=np.float64) - err2
+    return np.where(den > 0, np.sqrt(err2 / np.maximum(den, 1e-300)), np.inf)

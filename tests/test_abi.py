@@ -45,6 +45,7 @@ int main(void) {
          offsetof(msfem_batch, rhs_value));
   printf("%zu %zu %zu\\n", offsetof(msfem_batch, x0), offsetof(msfem_batch, vertices), offsetof(msfem_batch_out, resid));
   printf("%zu %zu\\n", offsetof(msfem_solver_opts, coarse_space), offsetof(msfem_solver_opts, q1_start));
+  printf("%zu %zu\\n", offsetof(msfem_solver_opts, energy_tol), offsetof(msfem_batch_out, energy_bound));
   return 0;
 }
 """)
@@ -55,14 +56,15 @@ int main(void) {
     assert vals == [C.sizeof(msfem.SolverOpts), C.sizeof(msfem.Batch), C.sizeof(msfem.BatchOut),
                     msfem.SolverOpts.rel_tol.offset, msfem.SolverOpts.variant.offset, msfem.Batch.rhs_value.offset,
                     msfem.Batch.x0.offset, msfem.Batch.vertices.offset, msfem.BatchOut.resid.offset,
-                    msfem.SolverOpts.coarse_space.offset, msfem.SolverOpts.q1_start.offset]
+                    msfem.SolverOpts.coarse_space.offset, msfem.SolverOpts.q1_start.offset,
+                    msfem.SolverOpts.energy_tol.offset, msfem.BatchOut.energy_bound.offset]
 
 
 def test_default_opts_and_version():
     o = msfem.default_opts()
     assert o.precision == msfem.PREC_FP64 and o.rel_tol == 1e-13 and o.max_iter > 1000
-    assert o.coarse_space == 1 and o.q1_start == 1
-    assert msfem.lib().msfem_version() == 2
+    assert o.coarse_space == 1 and o.q1_start == 1 and o.energy_tol == 0
+    assert msfem.lib().msfem_version() == 3
 
 
 def test_no_device_is_a_clean_error():

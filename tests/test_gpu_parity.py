@@ -395,3 +395,22 @@ def test_async_builds_on_two_streams_with_layout_change(ctx):
         assert np.array_equal(d["it"].numpy().reshape(w["iters"].shape), w["iters"])
         for arr in d.values():
             arr.free()
+
+
+@pytest.mark.parametrize("nc,bc", [(128, msfem.BC_ZERO_DIRICHLET), (128, msfem.BC_REFERENCE), (12, msfem.BC_REFERENCE),
+                                   (25, msfem.BC_ZERO_DIRICHLET)])
+def test_coarse_solve_sizes(ctx, nc, bc):
+    """The coarse CG (multigrid-preconditioned where the grid coarsens to a small
+    dense level: 128 -> 64 ... 2, 12 -> 6 -> 3; Jacobi-preconditioned for 25)
+    against the oracle's direct solve of the same coarse system (1e-10), on
+    the A^K, F^K of nc x nc lognormal cells of 16^2 fine cells."""
+    say(f"coarse solve nc={nc} bc={bc}")
+    case = C.lognormal_cells(dim=2, nc=nc, nf=16, seed=7)
+    out = C.gpu_build(ctx, case)
+    Kc, Fc, cm, cv = ctx.coarse_assemble(2, nc, out["Ae"], out["Fe"], bc)
+    u, it = ctx.coarse_solve(2, nc, Kc, Fc, cm, cv, rel_tol=1e-14)
+    ur = O.coarse_solve(2, nc, Kc, Fc, cm, cv)
+    say(f"  coarse CG iterations {it}")
+    assert rel(u, ur) < TOL, it
+    if nc != 25:
+        assert it <= 30  # multigrid preconditioning
