@@ -293,6 +293,22 @@ int msfem_coarse_solve_device(msfem_ctx *ctx, int dim, int nc, const double *Kc,
                               const uint8_t *cmask, const double *cval, double rel_tol, int max_iter,
                               double *u, int32_t *iters, void *stream);
 
+/* The same solve with no device allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The kernels of
+ * msfem_coarse_solve_device run with their scratch in a fixed 16 MiB device
+ * array of the library (one per GPU, held until the process ends, its address
+ * looked up when a context is created on that GPU); the call enqueues one
+ * kernel on `stream` and calls no hipMalloc, hipFree or other allocating
+ * routine.  Systems whose scratch exceeds the array (for power-of-two sizes:
+ * 2D beyond 256 x 256 cells, 3D beyond 32^3) are rejected with MSFEM_ERR_ARG.  Every call on one
+ * GPU shares that GPU's array, so calls that overlap on one GPU (different
+ * streams, no ordering between them) race; the kernels write every scratch
+ * value before reading it, so nothing needs clearing between calls. */
+int msfem_coarse_solve_device_noalloc(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc,
+                                      const uint8_t *cmask, const double *cval, double rel_tol, int max_iter,
+                                      double *u, int32_t *iters, void *stream);
+
 /* Per-cell weights w[c][i] = u[global node of vertex i of cell c]
  * (send_global_weights_to_cell, diffusion_problem_ms.tpp:301-325).  Host pointers. */
 int msfem_gather_weights(int dim, int nc, const double *u, double *w);

@@ -864,6 +864,10 @@ int msfem_ctx_create(int device, msfem_ctx **out)
   e = hipSetDevice(device);
   if (e != hipSuccess)
     return fail(nullptr, MSFEM_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+  // this device's copy of the allocation-free entry point's scratch (once per device)
+  e = noalloc_register_device(device);
+  if (e != hipSuccess)
+    return fail(nullptr, MSFEM_ERR_HIP, "hipGetSymbolAddress (fixed scratch): %s", hipGetErrorString(e));
   msfem_ctx *ctx = new msfem_ctx;
   ctx->device = device;
   e = hipStreamCreate(&ctx->stream);
@@ -1268,6 +1272,37 @@ int msfem_coarse_solve_device(msfem_ctx *ctx, int dim, int nc, const double *Kc,
   HIP_TRY(ctx, launch_coarse_solve(dim, nc, Kc, Fc, cmask, cval, rel_tol, max_iter, u, iters, work,
                                    (hipStream_t)stream));
   return retire_scratch(ctx, (hipStream_t)stream, capturing);
+}
+
+int msfem_coarse_solve_device_noalloc(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc,
+                                      const uint8_t *cmask, const double *cval, double rel_tol, int max_iter,
+                                      double *u, int32_t *iters, void *stream)
+{
+  if (!ctx)
+    return fail(nullptr, MSFEM_ERR_ARG, "ctx is NULL");
+  if ((dim != 2 && dim != 3) || nc < 1 || !(rel_tol > 0) || max_iter < 1 || !Kc || !Fc || !cmask || !cval || !u ||
+      !iters)
+    return fail(ctx, MSFEM_ERR_ARG, "invalid coarse solve arguments");
+  if ((dim == 2 && nc > 4096) || (dim == 3 && nc > 256))
+    return fail(ctx, MSFEM_ERR_ARG, "coarse system of %d^%d cells exceeds the fixed scratch of %zu bytes", nc, dim,
+                noalloc_scratch_bytes());
+  const size_t need = coarse_solve_work_doubles(dim, nc) * sizeof(double);
+  if (need > noalloc_scratch_bytes())
+    return fail(ctx, MSFEM_ERR_ARG,
+                "coarse system of %d^%d cells needs %zu bytes of scratch, more than the fixed %zu of the "
+                "allocation-free entry point (msfem_coarse_solve_device has no such limit)",
+                nc, dim, need, noalloc_scratch_bytes());
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  double *w = noalloc_scratch(ctx->device);
+  if (!w)
+    {
+      HIP_TRY(ctx, noalloc_register_device(ctx->device));
+      w = noalloc_scratch(ctx->device);
+    }
+  if (!w)
+    return fail(ctx, MSFEM_ERR_HIP, "fixed scratch of device %d unavailable", ctx->device);
+  HIP_TRY(ctx, launch_coarse_solve(dim, nc, Kc, Fc, cmask, cval, rel_tol, max_iter, u, iters, w, (hipStream_t)stream));
+  return MSFEM_OK;
 }
 
 int msfem_coarse_solve(msfem_ctx *ctx, int dim, int nc, const double *Kc, const double *Fc, const uint8_t *cmask,

@@ -93,6 +93,13 @@ size_t coarse_solve_work_doubles(int dim, int nc);
 hipError_t launch_reconstruct(int n_cells, int nb, const long long *bas_off, const long long *out_off,
                               const int *n_nodes, const double *bases, const double *w, double *u, hipStream_t st);
 
+// noalloc.hip: the fixed device scratch of the allocation-free entry point
+// (one copy per GPU).  noalloc_register_device looks up this device's address
+// (the device must be current); noalloc_scratch returns it (nullptr before).
+hipError_t noalloc_register_device(int device);
+double *noalloc_scratch(int device);
+size_t noalloc_scratch_bytes();
+
 // fine.hip: fine-scale standard FEM (element matrices + multi-workgroup FP64 CG
 // on a structured stencil matrix, scalars in device memory).
 struct FineCgState {

@@ -30,7 +30,7 @@ EXPORTS = [
     "msfem_ctx_set_profiling", "msfem_ctx_profile", "msfem_ctx_mark", "msfem_ctx_elapsed", "msfem_ctx_aux_stream", "msfem_ctx_mark_on", "msfem_ctx_wait_mark", "msfem_dev_alloc", "msfem_dev_free", "msfem_memcpy_h2d",
     "msfem_memcpy_d2h", "msfem_ctx_stream", "msfem_ctx_synchronize", "msfem_variant_count", "msfem_set_variant",
     "msfem_batch_build", "msfem_batch_build_device", "msfem_energy_error_device", "msfem_coarse_assemble", "msfem_coarse_assemble_device",
-    "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_gather_weights", "msfem_reconstruct_device",
+    "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_coarse_solve_device_noalloc", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
     "msfem_gen_perturbed_quads", "msfem_gen_coef_reference_quad",
@@ -108,6 +108,7 @@ def lib():
         L.msfem_coarse_assemble_device.argtypes = [vp, i32, i32, vp, vp, i32, vp, vp, vp, vp, vp]
         L.msfem_coarse_solve.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, i32, vp, vp]
         L.msfem_coarse_solve_device.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, i32, vp, vp, vp]
+        L.msfem_coarse_solve_device_noalloc.argtypes = [vp, i32, i32, vp, vp, vp, vp, dbl, i32, vp, vp, vp]
         L.msfem_gather_weights.argtypes = [i32, i32, vp, vp]
         L.msfem_reconstruct_device.argtypes = [vp, i32, i32, vp, vp, vp, vp, vp]
         L.msfem_philox4x32_10.argtypes = [vp, vp, vp]
