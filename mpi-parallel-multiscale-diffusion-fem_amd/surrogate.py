@@ -439,10 +439,12 @@ class TileSet:
         self.iters = ctx.alloc(n * 4, np.int32)
         self.status = ctx.alloc(n * 4, np.int32)
 
-    def solve(self, opts, x0_ptr=None, c0=0, c1=None):
+    def solve(self, opts, x0_ptr=None, c0=0, c1=None, bound_ptr=None):
         """Batched basis build of cells c0..c1-1 (default: all) into self.bases, on
         the context stream (enqueued; returns nothing).  x0_ptr: initial bases of
-        all n cells (the slice of c0..c1-1 is used)."""
+        all n cells (the slice of c0..c1-1 is used).  bound_ptr: device FP64
+        [n][4] for the a posteriori energy-norm bounds of all n cells (the slice
+        of c0..c1-1 is written)."""
         c1 = self.n if c1 is None else c1
         nb = 4 * (self.nf + 1) ** 2
         self.ctx.batch_build_device(2, self.n_fine[c0:c1], self.box.ptr + c0 * 4 * 8, msfem.COEF_SCALAR_CELL,
@@ -451,7 +453,8 @@ class TileSet:
                                     self.iters.ptr + c0 * 4 * 4, self.status.ptr + c0 * 4 * 4,
                                     rhs_kind=msfem.RHS_CONST, rhs_value=1.0,
                                     x0_ptr=None if x0_ptr is None else x0_ptr + c0 * nb * 8, opts=opts,
-                                    stream=self.ctx.stream)
+                                    stream=self.ctx.stream,
+                                    energy_bound_ptr=None if bound_ptr is None else bound_ptr + c0 * 4 * 8)
 
     def free(self):
         for a in (self.x, self.coef, self.box, self.bases, self.Ae, self.Fe, self.iters, self.status):
@@ -631,7 +634,7 @@ def energy_error(alpha, phi, ref):
     return float(np.max(np.sqrt(energy_norm2(alpha, phi - ref) / energy_norm2(alpha, ref))))
 
 
-def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512):
+def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512, bound_ptr=None):
     """One serving step -- prediction and warm-started basis build of all cells
     of `data` -- in `chunks` chunks: the inference of every chunk runs on the
     context's second stream (a mark after each), the build of chunk k on the
@@ -663,7 +666,7 @@ def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512):
 
     def build(k):
         ctx.wait_mark(mark0 + 1 + k, None)
-        data.solve(opts, x0.ptr, *bounds[k])
+        data.solve(opts, x0.ptr, *bounds[k], bound_ptr=bound_ptr)
 
     infer(0)
     for k in range(len(bounds)):
