@@ -371,7 +371,9 @@ public:
   void run()
   {
     const int nc = 1 << n_refine_, n = (int)std::pow(nc, dim), nb = 1 << dim;
-    std::vector<double> box((size_t)n * 2 * dim), quad;
+    std::vector<double> box((size_t)n * 2 * dim);
+    std::vector<double> &quad = quad_;
+    quad.clear();
     check(msfem_gen_unit_boxes(dim, nc, box.data()), nullptr);
     if (distortion_ > 0)
       {
@@ -443,6 +445,37 @@ public:
                     {"solution"}, pieces);
   }
 
+  // diffusion_problem_ms.tpp:330-380: the coarse solution "u" on the coarse mesh,
+  // "solution-ms_coarse-<d>d_refinements-<n_refine>.0000.vtu" (one process:
+  // subdomain 0) and its master record "..._refinements-<n_refine>.pvtu", in
+  // `dir`.  After run().
+  void output_global_coarse(const std::string &dir = "") const
+  {
+    if (solution_.empty())
+      throw Error(MSFEM_ERR_ARG, "output_global_coarse before run()");
+    const int nc = 1 << n_refine_, np = nc + 1;
+    const size_t nn = (size_t)std::pow(np, dim);
+    std::vector<double> pts(3 * nn, 0.0);
+    for (size_t k = 0; k < nn; ++k)
+      {
+        const int I[3] = {(int)(k % np), (int)((k / np) % np), dim == 3 ? (int)(k / ((size_t)np * np)) : 0};
+        for (int m = 0; m < dim; ++m)
+          pts[3 * k + m] = (double)I[m] / nc;
+        if (!quad_.empty()) // moved vertices: vertex of a cell that has this node
+          {
+            const int cx = I[0] < nc ? I[0] : nc - 1, cy = I[1] < nc ? I[1] : nc - 1;
+            const int v = (I[0] - cx) + 2 * (I[1] - cy);
+            pts[3 * k] = quad_[((size_t)cy * nc + cx) * 8 + 2 * v];
+            pts[3 * k + 1] = quad_[((size_t)cy * nc + cx) * 8 + 2 * v + 1];
+          }
+      }
+    const std::string base = std::string(dim == 2 ? "solution-ms_coarse-2d" : "solution-ms_coarse-3d") +
+                             "_refinements-" + std::to_string(n_refine_);
+    vtu::write_grid(DiffusionProblemBasis<dim>::join(dir, base + ".0000.vtu"), dim, nc, pts.data(),
+                    {{"u", solution_.data()}});
+    vtu::write_pvtu(DiffusionProblemBasis<dim>::join(dir, base + ".pvtu"), {"u"}, {base + ".0000.vtu"});
+  }
+
   // Coarse mesh with its interior vertices moved by up to amp/nc per axis
   // (msfem_gen_perturbed_quads; 2D, 0 < amp < 0.25): every cell is then a
   // general quadrilateral.  The boundary stays the unit square's, so the
@@ -478,6 +511,7 @@ private:
   std::string output_dir_;
   double distortion_ = 0;
   uint64_t distortion_seed_ = 0;
+  std::vector<double> quad_; // moved coarse cells' vertices of the last run (general cells)
 };
 
 } // namespace msfem

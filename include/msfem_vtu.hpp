@@ -6,6 +6,8 @@
 //     (include/base/diffusion_problem_basis.tpp:393-435)
 //   DiffusionProblemMultiscale::output_global_fine
 //     (include/base/diffusion_problem_ms.tpp:384-420, write_pvtu_record)
+//   DiffusionProblemMultiscale::output_global_coarse
+//     (include/base/diffusion_problem_ms.tpp:330-380)
 // A cell with nf fine cells per side is written as an UnstructuredGrid of
 // nf^dim VTK_QUAD (2D) or VTK_HEXAHEDRON (3D) elements on its (nf+1)^dim
 // lexicographic fine nodes (x fastest), with the given nodal fields as
@@ -96,6 +98,57 @@ inline void write_cell(const std::string &path, int dim, const double *box, int 
   std::fprintf(f, "</PointData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
   if (std::fclose(f) != 0)
     throw std::runtime_error("vtu::write_cell: write failed for " + path);
+}
+
+// A structured grid of n^dim cells (VTK_QUAD / VTK_HEXAHEDRON) on (n+1)^dim
+// nodes given by their coordinates points[(n+1)^dim][3] (x fastest), with
+// nodal fields: the coarse mesh of DiffusionProblemMultiscale::output_global_coarse.
+inline void write_grid(const std::string &path, int dim, int n, const double *points, const std::vector<Field> &fields)
+{
+  if ((dim != 2 && dim != 3) || n <= 0)
+    throw std::invalid_argument("vtu::write_grid: dim must be 2 or 3 and n > 0");
+  FILE *f = std::fopen(path.c_str(), "w");
+  if (!f)
+    throw std::runtime_error("vtu::write_grid: cannot open " + path);
+  const int np = n + 1;
+  const long long npts = dim == 2 ? (long long)np * np : (long long)np * np * np;
+  const long long ncells = dim == 2 ? (long long)n * n : (long long)n * n * n;
+  const int nv = dim == 2 ? 4 : 8;
+  std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+                  "byte_order=\"LittleEndian\">\n<UnstructuredGrid>\n");
+  std::fprintf(f, "<Piece NumberOfPoints=\"%lld\" NumberOfCells=\"%lld\">\n", npts, ncells);
+  std::fprintf(f, "<Points>\n<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n");
+  for (long long k = 0; k < npts; ++k)
+    std::fprintf(f, "%.17g %.17g %.17g\n", points[3 * k], points[3 * k + 1], points[3 * k + 2]);
+  std::fprintf(f, "</DataArray>\n</Points>\n<Cells>\n<DataArray type=\"Int64\" Name=\"connectivity\" "
+                  "format=\"ascii\">\n");
+  for (long long c = 0; c < ncells; ++c)
+    {
+      const long long cx = c % n, cy = (c / n) % n, cz = dim == 3 ? c / ((long long)n * n) : 0;
+      auto node = [&](int dx, int dy, int dz) { return (cx + dx) + np * ((cy + dy) + (long long)np * (cz + dz)); };
+      if (dim == 2)
+        std::fprintf(f, "%lld %lld %lld %lld\n", node(0, 0, 0), node(1, 0, 0), node(1, 1, 0), node(0, 1, 0));
+      else
+        std::fprintf(f, "%lld %lld %lld %lld %lld %lld %lld %lld\n", node(0, 0, 0), node(1, 0, 0), node(1, 1, 0),
+                     node(0, 1, 0), node(0, 0, 1), node(1, 0, 1), node(1, 1, 1), node(0, 1, 1));
+    }
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n");
+  for (long long c = 1; c <= ncells; ++c)
+    std::fprintf(f, "%lld\n", c * nv);
+  std::fprintf(f, "</DataArray>\n<DataArray type=\"UInt8\" Name=\"types\" format=\"ascii\">\n");
+  for (long long c = 0; c < ncells; ++c)
+    std::fprintf(f, "%d\n", dim == 2 ? 9 : 12);
+  std::fprintf(f, "</DataArray>\n</Cells>\n<PointData>\n");
+  for (const Field &fd : fields)
+    {
+      std::fprintf(f, "<DataArray type=\"Float64\" Name=\"%s\" format=\"ascii\">\n", fd.name.c_str());
+      for (long long k = 0; k < npts; ++k)
+        std::fprintf(f, "%.17g\n", fd.values[k]);
+      std::fprintf(f, "</DataArray>\n");
+    }
+  std::fprintf(f, "</PointData>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n");
+  if (std::fclose(f) != 0)
+    throw std::runtime_error("vtu::write_grid: write failed for " + path);
 }
 
 // Parallel master record naming the per-cell pieces (paths relative to it).

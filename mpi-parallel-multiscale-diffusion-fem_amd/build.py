@@ -12,7 +12,7 @@ OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmsfem_hip.so")
 ARCH = os.environ.get("MSFEM_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["local2d.hip", "local3d.hip", "element.hip", "coarse.hip", "fine.hip", "energy.hip", "noalloc.hip", "nn.hip", "engine.hip", "msfem_gen.cpp"]
+SOURCES = ["local2d.hip", "local3d.hip", "element.hip", "coarse.hip", "fine.hip", "energy.hip", "generic.hip", "noalloc.hip", "nn.hip", "engine.hip", "msfem_gen.cpp"]
 
 # -packed-fp32-ops: the compiler must not form packed FP32 instructions.
 HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",

@@ -112,8 +112,10 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
         }
       sM[s][ab] = v;
     }
+  // (beyond kElemMaxNF fine cells per side the factors are evaluated in place)
+  const bool sin_table = nf <= kElemMaxNF;
   if constexpr (!QUAD)
-    if (a.rhs_kind == 1)
+    if (a.rhs_kind == 1 && sin_table)
       for (int t = threadIdx.x; t < DIM * 2 * nf; t += blockDim.x)
         {
           const int k = t / (2 * nf), fi = t % (2 * nf) / 2;
@@ -220,7 +222,9 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
                   fv = 1.0;
                   const int fi[3] = {fx, fy, fz};
                   for (int k = 0; k < DIM; ++k)
-                    fv *= ssin[k][2 * fi[k] + ((q >> k) & 1)];
+                    fv *= sin_table ? ssin[k][2 * fi[k] + ((q >> k) & 1)]
+                                    : sin(M_PI * (lo[k] + (bx[2 * k + 1] - bx[2 * k]) * ((double)fi[k] / nf) +
+                                                  h[k] * (((q >> k) & 1) ? 0.5 + gs : 0.5 - gs)));
                 }
 #pragma unroll
               for (int aa = 0; aa < NV; ++aa)
@@ -475,7 +479,7 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
     hipLaunchKernelGGL((element_kernel<2, false, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.dim == 2)
     hipLaunchKernelGGL((element_kernel<2, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
-  else if (a.coef_kind == 0) // scalar per fine cell
+  else if (a.coef_kind == 0 && a.nf <= kElem3MaxNF) // scalar per fine cell
     hipLaunchKernelGGL(element3d_kernel, dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else
     hipLaunchKernelGGL((element_kernel<3, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);

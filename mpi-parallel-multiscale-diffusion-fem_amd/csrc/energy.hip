@@ -27,8 +27,28 @@ namespace {
 
 constexpr int kEnergyThreads = 256;
 
+// Smallest eigenvalue of a symmetric 3x3 matrix (trigonometric closed form of
+// the characteristic cubic), shrunk by 1e-10 relative so that its rounding
+// cannot lift it above the true value.
+__device__ inline double sym3_lambda_min(double a11, double a12, double a13, double a22, double a23, double a33)
+{
+  const double p1 = a12 * a12 + a13 * a13 + a23 * a23;
+  const double q = (a11 + a22 + a33) / 3.0;
+  const double b11 = a11 - q, b22 = a22 - q, b33 = a33 - q;
+  const double p2 = b11 * b11 + b22 * b22 + b33 * b33 + 2.0 * p1;
+  if (!(p2 > 0))
+    return q;
+  const double p = sqrt(p2 / 6.0), ip = 1.0 / p;
+  const double c11 = b11 * ip, c22 = b22 * ip, c33 = b33 * ip, c12 = a12 * ip, c13 = a13 * ip, c23 = a23 * ip;
+  const double det = c11 * (c22 * c33 - c23 * c23) - c12 * (c12 * c33 - c23 * c13) + c13 * (c12 * c23 - c22 * c13);
+  const double r = fmin(1.0, fmax(-1.0, 0.5 * det));
+  const double phi = acos(r) / 3.0;
+  const double lmin = q + 2.0 * p * cos(phi + 2.0 * M_PI / 3.0);
+  return lmin - 1e-10 * fabs(q) - 1e-10 * p;
+}
+
 // lambda_min of fine cell f and the entries of A^-1 (2D: b11, b12, b22; 3D:
-// the scalar 1/lambda on the diagonal) used in D_i.
+// the scalar 1/lambda on the diagonal, an upper bound of A^-1) used in D_i.
 template <int DIM>
 __device__ __forceinline__ double cell_lambda(int kind, const double *cf, int f, double (&B)[3])
 {
@@ -61,6 +81,20 @@ __device__ __forceinline__ double cell_lambda(int kind, const double *cf, int f,
           const double a = t[0], b = t[1], c = t[2];
           const double det = a * c - b * b, m = 0.5 * (a + c), d = sqrt(0.25 * (a - c) * (a - c) + b * b);
           lam = fmin(lam, det / (m + d));
+        }
+    }
+  else if (DIM == 3 && kind == 1) // tensor per fine cell (a11, a12, a13, a22, a23, a33)
+    {
+      const double *t = cf + (size_t)f * NS;
+      lam = sym3_lambda_min(t[0], t[1], t[2], t[3], t[4], t[5]);
+    }
+  else if (DIM == 3 && kind == 3) // tensor at the quadrature points
+    {
+      lam = 1e300;
+      for (int q = 0; q < NQ; ++q)
+        {
+          const double *t = cf + ((size_t)f * NQ + q) * NS;
+          lam = fmin(lam, sym3_lambda_min(t[0], t[1], t[2], t[3], t[4], t[5]));
         }
     }
   B[0] = 1.0 / lam;

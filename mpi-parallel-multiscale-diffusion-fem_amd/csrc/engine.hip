@@ -191,12 +191,16 @@ long long variant_key(int dim, int nf, int precision, int coef_kind)
   return ((long long)dim * 1024 + nf) * 64 + precision * 8 + coef_kind;
 }
 
+// Resolutions with tuned kernels (launch-shape variants) ...
 bool nf_supported(int dim, int nf)
 {
   if (dim == 2)
     return nf == 16 || nf == 32 || nf == 64 || nf == 128;
   return nf == 8 || nf == 16;
 }
+
+// ... and every resolution the build accepts (the others run the generic solve).
+bool nf_valid(int dim, int nf) { return nf >= 1 && nf <= (dim == 2 ? 1024 : 64); }
 
 // Local-solve storage mode of a class.  Scalar and tensor coefficients per
 // fine cell of box cells are cached in registers (up to 64^2); everything
@@ -206,6 +210,8 @@ bool nf_supported(int dim, int nf)
 // 32^2, whose lane groups hold several problems per workgroup).
 int class_mode(int dim, int coef_kind, int nf, bool general)
 {
+  if (!nf_supported(dim, nf) || (dim == 3 && coef_kind != MSFEM_COEF_SCALAR_CELL))
+    return kModeGeneric;
   if (dim == 3)
     return kModeScalar;
   if (!general && coef_kind == MSFEM_COEF_SCALAR_CELL)
@@ -244,8 +250,6 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
     return fail(ctx, MSFEM_ERR_ARG, "general (vertex) cells are supported in 2D only");
   if (coef_stride(in->dim, in->coef_kind) < 0)
     return fail(ctx, MSFEM_ERR_ARG, "unknown coefficient kind %d", in->coef_kind);
-  if (in->dim == 3 && in->coef_kind != MSFEM_COEF_SCALAR_CELL)
-    return fail(ctx, MSFEM_ERR_ARG, "3D supports MSFEM_COEF_SCALAR_CELL only");
   if (in->rhs_kind != MSFEM_RHS_CONST && in->rhs_kind != MSFEM_RHS_SINSIN)
     return fail(ctx, MSFEM_ERR_ARG, "unknown rhs kind %d", in->rhs_kind);
   if (o->precision != MSFEM_PREC_FP64 && o->precision != MSFEM_PREC_FP32_REFINE)
@@ -254,6 +258,8 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
     return fail(ctx, MSFEM_ERR_ARG, "variant must be >= 0");
   for (int c = 0; c < in->n_cells && o->variant > 0; ++c)
     {
+      if (class_mode(in->dim, in->coef_kind, in->n_fine[c], in->vertices != nullptr) == kModeGeneric)
+        continue; // the generic solve has one launch shape
       const int n = in->dim == 2 ? local2d_variants(in->n_fine[c]) : local3d_variants(in->n_fine[c]);
       if (o->variant > n)
         return fail(ctx, MSFEM_ERR_ARG, "variant %d does not exist for %d fine cells per side", o->variant,
@@ -266,9 +272,9 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
   if (o->energy_tol > 0 && in->vertices)
     return fail(ctx, MSFEM_ERR_ARG, "the energy stopping rule needs box cells (its eigenvalue bound is for boxes)");
   for (int c = 0; c < in->n_cells; ++c)
-    if (!nf_supported(in->dim, in->n_fine[c]))
-      return fail(ctx, MSFEM_ERR_ARG, "cell %d: unsupported fine resolution %d for dim %d", c, in->n_fine[c],
-                  in->dim);
+    if (!nf_valid(in->dim, in->n_fine[c]))
+      return fail(ctx, MSFEM_ERR_ARG, "cell %d: unsupported fine resolution %d for dim %d (2D: 1..1024, 3D: 1..64)",
+                  c, in->n_fine[c], in->dim);
   return MSFEM_OK;
 }
 
@@ -301,9 +307,11 @@ int make_plan(msfem_ctx *ctx, const msfem_batch *in)
       p.bas_off[c] = p.bas_total;
       p.bas_total += (long long)p.nb * ipow(nf + 1, in->dim);
       p.cond_off[c] = p.cond_total;
-      const int mode = in->dim == 2 ? class_mode(2, in->coef_kind, nf, general) : kModeScalar;
+      const int mode = class_mode(in->dim, in->coef_kind, nf, general);
       if (mode == kModeGlobal) // 6 FP64 conductances per fine cell
         p.cond_total += 6 * ipow(nf, 2);
+      if (mode == kModeGeneric) // QGauss(2) element matrix per fine cell
+        p.cond_total += (long long)generic_elem_doubles(in->dim, nf);
       if (mode == kModeStream) // 4 FP64 + 4 FP32 coupling planes
         p.cond_total += 6 * (long long)coupling_plane_stride(nf);
       cls[nf].push_back(c);
@@ -390,7 +398,7 @@ int class_variant(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
 // Whether size class nf runs the two-level local solve.
 bool class_coarse(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, int nf)
 {
-  return o->coarse_space && in->dim == 2 &&
+  return o->coarse_space && in->dim == 2 && build_mode(in, nf) != kModeGeneric &&
          local2d_coarse_space(nf, build_mode(in, nf), class_variant(ctx, in, o, nf));
 }
 
@@ -422,6 +430,18 @@ size_t stream_work_doubles(msfem_ctx *ctx, const msfem_batch *in, const msfem_so
   return n;
 }
 
+// Doubles of the generic classes' per-problem work vectors before size class k_end.
+size_t generic_work_total(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k_end)
+{
+  const Plan &p = ctx->plan;
+  size_t n = 0;
+  for (size_t k = 0; k < k_end && k < p.class_nf.size(); ++k)
+    if (build_mode(in, p.class_nf[k]) == kModeGeneric)
+      n += p.class_cells[k].size() * (size_t)(o->partition_of_unity ? p.nb - 1 : p.nb) *
+           generic_work_doubles(in->dim, p.class_nf[k]);
+  return n;
+}
+
 // Per-class preparation before the solve: conductances / node couplings,
 // general cells' boundary data, the two-level inverse coarse matrices.
 int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k, double *bases,
@@ -439,6 +459,9 @@ int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solv
   if (mode == kModeStream)
     HIP_TRY(ctx, launch_prep_couplings_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
                                           d_off(ctx, 0), cond, d_off(ctx, 2), fp32, st));
+  if (mode == kModeGeneric)
+    HIP_TRY(ctx, launch_generic_prep(in->dim, ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
+                                     d_off(ctx, 0), cond, d_off(ctx, 2), st));
   if (in->vertices) // the solve kernels read the boundary data from the bases
     HIP_TRY(ctx, launch_quad_dirichlet_2d(ncls, cells, nf, in->vertices, bases, d_off(ctx, 1), st));
   if (class_coarse(ctx, in, o, nf))
@@ -499,7 +522,14 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.cinv = nullptr;
   if (class_coarse(ctx, in, o, nf))
     a.cinv = (const double *)ctx->bufs["coarse_inv"].p + coarse_inv_doubles(ctx, in, o, k);
-  if (in->dim == 2)
+  if (mode == kModeGeneric)
+    {
+      double *gw = (double *)ctx->bufs["gen_work"].p;
+      if (!gw)
+        return fail(ctx, MSFEM_ERR_HIP, "generic-solve scratch missing");
+      HIP_TRY(ctx, launch_generic_solve(in->dim, nf, a, gw + generic_work_total(ctx, in, o, k), st));
+    }
+  else if (in->dim == 2)
     HIP_TRY(ctx, launch_local_solve_2d(fp32, mode, nf, variant, a, st));
   else
     HIP_TRY(ctx, launch_local_solve_3d(fp32, nf, variant, a, st));
@@ -698,6 +728,15 @@ int enqueue_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
     {
       void *d;
       rc = scratch(ctx, "coarse_inv", cinv_doubles * sizeof(double), &d);
+      if (rc)
+        return rc;
+    }
+  // per-problem vectors of the generic classes (allocated before any capture)
+  const size_t gen_doubles = generic_work_total(ctx, in, o, p.class_nf.size());
+  if (gen_doubles > 0)
+    {
+      void *d;
+      rc = scratch(ctx, "gen_work", gen_doubles * sizeof(double), &d);
       if (rc)
         return rc;
     }

@@ -33,6 +33,16 @@ hipError_t launch_quad_dirichlet_2d(int n_cells_class, const int *cells, int nf,
 int local3d_variants(int nf);
 hipError_t launch_local_solve_3d(bool fp32, int nf, int variant, const LocalArgs &a, hipStream_t st);
 
+// generic.hip: the local solve for resolutions and coefficient kinds outside the
+// tuned kernels.  Element matrices [nf^d][2^d][2^d] per cell (offsets elem_off,
+// generic_elem_doubles each); work: generic_work_doubles per problem.
+size_t generic_elem_doubles(int dim, int nf);
+size_t generic_work_doubles(int dim, int nf);
+hipError_t launch_generic_prep(int dim, int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                               const double *verts, const double *coef, const long long *coef_off, double *elem,
+                               const long long *elem_off, hipStream_t st);
+hipError_t launch_generic_solve(int dim, int nf, const LocalArgs &a, double *work, hipStream_t st);
+
 // element.hip: A^K, F^K of every cell of one size class; partition-of-unity fix-up.
 struct ElementArgs {
   int n_cells_class;

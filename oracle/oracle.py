@@ -367,7 +367,9 @@ def _cell_lambda_and_inverse(dim, kind, cdata):
         return A
     if kind == COEF_TENSOR_CELL:
         A = full(cdata[:ns])
-        return float(np.linalg.eigvalsh(A)[0]), np.linalg.inv(A)
+        lam = float(np.linalg.eigvalsh(A)[0])
+        # 3D: I / lambda_min (>= A^-1: a valid, simpler upper bound in D)
+        return lam, (np.linalg.inv(A) if dim == 2 else np.eye(dim) / lam)
     lam = min(float(np.linalg.eigvalsh(full(cdata[q * ns:(q + 1) * ns]))[0]) for q in range(nq))
     return lam, np.eye(dim) / lam
 
