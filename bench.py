@@ -54,8 +54,9 @@ PEAK_FP32_TFLOPS = 157.3      # MI355X published FP32 peak
 PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
 # HBM bytes per launch of the dominant kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by a plain copy of known size (--copy-calibration):
-# profiles/local_solve_2d_lognormal64_hbm_traffic.md
-TRAFFIC_BYTES = {("lognormal64", 0): 2.168743e9}
+# profiles/local_solve_2d_lognormal64_hbm_traffic.md (FP64, 4096 cells) and
+# profiles/local_solve_2d_fp32_grf128_hbm_traffic.md (FP32 energy rule, 16384 cells)
+TRAFFIC_BYTES = {("lognormal64", 0): 2.168743e9, ("surrogate128", 1): 5.598081e9}
 METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 TAU = 5e-7  # energy stopping rule of the FP32 legs: a 2x margin on the 1e-6 relative energy-norm bar
 WEIGHTS = os.path.join(PKG, "surrogate_weights.npz")
@@ -230,8 +231,11 @@ def bench_headline(args):
             "pays_for_itself": bool(infer_ms < saved_ms)}
     rl_solve = {"bound": "mfma", "engine": "valu", "kernel": "local_solve_2d<float, 64, ...> (FP32 CG + FP64 "
                 "refinement, energy rule)", "achieved": round(solve_tf, 3), "peak": PEAK_FP32_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(solve_tf / PEAK_FP32_TFLOPS, 4), "traffic": None,
-                "kernel_ms": round(solve_ms, 4), "flops_per_launch": flops_solve}
+                "unit": "TFLOP/s", "frac": round(solve_tf / PEAK_FP32_TFLOPS, 4),
+                "traffic": TRAFFIC_BYTES[("surrogate128", msfem.PREC_FP32_REFINE)],
+                "kernel_ms": round(solve_ms, 4), "flops_per_launch": flops_solve,
+                "hbm_GBps_counted": round(TRAFFIC_BYTES[("surrogate128", msfem.PREC_FP32_REFINE)] / (solve_ms * 1e-3) /
+                                          1e9, 1)}
     rl_inf = {"bound": "mfma", "engine": "mfma bf16", "kernel": "surrogate inference (direct 3x3 BF16-MFMA "
               "convolutions + glue, all 16384 tiles)", "achieved": round(inf_tf, 3), "peak": PEAK_BF16_MFMA_TFLOPS,
               "unit": "TFLOP/s", "frac": round(inf_tf / PEAK_BF16_MFMA_TFLOPS, 4), "traffic": None,

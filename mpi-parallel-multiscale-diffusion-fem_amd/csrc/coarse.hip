@@ -224,12 +224,32 @@ template <int DIM>
 struct MgGrid {
   int n; // cells per side; nodes (n+1)^DIM, x fastest
   __device__ __forceinline__ int np() const { return n + 1; }
+  // node / p by a float reciprocal and an exact one-step fix-up (node < 2^24)
+  __device__ __forceinline__ static int divp(int node, int p)
+  {
+    int q = (int)((float)node * (1.0f / (float)p));
+    if (q * p > node)
+      --q;
+    else if ((q + 1) * p <= node)
+      ++q;
+    return q;
+  }
   __device__ __forceinline__ void coords(int node, int (&I)[3]) const
   {
     const int p = n + 1;
-    I[0] = node % p;
-    I[1] = (node / p) % p;
-    I[2] = DIM == 3 ? node / (p * p) : 0;
+    const int q = divp(node, p);
+    I[0] = node - q * p;
+    if (DIM == 2)
+      {
+        I[1] = q;
+        I[2] = 0;
+      }
+    else
+      {
+        const int q2 = divp(q, p);
+        I[1] = q - q2 * p;
+        I[2] = q2;
+      }
   }
   __device__ __forceinline__ int index(const int (&I)[3]) const
   {
@@ -270,9 +290,12 @@ __global__ void __launch_bounds__(kCoarseThreads)
     return cmask[G0.index(I)] != 0;
   };
   auto Aof = [&](int l) -> const double * { return l == 0 ? Kc : w + g.off_A[l]; };
-  // (A_l v)_node over the neighbours inside the grid (rows of constrained nodes: 0)
+  // constrained node of level l after the set-up: its Jacobi entry is exactly 0
+  auto fixed = [&](int l, int node) -> bool { return w[g.off_d[l] + node] == 0.0; };
+  // (A_l v)_node over the neighbours inside the grid (rows of constrained nodes: 0);
+  // after the set-up
   auto apply = [&](int l, const double *v, int node) -> double {
-    if (masked(l, node))
+    if (fixed(l, node))
       return 0.0;
     MgGrid<DIM> Gl{g.n[l]};
     int I[3];
@@ -432,20 +455,20 @@ __global__ void __launch_bounds__(kCoarseThreads)
           x[n] = kMgOmega * d[n] * b[n];
         __syncthreads();
         for (int n = tid; n < nn; n += kCoarseThreads)
-          t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+          t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
         __syncthreads();
         for (int n = tid; n < nn; n += kCoarseThreads)
           x[n] += kMgOmega * d[n] * t[n];
         __syncthreads();
         for (int n = tid; n < nn; n += kCoarseThreads)
-          t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+          t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
         __syncthreads();
         MgGrid<DIM> Gf{g.n[l]}, Gc{g.n[l + 1]};
         double *bc = w + g.off_b[l + 1];
         for (int c = tid; c < (int)g.nn[l + 1]; c += kCoarseThreads)
           {
             double sres = 0;
-            if (!masked(l + 1, c))
+            if (!fixed(l + 1, c))
               {
                 int I[3];
                 Gc.coords(c, I);
@@ -504,7 +527,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
         MgGrid<DIM> Gf{g.n[l]}, Gc{g.n[l + 1]};
         for (int n = tid; n < nn; n += kCoarseThreads)
           {
-            if (masked(l, n))
+            if (fixed(l, n))
               continue;
             int j[3];
             Gf.coords(n, j);
@@ -533,7 +556,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
         for (int sweep = 0; sweep < 2; ++sweep)
           {
             for (int n = tid; n < nn; n += kCoarseThreads)
-              t[n] = masked(l, n) ? 0.0 : b[n] - apply(l, x, n);
+              t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
             __syncthreads();
             for (int n = tid; n < nn; n += kCoarseThreads)
               x[n] += kMgOmega * d[n] * t[n];

@@ -14,9 +14,9 @@ enum CondMode : int {
   kModeTensor = 1, // 2D symmetric tensor per fine cell, 4 conductances cached in registers
   kModeGlobal = 2, // per-fine-cell element conductances read from global memory (prep buffer)
   kModeStream = 3, // FP32 inner CG: 4 FP32 conductances per fine cell streamed from global memory
+                   // (planar prep buffer), the CG product vector w kept in LDS
   kModeGeneric = 4 // any resolution / coefficient kind: QGauss(2) element matrices in global memory,
                    // one workgroup per problem, FP64 (generic.hip)
-                   // (planar prep buffer), the CG product vector w kept in LDS
 };
 
 // kModeStream node couplings: four planes (aE, aN, aNE, aNW: the couplings of
