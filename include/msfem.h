@@ -27,13 +27,14 @@
  *   - dim is 2 or 3; nb = 2^dim bases per coarse cell; basis i belongs to
  *     coarse vertex i, lexicographic with x fastest (deal.II vertex order).
  *   - a coarse cell is an axis-aligned box: box[c] = {lo_x, hi_x, lo_y, hi_y
- *     [, lo_z, hi_z]}, or (2D, msfem_batch.vertices) a general convex
- *     quadrilateral given by its four vertices in deal.II order; its fine grid
- *     is the bilinear image of the uniform nf x nf grid (what
- *     GridGenerator::general_cell + refine_global produce,
+ *     [, lo_z, hi_z]}, or (msfem_batch.vertices) a general cell given by its
+ *     2^dim vertices in deal.II order: a convex quadrilateral (2D) or a
+ *     hexahedron (3D); its fine grid is the bi/trilinear image of the uniform
+ *     grid (what GridGenerator::general_cell + refine_global produce,
  *     include/base/diffusion_problem_basis.tpp:92-99), every fine cell is
- *     integrated with the bilinear mapping, and the Dirichlet data is BasisQ1's
- *     N_i(x) = c_0i + c_1i x + c_2i y + c_3i xy with N_i(v_j) = delta_ij
+ *     integrated with the bi/trilinear mapping, and the Dirichlet data is
+ *     BasisQ1's N_i(x) = sum_m c_mi mono_m(x), mono = (1, x, y, xy) in 2D and
+ *     (1, x, y, z, xy, yz, xz, xyz) in 3D, with N_i(v_j) = delta_ij
  *     (include/coefficients/basis_q1.tpp:26-46, :86-96) at the mapped nodes.
  *   - a cell with nf fine cells per side has (nf+1)^dim fine nodes,
  *     lexicographic, x fastest.  Its bases are stored as [nb][(nf+1)^dim],
@@ -139,8 +140,8 @@ typedef struct msfem_solver_opts {
 typedef struct msfem_batch {
   int32_t dim;
   int32_t n_cells;
-  const int32_t *n_fine;      /* HOST array [n_cells]: fine cells per side (2D: 16,32,64,128;
-                                 3D: 8,16) */
+  const int32_t *n_fine;      /* HOST array [n_cells]: fine cells per side (2D: 1..1024,
+                                 3D: 1..64; the reference's 2^n_refine_local) */
   const double *box;          /* [n_cells][dim][2] */
   int32_t coef_kind;          /* MSFEM_COEF_* */
   const double *coef;         /* concatenated coefficient tiles */
@@ -150,10 +151,16 @@ typedef struct msfem_batch {
                                  values only are read; a non-finite value is taken as 0.
                                  A problem whose initial FP64 residual already meets the
                                  tolerance takes 0 iterations (accepted as given). */
-  const double *vertices;     /* optional [n_cells][4][2]: general quadrilateral cells
-                                 (2D only; box is then ignored and may be NULL), deal.II
-                                 vertex order (0,0),(1,0),(0,1),(1,1) of the parametric
-                                 square; each cell convex with positive orientation.
+  const double *vertices;     /* optional general cells (box is then ignored and may be
+                                 NULL), deal.II vertex order (vertex v at the parametric
+                                 corner (v&1, (v>>1)&1, (v>>2)&1)):
+                                 2D [n_cells][4][2] quadrilaterals, each convex with
+                                 positive orientation;
+                                 3D [n_cells][8][3] hexahedra whose trilinear map's
+                                 Jacobian determinant has positive Bernstein
+                                 coefficients (a sufficient test for a positive
+                                 determinant everywhere; msfem_batch_build checks both,
+                                 the device entry points trust the caller).
                                  NULL: the cells are the boxes. */
 } msfem_batch;
 
@@ -351,6 +358,16 @@ int msfem_gen_perturbed_quads(int nc, double amp, uint64_t seed, double *vertice
 /* MatrixCoeff<2> at the mapped QGauss(2) points of the fine cells of general
  * quadrilaterals (vertices[n_cells][4][2]) -> TENSOR_QP tiles. */
 int msfem_gen_coef_reference_quad(int n_cells, const int32_t *n_fine, const double *vertices, double *coef);
+/* Unit cube split into nc^3 general hexahedra: the interior vertices of the
+ * uniform mesh are moved by amp/nc * (2u_k - 1) along axis k, u_0..u_2 uniform
+ * from Philox(ctr={vertex, 0, 3, 0}, key=seed) (0 <= amp < 1/6 keeps every
+ * Jacobian matrix strictly diagonally dominant, so every cell is positively
+ * oriented and passes the build's orientation test) -> vertices[nc^3][8][3],
+ * deal.II vertex order, cells x fastest.  Test meshes for general hexahedra. */
+int msfem_gen_perturbed_hexes(int nc, double amp, uint64_t seed, double *vertices);
+/* MatrixCoeff<3> at the mapped QGauss(2) points of the fine cells of general
+ * hexahedra (vertices[n_cells][8][3]) -> TENSOR_QP tiles. */
+int msfem_gen_coef_reference_hex(int n_cells, const int32_t *n_fine, const double *vertices, double *coef);
 /* alpha = 2 + sin(2 pi x/eps) cos(2 pi y/eps) at the QGauss(2) points -> SCALAR_QP tiles (2D). */
 int msfem_gen_coef_periodic(int n_cells, const int32_t *n_fine, const double *box, double eps, double *coef);
 /* alpha = exp(clamp(g, -clip, clip)), g ~ N(0,1) i.i.d. per fine cell -> SCALAR_CELL tiles.

@@ -12,8 +12,8 @@
 // registered cell in one msfem_batch_build call on the GPU.  The call sequence
 // of the reference driver is unchanged.
 //
-// Differences forced by the data path: a cell is described by its box, or (2D)
-// by its four vertices for a general quadrilateral, instead of a deal.II cell
+// Differences forced by the data path: a cell is described by its box, or by
+// its 2^dim vertices for a general quadrilateral / hexahedron, instead of a deal.II cell
 // iterator (the reference reads the vertices from the iterator,
 // diffusion_problem_basis.tpp:43-49), the communicator argument is gone (the local
 // problem never used it, diffusion_problem_basis.hpp:198), FullMatrix<double>
@@ -79,9 +79,9 @@ struct ProblemData {
   // Fills the concatenated coefficient tiles of the given cells; nullptr =
   // the reference coefficient.
   void (*fill)(int dim, int n_cells, const int32_t *n_fine, const double *box, double *coef) = nullptr;
-  // The same for a batch of general quadrilaterals (vertices [n][4][2]); a
-  // batch with such cells and a custom coefficient needs it.
-  void (*fill_quad)(int n_cells, const int32_t *n_fine, const double *vertices, double *coef) = nullptr;
+  // The same for a batch of general cells (vertices [n][2^dim][dim]); a batch
+  // with such cells and a custom coefficient needs it.
+  void (*fill_general)(int dim, int n_cells, const int32_t *n_fine, const double *vertices, double *coef) = nullptr;
 };
 
 template <int dim>
@@ -113,12 +113,13 @@ public:
     built_ = false;
     return (int)n_fine_.size() - 1;
   }
-  // A general quadrilateral (2D): its vertices in deal.II order.  The whole
-  // batch is then built as general cells (boxes as their four vertices).
+  // A general cell -- quadrilateral (2D) or hexahedron (3D) -- by its 2^dim
+  // vertices in deal.II order.  The whole batch is then built as general cells
+  // (boxes as their 2^dim vertices).
   int add_general(unsigned n_refine_local, const std::vector<std::array<double, dim>> &corner_points)
   {
-    if (dim != 2 || (int)corner_points.size() != nb)
-      throw Error(MSFEM_ERR_ARG, "general cells: 2D only, four corner points");
+    if ((int)corner_points.size() != nb)
+      throw Error(MSFEM_ERR_ARG, "general cells: 2^dim corner points");
     std::array<double, 2 * dim> bbox;
     std::array<double, nb * dim> v;
     for (int k = 0; k < dim; ++k)
@@ -168,12 +169,14 @@ public:
         vert.resize((size_t)n * nb * dim);
         for (int c = 0; c < n; ++c)
           std::copy(verts_[c].begin(), verts_[c].end(), vert.begin() + (size_t)c * nb * dim);
-        if (data_.fill_quad)
-          data_.fill_quad(n, n_fine_.data(), vert.data(), coef.data());
+        if (data_.fill_general)
+          data_.fill_general(dim, n, n_fine_.data(), vert.data(), coef.data());
         else if (data_.fill)
-          throw Error(MSFEM_ERR_ARG, "a batch of general cells needs ProblemData::fill_quad");
-        else
+          throw Error(MSFEM_ERR_ARG, "a batch of general cells needs ProblemData::fill_general");
+        else if (dim == 2)
           check(msfem_gen_coef_reference_quad(n, n_fine_.data(), vert.data(), coef.data()), nullptr);
+        else
+          check(msfem_gen_coef_reference_hex(n, n_fine_.data(), vert.data(), coef.data()), nullptr);
       }
     else if (data_.fill)
       data_.fill(dim, n, n_fine_.data(), box.data(), coef.data());
@@ -225,8 +228,8 @@ public:
     : batch_(&batch), id_(batch.add(n_refine_local, box)), local_subdomain_(local_subdomain), box_(box)
   {
   }
-  // A general quadrilateral (2D): the cell's vertices in deal.II order, as
-  // corner_points in the reference (diffusion_problem_basis.tpp:43-49).
+  // A general cell (quadrilateral or hexahedron): its vertices in deal.II
+  // order, as corner_points in the reference (diffusion_problem_basis.tpp:43-49).
   DiffusionProblemBasis(unsigned n_refine_local, const std::vector<std::array<double, dim>> &corner_points,
                         unsigned local_subdomain, BasisBatch<dim> &batch)
     : batch_(&batch), id_(batch.add_general(n_refine_local, corner_points)), local_subdomain_(local_subdomain),
@@ -372,13 +375,16 @@ public:
   {
     const int nc = 1 << n_refine_, n = (int)std::pow(nc, dim), nb = 1 << dim;
     std::vector<double> box((size_t)n * 2 * dim);
-    std::vector<double> &quad = quad_;
-    quad.clear();
+    std::vector<double> &verts = verts_;
+    verts.clear();
     check(msfem_gen_unit_boxes(dim, nc, box.data()), nullptr);
     if (distortion_ > 0)
       {
-        quad.resize((size_t)n * 8);
-        check(msfem_gen_perturbed_quads(nc, distortion_, distortion_seed_, quad.data()), nullptr);
+        verts.resize((size_t)n * nb * dim);
+        if (dim == 2)
+          check(msfem_gen_perturbed_quads(nc, distortion_, distortion_seed_, verts.data()), nullptr);
+        else
+          check(msfem_gen_perturbed_hexes(nc, distortion_, distortion_seed_, verts.data()), nullptr);
       }
     cells_.clear();
     for (int c = 0; c < n; ++c) // initialize_and_compute_basis (diffusion_problem_ms.tpp:42-88)
@@ -388,7 +394,7 @@ public:
             std::vector<std::array<double, dim>> corners(nb);
             for (int i = 0; i < nb; ++i)
               for (int k = 0; k < dim; ++k)
-                corners[i][k] = quad[(size_t)c * 8 + 2 * i + k];
+                corners[i][k] = verts[((size_t)c * nb + i) * dim + k];
             cells_.emplace(c, DiffusionProblemBasis<dim>(n_refine_local_, corners, 0, batch_));
           }
         else
@@ -461,12 +467,17 @@ public:
         const int I[3] = {(int)(k % np), (int)((k / np) % np), dim == 3 ? (int)(k / ((size_t)np * np)) : 0};
         for (int m = 0; m < dim; ++m)
           pts[3 * k + m] = (double)I[m] / nc;
-        if (!quad_.empty()) // moved vertices: vertex of a cell that has this node
+        if (!verts_.empty()) // moved vertices: vertex of a cell that has this node
           {
-            const int cx = I[0] < nc ? I[0] : nc - 1, cy = I[1] < nc ? I[1] : nc - 1;
-            const int v = (I[0] - cx) + 2 * (I[1] - cy);
-            pts[3 * k] = quad_[((size_t)cy * nc + cx) * 8 + 2 * v];
-            pts[3 * k + 1] = quad_[((size_t)cy * nc + cx) * 8 + 2 * v + 1];
+            int cc[3] = {0, 0, 0}, v = 0;
+            for (int m = 0; m < dim; ++m)
+              {
+                cc[m] = I[m] < nc ? I[m] : nc - 1;
+                v += (I[m] - cc[m]) << m;
+              }
+            const size_t cell = ((size_t)cc[2] * nc + cc[1]) * nc + cc[0];
+            for (int m = 0; m < dim; ++m)
+              pts[3 * k + m] = verts_[(cell * (1 << dim) + v) * dim + m];
           }
       }
     const std::string base = std::string(dim == 2 ? "solution-ms_coarse-2d" : "solution-ms_coarse-3d") +
@@ -477,13 +488,15 @@ public:
   }
 
   // Coarse mesh with its interior vertices moved by up to amp/nc per axis
-  // (msfem_gen_perturbed_quads; 2D, 0 < amp < 0.25): every cell is then a
-  // general quadrilateral.  The boundary stays the unit square's, so the
-  // coarse boundary data is unchanged.  amp = 0: the uniform mesh.
+  // (msfem_gen_perturbed_quads, 0 < amp < 0.25, or msfem_gen_perturbed_hexes,
+  // 0 < amp < 1/6): every cell is then a general quadrilateral / hexahedron.
+  // The boundary stays the unit square's / cube's, so the coarse boundary data
+  // is unchanged.  amp = 0: the uniform mesh.
   void set_mesh_distortion(double amp, uint64_t seed)
   {
-    if (amp != 0 && (dim != 2 || !(amp > 0 && amp < 0.25)))
-      throw Error(MSFEM_ERR_ARG, "mesh distortion: 2D only, 0 < amp < 0.25");
+    const double amax = dim == 2 ? 0.25 : 1.0 / 6.0;
+    if (amp != 0 && !(amp > 0 && amp < amax))
+      throw Error(MSFEM_ERR_ARG, "mesh distortion: 0 < amp < 0.25 (2D) or 1/6 (3D)");
     distortion_ = amp;
     distortion_seed_ = seed;
   }
@@ -511,7 +524,7 @@ private:
   std::string output_dir_;
   double distortion_ = 0;
   uint64_t distortion_seed_ = 0;
-  std::vector<double> quad_; // moved coarse cells' vertices of the last run (general cells)
+  std::vector<double> verts_; // moved coarse cells' vertices of the last run (general cells)
 };
 
 } // namespace msfem

@@ -30,10 +30,11 @@ struct Field {
   const double *values; // (nf+1)^dim nodal values, lexicographic
 };
 
-// box: {lo_x, hi_x, lo_y, hi_y[, lo_z, hi_z]}; quad (2D, or nullptr): four
-// vertices {x0, y0, ..., x3, y3} in deal.II order, used instead of the box.
+// box: {lo_x, hi_x, lo_y, hi_y[, lo_z, hi_z]}; verts (or nullptr): a general
+// cell's 2^dim vertices {x0, y0[, z0], ...} in deal.II order, used instead of
+// the box (the fine nodes are the cell's bi/trilinear map on the parametric grid).
 inline void write_cell(const std::string &path, int dim, const double *box, int nf, const std::vector<Field> &fields,
-                       const double *quad = nullptr)
+                       const double *verts = nullptr)
 {
   if ((dim != 2 && dim != 3) || nf <= 0)
     throw std::invalid_argument("vtu::write_cell: dim must be 2 or 3 and nf > 0");
@@ -52,12 +53,21 @@ inline void write_cell(const std::string &path, int dim, const double *box, int 
     {
       const int X = (int)(k % np), Y = (int)((k / np) % np), Z = dim == 3 ? (int)(k / ((long long)np * np)) : 0;
       double x, y, z = 0.0;
-      if (quad)
+      if (verts) // the cell's multilinear map at the parametric node
         {
-          const double xi = (double)X / nf, eta = (double)Y / nf;
-          const double w[4] = {(1 - xi) * (1 - eta), xi * (1 - eta), (1 - xi) * eta, xi * eta};
-          x = w[0] * quad[0] + w[1] * quad[2] + w[2] * quad[4] + w[3] * quad[6];
-          y = w[0] * quad[1] + w[1] * quad[3] + w[2] * quad[5] + w[3] * quad[7];
+          const double xi[3] = {(double)X / nf, (double)Y / nf, (double)Z / nf};
+          double p[3] = {0, 0, 0};
+          for (int v = 0; v < nv; ++v)
+            {
+              double w = 1;
+              for (int m = 0; m < dim; ++m)
+                w *= ((v >> m) & 1) ? xi[m] : 1 - xi[m];
+              for (int k = 0; k < dim; ++k)
+                p[k] += w * verts[v * dim + k];
+            }
+          x = p[0];
+          y = p[1];
+          z = p[2];
         }
       else
         {

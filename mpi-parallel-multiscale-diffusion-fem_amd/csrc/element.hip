@@ -33,7 +33,7 @@ __device__ __forceinline__ int sym_index(int dim, int i, int j)
   return 5;
 }
 
-// QUAD: general quadrilaterals (2D, a.verts), geometry per fine cell
+// QUAD: general cells (a.verts: quadrilaterals, hexahedra), geometry per fine cell
 // QPK: the coefficient kind may be one given at the quadrature points; false
 // compiles only the per-fine-cell kinds (their Gauss sums are folded into unit
 // matrices), which keeps the common instance's registers (and so its occupancy)
@@ -51,9 +51,9 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
 
   const int cell = a.cells[blockIdx.x];
   const int nf = a.nf, np = nf + 1;
-  // general quadrilateral (2D): geometry per fine cell from the bilinear map;
-  // the box data below is then unused (unit placeholders)
-  const double *vq = QUAD ? a.verts + (size_t)cell * 8 : nullptr;
+  // general cell: geometry per fine cell from the bi/trilinear map; the box
+  // data below is then unused (unit placeholders)
+  const double *vq = QUAD ? a.verts + (size_t)cell * NV * DIM : nullptr;
   const double unit_box[6] = {0, 1, 0, 1, 0, 1};
   const double *bx = vq ? unit_box : a.box + (size_t)cell * 2 * DIM;
   double h[3] = {1, 1, 1}, lo[3] = {0, 0, 0};
@@ -139,15 +139,16 @@ __global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
       const int fx = f % nf, fy = (f / nf) % nf, fz = DIM == 3 ? f / (nf * nf) : 0;
       double Ke[NV * NV];
       double be[NV];
-      if constexpr (QUAD)
+      if constexpr (QUAD && DIM == 2)
         {
-          static_assert(DIM == 2, "general cells are 2D");
           double K4[4][4];
           qg_element_quad(a.coef_kind, cf, f, nf, QuadMap(vq), K4, a.rhs_kind, a.rhs_value, be);
 #pragma unroll
           for (int ab = 0; ab < 16; ++ab)
             Ke[ab] = K4[ab / 4][ab % 4];
         }
+      else if constexpr (QUAD)
+        qg_element_hex(a.coef_kind, cf, f, nf, HexMap(vq), Ke, a.rhs_kind, a.rhs_value, be);
       else if (!QPK || a.coef_kind == 0 || a.coef_kind == 1)
         {
           double cs[NS];
@@ -475,6 +476,8 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
     return hipSuccess;
   if (a.dim == 2 && a.verts)
     hipLaunchKernelGGL((element_kernel<2, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else if (a.verts) // general hexahedra
+    hipLaunchKernelGGL((element_kernel<3, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.dim == 2 && a.coef_kind <= 1) // per-fine-cell coefficient
     hipLaunchKernelGGL((element_kernel<2, false, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.dim == 2)

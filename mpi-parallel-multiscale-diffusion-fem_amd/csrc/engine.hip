@@ -210,7 +210,7 @@ bool nf_valid(int dim, int nf) { return nf >= 1 && nf <= (dim == 2 ? 1024 : 64);
 // 32^2, whose lane groups hold several problems per workgroup).
 int class_mode(int dim, int coef_kind, int nf, bool general)
 {
-  if (!nf_supported(dim, nf) || (dim == 3 && coef_kind != MSFEM_COEF_SCALAR_CELL))
+  if (!nf_supported(dim, nf) || (dim == 3 && (coef_kind != MSFEM_COEF_SCALAR_CELL || general)))
     return kModeGeneric;
   if (dim == 3)
     return kModeScalar;
@@ -238,6 +238,55 @@ bool quad_ok(const double *v)
   return true;
 }
 
+// A hexahedron (deal.II vertex order) whose trilinear map has a positive
+// Jacobian determinant everywhere, certified by a sufficient test: det J is of
+// degree <= 2 in each parametric variable, and its 27 Bernstein coefficients on
+// [0,1]^3 are all positive (then so is det J, a convex combination of them).
+// Column l of J is multilinear in the other two variables with the cell's four
+// edge vectors along l as Bernstein coefficients, so the coefficient at
+// (a, b, c) in {0,1,2}^3 is the weighted mean of the triple products
+// e_x(j,k) . (e_y(i,k') x e_z(i',j')) over i + i' = a, j + j' = b, k + k' = c
+// (weight 1/2 for each mixed pair, from t(1-t) = B_1^2 / 2).  The eight corner
+// coefficients are the corner determinants, so every negatively oriented or
+// degenerate corner is rejected.
+bool hex_ok(const double *v)
+{
+  auto P = [&](int x, int y, int z) { return v + 3 * (x + 2 * y + 4 * z); };
+  auto edge = [&](const double *p, const double *q, double (&e)[3]) {
+    for (int k = 0; k < 3; ++k)
+      e[k] = q[k] - p[k];
+  };
+  double ex[2][2][3], ey[2][2][3], ez[2][2][3]; // ex[j][k]: edge along x at y = j, z = k; etc.
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      {
+        edge(P(0, i, j), P(1, i, j), ex[i][j]);
+        edge(P(i, 0, j), P(i, 1, j), ey[i][j]);
+        edge(P(i, j, 0), P(i, j, 1), ez[i][j]);
+      }
+  for (int a = 0; a <= 2; ++a)
+    for (int b = 0; b <= 2; ++b)
+      for (int c = 0; c <= 2; ++c)
+        {
+          double sum = 0;
+          for (int i = 0; i < 2; ++i)
+            for (int j = 0; j < 2; ++j)
+              for (int k = 0; k < 2; ++k)
+                {
+                  const int i2 = a - i, j2 = b - j, k2 = c - k;
+                  if (i2 < 0 || i2 > 1 || j2 < 0 || j2 > 1 || k2 < 0 || k2 > 1)
+                    continue;
+                  const double *e1 = ex[j][k], *e2 = ey[i][k2], *e3 = ez[i2][j2];
+                  const double t = e1[0] * (e2[1] * e3[2] - e2[2] * e3[1]) - e1[1] * (e2[0] * e3[2] - e2[2] * e3[0]) +
+                                   e1[2] * (e2[0] * e3[1] - e2[1] * e3[0]);
+                  sum += t * (i != i2 ? 0.5 : 1.0) * (j != j2 ? 0.5 : 1.0) * (k != k2 ? 0.5 : 1.0);
+                }
+          if (!(sum > 0) || !std::isfinite(sum))
+            return false;
+        }
+  return true;
+}
+
 int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
 {
   if (!in || !o)
@@ -246,8 +295,6 @@ int validate(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o)
     return fail(ctx, MSFEM_ERR_ARG, "dim must be 2 or 3 (got %d)", in->dim);
   if (in->n_cells < 0 || (in->n_cells > 0 && (!in->n_fine || (!in->box && !in->vertices) || !in->coef)))
     return fail(ctx, MSFEM_ERR_ARG, "n_cells < 0 or a required array is NULL");
-  if (in->vertices && in->dim != 2)
-    return fail(ctx, MSFEM_ERR_ARG, "general (vertex) cells are supported in 2D only");
   if (coef_stride(in->dim, in->coef_kind) < 0)
     return fail(ctx, MSFEM_ERR_ARG, "unknown coefficient kind %d", in->coef_kind);
   if (in->rhs_kind != MSFEM_RHS_CONST && in->rhs_kind != MSFEM_RHS_SINSIN)
@@ -462,8 +509,10 @@ int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solv
   if (mode == kModeGeneric)
     HIP_TRY(ctx, launch_generic_prep(in->dim, ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
                                      d_off(ctx, 0), cond, d_off(ctx, 2), st));
-  if (in->vertices) // the solve kernels read the boundary data from the bases
+  if (in->vertices && in->dim == 2) // the solve kernels read the boundary data from the bases
     HIP_TRY(ctx, launch_quad_dirichlet_2d(ncls, cells, nf, in->vertices, bases, d_off(ctx, 1), st));
+  else if (in->vertices)
+    HIP_TRY(ctx, launch_hex_dirichlet_3d(ncls, cells, nf, in->vertices, bases, d_off(ctx, 1), st));
   if (class_coarse(ctx, in, o, nf))
     {
       double *base = (double *)ctx->bufs["coarse_inv"].p;
@@ -1145,11 +1194,14 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
     return MSFEM_OK;
   if (!out || !out->Ae || !out->Fe)
     return fail(ctx, MSFEM_ERR_ARG, "Ae and Fe outputs are required");
+  const size_t nvd = (size_t)(1 << in->dim) * in->dim; // doubles per general cell
   if (in->vertices)
     for (int c = 0; c < in->n_cells; ++c)
-      if (!quad_ok(in->vertices + (size_t)c * 8))
-        return fail(ctx, MSFEM_ERR_ARG, "cell %d: the vertices do not form a convex, positively oriented "
-                    "quadrilateral (deal.II vertex order)", c);
+      if (in->dim == 2 ? !quad_ok(in->vertices + (size_t)c * nvd) : !hex_ok(in->vertices + (size_t)c * nvd))
+        return fail(ctx, MSFEM_ERR_ARG, in->dim == 2 ? "cell %d: the vertices do not form a convex, positively "
+                    "oriented quadrilateral (deal.II vertex order)" : "cell %d: the trilinear map of the vertices "
+                    "is not certified positively oriented (deal.II vertex order; Jacobian determinant's Bernstein "
+                    "coefficients must be > 0)", c);
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int dim = in->dim, nb = 1 << dim, n = in->n_cells;
   long long coef_total = 0, bas_total = 0;
@@ -1161,7 +1213,7 @@ int msfem_batch_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_
   HostStage s{ctx};
   msfem_batch din = *in;
   din.box = s.up("h_box", in->box, (size_t)n * 2 * dim);
-  din.vertices = s.up("h_vertices", in->vertices, (size_t)n * 8);
+  din.vertices = s.up("h_vertices", in->vertices, (size_t)n * nvd);
   din.coef = s.up("h_coef", in->coef, (size_t)coef_total);
   din.x0 = s.up("h_x0", in->x0, (size_t)bas_total);
   msfem_batch_out dout;

@@ -8,8 +8,8 @@
 // include/coefficients/matrix_coeff.tpp:28-41,66-91).
 //
 // prep: every fine cell's QGauss(2) element matrix (diffusion_problem_basis.tpp:
-// 207-222; general quadrilaterals with the bilinear mapping) into the class's
-// element buffer, shared by the cell's 2^d bases.
+// 207-222; general quadrilaterals and hexahedra with the bi/trilinear mapping)
+// into the class's element buffer, shared by the cell's 2^d bases.
 // solve: one 256-thread workgroup per (cell, basis) problem; FP64
 // Jacobi-preconditioned CG on the interior equations with the iterate in the
 // bases block and r, p, q, D^-1 in per-problem scratch; the operator gathers,
@@ -74,8 +74,10 @@ __device__ inline void qg_element_3d(int kind, const double *cf, int f, const do
     }
 }
 
+constexpr int kGenPrepThreads = 256;
+
 template <int DIM>
-__global__ void generic_prep_kernel(int n_cells_class, const int *cells, int nf, int kind, const double *box,
+__global__ void __launch_bounds__(kGenPrepThreads) generic_prep_kernel(int n_cells_class, const int *cells, int nf, int kind, const double *box,
                                     const double *verts, const double *coef, const long long *coef_off, double *elem,
                                     const long long *elem_off)
 {
@@ -101,6 +103,13 @@ __global__ void generic_prep_kernel(int n_cells_class, const int *cells, int nf,
           for (int u = 0; u < 4; ++u)
             for (int v = 0; v < 4; ++v)
               E[u * 4 + v] = K[u][v];
+        }
+      else if (verts)
+        {
+          double K[64];
+          qg_element_hex(kind, cf, f, nf, HexMap(verts + (size_t)cell * 24), K, 0, 0.0, nullptr);
+          for (int uv = 0; uv < 64; ++uv)
+            E[uv] = K[uv];
         }
       else
         {
@@ -220,7 +229,7 @@ __global__ void __launch_bounds__(kGenThreads) generic_solve_kernel(LocalArgs a,
       double v = 0.0;
       if (a.x0)
         v = a.x0[a.bas_off[cell] + (size_t)basis * nn + n];
-      else if (a.q1_start && !general)
+      else if (a.q1_start && (!general || DIM == 3)) // (hexahedra: the parametric nodal Q1 function)
         v = nodal_q1(basis, DIM, I[0], I[1], I[2], nf);
       else if (a.q1_start) // transfinite interpolant of the boundary values over the parametric grid
         {
@@ -334,6 +343,97 @@ __global__ void __launch_bounds__(kGenThreads) generic_solve_kernel(LocalArgs a,
     }
 }
 
+// Dirichlet data of general hexahedra: BasisQ1 (include/coefficients/basis_q1.tpp:
+// 26-46, :86-96) is N_i(x) = sum_m C[m][i] mono_m(x), mono = (1, x, y, z, xy,
+// yz, xz, xyz), C = M^{-1}, M[j][m] = mono_m(v_j).  One workgroup per cell:
+// one thread inverts M (Gauss-Jordan with partial pivoting, as deal.II's
+// FullMatrix::invert) in LDS, then the workgroup evaluates N_i at the mapped
+// boundary nodes F(I / nf) and writes them into every basis block.  Boundary
+// node k: the planes z = 0 and z = nf whole (2 (nf+1)^2 nodes), then the ring
+// of each inner plane z = 1 .. nf-1 (4 nf nodes each).
+constexpr int kHexBcThreads = 256;
+
+__global__ void __launch_bounds__(kHexBcThreads) hex_dirichlet_3d(const int *cells, int nf, const double *verts,
+                                                                   double *bases, const long long *bas_off)
+{
+  __shared__ double a[8][16];
+  const int cell = cells[blockIdx.x];
+  const double *v = verts + (size_t)cell * 24;
+  if (threadIdx.x == 0)
+    {
+      for (int r = 0; r < 8; ++r)
+        {
+          const double x = v[3 * r], y = v[3 * r + 1], z = v[3 * r + 2];
+          const double mono[8] = {1.0, x, y, z, x * y, y * z, x * z, x * y * z};
+          for (int c = 0; c < 8; ++c)
+            {
+              a[r][c] = mono[c];
+              a[r][8 + c] = r == c ? 1.0 : 0.0;
+            }
+        }
+      for (int c = 0; c < 8; ++c)
+        {
+          int piv = c;
+          for (int r = c + 1; r < 8; ++r)
+            if (fabs(a[r][c]) > fabs(a[piv][c]))
+              piv = r;
+          if (piv != c)
+            for (int m = 0; m < 16; ++m)
+              {
+                const double t = a[c][m];
+                a[c][m] = a[piv][m];
+                a[piv][m] = t;
+              }
+          const double d = 1.0 / a[c][c];
+          for (int m = 0; m < 16; ++m)
+            a[c][m] *= d;
+          for (int r = 0; r < 8; ++r)
+            if (r != c)
+              {
+                const double fct = a[r][c];
+                for (int m = 0; m < 16; ++m)
+                  a[r][m] -= fct * a[c][m];
+              }
+        }
+    }
+  __syncthreads();
+  const HexMap map(v);
+  const int np = nf + 1;
+  const size_t nn = (size_t)np * np * np;
+  const int nplane = np * np, nring = 4 * nf;
+  const int nbnd = 2 * nplane + (nf - 1) * nring;
+  double *phi = bases + bas_off[cell];
+  for (int k = threadIdx.x; k < nbnd; k += kHexBcThreads)
+    {
+      int X, Y, Z;
+      if (k < 2 * nplane)
+        {
+          Z = k < nplane ? 0 : nf;
+          X = (k % nplane) % np;
+          Y = (k % nplane) / np;
+        }
+      else
+        {
+          const int kk = k - 2 * nplane;
+          Z = 1 + kk / nring;
+          const int side = kk % nring / nf, j = kk % nring % nf;
+          X = side == 0 ? j : side == 1 ? nf : side == 2 ? nf - j : 0;
+          Y = side == 0 ? 0 : side == 1 ? j : side == 2 ? nf : nf - j;
+        }
+      double x[3];
+      map.point((double)X / nf, (double)Y / nf, (double)Z / nf, x);
+      const double mono[8] = {1.0, x[0], x[1], x[2], x[0] * x[1], x[1] * x[2], x[0] * x[2], x[0] * x[1] * x[2]};
+      const size_t node = (size_t)X + (size_t)np * (Y + (size_t)np * Z);
+      for (int i = 0; i < 8; ++i)
+        {
+          double val = 0;
+          for (int m = 0; m < 8; ++m)
+            val += a[m][8 + i] * mono[m];
+          phi[(size_t)i * nn + node] = val;
+        }
+    }
+}
+
 } // namespace
 
 size_t generic_elem_doubles(int dim, int nf)
@@ -355,15 +455,25 @@ hipError_t launch_generic_prep(int dim, int n_cells_class, const int *cells, int
   const long long total = (long long)n_cells_class * (dim == 2 ? (long long)nf * nf : (long long)nf * nf * nf);
   if (total == 0)
     return hipSuccess;
-  long long grid = (total + 255) / 256;
+  long long grid = (total + kGenPrepThreads - 1) / kGenPrepThreads;
   if (grid > 65536)
     grid = 65536;
   if (dim == 2)
-    hipLaunchKernelGGL(generic_prep_kernel<2>, dim3((unsigned)grid), dim3(256), 0, st, n_cells_class, cells, nf,
-                       coef_kind, box, verts, coef, coef_off, elem, elem_off);
+    hipLaunchKernelGGL(generic_prep_kernel<2>, dim3((unsigned)grid), dim3(kGenPrepThreads), 0, st, n_cells_class,
+                       cells, nf, coef_kind, box, verts, coef, coef_off, elem, elem_off);
   else
-    hipLaunchKernelGGL(generic_prep_kernel<3>, dim3((unsigned)grid), dim3(256), 0, st, n_cells_class, cells, nf,
-                       coef_kind, box, verts, coef, coef_off, elem, elem_off);
+    hipLaunchKernelGGL(generic_prep_kernel<3>, dim3((unsigned)grid), dim3(kGenPrepThreads), 0, st, n_cells_class,
+                       cells, nf, coef_kind, box, verts, coef, coef_off, elem, elem_off);
+  return hipGetLastError();
+}
+
+hipError_t launch_hex_dirichlet_3d(int n_cells_class, const int *cells, int nf, const double *verts, double *bases,
+                                  const long long *bas_off, hipStream_t st)
+{
+  if (n_cells_class == 0)
+    return hipSuccess;
+  hipLaunchKernelGGL(hex_dirichlet_3d, dim3(n_cells_class), dim3(kHexBcThreads), 0, st, cells, nf, verts, bases,
+                     bas_off);
   return hipGetLastError();
 }
 

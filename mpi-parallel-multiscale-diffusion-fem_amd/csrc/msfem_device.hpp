@@ -51,8 +51,8 @@ struct LocalArgs {
   int nb_solve;               // bases solved per cell
   int nb;                     // 2^dim
   const int *cells;           // [cells_in_class] batch cell index
-  const double *box;          // [n_cells][dim][2] (unused for general quadrilaterals)
-  const double *verts;        // [n_cells][4][2] general quadrilaterals (2D) or nullptr: boxes
+  const double *box;          // [n_cells][dim][2] (unused for general cells)
+  const double *verts;        // [n_cells][2^d][d] general quadrilaterals / hexahedra or nullptr: boxes
   const double *coef;         // concatenated coefficient tiles
   const long long *coef_off;  // [n_cells] tile offsets (elements)
   const double *cond;         // kModeGlobal: concatenated conductance tiles
@@ -474,6 +474,147 @@ __device__ inline void qg_element_quad(int coef_kind, const double *cf, int f, i
               fv = sin(M_PI * x) * sin(M_PI * y);
             }
           for (int u = 0; u < 4; ++u)
+            be[u] += val[u] * fv * jxw;
+        }
+    }
+}
+
+// Trilinear map of a hexahedron with vertices v[8][3] in deal.II order (vertex
+// w at the parametric corner (w&1, (w>>1)&1, (w>>2)&1)), in monomial form
+// F = c0 + xi c1 + eta c2 + zeta c3 + xi eta c4 + eta zeta c5 + xi zeta c6
+//     + xi eta zeta c7.
+struct HexMap {
+  double c[8][3];
+  __device__ explicit HexMap(const double *v)
+  {
+    for (int k = 0; k < 3; ++k)
+      {
+        const double v0 = v[k], v1 = v[3 + k], v2 = v[6 + k], v3 = v[9 + k];
+        const double v4 = v[12 + k], v5 = v[15 + k], v6 = v[18 + k], v7 = v[21 + k];
+        c[0][k] = v0;
+        c[1][k] = v1 - v0;
+        c[2][k] = v2 - v0;
+        c[3][k] = v4 - v0;
+        c[4][k] = v3 - v2 - v1 + v0;
+        c[5][k] = v6 - v4 - v2 + v0;
+        c[6][k] = v5 - v4 - v1 + v0;
+        c[7][k] = v7 - v6 - v5 - v3 + v4 + v2 + v1 - v0;
+      }
+  }
+  __device__ void point(double xi, double eta, double zeta, double (&x)[3]) const
+  {
+    for (int k = 0; k < 3; ++k)
+      x[k] = c[0][k] + xi * c[1][k] + eta * c[2][k] + zeta * c[3][k] + xi * eta * c[4][k] + eta * zeta * c[5][k] +
+             xi * zeta * c[6][k] + xi * eta * zeta * c[7][k];
+  }
+  // J[k][l] = d x_k / d xi_l
+  __device__ void jacobian(double xi, double eta, double zeta, double (&J)[3][3]) const
+  {
+    for (int k = 0; k < 3; ++k)
+      {
+        J[k][0] = c[1][k] + eta * c[4][k] + zeta * c[6][k] + eta * zeta * c[7][k];
+        J[k][1] = c[2][k] + xi * c[4][k] + zeta * c[5][k] + xi * zeta * c[7][k];
+        J[k][2] = c[3][k] + eta * c[5][k] + xi * c[6][k] + xi * eta * c[7][k];
+      }
+  }
+};
+
+// QGauss(2) element matrix (and optionally load) of fine cell f of a general
+// hexahedron refined nf times per side (diffusion_problem_basis.tpp:161-242 on
+// the mesh of GridGenerator::general_cell + refine_global, :92-99): the fine
+// cell is the image of the parametric box [fx, fx+1] x [fy, fy+1] x [fz, fz+1]
+// / nf under the coarse trilinear map F, also its own trilinear mapping, so its
+// Jacobian at reference point s is dF/dxi / nf at xi = (f + s) / nf;
+// grad psi = J^{-T} grad_s psi (J^{-1} by cofactors), JxW = det J / 8;
+// coefficient and f at the mapped point.  K: [8 x 8] row-major.  be may be nullptr.
+__device__ inline void qg_element_hex(int coef_kind, const double *cf, int f, int nf, const HexMap &m,
+                                      double (&K)[64], int rhs_kind, double rhs_value, double *be)
+{
+#pragma unroll
+  for (int uv = 0; uv < 64; ++uv)
+    K[uv] = 0;
+  if (be)
+    for (int u = 0; u < 8; ++u)
+      be[u] = 0;
+  const double gs = 0.5 / sqrt(3.0), inv = 1.0 / nf;
+  const int fi[3] = {f % nf, (f / nf) % nf, f / (nf * nf)};
+  for (int q = 0; q < 8; ++q)
+    {
+      double s[3], xi[3];
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        {
+          s[k] = ((q >> k) & 1) ? 0.5 + gs : 0.5 - gs;
+          xi[k] = (fi[k] + s[k]) * inv;
+        }
+      double J[3][3];
+      m.jacobian(xi[0], xi[1], xi[2], J);
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          J[k][l] *= inv;
+      double C[3][3]; // cofactors: J^{-1} = C^T / det
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          {
+            const int k1 = (k + 1) % 3, k2 = (k + 2) % 3, l1 = (l + 1) % 3, l2 = (l + 2) % 3;
+            C[k][l] = J[k1][l1] * J[k2][l2] - J[k1][l2] * J[k2][l1];
+          }
+      const double det = J[0][0] * C[0][0] + J[0][1] * C[0][1] + J[0][2] * C[0][2], rdet = 1.0 / det;
+      double A[3][3];
+      const int sidx[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int l = 0; l < 3; ++l)
+          switch (coef_kind)
+            {
+              case 0: A[k][l] = k == l ? cf[f] : 0.0; break;
+              case 1: A[k][l] = cf[6 * (size_t)f + sidx[k][l]]; break;
+              case 2: A[k][l] = k == l ? cf[8 * (size_t)f + q] : 0.0; break;
+              default: A[k][l] = cf[48 * (size_t)f + 6 * q + sidx[k][l]]; break;
+            }
+      double g[8][3], val[8];
+#pragma unroll
+      for (int v = 0; v < 8; ++v)
+        {
+          double f1[3], gr[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            f1[k] = ((v >> k) & 1) ? s[k] : 1.0 - s[k];
+          gr[0] = ((v & 1) ? 1.0 : -1.0) * f1[1] * f1[2];
+          gr[1] = ((v & 2) ? 1.0 : -1.0) * f1[0] * f1[2];
+          gr[2] = ((v & 4) ? 1.0 : -1.0) * f1[0] * f1[1];
+          val[v] = f1[0] * f1[1] * f1[2];
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            g[v][k] = (C[k][0] * gr[0] + C[k][1] * gr[1] + C[k][2] * gr[2]) * rdet;
+        }
+      const double jxw = 0.125 * det;
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        {
+          double Ag[3];
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            Ag[k] = (A[k][0] * g[u][0] + A[k][1] * g[u][1] + A[k][2] * g[u][2]) * jxw;
+#pragma unroll
+          for (int v = 0; v < 8; ++v)
+            K[u * 8 + v] += Ag[0] * g[v][0] + Ag[1] * g[v][1] + Ag[2] * g[v][2];
+        }
+      if (be)
+        {
+          double fv = rhs_value;
+          if (rhs_kind == 1)
+            {
+              double x[3];
+              m.point(xi[0], xi[1], xi[2], x);
+              fv = sin(M_PI * x[0]) * sin(M_PI * x[1]) * sin(M_PI * x[2]);
+            }
+          for (int u = 0; u < 8; ++u)
             be[u] += val[u] * fv * jxw;
         }
     }

@@ -228,6 +228,76 @@ int msfem_gen_coef_reference_quad(int n_cells, const int32_t *n_fine, const doub
   return MSFEM_OK;
 }
 
+int msfem_gen_perturbed_hexes(int nc, double amp, uint64_t seed, double *vertices)
+{
+  if (nc <= 0 || !(amp >= 0) || !(amp < 1.0 / 6.0) || !vertices)
+    return MSFEM_ERR_ARG;
+  const int np = nc + 1;
+  const uint32_t key[2] = {(uint32_t)seed, (uint32_t)(seed >> 32)};
+  std::vector<double> xyz((size_t)np * np * np * 3);
+  for (int l = 0; l < np; ++l)
+    for (int j = 0; j < np; ++j)
+      for (int i = 0; i < np; ++i)
+        {
+          const size_t n = ((size_t)l * np + j) * np + i;
+          double x[3] = {(double)i / nc, (double)j / nc, (double)l / nc};
+          if (i > 0 && i < nc && j > 0 && j < nc && l > 0 && l < nc)
+            {
+              const uint32_t ctr[4] = {(uint32_t)n, 0u, 3u, 0u};
+              uint32_t u[4];
+              philox(ctr, key, u);
+              for (int k = 0; k < 3; ++k)
+                x[k] += amp * (2.0 * unit_open(u[k]) - 1.0) / nc;
+            }
+          for (int k = 0; k < 3; ++k)
+            xyz[3 * n + k] = x[k];
+        }
+  for (int c = 0; c < nc * nc * nc; ++c)
+    {
+      const int cx = c % nc, cy = (c / nc) % nc, cz = c / (nc * nc);
+      for (int v = 0; v < 8; ++v)
+        {
+          const size_t n = ((size_t)(cz + (v >> 2)) * np + cy + ((v >> 1) & 1)) * np + cx + (v & 1);
+          for (int k = 0; k < 3; ++k)
+            vertices[(size_t)c * 24 + 3 * v + k] = xyz[3 * n + k];
+        }
+    }
+  return MSFEM_OK;
+}
+
+int msfem_gen_coef_reference_hex(int n_cells, const int32_t *n_fine, const double *vertices, double *coef)
+{
+  if (n_cells < 0 || !n_fine || !vertices || !coef)
+    return MSFEM_ERR_ARG;
+  size_t off = 0;
+  for (int c = 0; c < n_cells; ++c)
+    {
+      const int nf = n_fine[c];
+      const double *v = vertices + (size_t)c * 24;
+      for (int f = 0; f < nf * nf * nf; ++f)
+        {
+          const int fi[3] = {f % nf, (f / nf) % nf, f / (nf * nf)};
+          for (int q = 0; q < 8; ++q)
+            {
+              double xi[3], p[3] = {0, 0, 0};
+              for (int k = 0; k < 3; ++k)
+                xi[k] = (fi[k] + gauss_pt((q >> k) & 1)) / nf;
+              for (int w = 0; w < 8; ++w)
+                {
+                  double wt = 1;
+                  for (int m = 0; m < 3; ++m)
+                    wt *= ((w >> m) & 1) ? xi[m] : 1 - xi[m];
+                  for (int k = 0; k < 3; ++k)
+                    p[k] += wt * v[3 * w + k];
+                }
+              matrix_coeff(3, p, coef + off);
+              off += 6;
+            }
+        }
+    }
+  return MSFEM_OK;
+}
+
 int msfem_gen_coef_periodic(int n_cells, const int32_t *n_fine, const double *box, double eps, double *coef)
 {
   if (n_cells < 0 || !n_fine || !box || !coef || !(eps > 0))

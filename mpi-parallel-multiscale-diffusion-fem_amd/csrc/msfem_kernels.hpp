@@ -42,6 +42,10 @@ hipError_t launch_generic_prep(int dim, int n_cells_class, const int *cells, int
                                const double *verts, const double *coef, const long long *coef_off, double *elem,
                                const long long *elem_off, hipStream_t st);
 hipError_t launch_generic_solve(int dim, int nf, const LocalArgs &a, double *work, hipStream_t st);
+// Dirichlet data N_i (BasisQ1 by monomial inversion) of general hexahedra at
+// their boundary fine nodes, written into every basis block.
+hipError_t launch_hex_dirichlet_3d(int n_cells_class, const int *cells, int nf, const double *verts, double *bases,
+                                   const long long *bas_off, hipStream_t st);
 
 // element.hip: A^K, F^K of every cell of one size class; partition-of-unity fix-up.
 struct ElementArgs {
@@ -50,7 +54,7 @@ struct ElementArgs {
   int dim, nf, nb;
   int coef_kind;              // coefficient kind of the batch (MSFEM_COEF_*)
   const double *box;
-  const double *verts;        // general quadrilaterals [n_cells][4][2] (2D) or nullptr
+  const double *verts;        // general cells [n_cells][2^d][d] or nullptr
   const double *coef;
   const long long *coef_off;
   const double *cond;         // 2D conductance tiles when the class used kModeGlobal (else nullptr)

@@ -33,7 +33,8 @@ EXPORTS = [
     "msfem_coarse_solve", "msfem_coarse_solve_device", "msfem_coarse_solve_device_noalloc", "msfem_gather_weights", "msfem_reconstruct_device",
     "msfem_philox4x32_10", "msfem_gen_unit_boxes", "msfem_gen_coef_reference", "msfem_gen_coef_periodic",
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
-    "msfem_gen_perturbed_quads", "msfem_gen_coef_reference_quad",
+    "msfem_gen_perturbed_quads", "msfem_gen_coef_reference_quad", "msfem_gen_perturbed_hexes",
+    "msfem_gen_coef_reference_hex",
     "msfem_fine_solve", "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_conv_bf16", "msfem_nn_conv3x3_pack", "msfem_nn_conv3x3_bf16", "msfem_nn_conv_small", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
     "msfem_nn_bias_grad", "msfem_nn_pool2", "msfem_nn_up2", "msfem_nn_axpy", "msfem_nn_axpby", "msfem_nn_to_bf16",
     "msfem_nn_mse", "msfem_nn_adam", "msfem_nn_delta_to_bases", "msfem_nn_bases_to_delta",
@@ -122,6 +123,8 @@ def lib():
         L.msfem_gen_grf_log.argtypes = [i32, i32, ctypes.c_uint64, dbl, dbl, vp]
         L.msfem_gen_perturbed_quads.argtypes = [i32, dbl, ctypes.c_uint64, vp]
         L.msfem_gen_coef_reference_quad.argtypes = [i32, vp, vp, vp]
+        L.msfem_gen_perturbed_hexes.argtypes = [i32, dbl, ctypes.c_uint64, vp]
+        L.msfem_gen_coef_reference_hex.argtypes = [i32, vp, vp, vp]
         L.msfem_fine_solve.argtypes = [vp, i32, i32, i32, vp, i32, dbl, i32, dbl, i32, vp, vp]
         _lib = L
     return _lib
@@ -253,7 +256,8 @@ class Context:
     # ---- the hot path, host arrays ------------------------------------------------
     def batch_build(self, dim, n_fine, box, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, x0=None,
                     opts=None, want_bases=True, raise_noconv=True, vertices=None, energy_bound=False):
-        """vertices: [n][4][2] general quadrilaterals (2D; box is then ignored, may be None).
+        """vertices: general cells, [n][4][2] quadrilaterals (2D) or [n][8][3] hexahedra (3D);
+        box is then ignored and may be None.
         energy_bound: also return the a posteriori bound of each basis's relative
         energy-norm error ("energy_bound", [n][nb])."""
         n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
@@ -442,6 +446,21 @@ def coef_reference_quad(n_fine, vertices):
     vertices = np.ascontiguousarray(vertices, dtype=np.float64)
     out = np.zeros(_tile_total(2, n_fine, COEF_TENSOR_QP))
     _check(lib().msfem_gen_coef_reference_quad(len(n_fine), _p(n_fine), _p(vertices), _p(out)))
+    return out
+
+
+def perturbed_hexes(nc, amp, seed):
+    """[nc^3][8][3] vertices of the unit cube's nc^3 mesh with moved interior vertices (amp < 1/6)."""
+    out = np.zeros((nc ** 3, 8, 3))
+    _check(lib().msfem_gen_perturbed_hexes(nc, amp, seed, _p(out)))
+    return out
+
+
+def coef_reference_hex(n_fine, vertices):
+    n_fine = np.ascontiguousarray(n_fine, dtype=np.int32)
+    vertices = np.ascontiguousarray(vertices, dtype=np.float64)
+    out = np.zeros(_tile_total(3, n_fine, COEF_TENSOR_QP))
+    _check(lib().msfem_gen_coef_reference_hex(len(n_fine), _p(n_fine), _p(vertices), _p(out)))
     return out
 
 

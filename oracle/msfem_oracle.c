@@ -26,11 +26,11 @@
  *       diffusion_problem_ms.tpp:260-296
  *   - fine reconstruction u|_K = sum_i w_i phi_i
  *       diffusion_problem_basis.tpp:354-377
- *   - general (non-box) quadrilateral coarse cells, 2D: the fine mesh of
- *       GridGenerator::general_cell + refine_global (diffusion_problem_basis.tpp:
- *       92-99), FE_Q(1) with the bilinear mapping and QGauss(2) on every fine
- *       quadrilateral (:161-242), Dirichlet data N_i of BasisQ1 at the mapped
- *       boundary nodes (basis_q1.tpp:26-46, :86-96)
+ *   - general (non-box) coarse cells, quadrilaterals (2D) and hexahedra (3D):
+ *       the fine mesh of GridGenerator::general_cell + refine_global
+ *       (diffusion_problem_basis.tpp:92-99), FE_Q(1) with the bi/trilinear
+ *       mapping and QGauss(2) on every fine cell (:161-242), Dirichlet data N_i
+ *       of BasisQ1 at the mapped boundary nodes (basis_q1.tpp:26-46, :86-96)
  *   - the reference default coefficient MatrixCoeff (with PI_D, k=57,
  *       scale 0.9999) in long double: include/coefficients/matrix_coeff.hpp:45-49,
  *       matrix_coeff.tpp:17-91, coefficients.h:21
@@ -211,24 +211,45 @@ void orc_box_vertices(int dim, const double *box, double *verts)
       verts[v * dim + k] = ((v >> k) & 1) ? box[2 * k + 1] : box[2 * k];
 }
 
-/* Bilinear map of a quadrilateral with vertices quad[4][2] (deal.II order) at
- * the parametric point (xi, eta) in [0,1]^2. */
-static void quad_map(const double *quad, double xi, double eta, double *x)
+/* Multilinear map of a general cell with vertices verts[2^d][d] in deal.II
+ * order (vertex v sits at the parametric corner (v&1, (v>>1)&1, (v>>2)&1)) at
+ * the parametric point xi[d] in [0,1]^d: bilinear for a quadrilateral,
+ * trilinear for a hexahedron. */
+static void cell_map(int dim, const double *verts, const double *xi, double *x)
 {
-  for (int k = 0; k < 2; ++k)
-    x[k] = (1 - xi) * (1 - eta) * quad[k] + xi * (1 - eta) * quad[2 + k] + (1 - xi) * eta * quad[4 + k] +
-           xi * eta * quad[6 + k];
+  for (int k = 0; k < dim; ++k)
+    x[k] = 0;
+  for (int v = 0; v < (1 << dim); ++v)
+    {
+      double w = 1;
+      for (int m = 0; m < dim; ++m)
+        w *= ((v >> m) & 1) ? xi[m] : 1 - xi[m];
+      for (int k = 0; k < dim; ++k)
+        x[k] += w * verts[v * dim + k];
+    }
 }
 
-/* Fine node (X, Y) of a general quadrilateral refined nf times per side.
+/* Fine node I[d] of a general cell refined nf times per side.
  * GridGenerator::general_cell + refine_global (diffusion_problem_basis.tpp:
- * 92-99) put each new vertex at an edge midpoint or at the mean of a cell's four
- * vertices; on a bilinear cell both are the bilinear map's values at the
- * parametric midpoints, so the refined vertices are the map's values on the
- * uniform parametric grid. */
+ * 92-99) put each new vertex at an edge midpoint, at the mean of a face's four
+ * vertices or (3D) at the mean of a cell's eight vertices; on a multilinear
+ * cell these are the map's values at the parametric midpoints, and a child of
+ * a multilinear cell is the image of its parametric sub-box under the parent's
+ * map (the restriction is again multilinear), so the refined vertices are the
+ * map's values on the uniform parametric grid. */
+void orc_cell_node(int dim, const double *verts, int nf, const int *I, double *x)
+{
+  double xi[3];
+  for (int k = 0; k < dim; ++k)
+    xi[k] = (double)I[k] / nf;
+  cell_map(dim, verts, xi, x);
+}
+
+/* The same for a quadrilateral (2D): fine node (X, Y). */
 void orc_quad_node(const double *quad, int nf, int X, int Y, double *x)
 {
-  quad_map(quad, (double)X / nf, (double)Y / nf, x);
+  const int I[2] = {X, Y};
+  orc_cell_node(2, quad, nf, I, x);
 }
 
 /* ------------------------------------------------------------------ */
@@ -501,6 +522,107 @@ void orc_local_assemble_quad(int nf, const double *quad, int coef_kind, const do
     }
 }
 
+/* Element matrix and load of one fine hexahedron with vertices fv[8][3]
+ * (deal.II order): FE_Q(1) with the trilinear mapping (MappingQ1), QGauss(2)
+ * (diffusion_problem_basis.tpp:163-222).  At each quadrature point
+ * J_kl = sum_a fv[a][k] d psi_a / d xi_l, grad psi_a = J^{-T} grad_xi psi_a
+ * (J^{-1} by cofactors), JxW = det J / 8; coefficient and f at the mapped point. */
+void orc_fine_element_hex(const double *fv, int coef_kind, const double *cdata, int rhs_kind, double rhs_value,
+                          double *Ke /* [8][8] */, double *be /* [8] */)
+{
+  memset(Ke, 0, sizeof(double) * 64);
+  memset(be, 0, sizeof(double) * 8);
+  for (int q = 0; q < 8; ++q)
+    {
+      double xi[3];
+      for (int k = 0; k < 3; ++k)
+        xi[k] = gauss_pt((q >> k) & 1);
+      double val[8], gr[8][3];
+      for (int a = 0; a < 8; ++a)
+        {
+          double f1[3], d1[3];
+          for (int k = 0; k < 3; ++k)
+            {
+              f1[k] = ((a >> k) & 1) ? xi[k] : 1 - xi[k];
+              d1[k] = ((a >> k) & 1) ? 1.0 : -1.0;
+            }
+          val[a]   = f1[0] * f1[1] * f1[2];
+          gr[a][0] = d1[0] * f1[1] * f1[2];
+          gr[a][1] = f1[0] * d1[1] * f1[2];
+          gr[a][2] = f1[0] * f1[1] * d1[2];
+        }
+      double J[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, xq[3] = {0, 0, 0};
+      for (int a = 0; a < 8; ++a)
+        for (int k = 0; k < 3; ++k)
+          {
+            xq[k] += val[a] * fv[3 * a + k];
+            for (int l = 0; l < 3; ++l)
+              J[k][l] += fv[3 * a + k] * gr[a][l];
+          }
+      /* cofactor matrix C_kl = (-1)^(k+l) minor_kl; J^{-1} = C^T / det */
+      double Cf[3][3];
+      for (int k = 0; k < 3; ++k)
+        for (int l = 0; l < 3; ++l)
+          {
+            const int k1 = (k + 1) % 3, k2 = (k + 2) % 3, l1 = (l + 1) % 3, l2 = (l + 2) % 3;
+            Cf[k][l] = J[k1][l1] * J[k2][l2] - J[k1][l2] * J[k2][l1];
+          }
+      const double det = J[0][0] * Cf[0][0] + J[0][1] * Cf[0][1] + J[0][2] * Cf[0][2];
+      /* (J^{-T} g)_k = sum_l (J^{-1})_lk g_l = sum_l C_kl g_l / det */
+      double g[8][3];
+      for (int a = 0; a < 8; ++a)
+        for (int k = 0; k < 3; ++k)
+          g[a][k] = (Cf[k][0] * gr[a][0] + Cf[k][1] * gr[a][1] + Cf[k][2] * gr[a][2]) / det;
+      double A[3][3];
+      qp_tensor(3, coef_kind, cdata, q, A);
+      const double f = rhs_eval(3, rhs_kind, rhs_value, xq), jxw = det * 0.125;
+      for (int a = 0; a < 8; ++a)
+        {
+          for (int b = 0; b < 8; ++b)
+            {
+              double t = 0;
+              for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j)
+                  t += g[a][i] * A[i][j] * g[b][j];
+              Ke[a * 8 + b] += t * jxw;
+            }
+          be[a] += val[a] * f * jxw;
+        }
+    }
+}
+
+/* Local system of a general hexahedron (3D), as orc_local_assemble. */
+void orc_local_assemble_hex(int nf, const double *hex, int coef_kind, const double *tile, int rhs_kind,
+                            double rhs_value, double *Kst, double *b)
+{
+  const int np = nf + 1, nn = np * np * np;
+  const int stride = orc_coef_stride(3, coef_kind);
+  memset(Kst, 0, sizeof(double) * (size_t)nn * 27);
+  memset(b, 0, sizeof(double) * (size_t)nn);
+  for (int c = 0; c < nf * nf * nf; ++c)
+    {
+      const int ci[3] = {c % nf, (c / nf) % nf, c / (nf * nf)};
+      double fv[24], Ke[64], be[8];
+      for (int a = 0; a < 8; ++a)
+        {
+          const int I[3] = {ci[0] + (a & 1), ci[1] + ((a >> 1) & 1), ci[2] + ((a >> 2) & 1)};
+          orc_cell_node(3, hex, nf, I, fv + 3 * a);
+        }
+      orc_fine_element_hex(fv, coef_kind, tile + (size_t)c * stride, rhs_kind, rhs_value, Ke, be);
+      for (int a = 0; a < 8; ++a)
+        {
+          const int na = (ci[0] + (a & 1)) + np * ((ci[1] + ((a >> 1) & 1)) + np * (ci[2] + ((a >> 2) & 1)));
+          for (int bb = 0; bb < 8; ++bb)
+            {
+              const int ox = (bb & 1) - (a & 1), oy = ((bb >> 1) & 1) - ((a >> 1) & 1);
+              const int oz = ((bb >> 2) & 1) - ((a >> 2) & 1);
+              Kst[(size_t)na * 27 + (ox + 1) + 3 * (oy + 1) + 9 * (oz + 1)] += Ke[a * 8 + bb];
+            }
+          b[na] += be[a];
+        }
+    }
+}
+
 /* y = K x on all nodes */
 void orc_stencil_apply(int dim, int nf, const double *Kst, const double *x, double *y)
 {
@@ -534,12 +656,13 @@ static int is_boundary_node(int dim, int nf, int n)
 }
 
 /* Dirichlet data g_i at all nodes (interior entries = 0): BasisQ1 by monomial
- * inversion evaluated at the fine node coordinates (diffusion_problem_basis.tpp:119-135). */
-static void dirichlet_data(int dim, int nf, const double *box, const double *quad, int index, double *g)
+ * inversion evaluated at the fine node coordinates (diffusion_problem_basis.tpp:119-135).
+ * verts: a general cell's vertices [2^d][d], or NULL for the box. */
+static void dirichlet_data(int dim, int nf, const double *box, const double *gverts, int index, double *g)
 {
   double verts[24], coeff[64];
-  if (quad)
-    memcpy(verts, quad, sizeof(double) * 8);
+  if (gverts)
+    memcpy(verts, gverts, sizeof(double) * (size_t)(1 << dim) * dim);
   else
     orc_box_vertices(dim, box, verts);
   orc_basis_q1_coeffs(dim, verts, coeff);
@@ -553,8 +676,8 @@ static void dirichlet_data(int dim, int nf, const double *box, const double *qua
         }
       int I[3] = {n % np, (n / np) % np, dim == 3 ? n / (np * np) : 0};
       double p[3];
-      if (quad)
-        orc_quad_node(quad, nf, I[0], I[1], p);
+      if (gverts)
+        orc_cell_node(dim, gverts, nf, I, p);
       else
         for (int k = 0; k < dim; ++k)
           p[k] = box[2 * k] + (box[2 * k + 1] - box[2 * k]) * ((double)I[k] / nf);
@@ -571,6 +694,12 @@ void orc_dirichlet_data(int dim, int nf, const double *box, int index, double *g
 void orc_dirichlet_data_quad(int nf, const double *quad, int index, double *g)
 {
   dirichlet_data(2, nf, NULL, quad, index, g);
+}
+
+/* ... and of a general hexahedron's (3D). */
+void orc_dirichlet_data_hex(int nf, const double *hex, int index, double *g)
+{
+  dirichlet_data(3, nf, NULL, hex, index, g);
 }
 
 /* ------------------------------------------------------------------ */
@@ -631,7 +760,7 @@ static void band_solve(const band_t *B, double *x)
 }
 
 /* Basis problems by direct solve: phi[i][n], i < 2^d.  Returns 0 or 1 (not SPD). */
-static int local_bases_direct(int dim, int nf, const double *box, const double *quad, const double *Kst,
+static int local_bases_direct(int dim, int nf, const double *box, const double *verts, const double *Kst,
                               double *phi)
 {
   const int np = nf + 1, m = nf - 1, ns3 = ipow(3, dim), nb = 1 << dim;
@@ -673,7 +802,7 @@ static int local_bases_direct(int dim, int nf, const double *box, const double *
   double *x = (double *)malloc(sizeof(double) * (ni > 0 ? ni : 1));
   for (int ib = 0; ib < nb && rc == 0; ++ib)
     {
-      dirichlet_data(dim, nf, box, quad, ib, g);
+      dirichlet_data(dim, nf, box, verts, ib, g);
       for (int i = 0; i < ni; ++i)
         {
           int n = inode[i];
@@ -711,12 +840,17 @@ int orc_local_bases_direct_quad(int nf, const double *quad, const double *Kst, d
   return local_bases_direct(2, nf, NULL, quad, Kst, phi);
 }
 
+int orc_local_bases_direct_hex(int nf, const double *hex, const double *Kst, double *phi)
+{
+  return local_bases_direct(3, nf, NULL, hex, Kst, phi);
+}
+
 /* ------------------------------------------------------------------ */
 /* The reference's own recipe: condense, CG + SSOR(omega), absolute    */
 /* tolerance, iteration cap (diffusion_problem_basis.tpp:295-317,      */
 /* :450-465).  Returns the number of bases that did not converge.      */
 /* ------------------------------------------------------------------ */
-static int local_bases_ssor_pcg(int dim, int nf, const double *box, const double *quad, const double *Kst,
+static int local_bases_ssor_pcg(int dim, int nf, const double *box, const double *verts, const double *Kst,
                                 double omega, double abs_tol, int max_iter, double *phi, int *iters)
 {
   const int np = nf + 1, ns3 = ipow(3, dim), nb = 1 << dim, nn = ipow(np, dim);
@@ -751,7 +885,7 @@ static int local_bases_ssor_pcg(int dim, int nf, const double *box, const double
   for (int ib = 0; ib < nb; ++ib)
     {
       double *x = phi + (size_t)ib * nn;
-      dirichlet_data(dim, nf, box, quad, ib, g);
+      dirichlet_data(dim, nf, box, verts, ib, g);
       /* condensed rhs (diffusion_problem_basis.tpp:455,461): -K_IB g on free
        * rows, diag*g on constrained rows */
       for (int n = 0; n < nn; ++n)
@@ -943,32 +1077,44 @@ int orc_cell_build(int dim, int nf, const double *box, int coef_kind, const doub
   return rc;
 }
 
-/* One general quadrilateral coarse cell (2D), as orc_cell_build. */
-int orc_cell_build_quad(int nf, const double *quad, int coef_kind, const double *tile, int rhs_kind,
-                        double rhs_value, int solver, double omega, double abs_tol, int max_iter, double *phi,
-                        double *Ae, double *Fe, int *iters)
+/* One general coarse cell (quadrilateral in 2D, hexahedron in 3D), as
+ * orc_cell_build; verts [2^d][d]. */
+static int cell_build_general(int dim, int nf, const double *verts, int coef_kind, const double *tile,
+                              int rhs_kind, double rhs_value, int solver, double omega, double abs_tol,
+                              int max_iter, double *phi, double *Ae, double *Fe, int *iters)
 {
-  const int nn = (nf + 1) * (nf + 1);
-  double *Kst = (double *)malloc(sizeof(double) * (size_t)nn * 9);
+  const int nn = orc_n_nodes(dim, nf), ns3 = ipow(3, dim), nb = 1 << dim;
+  double *Kst = (double *)malloc(sizeof(double) * (size_t)nn * ns3);
   double *b   = (double *)malloc(sizeof(double) * nn);
-  double *ph  = phi ? phi : (double *)malloc(sizeof(double) * (size_t)4 * nn);
-  orc_local_assemble_quad(nf, quad, coef_kind, tile, rhs_kind, rhs_value, Kst, b);
+  double *ph  = phi ? phi : (double *)malloc(sizeof(double) * (size_t)nb * nn);
+  if (dim == 2)
+    orc_local_assemble_quad(nf, verts, coef_kind, tile, rhs_kind, rhs_value, Kst, b);
+  else
+    orc_local_assemble_hex(nf, verts, coef_kind, tile, rhs_kind, rhs_value, Kst, b);
   int rc;
   if (solver == ORC_SOLVER_DIRECT)
     {
-      rc = local_bases_direct(2, nf, NULL, quad, Kst, ph);
+      rc = local_bases_direct(dim, nf, NULL, verts, Kst, ph);
       if (iters)
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < nb; ++i)
           iters[i] = 0;
     }
   else
-    rc = local_bases_ssor_pcg(2, nf, NULL, quad, Kst, omega, abs_tol, max_iter, ph, iters);
-  orc_element(2, nf, Kst, b, ph, Ae, Fe);
+    rc = local_bases_ssor_pcg(dim, nf, NULL, verts, Kst, omega, abs_tol, max_iter, ph, iters);
+  orc_element(dim, nf, Kst, b, ph, Ae, Fe);
   free(Kst);
   free(b);
   if (!phi)
     free(ph);
   return rc;
+}
+
+int orc_cell_build_quad(int nf, const double *quad, int coef_kind, const double *tile, int rhs_kind,
+                        double rhs_value, int solver, double omega, double abs_tol, int max_iter, double *phi,
+                        double *Ae, double *Fe, int *iters)
+{
+  return cell_build_general(2, nf, quad, coef_kind, tile, rhs_kind, rhs_value, solver, omega, abs_tol, max_iter,
+                            phi, Ae, Fe, iters);
 }
 
 /* Many cells, OpenMP over cells (each cell is independent, as the reference's
@@ -1009,19 +1155,21 @@ int orc_batch_build(int dim, int n_cells, const double *box, const int *nf, int 
   return fails;
 }
 
-/* The same for general quadrilaterals (2D): quad [n][4][2]. */
-int orc_batch_build_quad(int n_cells, const double *quad, const int *nf, int coef_kind, const double *coef,
-                         int rhs_kind, double rhs_value, int solver, double omega, double abs_tol, int max_iter,
-                         int n_threads, double *bases, double *Ae, double *Fe, int *iters)
+/* The same for general cells: quadrilaterals (2D, verts [n][4][2]) or
+ * hexahedra (3D, verts [n][8][3]). */
+int orc_batch_build_general(int dim, int n_cells, const double *verts, const int *nf, int coef_kind,
+                            const double *coef, int rhs_kind, double rhs_value, int solver, double omega,
+                            double abs_tol, int max_iter, int n_threads, double *bases, double *Ae, double *Fe,
+                            int *iters)
 {
-  const int stride = orc_coef_stride(2, coef_kind);
+  const int nb = 1 << dim, stride = orc_coef_stride(dim, coef_kind);
   size_t *coff = (size_t *)malloc(sizeof(size_t) * (n_cells + 1));
   size_t *boff = (size_t *)malloc(sizeof(size_t) * (n_cells + 1));
   coff[0] = boff[0] = 0;
   for (int c = 0; c < n_cells; ++c)
     {
-      coff[c + 1] = coff[c] + (size_t)nf[c] * nf[c] * stride;
-      boff[c + 1] = boff[c] + (size_t)4 * (nf[c] + 1) * (nf[c] + 1);
+      coff[c + 1] = coff[c] + (size_t)ipow(nf[c], dim) * stride;
+      boff[c + 1] = boff[c] + (size_t)nb * orc_n_nodes(dim, nf[c]);
     }
   int fails = 0;
 #ifdef _OPENMP
@@ -1031,15 +1179,24 @@ int orc_batch_build_quad(int n_cells, const double *quad, const int *nf, int coe
 #endif
   for (int c = 0; c < n_cells; ++c)
     {
-      int rc = orc_cell_build_quad(nf[c], quad + (size_t)c * 8, coef_kind, coef + coff[c], rhs_kind, rhs_value,
-                                   solver, omega, abs_tol, max_iter, bases ? bases + boff[c] : NULL,
-                                   Ae + (size_t)c * 16, Fe + (size_t)c * 4, iters ? iters + (size_t)c * 4 : NULL);
+      int rc = cell_build_general(dim, nf[c], verts + (size_t)c * nb * dim, coef_kind, coef + coff[c], rhs_kind,
+                                  rhs_value, solver, omega, abs_tol, max_iter, bases ? bases + boff[c] : NULL,
+                                  Ae + (size_t)c * nb * nb, Fe + (size_t)c * nb,
+                                  iters ? iters + (size_t)c * nb : NULL);
       if (rc)
         fails += 1;
     }
   free(coff);
   free(boff);
   return fails;
+}
+
+int orc_batch_build_quad(int n_cells, const double *quad, const int *nf, int coef_kind, const double *coef,
+                         int rhs_kind, double rhs_value, int solver, double omega, double abs_tol, int max_iter,
+                         int n_threads, double *bases, double *Ae, double *Fe, int *iters)
+{
+  return orc_batch_build_general(2, n_cells, quad, nf, coef_kind, coef, rhs_kind, rhs_value, solver, omega,
+                                 abs_tol, max_iter, n_threads, bases, Ae, Fe, iters);
 }
 
 int orc_max_threads(void)

@@ -18,6 +18,7 @@ msfem_oracle.c):
   reconstruct                        diffusion_problem_basis.tpp:354-377
   *_quad (general quadrilaterals)    diffusion_problem_basis.tpp:92-99, :161-242 with the
                                      bilinear mapping; basis_q1.tpp:26-46 at mapped nodes
+  *_hex (general hexahedra)          the same in 3D with the trilinear mapping
 """
 import ctypes
 import os
@@ -84,6 +85,13 @@ def _declare(L):
         "orc_local_bases_direct_quad": (_i, [_i, _dp, _dp, _dp]),
         "orc_cell_build_quad": (_i, [_i, _dp, _i, _dp, _i, _d, _i, _d, _d, _i, _dp, _dp, _dp, _ip]),
         "orc_batch_build_quad": (_i, [_i, _dp, _ip, _i, _dp, _i, _d, _i, _d, _d, _i, _i, _dp, _dp, _dp, _ip]),
+        "orc_cell_node": (None, [_i, _dp, _i, _ip, _dp]),
+        "orc_fine_element_hex": (None, [_dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_local_assemble_hex": (None, [_i, _dp, _i, _dp, _i, _d, _dp, _dp]),
+        "orc_dirichlet_data_hex": (None, [_i, _dp, _i, _dp]),
+        "orc_local_bases_direct_hex": (_i, [_i, _dp, _dp, _dp]),
+        "orc_batch_build_general": (_i, [_i, _i, _dp, _ip, _i, _dp, _i, _d, _i, _d, _d, _i, _i, _dp, _dp, _dp,
+                                         _ip]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -316,6 +324,69 @@ def batch_build_quad(quad, nf, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.
                                        rhs_kind, rhs_value, solver, omega, abs_tol, max_iter, n_threads,
                                        _ptr(bases), _ptr(Ae), _ptr(Fe), _ptr(its, _ip))
     return {"Ae": Ae.reshape(n, 4, 4), "Fe": Fe.reshape(n, 4), "iters": its.reshape(n, 4), "bases": bases,
+            "fails": fails}
+
+
+# ---- general hexahedral cells (3D): vertices hex[8][3] in deal.II order ----
+def hex_nodes(hexa, nf):
+    """Fine node coordinates [(nf+1)^3][3] of a general hexahedron refined nf times."""
+    h = _d64(hexa).reshape(-1)
+    np1 = nf + 1
+    out = np.zeros((np1 ** 3, 3))
+    x = np.zeros(3)
+    I = np.zeros(3, np.int32)
+    for n in range(np1 ** 3):
+        I[:] = n % np1, (n // np1) % np1, n // (np1 * np1)
+        lib().orc_cell_node(3, _ptr(h), nf, _ptr(I, _ip), _ptr(x))
+        out[n] = x
+    return out
+
+
+def fine_element_hex(fv, coef_kind, cdata, rhs_kind=RHS_CONST, rhs_value=1.0):
+    Ke = np.zeros(64)
+    be = np.zeros(8)
+    lib().orc_fine_element_hex(_ptr(_d64(fv).reshape(-1)), coef_kind, _ptr(_d64(cdata)), rhs_kind, rhs_value,
+                               _ptr(Ke), _ptr(be))
+    return Ke.reshape(8, 8), be
+
+
+def local_assemble_hex(nf, hexa, coef_kind, tile, rhs_kind=RHS_CONST, rhs_value=1.0):
+    nn = n_nodes(3, nf)
+    K = np.zeros(nn * 27)
+    b = np.zeros(nn)
+    lib().orc_local_assemble_hex(nf, _ptr(_d64(hexa).reshape(-1)), coef_kind, _ptr(_d64(tile).reshape(-1)),
+                                 rhs_kind, rhs_value, _ptr(K), _ptr(b))
+    return K.reshape(nn, 27), b
+
+
+def dirichlet_data_hex(nf, hexa, index):
+    g = np.zeros(n_nodes(3, nf))
+    lib().orc_dirichlet_data_hex(nf, _ptr(_d64(hexa).reshape(-1)), index, _ptr(g))
+    return g
+
+
+def local_bases_direct_hex(nf, hexa, K):
+    phi = np.zeros(8 * n_nodes(3, nf))
+    rc = lib().orc_local_bases_direct_hex(nf, _ptr(_d64(hexa).reshape(-1)), _ptr(_d64(K).reshape(-1)), _ptr(phi))
+    if rc:
+        raise ValueError("local matrix not SPD")
+    return phi.reshape(8, -1)
+
+
+def batch_build_hex(hexa, nf, coef_kind, coef, rhs_kind=RHS_CONST, rhs_value=1.0, solver=SOLVER_DIRECT,
+                    omega=1.6, abs_tol=1e-12, max_iter=1000, n_threads=0, want_bases=False):
+    """batch_build for general hexahedra: hexa [n][8][3]."""
+    h = _d64(hexa).reshape(-1)
+    nf = np.ascontiguousarray(nf, dtype=np.int32)
+    n = nf.shape[0]
+    Ae = np.zeros(n * 64)
+    Fe = np.zeros(n * 8)
+    its = np.zeros(n * 8, dtype=np.int32)
+    bases = np.zeros(int(sum(8 * (int(k) + 1) ** 3 for k in nf))) if want_bases else None
+    fails = lib().orc_batch_build_general(3, n, _ptr(h), _ptr(nf, _ip), coef_kind, _ptr(_d64(coef).reshape(-1)),
+                                          rhs_kind, rhs_value, solver, omega, abs_tol, max_iter, n_threads,
+                                          _ptr(bases), _ptr(Ae), _ptr(Fe), _ptr(its, _ip))
+    return {"Ae": Ae.reshape(n, 8, 8), "Fe": Fe.reshape(n, 8), "iters": its.reshape(n, 8), "bases": bases,
             "fails": fails}
 
 

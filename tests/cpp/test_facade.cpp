@@ -5,8 +5,8 @@
 // Usage: test_facade [--dim=3] <n_refine> <n_refine_local> <out.bin> [<vtu_dir> [<distortion>]]
 // (with vtu_dir other than "-": also the fine and coarse output,
 // DiffusionProblemMultiscale::output_global_fine / output_global_coarse; with
-// distortion (2D): the coarse mesh's interior vertices moved by up to
-// distortion/nc, seed 1, so every cell is a general quadrilateral)
+// distortion: the coarse mesh's interior vertices moved by up to
+// distortion/nc, seed 1, so every cell is a general quadrilateral / hexahedron)
 #include <cstdio>
 #include <cstdlib>
 #include <string>

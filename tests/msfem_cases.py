@@ -10,8 +10,9 @@ from workloads import split_bases
 def oracle_build(case, want_bases=True, solver=0, n_threads=0):
     import oracle as O
     if case.get("vertices") is not None:
-        return O.batch_build_quad(case["vertices"], case["n_fine"], case["coef_kind"], case["coef"], case["rhs_kind"],
-                                  case["rhs_value"], solver=solver, n_threads=n_threads, want_bases=want_bases)
+        build = O.batch_build_quad if case["dim"] == 2 else O.batch_build_hex
+        return build(case["vertices"], case["n_fine"], case["coef_kind"], case["coef"], case["rhs_kind"],
+                     case["rhs_value"], solver=solver, n_threads=n_threads, want_bases=want_bases)
     return O.batch_build(case["dim"], case["box"], case["n_fine"], case["coef_kind"], case["coef"], case["rhs_kind"],
                          case["rhs_value"], solver=solver, n_threads=n_threads, want_bases=want_bases)
 
@@ -28,8 +29,9 @@ def energy_error(case, gpu_flat, ref_flat):
         nf = int(nf)
         n = nf ** dim * stride
         if case.get("vertices") is not None:
-            K, _ = O.local_assemble_quad(nf, case["vertices"][c], case["coef_kind"], case["coef"][off:off + n],
-                                         case["rhs_kind"], case["rhs_value"])
+            assemble = O.local_assemble_quad if dim == 2 else O.local_assemble_hex
+            K, _ = assemble(nf, case["vertices"][c], case["coef_kind"], case["coef"][off:off + n], case["rhs_kind"],
+                            case["rhs_value"])
         else:
             K, _ = O.local_assemble(dim, nf, case["box"][c], case["coef_kind"], case["coef"][off:off + n],
                                     case["rhs_kind"], case["rhs_value"])

@@ -131,6 +131,53 @@ def test_facade_reference_pipeline_3d(tmp_path, nr, nrl):
     assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-10
 
 
+def test_facade_general_hexahedra(tmp_path):
+    """DiffusionProblemMultiscale<3> with set_mesh_distortion(0.15, seed 1): every
+    coarse cell a general hexahedron (DiffusionProblemBasis<3> from its eight
+    corner points); element matrices, loads and the coarse solution against the
+    oracle's hexahedron path, the fine output's corner points at the moved
+    vertices and the coarse output's points at the moved mesh."""
+    import xml.etree.ElementTree as ET
+    print("[gpu-case] facade general hexahedra n_refine=2 n_refine_local=3 distortion=0.15", flush=True)
+    out, vdir = tmp_path / "facade3h.bin", tmp_path / "vtu"
+    vdir.mkdir()
+    p = subprocess.run([EXE, "--dim=3", "2", "3", str(out), str(vdir), "0.15"], capture_output=True, text=True,
+                       timeout=200)
+    assert p.returncode == 0, p.stdout + p.stderr
+    raw = out.read_bytes()
+    nu = int(np.frombuffer(raw[:8], np.int64)[0])
+    u = np.frombuffer(raw[8:8 + 8 * nu], np.float64)
+    nc_cells = int(np.frombuffer(raw[8 + 8 * nu:16 + 8 * nu], np.int64)[0])
+    rest = np.frombuffer(raw[16 + 8 * nu:], np.float64).reshape(nc_cells, 72)
+    Ae, Fe = rest[:, :64].reshape(-1, 8, 8), rest[:, 64:]
+    nc, nf = 4, 8
+    hexa = msfem.perturbed_hexes(nc, 0.15, 1)
+    n_fine = np.full(nc ** 3, nf, np.int32)
+    coef = msfem.coef_reference_hex(n_fine, hexa)
+    r = O.batch_build_hex(hexa, n_fine, O.COEF_TENSOR_QP, coef, O.RHS_CONST, 2.0)
+    assert np.abs(Ae - r["Ae"]).max() / np.abs(r["Ae"]).max() < 1e-10
+    assert np.abs(Fe - r["Fe"]).max() / np.abs(r["Fe"]).max() < 1e-10
+    Kc, Fc, cm, cv = O.coarse_assemble(3, nc, r["Ae"], r["Fe"], 1)
+    ur = O.coarse_solve(3, nc, Kc, Fc, cm, cv)
+    assert np.abs(u - ur).max() / np.abs(ur).max() < 1e-10
+    master = ET.parse(vdir / "solution-ms_fine-3d.pvtu").getroot()
+    pieces = [pc.get("Source") for pc in master.iter("Piece")]
+    assert len(pieces) == nc ** 3
+    np1 = nf + 1
+    corners = [x + np1 * (y + np1 * z) for z in (0, nf) for y in (0, nf) for x in (0, nf)]
+    for c, src in enumerate(pieces):  # cells in id order
+        pts = np.array(ET.parse(vdir / src).getroot().find("UnstructuredGrid/Piece/Points/DataArray").text.split(),
+                       float).reshape(-1, 3)
+        assert np.allclose(pts[corners], hexa[c], atol=1e-15)
+    coarse = ET.parse(vdir / "solution-ms_coarse-3d_refinements-2.0000.vtu").getroot()
+    pts = np.array(coarse.find("UnstructuredGrid/Piece/Points/DataArray").text.split(), float).reshape(-1, 3)
+    # coarse node (i, j, k) is vertex 0 of cell (i, j, k) where that cell exists
+    for k in range(nc):
+        for j in range(nc):
+            for i in range(nc):
+                assert np.allclose(pts[i + (nc + 1) * (j + (nc + 1) * k)], hexa[i + nc * (j + nc * k)][0], atol=1e-15)
+
+
 @pytest.mark.parametrize("distortion", [None, "0.2"])
 def test_facade_coarse_output(tmp_path, distortion):
     """output_global_coarse() (diffusion_problem_ms.tpp:330-380): the coarse
