@@ -150,6 +150,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // per-wave coarse correction (the FP64 128^2 instance has no LDS left for it)
   constexpr bool WAVE_CS = CS && !(sizeof(T) == 8 && NF == 128);
   __shared__ T swrc[WAVE_CS ? SW * kWave : 1];         // [wave][coarse dof]: P^T r, wave-private
+  // FP32 two-level box tiles keep the inner iterate x in thread-private LDS
+  // slots ([chunk of 4 nodes][thread][4]: 16-byte accesses, consecutive lanes
+  // on consecutive banks) instead of registers: x is only ever updated
+  // (x += alpha p) and read once at the end, and the freed registers let two
+  // workgroups share a CU
+  constexpr bool XLDS = MODE != kModeStream && CS && sizeof(T) == 4 && (R * CPL) % 4 == 0;
+  __shared__ float4 sx[XLDS ? R * CPL / 4 * kWave * SW : 1];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -264,7 +271,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // kModeStream keeps the diagonal as packed BF16 pairs (8 registers instead
   // of 16); any positive diagonal is a valid Jacobi preconditioner, and a
   // nonzero value stays nonzero, so the active-node mask is unchanged.
-  constexpr bool DPK = MODE == kModeStream;
+  constexpr bool DPK = MODE == kModeStream || (XLDS && WPE >= 4);
   uint32_t dpk[DPK ? (R * CPL + 1) / 2 : 1];
   if constexpr (DPK)
     {
@@ -500,14 +507,21 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // the entries 13 h .. 13 h + 12, a 4-lane butterfly completes it (every lane
   // of the quad holds the row's value).  Every wave evaluates a shared coarse
   // row in the same order, so the waves agree bitwise.
+  // With four waves per SIMD (128 registers) the lane's LDS addresses of the
+  // coarse product are recomputed here from an opaque copy of the lane index:
+  // hoisted out of the iteration loop they would be kept live and spilled.
+  constexpr bool OPAQUE_CA = WPE >= 4;
   auto coarse_wave_acc = [&]() -> T {
     T acc = 0;
     if constexpr (WAVE_CS)
       {
+        int lp = pl;
+        if constexpr (OPAQUE_CA)
+          asm volatile("" : "+v"(lp));
         T *wrc = swrc + wave * kWave;
-        if (pl < kCoarseDofs)
+        if (lp < kCoarseDofs)
           {
-            const int jx = pl % kCoarseSide + 1, jy = pl / kCoarseSide + 1;
+            const int jx = lp % kCoarseSide + 1, jy = lp / kCoarseSide + 1;
             T rc = 0;
 #pragma unroll
             for (int b = 0; b < WB; ++b)
@@ -516,12 +530,12 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 rc += (spart[spart_at(wl, 0, jx, 0)] + spart[spart_at(wl, 0, jx - 1, 1)]) +
                       (spart[spart_at(wu, 1, jx, 0)] + spart[spart_at(wu, 1, jx - 1, 1)]);
               }
-            wrc[pl] = rc;
+            wrc[lp] = rc;
           }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int q = pl >> 2, h = pl & 3;
+        const int q = lp >> 2, h = lp & 3;
         const int jyq = cband + (q >= kCoarseSide ? 1 : 0), jxq = q % kCoarseSide + 1;
         if (q < 2 * kCoarseSide && jyq >= 1 && jyq <= kCoarseSide)
           {
@@ -634,6 +648,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
             pr[i][c] = 0;
             sr[i][c] = 0;
           }
+      if constexpr (XLDS)
+#pragma unroll
+        for (int k = 0; k < R * CPL / 4; ++k)
+          sx[k * NT + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
       bool irun = run, first = true;
       T inv_gamma_prev = 1, inv_alpha_prev = 1; // 1 / gamma, 1 / alpha of the previous iteration
       if constexpr (MODE == kModeStream)
@@ -891,11 +909,15 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               // z = D^-1 r + P u_c (the interpolation vanishes on the tile boundary)
               T ci0[CPL], ci1[CPL];
               coarse_wave(ci0, ci1);
+              // the row weights sum to one: ci0 wya + ci1 wyb = ci0 + (ci1 - ci0) wyb
+#pragma unroll
+              for (int c = 0; c < CPL; ++c)
+                ci1[c] -= ci0[c];
 #pragma unroll
               for (int i = 0; i < R; ++i)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                  gown[i * GP + c] = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
+                  gown[i * GP + c] = rr[i][c] * (T)dget(i, c) + (ci0[c] + ci1[c] * wyb(i));
             }
           else
             {
@@ -985,8 +1007,21 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     const T z = (CS || sizeof(T) == 8) ? gown[i * GP + c] : rr[i][c] * (T)dget(i, c);
                     pr[i][c] = z + tb * pr[i][c];
                     sr[i][c] = wv[i][c] + tb * sr[i][c];
-                    xr[i][c] += ta * pr[i][c];
+                    if constexpr (!XLDS)
+                      xr[i][c] += ta * pr[i][c];
                     rr[i][c] -= ta * sr[i][c];
+                  }
+              if constexpr (XLDS)
+#pragma unroll
+                for (int k = 0; k < R * CPL / 4; ++k)
+                  {
+                    float4 v = sx[k * NT + tid];
+                    const int n = 4 * k;
+                    v.x += ta * pr[n / CPL][n % CPL];
+                    v.y += ta * pr[(n + 1) / CPL][(n + 1) % CPL];
+                    v.z += ta * pr[(n + 2) / CPL][(n + 2) % CPL];
+                    v.w += ta * pr[(n + 3) / CPL][(n + 3) % CPL];
+                    sx[k * NT + tid] = v;
                   }
               inv_gamma_prev = inv_gamma;
               inv_alpha_prev = denom * inv_gamma;
@@ -1007,7 +1042,16 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
             for (int c = 0; c < CPL; ++c)
               if (dget(i, c) != 0.0f)
-                phi[(Y0 + i) * NP + X0 + c] += (double)xr[i][c];
+                {
+                  T xv = xr[i][c];
+                  if constexpr (XLDS)
+                    {
+                      const int n = i * CPL + c;
+                      const float4 v = sx[(n / 4) * NT + tid];
+                      xv = (n & 3) == 0 ? v.x : (n & 3) == 1 ? v.y : (n & 3) == 2 ? v.z : v.w;
+                    }
+                  phi[(Y0 + i) * NP + X0 + c] += (double)xv;
+                }
         }
       __syncthreads();
     }
@@ -1348,6 +1392,8 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
 //   64:  1 = (SW 8, R 8, 2)            2 = (SW 4, R 16, 1)    3 = (SW 16, R 4, 4)   4 = (SW 8, R 8, 4)
 //        5 = (SW 4, R 16, 3)
 //   128: 1 = (SW 16, R 8, CPL 2, 4)
+// The two-level 64^2 box classes run shape 4 for id 4 and shape 1 otherwise,
+// except that id 0 gives FP32 shape 4 (two workgroups per CU).
 // kModeStream (one problem per workgroup, 64 lanes across):
 //   64:  1 = (SW 8, R 8, 2)            2 = (SW 4, R 16, 1)    3 = (SW 16, R 4, 4)
 //   128: 1 = (SW 16, R 8, CPL 2, 4)
@@ -1389,8 +1435,9 @@ hipError_t launch_mode(int nf, int variant, const LocalArgs &a, hipStream_t st)
         // the (8 waves, 8 rows) shapes also come in the two-level form (a.cinv set)
         if constexpr (MODE == kModeScalar || MODE == kModeTensor)
           if (a.cinv)
-            return variant == 4 ? launch_one<T, 64, 1, 8, 8, MODE, 4, true>(a, st)
-                                : launch_one<T, 64, 1, 8, 8, MODE, 2, true>(a, st);
+            return (variant == 4 || (variant == 0 && sizeof(T) == 4))
+                     ? launch_one<T, 64, 1, 8, 8, MODE, 4, true>(a, st)
+                     : launch_one<T, 64, 1, 8, 8, MODE, 2, true>(a, st);
         switch (variant)
           {
             case 2: return launch_one<T, 64, 1, 4, 16, MODE, 1>(a, st);

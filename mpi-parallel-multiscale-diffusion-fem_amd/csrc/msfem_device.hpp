@@ -336,16 +336,19 @@ __device__ __forceinline__ bool problem_sums(S (&v)[NV], bool flag, S *red, int 
     }
 }
 
+// a / b for the CG scalars.  FP32: a times the hardware reciprocal (v_rcp_f32,
+// within 1 ulp) -- two instructions instead of the scaled IEEE division
+// sequence; the CG coefficients tolerate the last-bit difference.
 template <typename S>
 __device__ __forceinline__ S quotient(S a, S b)
 {
   if constexpr (sizeof(S) == 4)
-    return __fdividef(a, b);
+    return a * __builtin_amdgcn_rcpf(b);
   else
     return a / b;
 }
 
-// 1 / b for the CG scalars (b > 0 and normal there).  FP32: as quotient.
+// 1 / b for the CG scalars (b > 0 and normal there).  FP32: v_rcp_f32.
 // FP64: the hardware estimate (v_rcp_f64) refined by two Newton steps
 // r <- r + r (1 - b r), each of which squares the relative error, so the
 // result is correctly rounded or one ulp off -- five instructions instead of
@@ -354,7 +357,7 @@ template <typename S>
 __device__ __forceinline__ S reciprocal(S b)
 {
   if constexpr (sizeof(S) == 4)
-    return __fdividef(1.0f, b);
+    return __builtin_amdgcn_rcpf(b);
   else
     {
       double r = __builtin_amdgcn_rcp(b);
