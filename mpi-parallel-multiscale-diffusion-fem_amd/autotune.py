@@ -7,6 +7,10 @@ the fastest, and writes autotune_table.json next to this file.  The engine
 applies the table through Context.load_variant_table() / msfem_set_variant.
 
     python autotune.py --workload lognormal64 --precision fp64
+    python autotune.py --workload grf128 --precision fp32
+
+Samples of a few thousand cells matter: the shapes differ in how many
+workgroups share a CU, which a small sample (few workgroups per CU) hides.
 """
 import argparse
 import json
@@ -93,15 +97,17 @@ def save(entries, path=TABLE):
 def main():
     import workloads as C
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="lognormal64", choices=["lognormal64", "tensor128", "lognormal3d"])
+    ap.add_argument("--workload", default="lognormal64", choices=["lognormal64", "grf128", "tensor128", "lognormal3d"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
-    ap.add_argument("--max-cells", type=int, default=1024)
+    ap.add_argument("--max-cells", type=int, default=4096)
     ap.add_argument("--out", default=TABLE)
     args = ap.parse_args()
     prec = msfem.PREC_FP64 if args.precision == "fp64" else msfem.PREC_FP32_REFINE
     rel_tol = 1e-13 if prec == msfem.PREC_FP64 else 1e-8
     if args.workload == "lognormal64":
         case = C.lognormal_cells()
+    elif args.workload == "grf128":
+        case = C.grf_cells()
     elif args.workload == "tensor128":
         case = C.tensor_cells()
     else:
