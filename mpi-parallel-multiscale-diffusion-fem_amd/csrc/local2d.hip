@@ -157,6 +157,9 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // workgroups share a CU
   constexpr bool XLDS = MODE != kModeStream && CS && sizeof(T) == 4 && (R * CPL) % 4 == 0;
   __shared__ float4 sx[XLDS ? R * CPL / 4 * kWave * SW : 1];
+  // with four waves per SIMD the recurrence s = K p goes there too
+  constexpr bool SLDS = XLDS && WPE >= 4;
+  __shared__ float4 ss[SLDS ? R * CPL / 4 * kWave * SW : 1];
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
@@ -194,14 +197,20 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     }
 
   // ---- fine-cell data cached in registers ----------------------------------
+  // CDPP (FP32 scalar two-level tiles at four waves per SIMD, one node column
+  // per lane): only the cells right of the lane's column are cached; the cells
+  // left of it are the left neighbour lane's, read with a one-lane rotation of
+  // the wave (DPP wave_ror:1).  Lane 63 (column 64, a boundary column whose
+  // results are masked) caches cell column 0 for lane 0.
+  constexpr bool CDPP = MODE == kModeScalar && CPL == 1 && LW == kWave && sizeof(T) == 4 && WPE >= 4 && CS;
   T cc[R + 1][CPL + 1][NCACHE];
 #pragma unroll
   for (int i = 0; i <= R; ++i)
 #pragma unroll
     for (int j = 0; j <= CPL; ++j)
       {
-        const int cx = X0 - 1 + j, cy = Y0 - 1 + i;
-        const bool ok = pvalid && cx < NF && cy < NF;
+        const int cx = (CDPP && pl == LW - 1) ? 0 : X0 - 1 + j, cy = Y0 - 1 + i;
+        const bool ok = pvalid && cx < NF && cy < NF && !(CDPP && j == 0);
         if constexpr (MODE == kModeScalar)
           cc[i][j][0] = ok ? (T)cf[cy * NF + cx] : (T)0;
         else if constexpr (MODE == kModeTensor)
@@ -220,13 +229,20 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         else
           cc[i][j][0] = 0;
       }
+  // scalar coefficient of cached cell (i, j)
+  auto ccs = [&](int i, int j) -> T {
+    if constexpr (CDPP)
+      if (j == 0)
+        return dpp_f32<0x13C>(cc[i][1][0]); // wave_ror:1: lane l reads lane l - 1 (lane 0: lane 63)
+    return cc[i][j][0];
+  };
   const Cond4 k4 = cond_scalar(1.0, rho); // scalar-mode unit conductances
   const T kh = (T)k4.ch, kv = (T)k4.cv, kd = (T)k4.cd, kc = (T)(k4.ch + k4.cv + k4.cd);
 
   auto cond_T = [&](int i, int j) -> Cond6<T> {
     if constexpr (MODE == kModeScalar)
       {
-        const T s = cc[i][j][0];
+        const T s = ccs(i, j);
         return {{s * kh, s * kv, s * kd, s * kd, s * kv, s * kh}};
       }
     else if constexpr (MODE == kModeTensor)
@@ -411,6 +427,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // row offsets (two ALU operations each): hoisted out of the iteration loop
   // they would be kept live and spilled.  64^2 tiles have the registers and
   // let the compiler keep them.
+  // With four waves per SIMD (128 registers) the lane-dependent LDS and global
+  // addresses used once per iteration or cycle are recomputed where used from
+  // opaque copies of the lane index: hoisted out of the loops they would be
+  // kept live and spilled.
+  constexpr bool OPAQUE_CA = WPE >= 4;
   constexpr bool OPAQUE_W = NF == 128;
   auto lane_off = [&]() {
     int v = (pl & 7) * CPL + 1;
@@ -457,10 +478,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           v4[k] = oct_sum_any<T>(v4[k]);
         if ((pl & 7) == 0)
           {
-            spart[spart_at(wave, 0, cg8, 0)] = v4[0];
-            spart[spart_at(wave, 0, cg8, 1)] = v4[1];
-            spart[spart_at(wave, 1, cg8, 0)] = v4[2];
-            spart[spart_at(wave, 1, cg8, 1)] = v4[3];
+            int g = cg8; // (OPAQUE_CA: the slot address is formed here, not hoisted and spilled)
+            if constexpr (OPAQUE_CA)
+              asm volatile("" : "+v"(g));
+            spart[spart_at(wave, 0, g, 0)] = v4[0];
+            spart[spart_at(wave, 0, g, 1)] = v4[1];
+            spart[spart_at(wave, 1, g, 0)] = v4[2];
+            spart[spart_at(wave, 1, g, 1)] = v4[3];
           }
       }
   };
@@ -507,10 +531,6 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // the entries 13 h .. 13 h + 12, a 4-lane butterfly completes it (every lane
   // of the quad holds the row's value).  Every wave evaluates a shared coarse
   // row in the same order, so the waves agree bitwise.
-  // With four waves per SIMD (128 registers) the lane's LDS addresses of the
-  // coarse product are recomputed here from an opaque copy of the lane index:
-  // hoisted out of the iteration loop they would be kept live and spilled.
-  constexpr bool OPAQUE_CA = WPE >= 4;
   auto coarse_wave_acc = [&]() -> T {
     T acc = 0;
     if constexpr (WAVE_CS)
@@ -651,7 +671,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       if constexpr (XLDS)
 #pragma unroll
         for (int k = 0; k < R * CPL / 4; ++k)
-          sx[k * NT + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+          {
+            sx[k * NT + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if constexpr (SLDS)
+              ss[k * NT + tid] = make_float4(0.f, 0.f, 0.f, 0.f);
+          }
       bool irun = run, first = true;
       T inv_gamma_prev = 1, inv_alpha_prev = 1; // 1 / gamma, 1 / alpha of the previous iteration
       if constexpr (MODE == kModeStream)
@@ -951,8 +975,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   T y;
                   if constexpr (MODE == kModeScalar)
                     {
-                      const T aNE = cc[i + 1][c + 1][0], aNW = cc[i + 1][c][0], aSE = cc[i][c + 1][0],
-                              aSW = cc[i][c][0];
+                      const T aNE = ccs(i + 1, c + 1), aNW = ccs(i + 1, c), aSE = ccs(i, c + 1), aSW = ccs(i, c);
                       y = kc * ((aNE + aNW) + (aSE + aSW)) * w1[c + 1];
                       y -= kh * ((aNE + aSE) * w1[c + 2] + (aNW + aSW) * w1[c]);
                       y -= kv * ((aNE + aNW) * w2[c + 1] + (aSE + aSW) * w0[c + 1]);
@@ -1006,10 +1029,28 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     // one LDS read instead of a conversion and a multiply)
                     const T z = (CS || sizeof(T) == 8) ? gown[i * GP + c] : rr[i][c] * (T)dget(i, c);
                     pr[i][c] = z + tb * pr[i][c];
-                    sr[i][c] = wv[i][c] + tb * sr[i][c];
+                    if constexpr (!SLDS)
+                      {
+                        sr[i][c] = wv[i][c] + tb * sr[i][c];
+                        rr[i][c] -= ta * sr[i][c];
+                      }
                     if constexpr (!XLDS)
                       xr[i][c] += ta * pr[i][c];
-                    rr[i][c] -= ta * sr[i][c];
+                  }
+              if constexpr (SLDS)
+#pragma unroll
+                for (int k = 0; k < R * CPL / 4; ++k)
+                  {
+                    float4 v = ss[k * NT + tid];
+                    float *vs = &v.x;
+#pragma unroll
+                    for (int m = 0; m < 4; ++m)
+                      {
+                        const int n = 4 * k + m, i = n / CPL, c = n % CPL;
+                        vs[m] = wv[i][c] + tb * vs[m];
+                        rr[i][c] -= ta * vs[m];
+                      }
+                    ss[k * NT + tid] = v;
                   }
               if constexpr (XLDS)
 #pragma unroll
@@ -1037,6 +1078,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       // ---- phi += d in FP64 (the FP64 streamed form has already added it) ---------
       if (run && !(MODE == kModeStream && sizeof(T) == 8))
         {
+          // (OPAQUE_CA: the lane's addresses are formed here, not hoisted out of
+          // the cycle loop and spilled)
+          double *phl = phi + X0;
+          if constexpr (OPAQUE_CA)
+            asm volatile("" : "+v"(phl));
 #pragma unroll
           for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -1050,7 +1096,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                       const float4 v = sx[(n / 4) * NT + tid];
                       xv = (n & 3) == 0 ? v.x : (n & 3) == 1 ? v.y : (n & 3) == 2 ? v.z : v.w;
                     }
-                  phi[(Y0 + i) * NP + X0 + c] += (double)xv;
+                  phl[(Y0 + i) * NP + c] += (double)xv;
                 }
         }
       __syncthreads();
