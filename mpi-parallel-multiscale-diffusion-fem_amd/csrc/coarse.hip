@@ -7,6 +7,8 @@
 //                reconstruction of DiffusionProblemBasis::set_global_weights
 //                (include/base/diffusion_problem_basis.tpp:354-377)
 // The coarse matrix is kept in structured stencil form Kc[node][3^dim].
+#include <algorithm>
+
 #include "msfem_device.hpp"
 #include "msfem_kernels.hpp"
 
@@ -265,6 +267,129 @@ struct MgGrid {
   }
 };
 
+// Constrained node of multigrid level l: the level-0 node at its position is
+// constrained (level l node I sits at level-0 node I * 2^l).
+template <int DIM>
+__device__ __forceinline__ bool mg_masked_from0(const MgLayout &g, int nc, const unsigned char *cmask, int l,
+                                                int node)
+{
+  if (l == 0)
+    return cmask[node] != 0;
+  MgGrid<DIM> Gl{g.n[l]}, G0{nc};
+  int I[3];
+  Gl.coords(node, I);
+  for (int k = 0; k < DIM; ++k)
+    I[k] <<= l;
+  return cmask[G0.index(I)] != 0;
+}
+
+// The flags of levels 1.. as byte arrays in the work buffer (one pass before
+// the Galerkin products, which then read one byte per test).
+template <int DIM>
+__global__ void __launch_bounds__(256) mg_mask_kernel(MgLayout g, int nc, const unsigned char *cmask, double *w)
+{
+  for (int l = 1; l < g.levels; ++l)
+    {
+      unsigned char *mk = (unsigned char *)(w + g.off_mask[l]);
+      for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < (int)g.nn[l]; n += gridDim.x * blockDim.x)
+        mk[n] = mg_masked_from0<DIM>(g, nc, cmask, l, n) ? 1 : 0;
+    }
+}
+
+__device__ __forceinline__ bool mg_masked(const MgLayout &g, const unsigned char *cmask, const double *w, int l,
+                                          int node)
+{
+  return (l == 0 ? cmask[node] : ((const unsigned char *)(w + g.off_mask[l]))[node]) != 0;
+}
+
+// Galerkin operator of level l >= 1, A_l = P^T A_{l-1} P with P the bilinear /
+// trilinear interpolation between consecutive levels, on the free nodes (rows
+// and columns of constrained nodes dropped), stored as a 3^DIM stencil per
+// node.  One thread per (coarse node, fine node of its support): each sums the
+// contributions of its fine node's row into a private stencil, the workgroup's
+// 3^DIM partial stencils of a coarse node are then added in a fixed order.  The
+// levels run one launch after another, each reading the one before.
+template <int DIM>
+constexpr int mg_galerkin_nodes() { return DIM == 2 ? 28 : 9; } // coarse nodes per workgroup
+
+template <int DIM>
+__global__ void __launch_bounds__(mg_galerkin_nodes<DIM>() * (DIM == 2 ? 9 : 27))
+  mg_galerkin_kernel(MgLayout g, int l, const double *Kc, const unsigned char *cmask, double *w)
+{
+  constexpr int S = DIM == 2 ? 9 : 27, NPB = mg_galerkin_nodes<DIM>();
+  __shared__ double part[NPB][S][S];
+  MgGrid<DIM> Gf{g.n[l - 1]}, Gc{g.n[l]};
+  const double *Af = l == 1 ? Kc : w + g.off_A[l - 1];
+  double *Ac = w + g.off_A[l];
+  const int cn = threadIdx.x / S, o = threadIdx.x % S;
+  const int c = blockIdx.x * NPB + cn;
+  double acc[S];
+#pragma unroll
+  for (int t = 0; t < S; ++t)
+    acc[t] = 0;
+  if (c < (int)g.nn[l] && !mg_masked(g, cmask, w, l, c))
+    {
+      int I[3];
+      Gc.coords(c, I);
+      // fine node i = 2I + o in the support of coarse node I
+      const int od[3] = {o % 3 - 1, (o / 3) % 3 - 1, DIM == 3 ? o / 9 - 1 : 0};
+      int i[3] = {2 * I[0] + od[0], 2 * I[1] + od[1], 2 * I[2] + od[2]};
+      if (Gf.inside(i) && !mg_masked(g, cmask, w, l - 1, Gf.index(i)))
+        {
+          const int fi = Gf.index(i);
+          double wi = 1;
+          for (int k = 0; k < DIM; ++k)
+            wi *= od[k] ? 0.5 : 1.0;
+          for (int t = 0; t < S; ++t) // fine neighbour j = i + s
+            {
+              int j[3] = {i[0] + t % 3 - 1, i[1] + (t / 3) % 3 - 1, DIM == 3 ? i[2] + t / 9 - 1 : 0};
+              if (!Gf.inside(j) || mg_masked(g, cmask, w, l - 1, Gf.index(j)))
+                continue;
+              const double a = wi * Af[(size_t)fi * S + t];
+              // coarse parents J of j: per axis one (even) or two (odd) with weight 1/2
+              int nodd = 0;
+              for (int k = 0; k < DIM; ++k)
+                nodd += j[k] & 1;
+              for (int comb = 0; comb < (1 << nodd); ++comb)
+                {
+                  int J[3] = {0, 0, 0}, bitpos = 0;
+                  double wj = 1;
+                  for (int k = 0; k < DIM; ++k)
+                    if (j[k] & 1)
+                      {
+                        J[k] = (j[k] >> 1) + ((comb >> bitpos) & 1);
+                        ++bitpos;
+                        wj *= 0.5;
+                      }
+                    else
+                      J[k] = j[k] >> 1;
+                  if (mg_masked(g, cmask, w, l, Gc.index(J)))
+                    continue;
+                  int tt = 0, mul = 1;
+                  for (int k = 0; k < DIM; ++k)
+                    {
+                      tt += (J[k] - I[k] + 1) * mul;
+                      mul *= 3;
+                    }
+                  acc[tt] += a * wj;
+                }
+            }
+        }
+    }
+#pragma unroll
+  for (int t = 0; t < S; ++t)
+    part[cn][o][t] = acc[t];
+  __syncthreads();
+  if (c < (int)g.nn[l])
+    {
+      const int t = o; // this thread adds stencil entry t of coarse node c over the support
+      double v = 0;
+      for (int k = 0; k < S; ++k)
+        v += part[cn][k][t];
+      Ac[(size_t)c * S + t] = v;
+    }
+}
+
 template <int DIM>
 __global__ void __launch_bounds__(kCoarseThreads)
   coarse_mgcg_kernel(MgLayout g, int nc, const double *Kc, const double *Fc, const unsigned char *cmask,
@@ -279,16 +404,7 @@ __global__ void __launch_bounds__(kCoarseThreads)
   double *cx = w + g.off_cx, *cr = w + g.off_cr, *cp = w + g.off_cp, *cq = w + g.off_cq;
 
   // constrained node of level l: the level-0 node at its position is constrained
-  auto masked = [&](int l, int node) -> bool {
-    if (l == 0)
-      return cmask[node] != 0;
-    MgGrid<DIM> Gl{g.n[l]};
-    int I[3];
-    Gl.coords(node, I);
-    for (int k = 0; k < DIM; ++k)
-      I[k] <<= l;
-    return cmask[G0.index(I)] != 0;
-  };
+  auto masked = [&](int l, int node) -> bool { return mg_masked(g, cmask, w, l, node); };
   auto Aof = [&](int l) -> const double * { return l == 0 ? Kc : w + g.off_A[l]; };
   // constrained node of level l after the set-up: its Jacobi entry is exactly 0
   auto fixed = [&](int l, int node) -> bool { return w[g.off_d[l] + node] == 0.0; };
@@ -311,75 +427,31 @@ __global__ void __launch_bounds__(kCoarseThreads)
     return s;
   };
 
-  // ---- set-up: Galerkin operators, Jacobi diagonals, coarsest Cholesky ------------
-  for (int l = 1; l < L; ++l)
-    {
-      MgGrid<DIM> Gf{g.n[l - 1]}, Gc{g.n[l]};
-      const double *Af = Aof(l - 1);
-      double *Ac = w + g.off_A[l];
-      for (int c = tid; c < (int)g.nn[l]; c += kCoarseThreads)
-        {
-          double acc[S];
-#pragma unroll
-          for (int t = 0; t < S; ++t)
-            acc[t] = 0;
-          if (!masked(l, c))
-            {
-              int I[3];
-              Gc.coords(c, I);
-              for (int o = 0; o < S; ++o) // fine node i = 2I + o in the support of coarse node I
-                {
-                  const int od[3] = {o % 3 - 1, (o / 3) % 3 - 1, DIM == 3 ? o / 9 - 1 : 0};
-                  int i[3] = {2 * I[0] + od[0], 2 * I[1] + od[1], 2 * I[2] + od[2]};
-                  if (!Gf.inside(i))
-                    continue;
-                  const int fi = Gf.index(i);
-                  if (masked(l - 1, fi))
-                    continue;
-                  double wi = 1;
-                  for (int k = 0; k < DIM; ++k)
-                    wi *= od[k] ? 0.5 : 1.0;
-                  for (int t = 0; t < S; ++t) // fine neighbour j = i + s
-                    {
-                      int j[3] = {i[0] + t % 3 - 1, i[1] + (t / 3) % 3 - 1, DIM == 3 ? i[2] + t / 9 - 1 : 0};
-                      if (!Gf.inside(j) || masked(l - 1, Gf.index(j)))
-                        continue;
-                      const double a = wi * Af[(size_t)fi * S + t];
-                      // coarse parents J of j: per axis one (even) or two (odd) with weight 1/2
-                      int nodd = 0;
-                      for (int k = 0; k < DIM; ++k)
-                        nodd += j[k] & 1;
-                      for (int comb = 0; comb < (1 << nodd); ++comb)
-                        {
-                          int J[3] = {0, 0, 0}, bitpos = 0;
-                          double wj = 1;
-                          for (int k = 0; k < DIM; ++k)
-                            if (j[k] & 1)
-                              {
-                                J[k] = (j[k] >> 1) + ((comb >> bitpos) & 1);
-                                ++bitpos;
-                                wj *= 0.5;
-                              }
-                            else
-                              J[k] = j[k] >> 1;
-                          if (masked(l, Gc.index(J)))
-                            continue;
-                          int tt = 0, mul = 1;
-                          for (int k = 0; k < DIM; ++k)
-                            {
-                              tt += (J[k] - I[k] + 1) * mul;
-                              mul *= 3;
-                            }
-                          acc[tt] += a * wj;
-                        }
-                    }
-                }
-            }
-          for (int t = 0; t < S; ++t)
-            Ac[(size_t)c * S + t] = acc[t];
-        }
-      __syncthreads();
-    }
+  // (A_l D_l v)_node: the operator applied to the Jacobi-scaled vector (d = 0 on
+  // constrained nodes; rows of constrained nodes: 0); after the set-up
+  auto apply_dv = [&](int l, const double *v, int node) -> double {
+    if (fixed(l, node))
+      return 0.0;
+    MgGrid<DIM> Gl{g.n[l]};
+    int I[3];
+    Gl.coords(node, I);
+    const double *A = Aof(l) + (size_t)node * S;
+    const double *d = w + g.off_d[l];
+    double s = 0;
+    for (int t = 0; t < S; ++t)
+      {
+        int J[3] = {I[0] + t % 3 - 1, I[1] + (t / 3) % 3 - 1, DIM == 3 ? I[2] + t / 9 - 1 : 0};
+        if (Gl.inside(J))
+          {
+            const int m = Gl.index(J);
+            s += A[t] * (d[m] * v[m]);
+          }
+      }
+    return s;
+  };
+
+  // ---- set-up: Jacobi diagonals, coarsest Cholesky (the Galerkin operators of
+  // levels 1.. come from mg_galerkin_kernel launches before this kernel) ------
   for (int l = 0; l < L; ++l)
     {
       double *d = w + g.off_d[l];
@@ -450,19 +522,25 @@ __global__ void __launch_bounds__(kCoarseThreads)
         const double *b = l == 0 ? b0 : w + g.off_b[l];
         double *x = w + g.off_x[l], *t = w + g.off_t[l];
         const double *d = w + g.off_d[l];
+        double *u = w + g.off_u[l];
         const int nn = (int)g.nn[l];
+        // two damped Jacobi sweeps from x = 0 in two passes, each residual from
+        // the previous one's neighbours (x1 = w D b, t1 = b - A x1 = b - w A D b;
+        // x2 = x1 + w D t1, t2 = b - A x2 = t1 - w A D t1): the values of the
+        // four-pass form up to rounding, two barriers fewer
         for (int n = tid; n < nn; n += kCoarseThreads)
-          x[n] = kMgOmega * d[n] * b[n];
+          {
+            x[n] = kMgOmega * d[n] * b[n];
+            t[n] = fixed(l, n) ? 0.0 : b[n] - kMgOmega * apply_dv(l, b, n);
+          }
         __syncthreads();
         for (int n = tid; n < nn; n += kCoarseThreads)
-          t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
+          {
+            x[n] += kMgOmega * d[n] * t[n];
+            u[n] = fixed(l, n) ? 0.0 : t[n] - kMgOmega * apply_dv(l, t, n);
+          }
         __syncthreads();
-        for (int n = tid; n < nn; n += kCoarseThreads)
-          x[n] += kMgOmega * d[n] * t[n];
-        __syncthreads();
-        for (int n = tid; n < nn; n += kCoarseThreads)
-          t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
-        __syncthreads();
+        t = u; // the residual restricted below
         MgGrid<DIM> Gf{g.n[l]}, Gc{g.n[l + 1]};
         double *bc = w + g.off_b[l + 1];
         for (int c = tid; c < (int)g.nn[l + 1]; c += kCoarseThreads)
@@ -553,15 +631,22 @@ __global__ void __launch_bounds__(kCoarseThreads)
             x[n] += v;
           }
         __syncthreads();
-        for (int sweep = 0; sweep < 2; ++sweep)
+        // two damped Jacobi sweeps in two passes (t = b - A x, x1 = x + w D t;
+        // x2 = x1 + w D (b - A x1) = x1 + w D (t - w A D t)), x1 in u
+        double *u = w + g.off_u[l];
+        for (int n = tid; n < nn; n += kCoarseThreads)
           {
-            for (int n = tid; n < nn; n += kCoarseThreads)
-              t[n] = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
-            __syncthreads();
-            for (int n = tid; n < nn; n += kCoarseThreads)
-              x[n] += kMgOmega * d[n] * t[n];
-            __syncthreads();
+            const double tv = fixed(l, n) ? 0.0 : b[n] - apply(l, x, n);
+            t[n] = tv;
+            u[n] = x[n] + kMgOmega * d[n] * tv;
           }
+        __syncthreads();
+        for (int n = tid; n < nn; n += kCoarseThreads)
+          {
+            const double sv = fixed(l, n) ? 0.0 : t[n] - kMgOmega * apply_dv(l, t, n);
+            x[n] = u[n] + kMgOmega * d[n] * sv;
+          }
+        __syncthreads();
       }
   };
 
@@ -711,9 +796,11 @@ MgLayout mg_layout(int dim, int nc)
     {
       g.off_x[l] = off, off += g.nn[l];
       g.off_t[l] = off, off += g.nn[l];
+      g.off_u[l] = off, off += g.nn[l];
       g.off_d[l] = off, off += g.nn[l];
       g.off_b[l] = off, off += l > 0 ? g.nn[l] : 0; // level 0's right-hand side is the CG residual
       g.off_A[l] = off, off += l > 0 ? g.nn[l] * S : 0; // level 0's operator is the coarse matrix itself
+      g.off_mask[l] = off, off += l > 0 ? (g.nn[l] + 7) / 8 : 0; // level 0's flags are the coarse mask
     }
   const long long nd = g.nn[L - 1];
   g.off_dense = off, off += nd * nd;
@@ -738,6 +825,21 @@ hipError_t launch_coarse_solve(int dim, int nc, const double *Kc, const double *
   const MgLayout g = mg_layout(dim, nc);
   if (g.levels > 0)
     {
+      if (dim == 2)
+        hipLaunchKernelGGL(mg_mask_kernel<2>, dim3(64), dim3(256), 0, st, g, nc, cmask, work);
+      else
+        hipLaunchKernelGGL(mg_mask_kernel<3>, dim3(64), dim3(256), 0, st, g, nc, cmask, work);
+      for (int l = 1; l < g.levels; ++l) // Galerkin operators, level by level, over the whole GPU
+        {
+          if (dim == 2)
+            hipLaunchKernelGGL(mg_galerkin_kernel<2>,
+                               dim3((unsigned)((g.nn[l] + mg_galerkin_nodes<2>() - 1) / mg_galerkin_nodes<2>())),
+                               dim3(mg_galerkin_nodes<2>() * 9), 0, st, g, l, Kc, cmask, work);
+          else
+            hipLaunchKernelGGL(mg_galerkin_kernel<3>,
+                               dim3((unsigned)((g.nn[l] + mg_galerkin_nodes<3>() - 1) / mg_galerkin_nodes<3>())),
+                               dim3(mg_galerkin_nodes<3>() * 27), 0, st, g, l, Kc, cmask, work);
+        }
       if (dim == 2)
         hipLaunchKernelGGL(coarse_mgcg_kernel<2>, dim3(1), dim3(kCoarseThreads), 0, st, g, nc, Kc, Fc, cmask, cval,
                            rel_tol, max_iter, u, iters, work);

@@ -92,7 +92,8 @@ struct MgLayout {
   int n[kMgMaxLevels];               // cells per side of level l (n[0] = nc)
   long long nn[kMgMaxLevels];        // nodes of level l
   long long off_x[kMgMaxLevels], off_t[kMgMaxLevels], off_d[kMgMaxLevels], off_b[kMgMaxLevels],
-    off_A[kMgMaxLevels];
+    off_A[kMgMaxLevels], off_u[kMgMaxLevels];
+  long long off_mask[kMgMaxLevels];  // level l >= 1: constrained-node flags, bytes at (char *)(w + off_mask[l])
   long long off_cx, off_cr, off_cp, off_cq; // CG vectors on level 0
   long long off_dense, off_dmap;     // coarsest Cholesky factor [F][F], free-node numbers
   long long total;
