@@ -1109,6 +1109,15 @@ static int cell_build_general(int dim, int nf, const double *verts, int coef_kin
   return rc;
 }
 
+/* Exported form of cell_build_general (general quadrilateral or hexahedron). */
+int orc_cell_build_general(int dim, int nf, const double *verts, int coef_kind, const double *tile, int rhs_kind,
+                           double rhs_value, int solver, double omega, double abs_tol, int max_iter, double *phi,
+                           double *Ae, double *Fe, int *iters)
+{
+  return cell_build_general(dim, nf, verts, coef_kind, tile, rhs_kind, rhs_value, solver, omega, abs_tol, max_iter,
+                            phi, Ae, Fe, iters);
+}
+
 int orc_cell_build_quad(int nf, const double *quad, int coef_kind, const double *tile, int rhs_kind,
                         double rhs_value, int solver, double omega, double abs_tol, int max_iter, double *phi,
                         double *Ae, double *Fe, int *iters)
