@@ -399,6 +399,22 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
   const int kh = 8 * (lane / 32);
   const int arow = (wm + lane % 32) * CP + kh;
 
+  // the epilogue's bias and skip values are loaded now, so that they arrive
+  // during the MFMAs instead of costing a memory round trip at the end
+  float eb[16]; // bias of the lane's 16 output rows (the same for every j)
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    eb[e] = bias ? bias[m0 + wm + 8 * (e / 4) + 4 * (lane / 32) + (e % 4)] : 0.0f;
+  float ea[TN][16];
+#pragma unroll
+  for (int j = 0; j < TN; ++j)
+#pragma unroll
+    for (int e = 0; e < 16; ++e)
+      {
+        const int row = m0 + wm + 8 * (e / 4) + 4 * (lane / 32) + (e % 4);
+        ea[j][e] = add ? add[(size_t)row * plane + p0 + wn + j * 32 + lane % 32] : 0.0f;
+      }
+
   fetch(0);
   stash();
   __syncthreads();
@@ -439,13 +455,13 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
         float v = acc[j][e];
         if (bias)
           {
-            v += bias[row];
+            v += eb[e];
             if (relu && v < 0.0f) // as bias_act_kernel (NaN passes)
               v = 0.0f;
           }
         const size_t o = (size_t)row * plane + p0 + wn + j * 32 + lane % 32;
         if (add) // skip connection: act(acc + bias) + add, one FP32 addition
-          v += add[o];
+          v += ea[j][e];
         out[o] = v;
       }
 }
@@ -480,6 +496,11 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
   const size_t plane = (size_t)B * HW;
   const size_t iplane = UP ? plane / 4 : plane;
 
+  // the bias of the lane's 16 output rows, once (every tile has the same rows)
+  float eb[16];
+#pragma unroll
+  for (int e = 0; e < 16; ++e)
+    eb[e] = bias ? bias[8 * (e / 4) + 4 * (lane / 32) + (e % 4)] : 0.0f;
   // the weights of every block, [blk][tap][co][ci], once
   for (int u = tid; u < NB * BM * 36; u += 256)
     {
@@ -587,7 +608,7 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
               float v = acc[e];
               if (bias)
                 {
-                  v += bias[row];
+                  v += eb[e];
                   if (relu && v < 0.0f) // as bias_act_kernel (NaN passes)
                     v = 0.0f;
                 }

@@ -388,7 +388,7 @@ class Surrogate:
         return s
 
     # ---- serving ------------------------------------------------------------------------
-    def predict_bases(self, x, n, bases_ptr, batch=512):
+    def predict_bases(self, x, n, bases_ptr, batch=2048):
         """x: device FP32 [n][tile*tile] normalized log-coefficients (g / CLIP);
         bases_ptr: device FP64 bases [n][4][(tile+1)^2], filled with N_i + dscale *
         prediction (the x0 of msfem_batch_build_device), `batch` tiles per forward
@@ -634,7 +634,7 @@ def energy_error(alpha, phi, ref):
     return float(np.max(np.sqrt(energy_norm2(alpha, phi - ref) / energy_norm2(alpha, ref))))
 
 
-def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512, bound_ptr=None):
+def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=2048, bound_ptr=None):
     """One serving step -- prediction and warm-started basis build of all cells
     of `data` -- in `chunks` chunks: the inference of every chunk runs on the
     context's second stream (a mark after each), the build of chunk k on the
@@ -676,7 +676,7 @@ def serve_pipelined(net, data, x0, opts, chunks=4, mark0=2, batch=512, bound_ptr
 
 
 def serve_report(ctx, net, n_side=128, seed=41, rel_tol=1e-8, reps=3, warmup=1, sample=64, log=None, data=None,
-                 pipe_batch=512):
+                 pipe_batch=2048):
     """Times one step -- the basis build of n_side^2 fresh GRF cells plus the
     coarse assembly -- five ways: plain FP64, FP32+FP64 refinement, and each of
     them warm-started from the network's prediction (inference included in the
