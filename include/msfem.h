@@ -432,6 +432,12 @@ int msfem_nn_conv_bf16(int Cin, int B, int H, int W, int kh, int kw, int pad, co
 int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stream);
 int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
                           float *out, const float *bias, int relu, const float *add, void *stream);
+/* The same, also writing pool (or nothing for NULL) = the 2x2 average pooling
+ * of out, [Cout][B][H/2][W/2], bitwise what msfem_nn_pool2 of out gives (an
+ * encoder layer and the pooling after it in one pass; H even). */
+int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in,
+                               int upsample, float *out, const float *bias, int relu, const float *add, float *pool,
+                               void *stream);
 /* Direct convolution for few output channels (Cout <= 8: a network's head)
  * on the VALU: exact BF16 x BF16 products of the rounded input and weights w
  * [Cout][Cin kh kw] summed in FP32; bias/relu as msfem_nn_conv_bf16.  The same

@@ -264,6 +264,34 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(int M, int N, int K, con
 
 typedef __bf16 bf16x8m_t __attribute__((ext_vector_type(8))); // v_mfma_f32_32x32x16_bf16 operand
 
+// 2x2 average pooling of one finished output tile -- BM channels x kGBN pixels
+// of whole rows (W >= 16) or whole W x W images (W <= 8), staged as floats
+// s[channel][pixel] in LDS -- into pool[c][b][y/2][x/2] (plane/4 values per
+// channel), in pool2_fwd_kernel's order of operations, so the pooled values
+// are bitwise those of msfem_nn_pool2 on the stored output.  The tile's pooled
+// pixels are contiguous from p0 / 4 (the tile starts on an even row).
+template <int BM, int W>
+__device__ __forceinline__ void pool_tile(const float *s, int m0, size_t plane, int p0, float *pool)
+{
+  constexpr int Q = kGBN / 4; // pooled pixels per tile
+  static_assert(W >= 16 ? (kGBN / W) % 2 == 0 : W % 2 == 0, "tiles of whole 2x2 windows");
+  for (int idx = threadIdx.x; idx < BM * Q; idx += blockDim.x)
+    {
+      const int r = idx / Q, q = idx % Q;
+      int p00;
+      if constexpr (W >= 16)
+        p00 = 2 * (q / (W / 2)) * W + 2 * (q % (W / 2));
+      else
+        {
+          constexpr int PER = W * W / 4;
+          const int qq = q % PER;
+          p00 = (q / PER) * W * W + 2 * (qq / (W / 2)) * W + 2 * (qq % (W / 2));
+        }
+      const float *t = s + r * kGBN + p00;
+      pool[(size_t)(m0 + r) * (plane / 4) + (size_t)p0 / 4 + q] = 0.25f * ((t[0] + t[1]) + (t[W] + t[W + 1]));
+    }
+}
+
 // Direct 3x3 convolution, padding 1, stride 1, on the matrix cores:
 //   out[co][b][y][x] = act(bias[co] + sum_{ci,ky,kx} w[co][ci][ky][kx] in[ci][b][y + ky - 1][x + kx - 1])
 // with BF16 operands and FP32 accumulation.  A workgroup owns BM output
@@ -281,7 +309,8 @@ typedef __bf16 bf16x8m_t __attribute__((ext_vector_type(8))); // v_mfma_f32_32x3
 // FP32 rounding only.
 template <int BM, int W, bool UP>
 __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H, const float *in, const uint16_t *wp,
-                                                           float *out, const float *bias, int relu, const float *add)
+                                                           float *out, const float *bias, int relu, const float *add,
+                                                           float *pool)
 {
   constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI; // images and rows per tile (W <= 8: H = W)
   constexpr int IR = TR + 2, IC = W + 2, NPIX = TI * IR * IC;
@@ -463,7 +492,15 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
         if (add) // skip connection: act(acc + bias) + add, one FP32 addition
           v += ea[j][e];
         out[o] = v;
+        if (pool) // the block loop ended with a barrier: sW is free
+          ((float *)sW)[(row - m0) * kGBN + wn + j * 32 + lane % 32] = v;
       }
+  if (pool) // (uniform)
+    {
+      static_assert(sizeof(sW) >= BM * kGBN * sizeof(float), "the staged tile fits the weight buffer");
+      __syncthreads();
+      pool_tile<BM, W>((const float *)sW, m0, plane, p0, pool);
+    }
 }
 
 // Persistent form of conv3x3_bf16_kernel for 32 output channels and NB <= 2
@@ -479,7 +516,7 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
 template <int W, bool UP, int NB>
 __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int ntiles, const float *in,
                                                             const uint16_t *wp, float *out, const float *bias,
-                                                            int relu, const float *add)
+                                                            int relu, const float *add, float *pool)
 {
   constexpr int BM = 32, K = NB * 32 * 9;
   constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI;
@@ -508,11 +545,14 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
       *(bf16x8_t *)&sW[((blk * 9 + j / 4) * BM + m) * CP + (j % 4) * 8] =
           *(const bf16x8_t *)(wp + (size_t)m * K + blk * 288 + j * 8);
     }
-  for (int h = tid; h < TI * IR * 2 * 4; h += 256) // halo columns: (staged row, side, octet)
-    {
-      const int rr = h / 8, side = h / 4 % 2, oct = h % 4;
-      *(bf16x8_t *)&sIn[(rr * IC + side * (IC - 1)) * CP + oct * 8] = bf16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
-    }
+  auto zero_halo = [&]() {
+    for (int h = tid; h < TI * IR * 2 * 4; h += 256) // halo columns: (staged row, side, octet)
+      {
+        const int rr = h / 8, side = h / 4 % 2, oct = h % 4;
+        *(bf16x8_t *)&sIn[(rr * IC + side * (IC - 1)) * CP + oct * 8] = bf16x8_t{0, 0, 0, 0, 0, 0, 0, 0};
+      }
+  };
+  zero_halo();
 
   // input staging as in conv3x3_bf16_kernel; iok describes the step whose
   // values are in the registers
@@ -599,6 +639,7 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
               acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
             }
         }
+      float pv[16]; // (pool: the tile's values, staged in sIn once every wave is done with it)
       if (blk == NB - 1) // the tile is complete: epilogue, then a fresh accumulator
         {
 #pragma unroll
@@ -616,10 +657,24 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
               if (add)
                 v += add[o];
               out[o] = v;
+              pv[e] = v;
               acc[e] = 0.0f;
             }
         }
       __syncthreads(); // every wave is done with sIn
+      if constexpr (W == 64) // (the host asks for pooling at W = 64 only: sIn holds the tile there)
+        if (pool && blk == NB - 1) // (uniform)
+          {
+            static_assert(sizeof(sIn) >= BM * kGBN * sizeof(float), "the staged tile fits the input buffer");
+            float *so = (float *)sIn;
+#pragma unroll
+            for (int e = 0; e < 16; ++e)
+              so[(8 * (e / 4) + 4 * (lane / 32) + (e % 4)) * kGBN + n] = pv[e];
+            __syncthreads();
+            pool_tile<BM, W>(so, 0, plane, tile * kGBN, pool);
+            __syncthreads();
+            zero_halo(); // the staged tile covered the halo columns; the next stash writes only the interior
+          }
       if (!more)
         break;
       stash();
@@ -1180,6 +1235,13 @@ int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stre
 int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
                           float *out, const float *bias, int relu, const float *add, void *stream)
 {
+  return msfem_nn_conv3x3_bf16_pool(Cin, B, H, W, wp, Cout, in, upsample, out, bias, relu, add, nullptr, stream);
+}
+
+int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in,
+                               int upsample, float *out, const float *bias, int relu, const float *add, float *pool,
+                               void *stream)
+{
   if (Cin <= 0 || Cin % 32 != 0 || B <= 0 || H <= 0 || Cout <= 0 || Cout % 32 != 0 || !wp || !in || !out ||
       (W != 4 && W != 8 && W != 16 && W != 32 && W != 64) || (W <= 8 ? H != W : H % (kGBN / W) != 0) ||
       ((uintptr_t)wp % 16) != 0 || ((uintptr_t)in % (upsample ? 8 : 16)) != 0 || (upsample && H % 2 != 0))
@@ -1191,7 +1253,9 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
   hipStream_t st = (hipStream_t)stream;
   const uint16_t *w = (const uint16_t *)wp;
   const int ntiles = (int)(plane / kGBN);
-  if (Cout == 32 && Cin == 32) // persistent form: weights staged once per workgroup
+  if (pool && (H % 2 != 0 || (long long)Cout * (plane / 4) >= (1ll << 31)))
+    return MSFEM_ERR_ARG;
+  if (Cout == 32 && Cin == 32 && (!pool || W == 64)) // persistent form: weights staged once per workgroup
     {
       // (with two input blocks the resident weights cost a third of the
       // occupancy, and the tile form is faster)
@@ -1199,7 +1263,8 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
       hipError_t e = hipSuccess;
 #define MSFEM_C3P_CASE(Wv, UPv, NBv)                                                                                  \
   if (W == Wv && (upsample != 0) == UPv && nb == NBv)                                                                \
-    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv>, ntiles, st, B, H, ntiles, in, w, out, bias, relu, add);
+    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv>, ntiles, st, B, H, ntiles, in, w, out, bias, relu, add,    \
+                     pool);
 #define MSFEM_C3P_W(Wv)                                                                                               \
   MSFEM_C3P_CASE(Wv, false, 1)                                                                                       \
   MSFEM_C3P_CASE(Wv, true, 1)
@@ -1219,10 +1284,10 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
     {                                                                                                                \
       if (upsample)                                                                                                  \
         hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, true>), grid, block, 0, st, Cin, B, H, in, w, out, bias,    \
-                           relu, add);                                                                               \
+                           relu, add, pool);                                                                         \
       else                                                                                                           \
         hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, false>), grid, block, 0, st, Cin, B, H, in, w, out, bias,   \
-                           relu, add);                                                                               \
+                           relu, add, pool);                                                                         \
     }
   MSFEM_C3_CASE(32, 4)
   MSFEM_C3_CASE(32, 8)

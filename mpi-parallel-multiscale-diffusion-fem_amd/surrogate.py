@@ -36,6 +36,7 @@ def _nn():
         L.msfem_nn_conv3x3_pack.argtypes = [i32, i32, vp, vp, vp]
         L.msfem_nn_conv_small.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, i32, vp]
         L.msfem_nn_conv3x3_bf16.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp]
+        L.msfem_nn_conv3x3_bf16_pool.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
         L.msfem_nn_act_bwd.argtypes = [u64, vp, vp, vp, i32, vp]
@@ -129,6 +130,15 @@ class Conv:
         at H/2 x W/2, upsampled inside the convolution's staging (no saved state)."""
         _ok(_nn().msfem_nn_conv3x3_bf16(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 1, out.ptr, self.b.ptr,
                                         1 if self.relu else 0, skip.ptr, self.st))
+
+    def forward_pool(self, x, B, H, W, buf, pool):
+        """Serving form of an encoder layer followed by 2x2 average pooling:
+        y = act(conv(x) + b) (kept, the skip) and pool = pool2(y) in one pass,
+        bitwise the values of forward + msfem_nn_pool2."""
+        y = buf.get(self.name + ".y", self.cout * B * H * W, np.float32)
+        _ok(_nn().msfem_nn_conv3x3_bf16_pool(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 0, y.ptr, self.b.ptr,
+                                             1 if self.relu else 0, None, pool.ptr, self.st))
+        return y
 
     def small_direct(self):
         """Whether msfem_nn_conv_small takes this layer (the head: at most 8 output channels)."""
@@ -272,14 +282,27 @@ class Surrogate:
         H = self.tile
         h, _, _ = self.enc[0].forward(x, B, H, H, buf)
         skips = []
-        h, _, _ = self.enc[1].forward(h, B, H, H, buf)
+        fused = not keep  # serving: each encoder layer writes its pooled output too
+        pend = None  # pooled output of the last encoder layer, when it was fused
+        if fused and self.enc[1].direct3x3(B, H, H):
+            pend = buf.get("pool1", self.chans[0] * B * (H // 2) ** 2, np.float32)
+            h = self.enc[1].forward_pool(h, B, H, H, buf, pend)
+        else:
+            h, _, _ = self.enc[1].forward(h, B, H, H, buf)
         skips.append((h, H))
         for l in range(1, self.levels):
             C = self.chans[l - 1]
-            p = buf.get(f"pool{l}", C * B * (H // 2) ** 2, np.float32)
-            _ok(L.msfem_nn_pool2(C * B, H, H, h.ptr, p.ptr, 0, st))
+            if pend is not None:
+                p, pend = pend, None
+            else:
+                p = buf.get(f"pool{l}", C * B * (H // 2) ** 2, np.float32)
+                _ok(L.msfem_nn_pool2(C * B, H, H, h.ptr, p.ptr, 0, st))
             H //= 2
-            h, _, _ = self.enc[l + 1].forward(p, B, H, H, buf)
+            if fused and l + 1 < self.levels and self.enc[l + 1].direct3x3(B, H, H):
+                pend = buf.get(f"pool{l + 1}", self.chans[l] * B * (H // 2) ** 2, np.float32)
+                h = self.enc[l + 1].forward_pool(p, B, H, H, buf, pend)
+            else:
+                h, _, _ = self.enc[l + 1].forward(p, B, H, H, buf)
             skips.append((h, H))
         self.skips = skips
         for l in range(self.levels - 2, -1, -1):

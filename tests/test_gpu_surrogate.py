@@ -272,6 +272,36 @@ def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
         a.free()
 
 
+@pytest.mark.parametrize("Cin,Cout,B,H,W", [
+    (32, 32, 2, 64, 64), (32, 32, 64, 64, 64), (32, 64, 1, 32, 32), (64, 128, 2, 16, 16), (128, 256, 4, 8, 8),
+    (256, 256, 16, 4, 4), (32, 32, 3, 32, 32), (64, 32, 2, 64, 64)])
+def test_conv3x3_fused_pool(ctx, Cin, Cout, B, H, W):
+    """msfem_nn_conv3x3_bf16_pool: the convolution's output is bitwise that of
+    msfem_nn_conv3x3_bf16 and the pooled output bitwise msfem_nn_pool2 of it --
+    the persistent kernel (32 -> 32 at 64^2, one tile and many tiles per
+    workgroup) and the tile kernel (BM 32 and 64, whole rows and whole images
+    per tile)."""
+    say(f"fused pool conv3x3 Cin={Cin} Cout={Cout} B={B} H={H} W={W}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(7 * Cin + Cout + B + H + W)
+    K, N = Cin * 9, B * H * W
+    x = ctx.to_device(rng.standard_normal((Cin, B, H, W)).astype(np.float32))
+    dw = ctx.to_device(bf16_bits(R.bf16((rng.standard_normal((Cout, K)) / np.sqrt(K)).astype(np.float32))))
+    db = ctx.to_device(rng.standard_normal(Cout).astype(np.float32))
+    wpk = ctx.alloc(Cout * K, np.uint16)
+    y1, y2 = ctx.alloc(Cout * N, np.float32), ctx.alloc(Cout * N, np.float32)
+    p1, p2 = ctx.alloc(Cout * N // 4, np.float32), ctx.alloc(Cout * N // 4, np.float32)
+    assert L.msfem_nn_conv3x3_pack(Cin, Cout, dw.ptr, wpk.ptr, st) == 0
+    assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, wpk.ptr, Cout, x.ptr, 0, y1.ptr, db.ptr, 1, None, st) == 0
+    assert L.msfem_nn_pool2(Cout * B, H, W, y1.ptr, p1.ptr, 0, st) == 0
+    assert L.msfem_nn_conv3x3_bf16_pool(Cin, B, H, W, wpk.ptr, Cout, x.ptr, 0, y2.ptr, db.ptr, 1, None, p2.ptr,
+                                        st) == 0
+    assert np.array_equal(y1.numpy(), y2.numpy())
+    assert np.array_equal(p1.numpy(), p2.numpy()) and np.abs(p1.numpy()).max() > 0
+    for a in (x, dw, db, wpk, y1, y2, p1, p2):
+        a.free()
+
+
 @pytest.mark.parametrize("Cin,B,H,W,up", [
     (32, 2, 64, 64, 0), (32, 1, 64, 64, 1), (64, 3, 32, 32, 0), (32, 2, 32, 16, 1), (32, 6, 8, 8, 1),
     (32, 8, 4, 4, 0), (64, 1, 64, 64, 1)])
