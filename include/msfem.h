@@ -438,6 +438,16 @@ int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout
 int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in,
                                int upsample, float *out, const float *bias, int relu, const float *add, float *pool,
                                void *stream);
+/* A network's first two layers in one pass: h = act0(conv3x3(in, w0) + b0)
+ * (one input channel -> 32, padding 1; w0 BF16 [32][9] as msfem_nn_conv_bf16
+ * takes it) is formed in the staging of out = act(conv3x3(h) + bias)
+ * (32 -> Cout = 32, wp packed by msfem_nn_conv3x3_pack), with the optional
+ * 2x2 average pooling of out as msfem_nn_conv3x3_bf16_pool.  in: [B][H][64]
+ * FP32, 16-byte aligned, H even.  Bitwise equal to msfem_nn_conv_bf16 (no
+ * split) followed by msfem_nn_conv3x3_bf16_pool, without h in memory. */
+int msfem_nn_conv3x3_first_bf16(int B, int H, int W, const void *w0, const float *b0, int relu0, const float *in,
+                                const void *wp, int Cout, float *out, const float *bias, int relu, float *pool,
+                                void *stream);
 /* Direct convolution for few output channels (Cout <= 8: a network's head)
  * on the VALU: exact BF16 x BF16 products of the rounded input and weights w
  * [Cout][Cin kh kw] summed in FP32; bias/relu as msfem_nn_conv_bf16.  The same

@@ -513,10 +513,20 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
 // loaded into registers during the MFMAs and the epilogue of the current one.
 // The sums run in the same order as in conv3x3_bf16_kernel (blocks, taps,
 // 16-channel steps), so the results are bitwise those of the tile form.
-template <int W, bool UP, int NB>
+// FIRST (W = 64, one block): the layer's input is itself computed in the
+// staging from a one-channel image, as a network's first layer (1 -> 32
+// channels, 3x3, padding 1) produces it: in is then [B][H][W] FP32, w0 that
+// layer's BF16 weights [32][9] and b0 its bias.  The staged rows are act(conv(in)
+// + b0) formed as the implicit GEMM (gemm_bf16_kernel with CONV) forms them:
+// the same v_mfma_f32_32x32x8 steps over k = 0..31 (zero beyond 9) on the
+// same operands, then bias and ReLU, so the result is bitwise that of the
+// two layers run one after the other, without the 32-channel intermediate in
+// memory.
+template <int W, bool UP, int NB, bool FIRST>
 __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int ntiles, const float *in,
                                                             const uint16_t *wp, float *out, const float *bias,
-                                                            int relu, const float *add, float *pool)
+                                                            int relu, const float *add, float *pool,
+                                                            const uint16_t *w0, const float *b0, int relu0)
 {
   constexpr int BM = 32, K = NB * 32 * 9;
   constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI;
@@ -524,8 +534,13 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
   constexpr int CP = 40;
   constexpr int NG = W / 4, IIT = TI * IR * NG * 4;
   static_assert(IIT <= 256 && kGBN % W == 0 && NB >= 1 && NB <= 2, "shape outside the persistent form");
+  static_assert(!FIRST || (W == 64 && !UP && NB == 1), "the fused first layer: 64-wide rows, one block");
+  constexpr int XR = IR + 2;                     // FIRST: one-channel rows staged (the layer's rows and their halo)
+  constexpr int XB = IR * W / 32;                // FIRST: 32-pixel blocks of the first layer's staged output
+  static_assert(!FIRST || (XR * NG <= 256 && XB % 4 == 0), "one one-channel item per thread, whole blocks per wave");
   __shared__ __align__(16) uint16_t sIn[NPIX * CP];
   __shared__ __align__(16) uint16_t sW[NB * 9 * BM * CP];
+  __shared__ __align__(16) uint16_t sX[FIRST ? XR * IC : 1]; // [row][column], zero halo columns
 
   const int tid = threadIdx.x, wave = tid / 64, lane = tid % 64;
   const int wn = wave * 32; // four waves along the pixels, one 32 x 32 block each
@@ -553,6 +568,28 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
       }
   };
   zero_halo();
+
+  // FIRST: the first layer's weights as this lane's MFMA A operands (row
+  // lane % 32, k = 8 q + 4 (lane / 32) + i, zero beyond k = 8) and the bias of
+  // its 16 output rows
+  bf16x4 a0[4];
+  float eb0[16];
+  if constexpr (FIRST)
+    {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          {
+            const int k = 8 * q + 4 * (lane / 32) + i;
+            a0[q][i] = k < 9 ? (short)w0[(lane % 32) * 9 + k] : (short)0;
+          }
+#pragma unroll
+      for (int e = 0; e < 16; ++e)
+        eb0[e] = b0 ? b0[8 * (e / 4) + 4 * (lane / 32) + (e % 4)] : 0.0f;
+      for (int h = tid; h < XR * 2; h += 256) // the one-channel rows' halo columns
+        sX[(h / 2) * IC + (h % 2) * (IC - 1)] = 0;
+    }
 
   // input staging as in conv3x3_bf16_kernel; iok describes the step whose
   // values are in the registers
@@ -600,6 +637,88 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
           }
       }
   };
+  // FIRST: item (one-channel row, 4-column group) of the rows y0 - 2 .. y0 + IR
+  // - 1 around the tile whose input is fetched (xt); the first layer's output
+  // rows y0 - 1 .. y0 + IR - 2 then go to sIn as act(acc + b0), zero outside
+  // the image
+  float4 rx = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  bool xok = false;
+  int xt = 0;
+  const int xr = tid / NG, xg = tid % NG;
+  auto fetch_first = [&](int tile) __attribute__((always_inline)) {
+    xt = tile;
+    const int p0 = tile * kGBN, b = p0 / HW, y0 = (p0 - b * HW) / W;
+    const int iy = y0 - 2 + xr;
+    xok = tid < XR * NG && iy >= 0 && iy < H;
+    if (xok)
+      rx = *(const float4 *)(in + (size_t)b * HW + (size_t)iy * W + 4 * xg);
+  };
+  auto stash_first = [&]() __attribute__((always_inline)) {
+    if (tid < XR * NG)
+      {
+        uint16_t *d = &sX[xr * IC + 1 + 4 * xg];
+        d[0] = xok ? f32_to_bf16(rx.x) : (uint16_t)0;
+        d[1] = xok ? f32_to_bf16(rx.y) : (uint16_t)0;
+        d[2] = xok ? f32_to_bf16(rx.z) : (uint16_t)0;
+        d[3] = xok ? f32_to_bf16(rx.w) : (uint16_t)0;
+      }
+    __syncthreads();
+    const int p0 = xt * kGBN, b = p0 / HW, y0 = (p0 - b * HW) / W;
+#pragma unroll
+    for (int jj = 0; jj < XB / 4; ++jj)
+      {
+        const int p = 32 * (wave + 4 * jj) + lane % 32, pr = p / W, pc = p % W;
+        f32x16 a;
+#pragma unroll
+        for (int e = 0; e < 16; ++e)
+          a[e] = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          {
+            bf16x4 bv;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              {
+                const int k = 8 * q + 4 * (lane / 32) + i; // tap (k / 3, k % 3)
+                bv[i] = k < 9 ? (short)sX[(pr + k / 3) * IC + pc + k % 3] : (short)0;
+              }
+            a = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a0[q], bv, a, 0, 0, 0);
+          }
+        const int y = y0 - 1 + pr;
+        const bool rok = y >= 0 && y < H;
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+          {
+            bf16x4_t v;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              {
+                float f = a[4 * g + i];
+                if (b0)
+                  {
+                    f += eb0[4 * g + i];
+                    if (relu0 && f < 0.0f) // as the implicit GEMM's epilogue (NaN passes)
+                      f = 0.0f;
+                  }
+                v[i] = rok ? f32_to_bf16(f) : (uint16_t)0;
+              }
+            *(bf16x4_t *)&sIn[(pr * IC + 1 + pc) * CP + 8 * g + 4 * (lane / 32)] = v;
+          }
+      }
+  };
+  // (always_inline: a lambda called twice and left out of line keeps its captures in scratch)
+  auto fetch_step = [&](int tile, int blk) __attribute__((always_inline)) {
+    if constexpr (FIRST)
+      fetch_first(tile);
+    else
+      fetch(tile, blk);
+  };
+  auto stash_step = [&]() __attribute__((always_inline)) {
+    if constexpr (FIRST)
+      stash_first();
+    else
+      stash();
+  };
 
   const int n = wn + lane % 32, img = n / (TR * W), r = n % (TR * W) / W;
   const int pb = ((img * IR + r) * IC + n % W) * CP;
@@ -613,8 +732,8 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
   int tile = blockIdx.x, blk = 0;
   if (tile >= ntiles) // (the host sizes the grid to at most ntiles)
     return;
-  fetch(tile, 0);
-  stash();
+  fetch_step(tile, 0);
+  stash_step();
   __syncthreads();
   for (;;)
     {
@@ -626,7 +745,7 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
         }
       const bool more = ntile < ntiles;
       if (more)
-        fetch(ntile, nblk); // in flight during the MFMAs and the epilogue below
+        fetch_step(ntile, nblk); // in flight during the MFMAs and the epilogue below
 #pragma unroll
       for (int tap = 0; tap < 9; ++tap)
         {
@@ -677,7 +796,7 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
           }
       if (!more)
         break;
-      stash();
+      stash_step();
       __syncthreads();
       tile = ntile;
       blk = nblk;
@@ -1263,8 +1382,8 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
       hipError_t e = hipSuccess;
 #define MSFEM_C3P_CASE(Wv, UPv, NBv)                                                                                  \
   if (W == Wv && (upsample != 0) == UPv && nb == NBv)                                                                \
-    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv>, ntiles, st, B, H, ntiles, in, w, out, bias, relu, add,    \
-                     pool);
+    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv, false>, ntiles, st, B, H, ntiles, in, w, out, bias, relu,  \
+                     add, pool, (const uint16_t *)nullptr, (const float *)nullptr, 0);
 #define MSFEM_C3P_W(Wv)                                                                                               \
   MSFEM_C3P_CASE(Wv, false, 1)                                                                                       \
   MSFEM_C3P_CASE(Wv, true, 1)
@@ -1301,6 +1420,23 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
   MSFEM_C3_CASE(64, 64)
 #undef MSFEM_C3_CASE
   return check_launch();
+}
+
+int msfem_nn_conv3x3_first_bf16(int B, int H, int W, const void *w0, const float *b0, int relu0, const float *in,
+                                const void *wp, int Cout, float *out, const float *bias, int relu, float *pool,
+                                void *stream)
+{
+  if (B <= 0 || H <= 0 || W != 64 || H % 2 != 0 || Cout != 32 || !w0 || !in || !wp || !out ||
+      ((uintptr_t)in % 16) != 0 || ((uintptr_t)wp % 16) != 0)
+    return MSFEM_ERR_ARG;
+  const long long plane = (long long)B * H * W;
+  if ((long long)Cout * plane >= (1ll << 31))
+    return MSFEM_ERR_ARG;
+  const int ntiles = (int)(plane / kGBN);
+  const hipError_t e = launch_tiles(conv3x3_tiles_kernel<64, false, 1, true>, ntiles, (hipStream_t)stream, B, H,
+                                    ntiles, in, (const uint16_t *)wp, out, bias, relu, (const float *)nullptr, pool,
+                                    (const uint16_t *)w0, b0, relu0);
+  return e == hipSuccess ? check_launch() : MSFEM_ERR_HIP;
 }
 
 int msfem_nn_conv_small(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,

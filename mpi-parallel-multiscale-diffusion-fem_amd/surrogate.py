@@ -37,6 +37,7 @@ def _nn():
         L.msfem_nn_conv_small.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, i32, vp]
         L.msfem_nn_conv3x3_bf16.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp]
         L.msfem_nn_conv3x3_bf16_pool.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp]
+        L.msfem_nn_conv3x3_first_bf16.argtypes = [i32, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
         L.msfem_nn_act_bwd.argtypes = [u64, vp, vp, vp, i32, vp]
@@ -274,20 +275,29 @@ class Surrogate:
     def forward(self, x, B, keep=True):
         """x: DeviceArray [1][B][tile][tile] FP32 (normalized log-coefficient).
         Returns the head output [4][B][tile-1][tile-1] (DeviceArray view).
-        keep=False (inference): decoder layers that the direct 3x3 kernel takes
-        read the upsampled input and add the skip inside the convolution; the
-        values are bitwise those of the training form, which keeps the
-        intermediates the backward pass needs."""
+        keep=False (inference): the first two layers run as one pass, encoder
+        layers write their pooled output beside their own, and decoder layers
+        that the direct 3x3 kernel takes read the upsampled input and add the
+        skip inside the convolution; the values are bitwise those of the
+        training form, which keeps the intermediates the backward pass needs."""
         L, st, buf = _nn(), self.st, self.buf
         H = self.tile
-        h, _, _ = self.enc[0].forward(x, B, H, H, buf)
         skips = []
         fused = not keep  # serving: each encoder layer writes its pooled output too
         pend = None  # pooled output of the last encoder layer, when it was fused
-        if fused and self.enc[1].direct3x3(B, H, H):
+        if fused and self.first_fused(H):  # the first two layers in one pass, pooled
+            pend = buf.get("pool1", self.chans[0] * B * (H // 2) ** 2, np.float32)
+            e0a, e0 = self.enc[0], self.enc[1]
+            h = buf.get(e0.name + ".y", e0.cout * B * H * H, np.float32)
+            _ok(L.msfem_nn_conv3x3_first_bf16(B, H, H, e0a.wbf.ptr, e0a.b.ptr, 1 if e0a.relu else 0, x.ptr,
+                                              e0.packed(), e0.cout, h.ptr, e0.b.ptr, 1 if e0.relu else 0, pend.ptr,
+                                              st))
+        elif fused and self.enc[1].direct3x3(B, H, H):
+            h, _, _ = self.enc[0].forward(x, B, H, H, buf)
             pend = buf.get("pool1", self.chans[0] * B * (H // 2) ** 2, np.float32)
             h = self.enc[1].forward_pool(h, B, H, H, buf, pend)
         else:
+            h, _, _ = self.enc[0].forward(x, B, H, H, buf)
             h, _, _ = self.enc[1].forward(h, B, H, H, buf)
         skips.append((h, H))
         for l in range(1, self.levels):
@@ -324,6 +334,13 @@ class Surrogate:
         out, _, _ = self.head.forward(h, B, H, H, buf)
         self.B = B
         return out
+
+    def first_fused(self, H):
+        """Whether msfem_nn_conv3x3_first_bf16 takes the first two layers (1 -> 32 -> 32
+        channels, 3x3, on 64-wide tiles)."""
+        a, b = self.enc[0], self.enc[1]
+        return (H == 64 and a.cin == 1 and a.k == 3 and a.pad == 1 and a.cout == 32 and b.cin == 32 and b.k == 3
+                and b.pad == 1 and b.cout == 32)
 
     def backward(self, dout):
         """dout: gradient of the loss w.r.t. the head output (consumed)."""

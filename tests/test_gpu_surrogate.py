@@ -302,6 +302,41 @@ def test_conv3x3_fused_pool(ctx, Cin, Cout, B, H, W):
         a.free()
 
 
+@pytest.mark.parametrize("B,H,pooled", [(1, 64, 1), (2, 64, 0), (64, 64, 1), (3, 16, 1)])
+def test_conv3x3_first_fused(ctx, B, H, pooled):
+    """msfem_nn_conv3x3_first_bf16 (a 1 -> 32 3x3 layer formed in the staging of
+    the 32 -> 32 one) is bitwise msfem_nn_conv_bf16 followed by
+    msfem_nn_conv3x3_bf16_pool, with one tile and many tiles per workgroup and
+    short images (the first layer's zero rows at both image edges)."""
+    say(f"fused first layer B={B} H={H} pool={pooled}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(11 * B + H)
+    W, N = 64, B * H * 64
+    x = ctx.to_device(rng.standard_normal((B, H, W)).astype(np.float32))
+    w0 = ctx.to_device(bf16_bits(R.bf16((rng.standard_normal((32, 9)) / 3).astype(np.float32))))
+    b0 = ctx.to_device((0.3 * rng.standard_normal(32)).astype(np.float32))
+    dw = ctx.to_device(bf16_bits(R.bf16((rng.standard_normal((32, 288)) / np.sqrt(288)).astype(np.float32))))
+    db = ctx.to_device(rng.standard_normal(32).astype(np.float32))
+    wpk = ctx.alloc(32 * 288, np.uint16)
+    h = ctx.alloc(32 * N, np.float32)
+    y1, y2 = ctx.alloc(32 * N, np.float32), ctx.alloc(32 * N, np.float32)
+    p1, p2 = ctx.alloc(32 * N // 4, np.float32), ctx.alloc(32 * N // 4, np.float32)
+    assert L.msfem_nn_conv3x3_pack(32, 32, dw.ptr, wpk.ptr, st) == 0
+    assert L.msfem_nn_conv_bf16(1, B, H, W, 3, 3, 1, w0.ptr, 32, x.ptr, h.ptr, 1, None, b0.ptr, 1, st) == 0
+    assert L.msfem_nn_conv3x3_bf16_pool(32, B, H, W, wpk.ptr, 32, h.ptr, 0, y1.ptr, db.ptr, 1, None,
+                                        p1.ptr if pooled else None, st) == 0
+    assert L.msfem_nn_conv3x3_first_bf16(B, H, W, w0.ptr, b0.ptr, 1, x.ptr, wpk.ptr, 32, y2.ptr, db.ptr, 1,
+                                         p2.ptr if pooled else None, st) == 0
+    ref = y1.numpy()
+    assert np.array_equal(ref, y2.numpy()) and (ref > 0).any() and (ref == 0).any()
+    if pooled:
+        assert np.array_equal(p1.numpy(), p2.numpy())
+    assert L.msfem_nn_conv3x3_first_bf16(B, H, 32, w0.ptr, b0.ptr, 1, x.ptr, wpk.ptr, 32, y2.ptr, db.ptr, 1, None,
+                                         st) == msfem.ERR_ARG
+    for a in (x, w0, b0, dw, db, wpk, h, y1, y2, p1, p2):
+        a.free()
+
+
 @pytest.mark.parametrize("Cin,B,H,W,up", [
     (32, 2, 64, 64, 0), (32, 1, 64, 64, 1), (64, 3, 32, 32, 0), (32, 2, 32, 16, 1), (32, 6, 8, 8, 1),
     (32, 8, 4, 4, 0), (64, 1, 64, 64, 1)])
