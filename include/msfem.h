@@ -433,18 +433,23 @@ int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stre
 int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
                           float *out, const float *bias, int relu, const float *add, void *stream);
 /* The same, also writing pool (or nothing for NULL) = the 2x2 average pooling
- * of out, [Cout][B][H/2][W/2], bitwise what msfem_nn_pool2 of out gives (an
- * encoder layer and the pooling after it in one pass; H even). */
-int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in,
-                               int upsample, float *out, const float *bias, int relu, const float *add, float *pool,
-                               void *stream);
+ * of out, [Cout][B][H/2][W/2] FP32, bitwise what msfem_nn_pool2 of out gives
+ * (an encoder layer and the pooling after it in one pass; H even), with the
+ * activations optionally stored as BF16 (io bit 0: in holds BF16 values, 4-
+ * or 8-byte aligned; bit 1: out receives each result rounded to BF16, to
+ * nearest even).  A consumer that rounds its input to BF16 anyway (the 3x3
+ * convolutions here, msfem_nn_head_bases) computes bitwise the same values
+ * from the BF16 store as from the FP32 one. */
+int msfem_nn_conv3x3_bf16_ex(int Cin, int B, int H, int W, const void *wp, int Cout, const void *in, int upsample,
+                             void *out, const float *bias, int relu, const float *add, float *pool, int io,
+                             void *stream);
 /* A network's first two layers in one pass: h = act0(conv3x3(in, w0) + b0)
  * (one input channel -> 32, padding 1; w0 BF16 [32][9] as msfem_nn_conv_bf16
  * takes it) is formed in the staging of out = act(conv3x3(h) + bias)
  * (32 -> Cout = 32, wp packed by msfem_nn_conv3x3_pack), with the optional
  * 2x2 average pooling of out as msfem_nn_conv3x3_bf16_pool.  in: [B][H][64]
  * FP32, 16-byte aligned, H even.  Bitwise equal to msfem_nn_conv_bf16 (no
- * split) followed by msfem_nn_conv3x3_bf16_pool, without h in memory. */
+ * split) followed by msfem_nn_conv3x3_bf16_ex, without h in memory. */
 int msfem_nn_conv3x3_first_bf16(int B, int H, int W, const void *w0, const float *b0, int relu0, const float *in,
                                 const void *wp, int Cout, float *out, const float *bias, int relu, float *pool,
                                 void *stream);
@@ -455,6 +460,14 @@ int msfem_nn_conv3x3_first_bf16(int B, int H, int W, const void *w0, const float
  * kh = kw in {1, 2, 3} and Cin kh kw C <= 16384 with C = 4 (Cout <= 4) or 8. */
 int msfem_nn_conv_small(int Cin, int B, int H, int W, int kh, int kw, int pad, const void *w, int Cout,
                         const float *in, float *out, const float *bias, int relu, void *stream);
+/* The basis surrogate's head written as bases: bases[b][c][Y (nf+1) + X] =
+ * N_c(X, Y) + (double)(scale * h_c) at the interior nodes of cell b, h = the
+ * 2x2 valid convolution (Cin -> 4, BF16 weights w [4][Cin 2 2]) + bias (+
+ * ReLU) of the activations in [Cin][B][nf][nf] at pixel (X-1, Y-1), and N_c
+ * on the boundary: bitwise msfem_nn_conv_small, msfem_nn_axpby(scale, h, 0)
+ * and msfem_nn_delta_to_bases in one pass.  in_bf16: in holds BF16 values. */
+int msfem_nn_head_bases(int Cin, int B, int nf, const void *w, const void *in, int in_bf16, const float *bias,
+                        int relu, float scale, double *bases, void *stream);
 /* adjoint of im2col (FP32, gather form) */
 int msfem_nn_col2im(int C, int B, int H, int W, int kh, int kw, int pad, const float *dcol, float *dx, void *stream);
 int msfem_nn_bias_act(int C, uint64_t N, const float *bias, float *y, int relu, void *stream);

@@ -307,9 +307,12 @@ __device__ __forceinline__ void pool_tile(const float *s, int m0, size_t plane, 
 // wp[co][blk][tap][c] = w[co][32 blk + c][tap].  The sum over k runs tap by
 // tap within a channel block, so results differ from the implicit GEMM's by
 // FP32 rounding only.
-template <int BM, int W, bool UP>
-__global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H, const float *in, const uint16_t *wp,
-                                                           float *out, const float *bias, int relu, const float *add,
+// IB: in holds BF16 values (they enter the staging as they are, i.e. as the
+// FP32 values they were rounded from would); OB: out receives the BF16
+// rounding of each FP32 result (pool and add stay FP32).
+template <int BM, int W, bool UP, bool IB, bool OB>
+__global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H, const void *in_, const uint16_t *wp,
+                                                           void *out_, const float *bias, int relu, const float *add,
                                                            float *pool)
 {
   constexpr int TI = W >= 16 ? 1 : kGBN / (W * W), TR = kGBN / W / TI; // images and rows per tile (W <= 8: H = W)
@@ -347,21 +350,39 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
   const int ioct = tid % 4, ig = tid / 4 % NG, irr = tid / 4 / NG; // irr = img IR + row
   const int iy = y0 - 1 + irr % IR;
   const bool iact = tid < IIT, iok = iact && iy >= 0 && iy < H;
+  constexpr bool ib = IB; // BF16 input: the raw pairs sit in the .x/.y registers
   const size_t iplane = UP ? plane / 4 : plane; // channel stride of in
-  const float *isrc =
-      in + (iok ? (UP ? (size_t)(b + irr / IR) * (HW / 4) + (size_t)(iy / 2) * (W / 2) + 2 * ig
-                      : (size_t)(b + irr / IR) * HW + (size_t)iy * W + 4 * ig)
-                : 0);
+  const size_t ioff = iok ? (UP ? (size_t)(b + irr / IR) * (HW / 4) + (size_t)(iy / 2) * (W / 2) + 2 * ig
+                                : (size_t)(b + irr / IR) * HW + (size_t)iy * W + 4 * ig)
+                          : 0;
   auto fetch = [&](int blk) {
     if (iok)
       {
-        const float *src = isrc + (size_t)(blk * 32 + ioct * 8) * iplane;
+        const size_t so = ioff + (size_t)(blk * 32 + ioct * 8) * iplane;
+        if constexpr (IB)
+          {
+            const uint16_t *src = (const uint16_t *)in_ + so;
 #pragma unroll
-        for (int e = 0; e < 8; ++e)
-          if constexpr (UP)
-            rh[e] = *(const float2 *)(src + e * iplane);
-          else
-            ri[e] = *(const float4 *)(src + e * iplane);
+            for (int e = 0; e < 8; ++e)
+              if constexpr (UP)
+                rh[e].x = __uint_as_float(*(const uint32_t *)(src + e * iplane));
+              else
+                {
+                  const uint2 q = *(const uint2 *)(src + e * iplane);
+                  ri[e].x = __uint_as_float(q.x);
+                  ri[e].y = __uint_as_float(q.y);
+                }
+          }
+        else
+          {
+            const float *src = (const float *)in_ + so;
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+              if constexpr (UP)
+                rh[e] = *(const float2 *)(src + e * iplane);
+              else
+                ri[e] = *(const float4 *)(src + e * iplane);
+          }
       }
 #pragma unroll
     for (int t = 0; t < WPT; ++t)
@@ -382,12 +403,27 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
 #pragma unroll
             for (int e = 0; e < 8; ++e)
               {
-                float f;
-                if constexpr (UP)
-                  f = px < 2 ? rh[e].x : rh[e].y;
+                uint16_t h;
+                if constexpr (IB && UP) // (no ternary over rh and ri: one of them has a single element)
+                  {
+                    const uint32_t q = __float_as_uint(rh[e].x);
+                    h = px < 2 ? (uint16_t)q : (uint16_t)(q >> 16);
+                  }
+                else if constexpr (IB)
+                  {
+                    const uint32_t q = __float_as_uint(px < 2 ? ri[e].x : ri[e].y);
+                    h = px % 2 == 0 ? (uint16_t)q : (uint16_t)(q >> 16);
+                  }
                 else
-                  f = px == 0 ? ri[e].x : px == 1 ? ri[e].y : px == 2 ? ri[e].z : ri[e].w;
-                v[e] = iok ? f32_to_bf16(f) : (uint16_t)0;
+                  {
+                    float f;
+                    if constexpr (UP)
+                      f = px < 2 ? rh[e].x : rh[e].y;
+                    else
+                      f = px == 0 ? ri[e].x : px == 1 ? ri[e].y : px == 2 ? ri[e].z : ri[e].w;
+                    h = f32_to_bf16(f);
+                  }
+                v[e] = iok ? h : (uint16_t)0;
               }
             *(bf16x8_t *)(dst + px * CP) = v;
           }
@@ -491,7 +527,10 @@ __global__ void __launch_bounds__(256) conv3x3_bf16_kernel(int Cin, int B, int H
         const size_t o = (size_t)row * plane + p0 + wn + j * 32 + lane % 32;
         if (add) // skip connection: act(acc + bias) + add, one FP32 addition
           v += ea[j][e];
-        out[o] = v;
+        if constexpr (OB)
+          ((uint16_t *)out_)[o] = f32_to_bf16(v);
+        else
+          ((float *)out_)[o] = v;
         if (pool) // the block loop ended with a barrier: sW is free
           ((float *)sW)[(row - m0) * kGBN + wn + j * 32 + lane % 32] = v;
       }
@@ -803,60 +842,72 @@ __global__ void __launch_bounds__(256) conv3x3_tiles_kernel(int B, int H, int nt
     }
 }
 
+// The sums of conv_small_kernel at output pixel (x, y) of image b: acc[c] =
+// sum_k bf16(in) w[k][c], k = (ci KS + ky) KS + kx in this order, KS x KS
+// windows of CU input channels loaded at once.  IB: in holds BF16 values
+// (the same sums as from the FP32 values they were rounded from).
+template <int CO, int KS, bool IB>
+__device__ __forceinline__ void small_conv_sums(int Cin, int B, int H, int W, int pad, int x, int y, int b,
+                                                const void *in, const float *swk, float (&acc)[CO])
+{
+  constexpr int CU = 8, T = KS * KS; // input channels loaded at once
+  const size_t plane = (size_t)B * H * W, base = (size_t)b * H * W;
+#pragma unroll
+  for (int c = 0; c < CO; ++c)
+    acc[c] = 0.0f;
+  for (int ci0 = 0; ci0 < Cin; ci0 += CU)
+    {
+      float v[CU][T];
+#pragma unroll
+      for (int u = 0; u < CU; ++u)
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+          {
+            const int yy = y + t / KS - pad, xx = x + t % KS - pad;
+            const bool ok = ci0 + u < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W;
+            const size_t o = ok ? base + (ci0 + u) * plane + yy * W + xx : 0;
+            if constexpr (IB)
+              v[u][t] = ok ? bf16_to_f32(((const uint16_t *)in)[o]) : 0.0f;
+            else
+              v[u][t] = ok ? ((const float *)in)[o] : 0.0f;
+          }
+#pragma unroll
+      for (int u = 0; u < CU; ++u)
+        if (ci0 + u < Cin)
+#pragma unroll
+          for (int t = 0; t < T; ++t)
+            {
+              const float f = IB ? v[u][t] : bf16_to_f32(f32_to_bf16(v[u][t]));
+              const float *wk = swk + ((ci0 + u) * T + t) * CO;
+#pragma unroll
+              for (int c = 0; c < CO; ++c)
+                acc[c] = fmaf(f, wk[c], acc[c]);
+            }
+    }
+}
+
 // Direct convolution with few output channels (a network's head) on the
 // VALU: one thread per output pixel computes CO channels (CO >= Cout; weights
 // of the channels beyond Cout are zero, their sums are not stored) from the
-// BF16-rounded input and weights, the KS x KS window of 8 input channels
-// loaded at once.  Weights
-// as FP32 in LDS, [k][CO].  Each BF16 x BF16 product is exact in FP32, as in
-// the MFMA path; the sums run in the order k = (ci KS + ky) KS + kx.
+// BF16-rounded input and weights (small_conv_sums).  Weights as FP32 in LDS,
+// [k][CO].  Each BF16 x BF16 product is exact in FP32, as in the MFMA path.
 template <int CO, int KS>
 __global__ void __launch_bounds__(256) conv_small_kernel(int Cin, int B, int H, int W, int pad, int Cout,
                                                          const float *in, const uint16_t *w, float *out,
                                                          const float *bias, int relu)
 {
-  constexpr int CU = 8, T = KS * KS; // input channels loaded at once
   extern __shared__ float swk[];
-  const int K = Cin * T;
+  const int K = Cin * KS * KS;
   for (int t = threadIdx.x; t < K * CO; t += blockDim.x)
     swk[t] = t % CO < Cout ? bf16_to_f32(w[(size_t)(t % CO) * K + t / CO]) : 0.0f;
   __syncthreads();
   const int Ho = H + 2 * pad - KS + 1, Wo = W + 2 * pad - KS + 1;
-  const size_t N = (size_t)B * Ho * Wo, plane = (size_t)B * H * W;
+  const size_t N = (size_t)B * Ho * Wo;
   for (size_t n = blockIdx.x * (size_t)blockDim.x + threadIdx.x; n < N; n += (size_t)gridDim.x * blockDim.x)
     {
       const int x = (int)(n % Wo), y = (int)(n / Wo % Ho), b = (int)(n / ((size_t)Wo * Ho));
       float acc[CO];
-#pragma unroll
-      for (int c = 0; c < CO; ++c)
-        acc[c] = 0.0f;
-      const float *src = in + (size_t)b * H * W;
-      for (int ci0 = 0; ci0 < Cin; ci0 += CU)
-        {
-          float v[CU][T];
-#pragma unroll
-          for (int u = 0; u < CU; ++u)
-#pragma unroll
-            for (int t = 0; t < T; ++t)
-              {
-                const int yy = y + t / KS - pad, xx = x + t % KS - pad;
-                v[u][t] = (ci0 + u < Cin && yy >= 0 && yy < H && xx >= 0 && xx < W)
-                              ? src[(ci0 + u) * plane + yy * W + xx]
-                              : 0.0f;
-              }
-#pragma unroll
-          for (int u = 0; u < CU; ++u)
-            if (ci0 + u < Cin)
-#pragma unroll
-              for (int t = 0; t < T; ++t)
-                {
-                  const float f = bf16_to_f32(f32_to_bf16(v[u][t]));
-                  const float *wk = swk + ((ci0 + u) * T + t) * CO;
-#pragma unroll
-                  for (int c = 0; c < CO; ++c)
-                    acc[c] = fmaf(f, wk[c], acc[c]);
-                }
-        }
+      small_conv_sums<CO, KS, false>(Cin, B, H, W, pad, x, y, b, in, swk, acc);
 #pragma unroll
       for (int c = 0; c < CO; ++c)
         if (c < Cout)
@@ -870,6 +921,52 @@ __global__ void __launch_bounds__(256) conv_small_kernel(int Cin, int B, int H, 
               }
             out[c * N + n] = r;
           }
+    }
+}
+
+// The serving head of the basis surrogate, written as bases: for node (X, Y)
+// of cell b's (nf + 1)^2 nodes and basis c < 4, bases[b][c][Y (nf + 1) + X] =
+// N_c(X, Y) + (double)(scale * h_c) at the interior nodes, with h_c the 2 x 2
+// valid convolution + bias (+ ReLU) at pixel (X - 1, Y - 1) of the nf x nf
+// activations as conv_small_kernel forms it, and N_c on the boundary: bitwise
+// conv_small_kernel, axpby_kernel (a = scale, b = 0) and
+// delta_to_bases_kernel in one pass.  One thread per node.
+template <bool IB>
+__global__ void __launch_bounds__(256) head_bases_kernel(int Cin, int B, int nf, const void *in, const uint16_t *w,
+                                                         const float *bias, int relu, float scale, double *bases)
+{
+  constexpr int CO = 4, KS = 2;
+  extern __shared__ float swk[];
+  const int K = Cin * KS * KS;
+  for (int t = threadIdx.x; t < K * CO; t += blockDim.x)
+    swk[t] = bf16_to_f32(w[(size_t)(t % CO) * K + t / CO]);
+  __syncthreads();
+  const int np = nf + 1;
+  const size_t per = (size_t)np * np, total = (size_t)B * per;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    {
+      const int b = (int)(t / per), k = (int)(t % per), X = k % np, Y = k / np;
+      const bool inner = X > 0 && X < nf && Y > 0 && Y < nf;
+      float acc[CO];
+      if (inner)
+        small_conv_sums<CO, KS, IB>(Cin, B, nf, nf, 0, X - 1, Y - 1, b, in, swk, acc);
+#pragma unroll
+      for (int c = 0; c < CO; ++c)
+        {
+          double v = nodal_q1(c, 2, X, Y, 0, nf);
+          if (inner)
+            {
+              float r = acc[c];
+              if (bias)
+                {
+                  r += bias[c];
+                  if (relu && r < 0.0f)
+                    r = 0.0f;
+                }
+              v += (double)(scale * r);
+            }
+          bases[((size_t)b * CO + c) * per + k] = v;
+        }
     }
 }
 
@@ -1354,16 +1451,17 @@ int msfem_nn_conv3x3_pack(int Cin, int Cout, const void *w, void *wp, void *stre
 int msfem_nn_conv3x3_bf16(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in, int upsample,
                           float *out, const float *bias, int relu, const float *add, void *stream)
 {
-  return msfem_nn_conv3x3_bf16_pool(Cin, B, H, W, wp, Cout, in, upsample, out, bias, relu, add, nullptr, stream);
+  return msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wp, Cout, in, upsample, out, bias, relu, add, nullptr, 0, stream);
 }
 
-int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int Cout, const float *in,
-                               int upsample, float *out, const float *bias, int relu, const float *add, float *pool,
-                               void *stream)
+int msfem_nn_conv3x3_bf16_ex(int Cin, int B, int H, int W, const void *wp, int Cout, const void *in, int upsample,
+                             void *out, const float *bias, int relu, const float *add, float *pool, int io,
+                             void *stream)
 {
+  const int ialign = (upsample ? 8 : 16) / ((io & 1) ? 2 : 1); // one staging load: 2 or 4 columns
   if (Cin <= 0 || Cin % 32 != 0 || B <= 0 || H <= 0 || Cout <= 0 || Cout % 32 != 0 || !wp || !in || !out ||
       (W != 4 && W != 8 && W != 16 && W != 32 && W != 64) || (W <= 8 ? H != W : H % (kGBN / W) != 0) ||
-      ((uintptr_t)wp % 16) != 0 || ((uintptr_t)in % (upsample ? 8 : 16)) != 0 || (upsample && H % 2 != 0))
+      ((uintptr_t)wp % 16) != 0 || ((uintptr_t)in % ialign) != 0 || (upsample && H % 2 != 0) || io < 0 || io > 3)
     return MSFEM_ERR_ARG;
   const long long plane = (long long)B * H * W;
   if (plane % kGBN != 0 || (long long)Cin * plane >= (1ll << 31) || (long long)Cout * plane >= (1ll << 31) ||
@@ -1374,7 +1472,7 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
   const int ntiles = (int)(plane / kGBN);
   if (pool && (H % 2 != 0 || (long long)Cout * (plane / 4) >= (1ll << 31)))
     return MSFEM_ERR_ARG;
-  if (Cout == 32 && Cin == 32 && (!pool || W == 64)) // persistent form: weights staged once per workgroup
+  if (Cout == 32 && Cin == 32 && (!pool || W == 64) && io == 0) // persistent form: weights staged once per workgroup
     {
       // (with two input blocks the resident weights cost a third of the
       // occupancy, and the tile form is faster)
@@ -1382,8 +1480,8 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
       hipError_t e = hipSuccess;
 #define MSFEM_C3P_CASE(Wv, UPv, NBv)                                                                                  \
   if (W == Wv && (upsample != 0) == UPv && nb == NBv)                                                                \
-    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv, false>, ntiles, st, B, H, ntiles, in, w, out, bias, relu,  \
-                     add, pool, (const uint16_t *)nullptr, (const float *)nullptr, 0);
+    e = launch_tiles(conv3x3_tiles_kernel<Wv, UPv, NBv, false>, ntiles, st, B, H, ntiles, (const float *)in, w,    \
+                     (float *)out, bias, relu, add, pool, (const uint16_t *)nullptr, (const float *)nullptr, 0);
 #define MSFEM_C3P_W(Wv)                                                                                               \
   MSFEM_C3P_CASE(Wv, false, 1)                                                                                       \
   MSFEM_C3P_CASE(Wv, true, 1)
@@ -1398,15 +1496,35 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
     }
   const int BM = Cout % 64 == 0 ? 64 : 32;
   const dim3 grid((unsigned)ntiles, (unsigned)(Cout / BM)), block(256);
+#define MSFEM_C3_LAUNCH(BMv, Wv, UPv, IBv, OBv)                                                                       \
+  hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, UPv, IBv, OBv>), grid, block, 0, st, Cin, B, H, in, w, out, bias,  \
+                     relu, add, pool)
+#define MSFEM_C3_UP(BMv, Wv, UPv)                                                                                     \
+  switch (io)                                                                                                        \
+    {                                                                                                                \
+    case 0:                                                                                                          \
+      MSFEM_C3_LAUNCH(BMv, Wv, UPv, false, false);                                                                   \
+      break;                                                                                                         \
+    case 1:                                                                                                          \
+      MSFEM_C3_LAUNCH(BMv, Wv, UPv, true, false);                                                                    \
+      break;                                                                                                         \
+    case 2:                                                                                                          \
+      MSFEM_C3_LAUNCH(BMv, Wv, UPv, false, true);                                                                    \
+      break;                                                                                                         \
+    default:                                                                                                         \
+      MSFEM_C3_LAUNCH(BMv, Wv, UPv, true, true);                                                                     \
+    }
 #define MSFEM_C3_CASE(BMv, Wv)                                                                                         \
   if (BM == BMv && W == Wv)                                                                                          \
     {                                                                                                                \
       if (upsample)                                                                                                  \
-        hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, true>), grid, block, 0, st, Cin, B, H, in, w, out, bias,    \
-                           relu, add, pool);                                                                         \
+        {                                                                                                            \
+          MSFEM_C3_UP(BMv, Wv, true)                                                                                 \
+        }                                                                                                            \
       else                                                                                                           \
-        hipLaunchKernelGGL((conv3x3_bf16_kernel<BMv, Wv, false>), grid, block, 0, st, Cin, B, H, in, w, out, bias,   \
-                           relu, add, pool);                                                                         \
+        {                                                                                                            \
+          MSFEM_C3_UP(BMv, Wv, false)                                                                                \
+        }                                                                                                            \
     }
   MSFEM_C3_CASE(32, 4)
   MSFEM_C3_CASE(32, 8)
@@ -1419,6 +1537,8 @@ int msfem_nn_conv3x3_bf16_pool(int Cin, int B, int H, int W, const void *wp, int
   MSFEM_C3_CASE(64, 32)
   MSFEM_C3_CASE(64, 64)
 #undef MSFEM_C3_CASE
+#undef MSFEM_C3_UP
+#undef MSFEM_C3_LAUNCH
   return check_launch();
 }
 
@@ -1465,6 +1585,23 @@ int msfem_nn_conv_small(int Cin, int B, int H, int W, int kh, int kw, int pad, c
   MSFEM_CS_CASE(8, 2)
   MSFEM_CS_CASE(8, 3)
 #undef MSFEM_CS_CASE
+  return check_launch();
+}
+
+int msfem_nn_head_bases(int Cin, int B, int nf, const void *w, const void *in, int in_bf16, const float *bias,
+                        int relu, float scale, double *bases, void *stream)
+{
+  if (Cin <= 0 || B <= 0 || nf < 2 || !w || !in || !bases || (long long)Cin * 4 * 4 * 4 > 65536 ||
+      (long long)Cin * B * nf * nf >= (1ll << 31) || (long long)B * 4 * (nf + 1) * (nf + 1) >= (1ll << 40))
+    return MSFEM_ERR_ARG;
+  const size_t total = (size_t)B * (nf + 1) * (nf + 1), lds = (size_t)Cin * 4 * 4 * sizeof(float);
+  hipStream_t st = (hipStream_t)stream;
+  if (in_bf16)
+    hipLaunchKernelGGL(head_bases_kernel<true>, dim3(grid_for(total)), dim3(256), lds, st, Cin, B, nf, in,
+                       (const uint16_t *)w, bias, relu, scale, bases);
+  else
+    hipLaunchKernelGGL(head_bases_kernel<false>, dim3(grid_for(total)), dim3(256), lds, st, Cin, B, nf, in,
+                       (const uint16_t *)w, bias, relu, scale, bases);
   return check_launch();
 }
 

@@ -35,7 +35,7 @@ EXPORTS = [
     "msfem_gen_coef_lognormal", "msfem_gen_resolution_classes", "msfem_gen_coef_tensor", "msfem_gen_grf_log",
     "msfem_gen_perturbed_quads", "msfem_gen_coef_reference_quad", "msfem_gen_perturbed_hexes",
     "msfem_gen_coef_reference_hex",
-    "msfem_fine_solve", "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_conv_bf16", "msfem_nn_conv3x3_pack", "msfem_nn_conv3x3_bf16", "msfem_nn_conv3x3_bf16_pool", "msfem_nn_conv3x3_first_bf16", "msfem_nn_conv_small", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
+    "msfem_fine_solve", "msfem_nn_gemm_bf16", "msfem_nn_gemm_bf16_splitk", "msfem_nn_im2col", "msfem_nn_conv_bf16", "msfem_nn_conv3x3_pack", "msfem_nn_conv3x3_bf16", "msfem_nn_conv3x3_bf16_ex", "msfem_nn_conv3x3_first_bf16", "msfem_nn_conv_small", "msfem_nn_head_bases", "msfem_nn_col2im", "msfem_nn_bias_act", "msfem_nn_act_bwd",
     "msfem_nn_bias_grad", "msfem_nn_pool2", "msfem_nn_up2", "msfem_nn_axpy", "msfem_nn_axpby", "msfem_nn_to_bf16",
     "msfem_nn_mse", "msfem_nn_adam", "msfem_nn_delta_to_bases", "msfem_nn_bases_to_delta",
 ]

@@ -36,7 +36,8 @@ def _nn():
         L.msfem_nn_conv3x3_pack.argtypes = [i32, i32, vp, vp, vp]
         L.msfem_nn_conv_small.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, i32, vp, vp, vp, i32, vp]
         L.msfem_nn_conv3x3_bf16.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp]
-        L.msfem_nn_conv3x3_bf16_pool.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, vp]
+        L.msfem_nn_conv3x3_bf16_ex.argtypes = [i32, i32, i32, i32, vp, i32, vp, i32, vp, vp, i32, vp, vp, i32, vp]
+        L.msfem_nn_head_bases.argtypes = [i32, i32, i32, vp, vp, i32, vp, i32, f32, vp, vp]
         L.msfem_nn_conv3x3_first_bf16.argtypes = [i32, i32, i32, vp, vp, i32, vp, vp, i32, vp, vp, i32, vp, vp]
         L.msfem_nn_col2im.argtypes = [i32, i32, i32, i32, i32, i32, i32, vp, vp, vp]
         L.msfem_nn_bias_act.argtypes = [i32, u64, vp, vp, i32, vp]
@@ -126,19 +127,20 @@ class Conv:
             self.wpk_ver = self.wver
         return self.wpk.ptr
 
-    def forward_up_skip(self, x, B, H, W, skip, out):
+    def forward_up_skip(self, x, B, H, W, skip, out, io=0):
         """Serving form of a decoder layer: out = act(conv(up2(x)) + b) + skip with x
-        at H/2 x W/2, upsampled inside the convolution's staging (no saved state)."""
-        _ok(_nn().msfem_nn_conv3x3_bf16(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 1, out.ptr, self.b.ptr,
-                                        1 if self.relu else 0, skip.ptr, self.st))
+        at H/2 x W/2, upsampled inside the convolution's staging (no saved state);
+        io as msfem_nn_conv3x3_bf16_ex (bit 0: x holds BF16, bit 1: out gets BF16)."""
+        _ok(_nn().msfem_nn_conv3x3_bf16_ex(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 1, out.ptr,
+                                           self.b.ptr, 1 if self.relu else 0, skip.ptr, None, io, self.st))
 
     def forward_pool(self, x, B, H, W, buf, pool):
         """Serving form of an encoder layer followed by 2x2 average pooling:
         y = act(conv(x) + b) (kept, the skip) and pool = pool2(y) in one pass,
         bitwise the values of forward + msfem_nn_pool2."""
         y = buf.get(self.name + ".y", self.cout * B * H * W, np.float32)
-        _ok(_nn().msfem_nn_conv3x3_bf16_pool(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 0, y.ptr, self.b.ptr,
-                                             1 if self.relu else 0, None, pool.ptr, self.st))
+        _ok(_nn().msfem_nn_conv3x3_bf16_ex(self.cin, B, H, W, self.packed(), self.cout, x.ptr, 0, y.ptr, self.b.ptr,
+                                           1 if self.relu else 0, None, pool.ptr, 0, self.st))
         return y
 
     def small_direct(self):
@@ -272,14 +274,20 @@ class Surrogate:
         return f
 
     # ---- forward / backward ---------------------------------------------------------
-    def forward(self, x, B, keep=True):
+    def forward(self, x, B, keep=True, bases=None):
         """x: DeviceArray [1][B][tile][tile] FP32 (normalized log-coefficient).
-        Returns the head output [4][B][tile-1][tile-1] (DeviceArray view).
+        Returns the head output [4][B][tile-1][tile-1] (DeviceArray view); with
+        `bases` (keep=False only: a device address of FP64 [B][4][(tile+1)^2])
+        the head writes N_i + dscale * prediction there instead
+        (msfem_nn_head_bases) and None is returned.
         keep=False (inference): the first two layers run as one pass, encoder
         layers write their pooled output beside their own, and decoder layers
         that the direct 3x3 kernel takes read the upsampled input and add the
-        skip inside the convolution; the values are bitwise those of the
-        training form, which keeps the intermediates the backward pass needs."""
+        skip inside the convolution, storing their output as BF16 where its only
+        consumer rounds it to BF16 anyway (the next such layer, the head written
+        as bases); the values are bitwise those of the training form, which
+        keeps the intermediates the backward pass needs."""
+        assert bases is None or not keep
         L, st, buf = _nn(), self.st, self.buf
         H = self.tile
         skips = []
@@ -315,13 +323,16 @@ class Surrogate:
                 h, _, _ = self.enc[l + 1].forward(p, B, H, H, buf)
             skips.append((h, H))
         self.skips = skips
+        hb = False  # h holds BF16 values
         for l in range(self.levels - 2, -1, -1):
             C = self.chans[l + 1]
             if not keep and self.dec[l].direct3x3(B, 2 * H, 2 * H):
                 H *= 2
-                out = buf.get(f"sum{l}", self.chans[l] * B * H * H, np.float32)
-                self.dec[l].forward_up_skip(h, B, H, H, skips[l][0], out)
-                h = out
+                ob = bases is not None if l == 0 else self.dec[l - 1].direct3x3(B, 2 * H, 2 * H)
+                out = buf.get(f"sum{l}" + ("h" if ob else ""), self.chans[l] * B * H * H,
+                              np.uint16 if ob else np.float32)
+                self.dec[l].forward_up_skip(h, B, H, H, skips[l][0], out, (1 if hb else 0) | (2 if ob else 0))
+                h, hb = out, ob
                 continue
             u = buf.get(f"up{l}", C * B * (2 * H) ** 2, np.float32)
             _ok(L.msfem_nn_up2(C * B, H, H, h.ptr, u.ptr, 0, st))
@@ -331,8 +342,14 @@ class Surrogate:
             # the sum goes to its own buffer: y (post-ReLU) is kept for the backward mask
             h = buf.get(f"sum{l}", self.chans[l] * B * H * H, np.float32)
             _ok(L.msfem_nn_axpby(self.chans[l] * B * H * H, 1.0, y.ptr, 1.0, sk.ptr, h.ptr, st))
-        out, _, _ = self.head.forward(h, B, H, H, buf)
         self.B = B
+        if bases is not None:
+            hd = self.head
+            assert hd.k == 2 and hd.pad == 0 and hd.cout == 4 and H == self.tile
+            _ok(L.msfem_nn_head_bases(hd.cin, B, H, hd.wbf.ptr, h.ptr, 1 if hb else 0, hd.b.ptr, 1 if hd.relu else 0,
+                                      self.dscale, bases, st))
+            return None
+        out, _, _ = self.head.forward(h, B, H, H, buf)
         return out
 
     def first_fused(self, H):
@@ -434,15 +451,11 @@ class Surrogate:
         prediction (the x0 of msfem_batch_build_device), `batch` tiles per forward
         pass.  Enqueued on the network's stream (the context stream unless
         set_stream chose another), not synchronized."""
-        L, st = _nn(), self.st
         t = self.tile
         nbase = 4 * (t + 1) ** 2
         for s in range(0, n, batch):
             b = min(batch, n - s)
-            out = self.forward(Ptr(x.ptr + s * t * t * 4), b, keep=False)
-            m = 4 * b * (t - 1) ** 2
-            _ok(L.msfem_nn_axpby(m, self.dscale, out.ptr, 0.0, None, out.ptr, st))
-            _ok(L.msfem_nn_delta_to_bases(b, t, out.ptr, bases_ptr + s * nbase * 8, st))
+            self.forward(Ptr(x.ptr + s * t * t * 4), b, keep=False, bases=bases_ptr + s * nbase * 8)
 
 
 class Ptr:

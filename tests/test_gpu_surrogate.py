@@ -276,7 +276,7 @@ def test_conv3x3_direct(ctx, Cin, Cout, B, H, W):
     (32, 32, 2, 64, 64), (32, 32, 64, 64, 64), (32, 64, 1, 32, 32), (64, 128, 2, 16, 16), (128, 256, 4, 8, 8),
     (256, 256, 16, 4, 4), (32, 32, 3, 32, 32), (64, 32, 2, 64, 64)])
 def test_conv3x3_fused_pool(ctx, Cin, Cout, B, H, W):
-    """msfem_nn_conv3x3_bf16_pool: the convolution's output is bitwise that of
+    """msfem_nn_conv3x3_bf16_ex with pooling: the convolution's output is bitwise that of
     msfem_nn_conv3x3_bf16 and the pooled output bitwise msfem_nn_pool2 of it --
     the persistent kernel (32 -> 32 at 64^2, one tile and many tiles per
     workgroup) and the tile kernel (BM 32 and 64, whole rows and whole images
@@ -294,8 +294,8 @@ def test_conv3x3_fused_pool(ctx, Cin, Cout, B, H, W):
     assert L.msfem_nn_conv3x3_pack(Cin, Cout, dw.ptr, wpk.ptr, st) == 0
     assert L.msfem_nn_conv3x3_bf16(Cin, B, H, W, wpk.ptr, Cout, x.ptr, 0, y1.ptr, db.ptr, 1, None, st) == 0
     assert L.msfem_nn_pool2(Cout * B, H, W, y1.ptr, p1.ptr, 0, st) == 0
-    assert L.msfem_nn_conv3x3_bf16_pool(Cin, B, H, W, wpk.ptr, Cout, x.ptr, 0, y2.ptr, db.ptr, 1, None, p2.ptr,
-                                        st) == 0
+    assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, x.ptr, 0, y2.ptr, db.ptr, 1, None, p2.ptr, 0,
+                                      st) == 0
     assert np.array_equal(y1.numpy(), y2.numpy())
     assert np.array_equal(p1.numpy(), p2.numpy()) and np.abs(p1.numpy()).max() > 0
     for a in (x, dw, db, wpk, y1, y2, p1, p2):
@@ -306,7 +306,7 @@ def test_conv3x3_fused_pool(ctx, Cin, Cout, B, H, W):
 def test_conv3x3_first_fused(ctx, B, H, pooled):
     """msfem_nn_conv3x3_first_bf16 (a 1 -> 32 3x3 layer formed in the staging of
     the 32 -> 32 one) is bitwise msfem_nn_conv_bf16 followed by
-    msfem_nn_conv3x3_bf16_pool, with one tile and many tiles per workgroup and
+    msfem_nn_conv3x3_bf16_ex, with one tile and many tiles per workgroup and
     short images (the first layer's zero rows at both image edges)."""
     say(f"fused first layer B={B} H={H} pool={pooled}")
     L, st = S._nn(), ctx.stream
@@ -323,8 +323,8 @@ def test_conv3x3_first_fused(ctx, B, H, pooled):
     p1, p2 = ctx.alloc(32 * N // 4, np.float32), ctx.alloc(32 * N // 4, np.float32)
     assert L.msfem_nn_conv3x3_pack(32, 32, dw.ptr, wpk.ptr, st) == 0
     assert L.msfem_nn_conv_bf16(1, B, H, W, 3, 3, 1, w0.ptr, 32, x.ptr, h.ptr, 1, None, b0.ptr, 1, st) == 0
-    assert L.msfem_nn_conv3x3_bf16_pool(32, B, H, W, wpk.ptr, 32, h.ptr, 0, y1.ptr, db.ptr, 1, None,
-                                        p1.ptr if pooled else None, st) == 0
+    assert L.msfem_nn_conv3x3_bf16_ex(32, B, H, W, wpk.ptr, 32, h.ptr, 0, y1.ptr, db.ptr, 1, None,
+                                      p1.ptr if pooled else None, 0, st) == 0
     assert L.msfem_nn_conv3x3_first_bf16(B, H, W, w0.ptr, b0.ptr, 1, x.ptr, wpk.ptr, 32, y2.ptr, db.ptr, 1,
                                          p2.ptr if pooled else None, st) == 0
     ref = y1.numpy()
@@ -334,6 +334,112 @@ def test_conv3x3_first_fused(ctx, B, H, pooled):
     assert L.msfem_nn_conv3x3_first_bf16(B, H, 32, w0.ptr, b0.ptr, 1, x.ptr, wpk.ptr, 32, y2.ptr, db.ptr, 1, None,
                                          st) == msfem.ERR_ARG
     for a in (x, w0, b0, dw, db, wpk, h, y1, y2, p1, p2):
+        a.free()
+
+
+@pytest.mark.parametrize("Cin,Cout,B,H,W,up", [
+    (64, 32, 2, 64, 64, 1), (32, 32, 3, 64, 64, 0), (128, 64, 2, 32, 32, 1), (64, 64, 2, 32, 32, 0),
+    (256, 128, 4, 16, 16, 1), (256, 256, 8, 8, 8, 1), (256, 256, 16, 4, 4, 1), (32, 64, 16, 4, 4, 0)])
+def test_conv3x3_bf16_storage(ctx, Cin, Cout, B, H, W, up):
+    """msfem_nn_conv3x3_bf16_ex's BF16 activation storage: with io bit 1 the
+    output is bitwise the BF16 rounding of the FP32 output, and with io bit 0
+    the BF16 bits of the input give bitwise the output of the FP32 input, with
+    and without upsampling, skip and pooling, BM 32 and 64 and every tile shape."""
+    say(f"bf16 storage conv3x3 Cin={Cin} Cout={Cout} B={B} H={H} W={W} up={up}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(3 * Cin + Cout + B + H + up)
+    K, N = Cin * 9, B * H * W
+    Ni = N // 4 if up else N
+    xf = rng.standard_normal(Cin * Ni).astype(np.float32)
+    x, xb = ctx.to_device(xf), ctx.to_device(bf16_bits(R.bf16(xf)))
+    dw = ctx.to_device(bf16_bits(R.bf16((rng.standard_normal((Cout, K)) / np.sqrt(K)).astype(np.float32))))
+    db = ctx.to_device(rng.standard_normal(Cout).astype(np.float32))
+    sk = ctx.to_device(rng.standard_normal(Cout * N).astype(np.float32))
+    wpk = ctx.alloc(Cout * K, np.uint16)
+    assert L.msfem_nn_conv3x3_pack(Cin, Cout, dw.ptr, wpk.ptr, st) == 0
+    y0, y1 = ctx.alloc(Cout * N, np.float32), ctx.alloc(Cout * N, np.float32)
+    h2, h3 = ctx.alloc(Cout * N, np.uint16), ctx.alloc(Cout * N, np.uint16)
+    p0, p1 = ctx.alloc(Cout * N // 4, np.float32), ctx.alloc(Cout * N // 4, np.float32)
+    for add in (None, sk.ptr):
+        pool = (p0.ptr, p1.ptr) if add is None and not up else (None, None)
+        assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, x.ptr, up, y0.ptr, db.ptr, 1, add, pool[0], 0,
+                                          st) == 0
+        assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, xb.ptr, up, y1.ptr, db.ptr, 1, add, pool[1], 1,
+                                          st) == 0
+        assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, x.ptr, up, h2.ptr, db.ptr, 1, add, None, 2,
+                                          st) == 0
+        assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, xb.ptr, up, h3.ptr, db.ptr, 1, add, None, 3,
+                                          st) == 0
+        ref = y0.numpy()
+        assert np.abs(ref).max() > 0
+        assert np.array_equal(ref, y1.numpy())
+        assert np.array_equal(bf16_bits(R.bf16(ref)), h2.numpy()) and np.array_equal(h2.numpy(), h3.numpy())
+        if pool[0] is not None:
+            assert np.array_equal(p0.numpy(), p1.numpy())
+    assert L.msfem_nn_conv3x3_bf16_ex(Cin, B, H, W, wpk.ptr, Cout, xb.ptr, up, y1.ptr, db.ptr, 1, None, None, 4,
+                                      st) == msfem.ERR_ARG
+    for a in (x, xb, dw, db, sk, wpk, y0, y1, h2, h3, p0, p1):
+        a.free()
+
+
+@pytest.mark.parametrize("Cin,B,nf", [(32, 3, 64), (8, 2, 16), (13, 1, 5)])
+def test_head_bases(ctx, Cin, B, nf):
+    """msfem_nn_head_bases is bitwise msfem_nn_conv_small (2x2 valid, 4
+    channels) + msfem_nn_axpby(scale, h, 0) + msfem_nn_delta_to_bases, from FP32
+    and from BF16 activations, with and without ReLU."""
+    say(f"head as bases Cin={Cin} B={B} nf={nf}")
+    L, st = S._nn(), ctx.stream
+    rng = np.random.default_rng(Cin + B + nf)
+    K, ni, np1 = Cin * 4, nf - 1, nf + 1
+    xf = rng.standard_normal(Cin * B * nf * nf).astype(np.float32)
+    x, xb = ctx.to_device(xf), ctx.to_device(bf16_bits(R.bf16(xf)))
+    dw = ctx.to_device(bf16_bits(R.bf16((rng.standard_normal((4, K)) / np.sqrt(K)).astype(np.float32))))
+    db = ctx.to_device(rng.standard_normal(4).astype(np.float32))
+    h = ctx.alloc(4 * B * ni * ni, np.float32)
+    ref, got = ctx.alloc(B * 4 * np1 * np1, np.float64), ctx.alloc(B * 4 * np1 * np1, np.float64)
+    scale = np.float32(0.731)
+    for relu in (0, 1):
+        assert L.msfem_nn_conv_small(Cin, B, nf, nf, 2, 2, 0, dw.ptr, 4, x.ptr, h.ptr, db.ptr, relu, st) == 0
+        assert L.msfem_nn_axpby(4 * B * ni * ni, scale, h.ptr, 0.0, None, h.ptr, st) == 0
+        assert L.msfem_nn_delta_to_bases(B, nf, h.ptr, ref.ptr, st) == 0
+        want = ref.numpy()
+        for src, ib in ((x, 0), (xb, 1)):
+            assert L.msfem_nn_head_bases(Cin, B, nf, dw.ptr, src.ptr, ib, db.ptr, relu, scale, got.ptr, st) == 0
+            assert np.array_equal(got.numpy(), want)
+    assert L.msfem_nn_head_bases(Cin, B, 1, dw.ptr, x.ptr, 0, db.ptr, 0, scale, got.ptr, st) == msfem.ERR_ARG
+    for a in (x, xb, dw, db, h, ref, got):
+        a.free()
+
+
+def test_serving_forward_matches_training_form(ctx):
+    """The serving forward of the full-size network (first two layers fused,
+    pooled encoder outputs, decoder layers with upsampling and skip inside the
+    convolution and BF16 activations, head written as bases) gives bitwise the
+    bases of the training form's forward + msfem_nn_axpby +
+    msfem_nn_delta_to_bases, and without bases bitwise the training form's
+    head output.  Random weights and biases; 8 tiles (every layer takes its
+    serving kernel)."""
+    say("serving forward vs training form, 8 tiles of 64^2")
+    net = S.Surrogate(ctx, seed=5)
+    rng = np.random.default_rng(5)
+    for layer in net.layers:
+        w, b = layer.get()
+        layer.set(w, 0.1 * rng.standard_normal(b.shape))
+    net.dscale = 0.37
+    L, st = S._nn(), ctx.stream
+    B, t = 8, net.tile
+    m, nb = 4 * B * (t - 1) ** 2, B * 4 * (t + 1) ** 2
+    x = ctx.to_device((0.5 * rng.standard_normal(B * t * t)).astype(np.float32))
+    out = net.forward(x, B, keep=True)
+    d, ref, got = ctx.alloc(m, np.float32), ctx.alloc(nb, np.float64), ctx.alloc(nb, np.float64)
+    assert L.msfem_nn_axpby(m, net.dscale, out.ptr, 0.0, None, d.ptr, st) == 0
+    assert L.msfem_nn_delta_to_bases(B, t, d.ptr, ref.ptr, st) == 0
+    raw = out.read(0, m)
+    assert np.abs(raw).max() > 0
+    assert net.forward(x, B, keep=False, bases=got.ptr) is None
+    assert np.array_equal(got.numpy(), ref.numpy())
+    assert np.array_equal(net.forward(x, B, keep=False).read(0, m), raw)
+    for a in (x, d, ref, got):
         a.free()
 
 
