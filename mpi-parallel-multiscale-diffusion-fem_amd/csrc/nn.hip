@@ -928,28 +928,75 @@ __global__ void __launch_bounds__(256) conv_small_kernel(int Cin, int B, int H, 
 // of cell b's (nf + 1)^2 nodes and basis c < 4, bases[b][c][Y (nf + 1) + X] =
 // N_c(X, Y) + (double)(scale * h_c) at the interior nodes, with h_c the 2 x 2
 // valid convolution + bias (+ ReLU) at pixel (X - 1, Y - 1) of the nf x nf
-// activations as conv_small_kernel forms it, and N_c on the boundary: bitwise
-// conv_small_kernel, axpby_kernel (a = scale, b = 0) and
-// delta_to_bases_kernel in one pass.  One thread per node.
+// activations, summed as conv_small_kernel sums (k = 4 ci + 2 ky + kx
+// ascending, fmaf), and N_c on the boundary: bitwise conv_small_kernel,
+// axpby_kernel (a = scale, b = 0) and delta_to_bases_kernel in one pass.  One
+// thread per node; an interior node's window lies inside the image, so the
+// loads of 8 channels go out unconditionally, back to back.
 template <bool IB>
 __global__ void __launch_bounds__(256) head_bases_kernel(int Cin, int B, int nf, const void *in, const uint16_t *w,
                                                          const float *bias, int relu, float scale, double *bases)
 {
-  constexpr int CO = 4, KS = 2;
-  extern __shared__ float swk[];
-  const int K = Cin * KS * KS;
+  constexpr int CO = 4, CU = 8;
+  extern __shared__ float swk[]; // [k][CO]
+  const int K = Cin * 4;
   for (int t = threadIdx.x; t < K * CO; t += blockDim.x)
     swk[t] = bf16_to_f32(w[(size_t)(t % CO) * K + t / CO]);
   __syncthreads();
   const int np = nf + 1;
-  const size_t per = (size_t)np * np, total = (size_t)B * per;
+  const size_t per = (size_t)np * np, total = (size_t)B * per, plane = (size_t)B * nf * nf;
+  auto value = [&](size_t o) -> float { // the BF16-rounded activation, as a float
+    if constexpr (IB)
+      return bf16_to_f32(((const uint16_t *)in)[o]);
+    else
+      return bf16_to_f32(f32_to_bf16(((const float *)in)[o]));
+  };
   for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
     {
       const int b = (int)(t / per), k = (int)(t % per), X = k % np, Y = k / np;
       const bool inner = X > 0 && X < nf && Y > 0 && Y < nf;
-      float acc[CO];
+      float acc[CO] = {0.0f, 0.0f, 0.0f, 0.0f};
       if (inner)
-        small_conv_sums<CO, KS, IB>(Cin, B, nf, nf, 0, X - 1, Y - 1, b, in, swk, acc);
+        {
+          const size_t o0 = (size_t)b * nf * nf + (size_t)(Y - 1) * nf + (X - 1);
+          int ci0 = 0;
+          for (; ci0 + CU <= Cin; ci0 += CU)
+            {
+              float v[CU][4];
+#pragma unroll
+              for (int u = 0; u < CU; ++u)
+                {
+                  const size_t o = o0 + (ci0 + u) * plane;
+                  v[u][0] = value(o);
+                  v[u][1] = value(o + 1);
+                  v[u][2] = value(o + nf);
+                  v[u][3] = value(o + nf + 1);
+                }
+#pragma unroll
+              for (int u = 0; u < CU; ++u)
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                  {
+                    const float *wk = swk + ((ci0 + u) * 4 + q) * CO;
+#pragma unroll
+                    for (int c = 0; c < CO; ++c)
+                      acc[c] = fmaf(v[u][q], wk[c], acc[c]);
+                  }
+            }
+          for (; ci0 < Cin; ++ci0) // (Cin not a multiple of CU)
+            {
+              const size_t o = o0 + ci0 * plane;
+              const float v[4] = {value(o), value(o + 1), value(o + nf), value(o + nf + 1)};
+#pragma unroll
+              for (int q = 0; q < 4; ++q)
+                {
+                  const float *wk = swk + (ci0 * 4 + q) * CO;
+#pragma unroll
+                  for (int c = 0; c < CO; ++c)
+                    acc[c] = fmaf(v[q], wk[c], acc[c]);
+                }
+            }
+        }
 #pragma unroll
       for (int c = 0; c < CO; ++c)
         {
