@@ -602,8 +602,9 @@ def main(argv=None):
     ap.add_argument("--time-forward", type=int, default=0,
                     help="time N forward passes over random tiles (after one untimed pass)")
     ap.add_argument("--forward-batch", type=int, default=128, help="--time-forward: tiles per pass")
-    ap.add_argument("--forward-form", choices=("serving", "training", "both"), default="both",
-                    help="--time-forward: the serving form (fused decoder) and/or the training form")
+    ap.add_argument("--forward-form", choices=("serving", "training", "both", "predict"), default="both",
+                    help="--time-forward: the serving form (fused layers) and/or the training form, or "
+                         "predict: the serving form with the head written as bases (predict_bases' pass)")
     a = ap.parse_args(argv)
     if not a.load and not a.out:
         ap.error("--out (train) or --load (serve) is required")
@@ -620,18 +621,24 @@ def main(argv=None):
     if a.time_forward:
         fb = a.forward_batch
         x = ctx.to_device(np.random.default_rng(0).standard_normal(fb * net.tile ** 2).astype(np.float32))
-        forms = {"serving": (False,), "training": (True,), "both": (False, True)}[a.forward_form]
-        for keep in forms:
-            net.forward(x, fb, keep=keep)
+        forms = {"serving": ("serving",), "training": ("training",), "both": ("serving", "training"),
+                 "predict": ("predict",)}[a.forward_form]
+        bases = ctx.alloc(fb * 4 * (net.tile + 1) ** 2, np.float64) if "predict" in forms else None
+        for form in forms:
+            run = lambda: net.forward(x, fb, keep=form == "training",
+                                      bases=bases.ptr if form == "predict" else None)
+            run()
             ctx.synchronize()
             t = time.perf_counter()
             for _ in range(a.time_forward):
-                net.forward(x, fb, keep=keep)
+                run()
             ctx.synchronize()
             dt = (time.perf_counter() - t) / a.time_forward
-            log(f"{'training' if keep else 'serving'} forward, {fb} tiles: {dt * 1e3:.3f} ms "
+            log(f"{form} forward, {fb} tiles: {dt * 1e3:.3f} ms "
                 f"({net.flops_per_sample() * fb / dt / 1e12:.1f} TFLOP/s)")
         x.free()
+        if bases is not None:
+            bases.free()
     if a.serve:
         rep = serve_report(ctx, net, n_side=a.serve, seed=a.serve_seed, log=log)
         log(json.dumps(rep))
