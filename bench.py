@@ -30,11 +30,13 @@ reference's default MatrixCoeff problem), quad64 (64^2 general quadrilaterals).
 
 Algorithmic work per CG iteration of a problem with N_I interior nodes:
 F_iter = (2*3^d - 1 + 10) N_I flops (stencil + CG vector updates), plus
-8 N_I + 2*49^2 for the two-level correction where it runs.  The CPU baseline
-is the oracle's restatement of the reference's own recipe (CG + SSOR(1.6),
-absolute tolerance 1e-12, 1000 iterations;
+8 N_I + 2*49^2 for the two-level correction where it runs, plus f_ml(nf)
+(75 496 at 64^2) for the intermediate hat levels where they run.  The CPU
+baseline is the oracle's restatement of the reference's own recipe (CG +
+SSOR(1.6), absolute tolerance 1e-12, 1000 iterations;
 include/base/diffusion_problem_basis.tpp:295-317) on a sample of the same
-cells, one OpenMP thread per available core.
+cells, one MPI rank per host core (mpiexec; one OpenMP thread per core
+without MPICH).
 """
 import argparse
 import json
@@ -62,10 +64,28 @@ TAU = 5e-7  # energy stopping rule of the FP32 legs: a 2x margin on the 1e-6 rel
 WEIGHTS = os.path.join(PKG, "surrogate_weights.npz")
 
 
-def f_iter_2d(nf, two_level):
-    """Algorithmic flops of one CG iteration of one 2D local problem."""
+def f_ml(nf):
+    """Algorithmic flops per iteration of the intermediate hat levels (spacing 2
+    and 4) of a 2D problem of nf^2 fine cells: separable restriction (row
+    weights over the fine columns, then column weights), diagonal scaling,
+    separable bilinear prolongation and the addition to z."""
+    n = nf - 1
+    f = 2 * n * n  # z += the two levels' values
+    for s in (2, 4):
+        m = nf // s - 1  # interior coarse nodes per side
+        f += m * n * 2 * (2 * s - 1) + m * m * 2 * (2 * s - 1)  # restriction
+        f += m * m                                               # D_s^-1
+        f += m * n * 3 + n * n * 3                               # prolongation
+    return f
+
+
+def f_iter_2d(nf, coarse):
+    """Algorithmic flops of one CG iteration of one 2D local problem; coarse as
+    msfem_uses_coarse_space (0 Jacobi, 1 two-level, 2 with the intermediate levels)."""
     f = (2 * 9 - 1 + 10) * (nf - 1) ** 2
-    return f + (8 * (nf - 1) ** 2 + 2 * 49 * 49 if two_level else 0)
+    if coarse:
+        f += 8 * (nf - 1) ** 2 + 2 * 49 * 49
+    return f + (f_ml(nf) if coarse == 2 else 0)
 
 
 def workload(name):
@@ -221,8 +241,8 @@ def bench_headline(args):
     ctx.synchronize()
     coarse_ms = ctx.elapsed(4, 5) / 3  # device time (HIP events on the context stream)
 
-    two_level = bool(ctx.uses_coarse_space(2, nf, o32s, msfem.COEF_SCALAR_CELL))
-    flops_solve = float(np.mean(solve_its)) * f_iter_2d(nf, two_level)
+    coarse = ctx.uses_coarse_space(2, nf, o32s, msfem.COEF_SCALAR_CELL)
+    flops_solve = float(np.mean(solve_its)) * f_iter_2d(nf, coarse)
     solve_tf = flops_solve / (solve_ms * 1e-3) / 1e12
     flops_inf = float(net.flops_per_sample()) * n
     inf_tf = flops_inf / (infer_ms * 1e-3) / 1e12
@@ -389,7 +409,8 @@ def bench_workload(args):
     f_iter = (2 * s - 1 + 10) * (nf_arr - 1) ** dim                      # per cell
     cs = np.array([ctx.uses_coarse_space(dim, int(k), opts[main_prec], case["coef_kind"], quads is not None)
                    for k in nf_arr])
-    f_iter = f_iter + cs * (8 * (nf_arr - 1) ** dim + 2 * 49 * 49)
+    f_iter = f_iter + (cs > 0) * (8 * (nf_arr - 1) ** dim + 2 * 49 * 49)
+    f_iter = f_iter + (cs == 2) * np.array([f_ml(int(k)) for k in nf_arr])
     w = 8 if main_prec == msfem.PREC_FP64 else 4
     stride = msfem.coef_stride(dim, case["coef_kind"])
     b_iter = 11 * (nf_arr + 1) ** dim * w + nf_arr ** dim * stride * w   # per cell
@@ -417,7 +438,7 @@ def bench_workload(args):
         "config": {"workload": f"{args.workload}: {desc}", "coarse_cells": n, "dim": dim,
                    "fine_per_cell": classes, "local_solves": n * nb, "path": pname[main_prec],
                    "autotuned_classes": tuned, "launch": "HIP graph, one concurrent branch per size class",
-                   "two_level_cells": int(cs.sum())},
+                   "two_level_cells": int((cs > 0).sum()), "intermediate_level_cells": int((cs == 2).sum())},
         "roofline": {"bound": "mfma", "engine": "valu", "kernel": f"local_solve_{dim}d ({pname[main_prec]} CG)",
                      "achieved": round(achieved, 3), "peak": peak, "unit": "TFLOP/s",
                      "frac": round(achieved / peak, 4), "traffic": TRAFFIC_BYTES.get((args.workload, main_prec)),

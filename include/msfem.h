@@ -113,10 +113,13 @@ typedef struct msfem_solver_opts {
                                  msfem_ctx_mark) */
   int32_t coarse_space;       /* 1 (default): two-level preconditioner (Jacobi plus the
                                  bilinear coarse space of an 8 x 8 coarse grid per local
-                                 problem) for 64^2 classes of box cells with per-fine-cell
-                                 scalar or tensor coefficients at the default launch
-                                 shapes; 0: Jacobi only.  Same converged values, fewer
-                                 iterations. */
+                                 problem, solved exactly) for the 64^2 and 128^2 classes at
+                                 the default launch shapes; on 64^2 box cells with per-fine-
+                                 cell scalar or tensor coefficients also the intermediate
+                                 hat levels of spacing 2 and 4, scaled by their diagonals
+                                 (additive); 2: the two-level form without the
+                                 intermediate levels; 0: Jacobi only.  Same converged
+                                 values, fewer iterations. */
   int32_t q1_start;           /* without x0: 1 (default) = the interior starts at the Q1
                                  interpolant of the Dirichlet data (box cells: the nodal
                                  Q1 function itself; general cells: the transfinite
@@ -187,7 +190,8 @@ const char *msfem_last_error(const msfem_ctx *ctx); /* ctx may be NULL (creation
 void msfem_default_opts(msfem_solver_opts *opts);
 /* 1 if a class of nf fine cells per side with these options, coefficient kind
  * and geometry (general: vertex cells) runs the two-level local solve on this
- * context (its launch-shape table applied), else 0.  No counterpart in the
+ * context (its launch-shape table applied), 2 if its preconditioner also takes
+ * the intermediate hat levels (spacing 2 and 4), else 0.  No counterpart in the
  * reference; for callers that account for the solver's work. */
 int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_opts *opts, int coef_kind,
                             int general);

@@ -449,6 +449,26 @@ bool class_coarse(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
          local2d_coarse_space(nf, build_mode(in, nf), class_variant(ctx, in, o, nf));
 }
 
+// Whether size class nf's two-level solve also takes the intermediate hat
+// levels (spacing 2 and 4): coarse_space 1 on the 64^2 box classes.
+bool class_ml(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, int nf)
+{
+  const int mode = build_mode(in, nf);
+  return o->coarse_space == 1 && nf == 64 && (mode == kModeScalar || mode == kModeTensor) &&
+         class_coarse(ctx, in, o, nf);
+}
+
+// Floats of the intermediate levels' inverse diagonals before size class k_end.
+size_t ml_floats(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k_end)
+{
+  const Plan &p = ctx->plan;
+  size_t n = 0;
+  for (size_t k = 0; k < k_end && k < p.class_nf.size(); ++k)
+    if (class_ml(ctx, in, o, p.class_nf[k]))
+      n += p.class_cells[k].size() * (size_t)kMlFloats;
+  return n;
+}
+
 // Doubles of the two-level classes' inverse coarse matrices before size class
 // k_end (each such class owns a slice: concurrent graph branches share none).
 size_t coarse_inv_doubles(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, size_t k_end)
@@ -521,6 +541,14 @@ int launch_class_prepare(msfem_ctx *ctx, const msfem_batch *in, const msfem_solv
       HIP_TRY(ctx, launch_coarse_setup_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef,
                                           d_off(ctx, 0), base + coarse_inv_doubles(ctx, in, o, k), st));
     }
+  if (class_ml(ctx, in, o, nf))
+    {
+      float *mb = (float *)ctx->bufs["ml_inv"].p;
+      if (!mb)
+        return fail(ctx, MSFEM_ERR_HIP, "intermediate-level scratch missing");
+      HIP_TRY(ctx, launch_ml_diag_2d(ncls, cells, nf, in->coef_kind, in->box, in->vertices, in->coef, d_off(ctx, 0),
+                                     mb + ml_floats(ctx, in, o, k), st));
+    }
   if (ctx->energy_terms && !in->vertices)
     HIP_TRY(ctx, launch_energy_terms(in->dim, ncls, cells, nf, in->coef_kind, in->box, in->coef, d_off(ctx, 0),
                                      o->energy_tol, (double *)ctx->bufs["lam_lb"].p,
@@ -571,6 +599,9 @@ int launch_class_solve(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver
   a.cinv = nullptr;
   if (class_coarse(ctx, in, o, nf))
     a.cinv = (const double *)ctx->bufs["coarse_inv"].p + coarse_inv_doubles(ctx, in, o, k);
+  a.mlinv = nullptr;
+  if (class_ml(ctx, in, o, nf))
+    a.mlinv = (const float *)ctx->bufs["ml_inv"].p + ml_floats(ctx, in, o, k);
   if (mode == kModeGeneric)
     {
       double *gw = (double *)ctx->bufs["gen_work"].p;
@@ -780,6 +811,15 @@ int enqueue_build(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
       if (rc)
         return rc;
     }
+  // inverse diagonals of the intermediate levels (allocated before any capture)
+  const size_t mlf = ml_floats(ctx, in, o, p.class_nf.size());
+  if (mlf > 0)
+    {
+      void *d;
+      rc = scratch(ctx, "ml_inv", mlf * sizeof(float), &d);
+      if (rc)
+        return rc;
+    }
   // per-problem vectors of the generic classes (allocated before any capture)
   const size_t gen_doubles = generic_work_total(ctx, in, o, p.class_nf.size());
   if (gen_doubles > 0)
@@ -918,7 +958,7 @@ int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_
   b.coef_kind = coef_kind;
   static const double one_vertex_set[8] = {0, 0, 1, 0, 0, 1, 1, 1};
   b.vertices = general ? one_vertex_set : nullptr;
-  return class_coarse(ctx, &b, opts, nf) ? 1 : 0;
+  return class_ml(ctx, &b, opts, nf) ? 2 : class_coarse(ctx, &b, opts, nf) ? 1 : 0;
 }
 
 void msfem_default_opts(msfem_solver_opts *o)

@@ -160,11 +160,22 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // with four waves per SIMD the recurrence s = K p goes there too
   constexpr bool SLDS = XLDS && WPE >= 4;
   __shared__ float4 ss[SLDS ? R * CPL / 4 * kWave * SW : 1];
+  // Intermediate hat levels of the 64^2 two-level tiles (a.mlinv set): M^-1
+  // also gains P_2 D_2^-1 P_2^T + P_4 D_4^-1 P_4^T, P_s the bilinear hats of
+  // spacing s, D_s = diag(P_s^T K P_s).  Each wave restricts its 8 rows with DPP
+  // sums along the wave (x) and fixed row weights (y); the coarse rows it shares
+  // with the waves below and above are completed from their published partial
+  // rows after the restriction barrier; prolongation by DPP from the coarse
+  // values masked to their lanes.
+  constexpr bool ML = CS && NF == 64 && CPL == 1 && SW == 8 && R == 8 && MODE != kModeStream;
+  __shared__ T smlp[ML ? 4 * SW * kWave : 1]; // partial rows [B2, T2, B4, T4][wave][lane]
+  __shared__ float smld[ML ? kMlFloats : 1];  // D_2^-1 [31][31], then D_4^-1 [15][15]
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
   const int prob = xcd_block(blockIdx.x, gridDim.x) * PPW + slot;
   const bool pvalid = prob < a.n_problems;
+  const bool mlon = ML && a.mlinv != nullptr; // (uniform)
   int cell = 0, basis = 0;
   if (pvalid)
     {
@@ -328,6 +339,12 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         const double *ci = a.cinv + (size_t)(prob / a.nb_solve) * (kCoarseDofs * kCoarseDofs);
         for (int k = tid; k < kCoarseDofs * kCoarseDofs; k += blockDim.x)
           sci[(k / kCoarseDofs) * kSciStride + k % kCoarseDofs] = (T)ci[k];
+        if (mlon)
+          {
+            const float *mi = a.mlinv + (size_t)(prob / a.nb_solve) * kMlFloats;
+            for (int k = tid; k < kMlFloats; k += blockDim.x)
+              smld[k] = mi[k];
+          }
       }
   if (pvalid)
     for (int k = pl + LW * wave; k < NP * NP; k += LW * SW)
@@ -603,6 +620,89 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
         ci0[c] = u00 * wxa(c) + u10 * wxb(c);
         ci1[c] = u01 * wxa(c) + u11 * wxb(c);
       }
+  };
+
+  // ---- intermediate levels (ML) ------------------------------------------------------
+  auto ror1 = [](T v) -> T { return dpp_any<T, 0x13C>(v); }; // wave_ror:1: lane l reads lane l - 1 (lane 0: 63)
+  auto rol1 = [](T v) -> T { return dpp_any<T, 0x134>(v); }; // wave_rol:1: lane l reads lane l + 1 (lane 63: 0)
+  auto xw2 = [&](T v) -> T { return v + (T)0.5 * (ror1(v) + rol1(v)); }; // hat-2 weights along the wave
+  auto xw4 = [&](T v) -> T {                                              // hat-4 weights along the wave
+    const T u = xw2(v);
+    return u + (T)0.5 * (ror1(ror1(u)) + rol1(rol1(u)));
+  };
+  // the partial rows this wave shares with its neighbours: B2 = row 4w (from
+  // row y = 8w + 1), T2 = row 4w + 4 (rows 8w + 7, 8w + 8), B4 = row 2w (rows
+  // 8w + 1 .. 8w + 3), T4 = row 2w + 2 (rows 8w + 5 .. 8w + 8); lane l = column l + 1
+  auto restrict_ml = [&](T (&rv)[R][CPL]) {
+    if constexpr (ML)
+      if (mlon)
+        {
+          T *pp = smlp + wave * kWave + lane;
+          pp[0] = xw2((T)0.5 * rv[0][0]);
+          pp[SW * kWave] = xw2(rv[7][0] + (T)0.5 * rv[6][0]);
+          pp[2 * SW * kWave] = xw4(((T)0.75 * rv[0][0] + (T)0.5 * rv[1][0]) + (T)0.25 * rv[2][0]);
+          pp[3 * SW * kWave] = xw4((((T)0.25 * rv[4][0] + (T)0.5 * rv[5][0]) + (T)0.75 * rv[6][0]) + rv[7][0]);
+        }
+  };
+  // the intermediate levels' share of z on the lane's rows (after the
+  // restriction barrier): coarse values y_s = D_s^-1 (P_s^T r) at lanes
+  // x = s K (zero elsewhere, and on the tile boundary), spread to the wave's
+  // columns with the hat weights, then to its rows
+  auto ml_terms = [&](T (&rv)[R][CPL], T (&ml)[R]) {
+#pragma unroll
+    for (int i = 0; i < R; ++i)
+      ml[i] = 0;
+    if constexpr (ML)
+      if (mlon)
+        {
+          const T *pp = smlp + lane;
+          const int wlo = wave > 0 ? wave - 1 : 0, whi = wave < SW - 1 ? wave + 1 : SW - 1;
+          const int x = lane + 1;
+          const T r0 = rv[0][0], r1 = rv[1][0], r2 = rv[2][0], r3 = rv[3][0], r4 = rv[4][0], r5 = rv[5][0],
+                  r6 = rv[6][0];
+          T c2[5]; // level 2, coarse rows 4w .. 4w + 4
+          {
+            T q[5];
+            q[0] = wave > 0 ? pp[(SW + wlo) * kWave] + pp[wave * kWave] : (T)0;
+            q[1] = xw2(r1 + (T)0.5 * (r0 + r2));
+            q[2] = xw2(r3 + (T)0.5 * (r2 + r4));
+            q[3] = xw2(r5 + (T)0.5 * (r4 + r6));
+            q[4] = wave < SW - 1 ? pp[(SW + wave) * kWave] + pp[whi * kWave] : (T)0;
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+              {
+                const int L = 4 * wave + m;
+                const bool ok = (x & 1) == 0 && x <= 62 && L >= 1 && L <= 31;
+                const T y = ok ? q[m] * (T)smld[ok ? (L - 1) * 31 + (x >> 1) - 1 : 0] : (T)0;
+                c2[m] = y + (T)0.5 * (ror1(y) + rol1(y));
+              }
+          }
+          T c4[3]; // level 4, coarse rows 2w .. 2w + 2
+          {
+            T q[3];
+            q[0] = wave > 0 ? pp[(3 * SW + wlo) * kWave] + pp[(2 * SW + wave) * kWave] : (T)0;
+            q[1] = xw4((((((T)0.25 * r0 + (T)0.5 * r1) + (T)0.75 * r2) + r3) + (T)0.75 * r4) + (T)0.5 * r5 +
+                       (T)0.25 * r6);
+            q[2] = wave < SW - 1 ? pp[(3 * SW + wave) * kWave] + pp[(2 * SW + whi) * kWave] : (T)0;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+              {
+                const int L = 2 * wave + m;
+                const bool ok = (x & 3) == 0 && x <= 60 && L >= 1 && L <= 15;
+                const T y = ok ? q[m] * (T)smld[kMl2 + (ok ? (L - 1) * 15 + (x >> 2) - 1 : 0)] : (T)0;
+                const T a1 = ror1(y), b1 = rol1(y), a2 = ror1(a1), b2 = rol1(b1), a3 = ror1(a2), b3 = rol1(b2);
+                c4[m] = y + (T)0.75 * (a1 + b1) + (T)0.5 * (a2 + b2) + (T)0.25 * (a3 + b3);
+              }
+          }
+          ml[0] = (T)0.5 * (c2[0] + c2[1]) + ((T)0.75 * c4[0] + (T)0.25 * c4[1]);
+          ml[1] = c2[1] + (T)0.5 * (c4[0] + c4[1]);
+          ml[2] = (T)0.5 * (c2[1] + c2[2]) + ((T)0.25 * c4[0] + (T)0.75 * c4[1]);
+          ml[3] = c2[2] + c4[1];
+          ml[4] = (T)0.5 * (c2[2] + c2[3]) + ((T)0.75 * c4[1] + (T)0.25 * c4[2]);
+          ml[5] = c2[3] + (T)0.5 * (c4[1] + c4[2]);
+          ml[6] = (T)0.5 * (c2[3] + c2[4]) + ((T)0.25 * c4[1] + (T)0.75 * c4[2]);
+          ml[7] = c2[4] + c4[2];
+        }
   };
 
   T xr[R][CPL], rr[R][CPL], pr[R][CPL], sr[R][CPL];
@@ -919,7 +1019,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
       else
       {
       if constexpr (CS)
-        restrict_r(rr);
+        {
+          restrict_r(rr);
+          restrict_ml(rr);
+        }
       while (true)
         {
           // one problem per workgroup: irun is uniform, so the loop ends with
@@ -937,11 +1040,13 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
 #pragma unroll
               for (int c = 0; c < CPL; ++c)
                 ci1[c] -= ci0[c];
+              T ml[R];
+              ml_terms(rr, ml);
 #pragma unroll
               for (int i = 0; i < R; ++i)
 #pragma unroll
                 for (int c = 0; c < CPL; ++c)
-                  gown[i * GP + c] = rr[i][c] * (T)dget(i, c) + (ci0[c] + ci1[c] * wyb(i));
+                  gown[i * GP + c] = rr[i][c] * (T)dget(i, c) + (ci0[c] + ci1[c] * wyb(i)) + ml[i];
             }
           else
             {
@@ -1069,7 +1174,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               first = false;
               ++it_total;
               if constexpr (CS)
-                restrict_r(rr);
+                {
+                  restrict_r(rr);
+                  restrict_ml(rr);
+                }
             }
           if (!any)
             break;
@@ -1285,6 +1393,97 @@ __global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, c
 // order.  cinv: [cells in class][49][49].
 constexpr int kSetupThreads = 256;
 
+// Element matrix of fine cell f for the setup kernels: per-fine-cell
+// coefficients on a box take the exact Q1 element matrix a rho Kxx + c / rho
+// Kyy + b Kxy (rho = hy / hx; unit-square Kxx, Kyy and Kxy = (s t^T + t s^T) /
+// 4, s_u = +-1 by x side, t_u by y side), every other kind fine_element_2d.
+__device__ inline void setup_element_2d(int coef_kind, const double *cf, int f, int nf, const double *bx,
+                                        const double *vq, double (&K)[4][4])
+{
+  if (!vq && coef_kind <= 1)
+    {
+      const double a = coef_kind == 0 ? cf[f] : cf[3 * f], b = coef_kind == 0 ? 0.0 : cf[3 * f + 1];
+      const double c = coef_kind == 0 ? cf[f] : cf[3 * f + 2];
+      const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+      const double ka = a * rho * (1.0 / 6.0), kc = c / rho * (1.0 / 6.0), kb = 0.25 * b;
+      for (int u = 0; u < 4; ++u)
+        for (int v = 0; v < 4; ++v)
+          {
+            const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
+            const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
+            const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
+            const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
+            K[u][v] = ka * sx * ny + kc * sy * nx + kb * (su * tv + tu * sv);
+          }
+    }
+  else
+    fine_element_2d(coef_kind, cf, f, nf, bx, vq, K);
+}
+
+// Intermediate levels of the 64^2 two-level preconditioner: the diagonals of
+// P_s^T K P_s for the bilinear hats of spacing s = 2 (31 x 31 interior coarse
+// nodes) and s = 4 (15 x 15), inverted and stored as FP32, mlinv [cells in
+// class][kMlFloats] = D_2^-1 [L - 1][K - 1], then D_4^-1 [L - 1][K - 1] (node
+// (K, L) at fine (s K, s L)).  Each s x s block of fine cells contributes the
+// four diagonal entries w_m^T K_e w_m of its corners m (w_m = the corner's hat
+// at the fine vertices), summed over the block's cells in a fixed order; a
+// node's entry is the sum over its four blocks, in a fixed order.
+__global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, const int *cells, int coef_kind,
+                                                            const double *box, const double *verts, const double *coef,
+                                                            const long long *coef_off, float *mlinv)
+{
+  constexpr int NF = 64, NB2 = NF / 2, NB4 = NF / 4, NBLK = NB2 * NB2 + NB4 * NB4;
+  __shared__ double sdiag[NBLK][4];
+  const int cc = blockIdx.x, t = threadIdx.x;
+  if (cc >= n_cells_class)
+    return;
+  const int cell = cells[cc];
+  const double *bx = verts ? nullptr : box + (size_t)cell * 4;
+  const double *vq = verts ? verts + (size_t)cell * 8 : nullptr;
+  const double *cf = coef + coef_off[cell];
+  for (int b = t; b < NBLK; b += kSetupThreads)
+    {
+      const bool l2 = b < NB2 * NB2;
+      const int S = l2 ? 2 : 4, nb = l2 ? NB2 : NB4, bb = l2 ? b : b - NB2 * NB2;
+      const int bxi = bb % nb, byi = bb / nb;
+      double acc[4] = {0, 0, 0, 0};
+      for (int ly = 0; ly < S; ++ly)
+        for (int lx = 0; lx < S; ++lx)
+          {
+            double K[4][4];
+            setup_element_2d(coef_kind, cf, (byi * S + ly) * NF + bxi * S + lx, NF, bx, vq, K);
+            for (int m = 0; m < 4; ++m)
+              {
+                double w[4]; // corner m's hat at the fine cell's vertices
+                for (int v = 0; v < 4; ++v)
+                  {
+                    const double px = (double)(lx + (v & 1)) / S, py = (double)(ly + (v >> 1)) / S;
+                    w[v] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
+                  }
+                double q = 0;
+                for (int u = 0; u < 4; ++u)
+                  q += w[u] * (((K[u][0] * w[0] + K[u][1] * w[1]) + K[u][2] * w[2]) + K[u][3] * w[3]);
+                acc[m] += q;
+              }
+          }
+      for (int m = 0; m < 4; ++m)
+        sdiag[b][m] = acc[m];
+    }
+  __syncthreads();
+  float *out = mlinv + (size_t)cc * kMlFloats;
+  for (int e = t; e < kMl2 + kMl4; e += kSetupThreads)
+    {
+      const bool l2 = e < kMl2;
+      const int side = l2 ? NB2 - 1 : NB4 - 1, nb = l2 ? NB2 : NB4, off = l2 ? 0 : NB2 * NB2;
+      const int k = l2 ? e : e - kMl2, K = k % side + 1, L = k / side + 1;
+      // blocks (K-1, L-1) corner 3, (K, L-1) corner 2, (K-1, L) corner 1, (K, L) corner 0
+      const double d = ((sdiag[off + (L - 1) * nb + K - 1][3] + sdiag[off + (L - 1) * nb + K][2]) +
+                        sdiag[off + L * nb + K - 1][1]) +
+                       sdiag[off + L * nb + K][0];
+      out[e] = (float)(1.0 / d);
+    }
+}
+
 template <int NF>
 __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind,
                                                                 const double *box, const double *verts,
@@ -1311,28 +1510,7 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
       for (int lx = 0; lx < H; ++lx)
         {
           double K[4][4];
-          const int f = (byi * H + ly) * NF + bxi * H + lx;
-          if (!vq && coef_kind <= 1)
-            {
-              // per-fine-cell coefficient on a box: the exact Q1 element matrix
-              // a rho Kxx + c / rho Kyy + b Kxy (rho = hy / hx; unit-square Kxx,
-              // Kyy and Kxy = (s t^T + t s^T) / 4, s_u = +-1 by x side, t_u by y side)
-              const double a = coef_kind == 0 ? cf[f] : cf[3 * f], b = coef_kind == 0 ? 0.0 : cf[3 * f + 1];
-              const double c = coef_kind == 0 ? cf[f] : cf[3 * f + 2];
-              const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
-              const double ka = a * rho * (1.0 / 6.0), kc = c / rho * (1.0 / 6.0), kb = 0.25 * b;
-              for (int u = 0; u < 4; ++u)
-                for (int v = 0; v < 4; ++v)
-                  {
-                    const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
-                    const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
-                    const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
-                    const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
-                    K[u][v] = ka * sx * ny + kc * sy * nx + kb * (su * tv + tu * sv);
-                  }
-            }
-          else
-            fine_element_2d(coef_kind, cf, f, NF, bx, vq, K);
+          setup_element_2d(coef_kind, cf, (byi * H + ly) * NF + bxi * H + lx, NF, bx, vq, K);
           double Wt[4][4]; // [vertex a][block corner m]
           for (int v = 0; v < 4; ++v)
             {
@@ -1617,6 +1795,19 @@ hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int nf, i
                        coef_kind, box, verts, coef, coef_off, cinv);
   else
     return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_ml_diag_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                             const double *verts, const double *coef, const long long *coef_off, float *mlinv,
+                             hipStream_t st)
+{
+  if (n_cells_class == 0)
+    return hipSuccess;
+  if (nf != 64)
+    return hipErrorInvalidValue;
+  hipLaunchKernelGGL(ml_diag_2d, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells, coef_kind, box,
+                     verts, coef, coef_off, mlinv);
   return hipGetLastError();
 }
 

@@ -35,6 +35,10 @@ __host__ __device__ constexpr int stream_q_slots(int nf) { return nf * nf; }
 // of the 8 x 8 coarse grid (spacing kCoarseH fine cells) of the tile, its
 // kCoarseSide^2 interior nodes (zero Dirichlet on the tile boundary).
 constexpr int kCoarseH = 8, kCoarseSide = 7, kCoarseDofs = kCoarseSide * kCoarseSide;
+// Intermediate levels of the 64^2 two-level preconditioner: inverse diagonals
+// of the spacing-2 (31 x 31 interior nodes) and spacing-4 (15 x 15) hat levels
+// per cell, FP32, padded to a multiple of 4 floats.
+constexpr int kMl2 = 31 * 31, kMl4 = 15 * 15, kMlFloats = (kMl2 + kMl4 + 3) / 4 * 4;
 
 // Workgroup -> block index such that the blocks one XCD receives (hardware
 // dispatch is round-robin over the 8 XCDs) hold consecutive problems: the
@@ -64,6 +68,8 @@ struct LocalArgs {
   const double *x0;           // optional initial guesses (bases layout) or nullptr
   int q1_start;               // without x0: 1 = interior starts at the Q1 interpolant of g
   const double *cinv;         // two-level: [cells in class][kCoarseDofs^2] inverse coarse matrices, or nullptr
+  const float *mlinv;         // 64^2 two-level: [cells in class][kMlFloats] inverse diagonals of the
+                              // intermediate hat levels (spacing 2 and 4), or nullptr: two levels only
   int *iters;                 // [n_cells][nb] or nullptr
   int *status;                // [n_cells][nb] or nullptr
   double *resid;              // [n_cells][nb] or nullptr

@@ -16,6 +16,11 @@ bool local2d_coarse_space(int nf, int mode, int variant);
 hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
                                   const double *verts, const double *coef, const long long *coef_off, double *cinv,
                                   hipStream_t st);
+// the intermediate levels' inverse diagonals of a 64^2 two-level class
+// (mlinv [cells in class][kMlFloats])
+hipError_t launch_ml_diag_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,
+                             const double *verts, const double *coef, const long long *coef_off, float *mlinv,
+                             hipStream_t st);
 hipError_t launch_local_solve_2d(bool fp32, int mode, int nf, int variant, const LocalArgs &a, hipStream_t st);
 // verts: general quadrilaterals [n_cells][4][2] or nullptr (boxes)
 hipError_t launch_prep_couplings_2d(int n_cells_class, const int *cells, int nf, int coef_kind, const double *box,

@@ -198,8 +198,10 @@ class Context:
         return len(table["entries"])
 
     def uses_coarse_space(self, dim, nf, opts, coef_kind, general=False):
-        """Whether a class of nf fine cells per side runs the two-level local solve here."""
-        return bool(lib().msfem_uses_coarse_space(self.h, dim, nf, ctypes.byref(opts), coef_kind, 1 if general else 0))
+        """0: a class of nf fine cells per side runs Jacobi-PCG here; 1: the
+        two-level local solve; 2: the two-level solve with the intermediate hat
+        levels."""
+        return int(lib().msfem_uses_coarse_space(self.h, dim, nf, ctypes.byref(opts), coef_kind, 1 if general else 0))
 
     def set_profiling(self, on=True):
         _check(lib().msfem_ctx_set_profiling(self.h, 1 if on else 0), self.h)

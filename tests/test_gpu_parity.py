@@ -313,9 +313,11 @@ def test_every_launch_shape(ctx, prec):
 
 @pytest.mark.parametrize("kind", ["lognormal", "tensor"])
 def test_two_level_preconditioner(ctx, kind):
-    """64^2 cells with the two-level local solve (coarse_space=1, the default) and
-    with Jacobi alone (coarse_space=0): the same values (oracle, 1e-10; FP32 +
-    refinement at 1e-8 within 1e-6 energy norm) and at most 0.6x the iterations."""
+    """64^2 cells with the two-level local solve plus the intermediate hat levels
+    (coarse_space=1, the default), the two-level form alone (2) and Jacobi alone
+    (0): the same values (oracle, 1e-10; FP32 + refinement at 1e-8 within 1e-6
+    energy norm), the two-level form at most 0.6x Jacobi's iterations and the
+    intermediate levels at most 0.75x the two-level form's, on both launch shapes."""
     say(f"two-level preconditioner, {kind} 64^2")
     if kind == "lognormal":
         case = C.lognormal_cells(which=np.array([3, 700, 2222, 4000]))
@@ -323,16 +325,19 @@ def test_two_level_preconditioner(ctx, kind):
         case = C.tensor_cells(nc=128, which=np.arange(8), n_fine=[64] * 8)
     ref = C.oracle_build(case)
     its = {}
-    for cs in (0, 1):
+    for cs in (0, 2, 1):
         for variant in (1, 4):  # both launch shapes of the two-level form
             out = C.gpu_build(ctx, case, coarse_space=cs, variant=variant)
             check_against_oracle(case, out, ref)
             its[cs, variant] = out["iters"].mean()
-        assert ctx.uses_coarse_space(2, 64, msfem.default_opts(coarse_space=cs, variant=1), case["coef_kind"]) == bool(cs)
+        assert ctx.uses_coarse_space(2, 64, msfem.default_opts(coarse_space=cs, variant=1), case["coef_kind"]) == \
+            {0: 0, 2: 1, 1: 2}[cs]
         out = C.gpu_build(ctx, case, coarse_space=cs, precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
         assert (out["status"] == 0).all() and C.energy_error(case, out["bases"], ref["bases"]) < 1e-6
-    say(f"  mean iterations: Jacobi {its[0, 1]:.1f}, two-level {its[1, 1]:.1f} / {its[1, 4]:.1f}")
-    assert its[1, 1] < 0.6 * its[0, 1] and its[1, 4] < 0.6 * its[0, 4]
+    say(f"  mean iterations: Jacobi {its[0, 1]:.1f}, two-level {its[2, 1]:.1f} / {its[2, 4]:.1f}, "
+        f"with the intermediate levels {its[1, 1]:.1f} / {its[1, 4]:.1f}")
+    for v in (1, 4):
+        assert its[2, v] < 0.6 * its[0, v] and its[1, v] < 0.75 * its[2, v]
     # shapes outside the two-level table keep Jacobi
     assert not ctx.uses_coarse_space(2, 64, msfem.default_opts(variant=2), case["coef_kind"])
     assert not ctx.uses_coarse_space(2, 32, msfem.default_opts(), case["coef_kind"])
