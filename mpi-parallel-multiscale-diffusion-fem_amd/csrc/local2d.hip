@@ -660,24 +660,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           const int x = lane + 1;
           const T r0 = rv[0][0], r1 = rv[1][0], r2 = rv[2][0], r3 = rv[3][0], r4 = rv[4][0], r5 = rv[5][0],
                   r6 = rv[6][0];
-          T c2[5]; // level 2, coarse rows 4w .. 4w + 4
-          {
-            T q[5];
-            q[0] = wave > 0 ? pp[(SW + wlo) * kWave] + pp[wave * kWave] : (T)0;
-            q[1] = xw2(r1 + (T)0.5 * (r0 + r2));
-            q[2] = xw2(r3 + (T)0.5 * (r2 + r4));
-            q[3] = xw2(r5 + (T)0.5 * (r4 + r6));
-            q[4] = wave < SW - 1 ? pp[(SW + wave) * kWave] + pp[whi * kWave] : (T)0;
-#pragma unroll
-            for (int m = 0; m < 5; ++m)
-              {
-                const int L = 4 * wave + m;
-                const bool ok = (x & 1) == 0 && x <= 62 && L >= 1 && L <= 31;
-                const T y = ok ? q[m] * (T)smld[ok ? (L - 1) * 31 + (x >> 1) - 1 : 0] : (T)0;
-                c2[m] = y + (T)0.5 * (ror1(y) + rol1(y));
-              }
-          }
-          T c4[3]; // level 4, coarse rows 2w .. 2w + 2
+          const bool lane2 = (x & 1) == 0 && x <= 62, lane4 = (x & 3) == 0 && x <= 60;
+          const int i2 = (4 * wave - 1) * 31 + (x >> 1) - 1, i4 = kMl2 + (2 * wave - 1) * 15 + (x >> 2) - 1;
+          // level 4, coarse rows 2w .. 2w + 2: y4 at lanes x = 4 K, spread to
+          // the level-2 lanes (x = 2 K; zero at odd x)
+          T s4[3];
           {
             T q[3];
             q[0] = wave > 0 ? pp[(3 * SW + wlo) * kWave] + pp[(2 * SW + wave) * kWave] : (T)0;
@@ -688,20 +675,39 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
             for (int m = 0; m < 3; ++m)
               {
                 const int L = 2 * wave + m;
-                const bool ok = (x & 3) == 0 && x <= 60 && L >= 1 && L <= 15;
-                const T y = ok ? q[m] * (T)smld[kMl2 + (ok ? (L - 1) * 15 + (x >> 2) - 1 : 0)] : (T)0;
-                const T a1 = ror1(y), b1 = rol1(y), a2 = ror1(a1), b2 = rol1(b1), a3 = ror1(a2), b3 = rol1(b2);
-                c4[m] = y + (T)0.75 * (a1 + b1) + (T)0.5 * (a2 + b2) + (T)0.25 * (a3 + b3);
+                const bool ok = lane4 && L >= 1 && L <= 15;
+                const T y = ok ? q[m] * (T)smld[ok ? i4 + 15 * m : 0] : (T)0;
+                s4[m] = y + (T)0.5 * (ror1(ror1(y)) + rol1(rol1(y)));
               }
           }
-          ml[0] = (T)0.5 * (c2[0] + c2[1]) + ((T)0.75 * c4[0] + (T)0.25 * c4[1]);
-          ml[1] = c2[1] + (T)0.5 * (c4[0] + c4[1]);
-          ml[2] = (T)0.5 * (c2[1] + c2[2]) + ((T)0.25 * c4[0] + (T)0.75 * c4[1]);
-          ml[3] = c2[2] + c4[1];
-          ml[4] = (T)0.5 * (c2[2] + c2[3]) + ((T)0.75 * c4[1] + (T)0.25 * c4[2]);
-          ml[5] = c2[3] + (T)0.5 * (c4[1] + c4[2]);
-          ml[6] = (T)0.5 * (c2[3] + c2[4]) + ((T)0.25 * c4[1] + (T)0.75 * c4[2]);
-          ml[7] = c2[4] + c4[2];
+          // level 2, coarse rows 4w .. 4w + 4, plus the level-4 values
+          // interpolated to them (P_4 = P_2 P_24: one prolongation from level 2)
+          T c2[5];
+          {
+            T q[5];
+            q[0] = wave > 0 ? pp[(SW + wlo) * kWave] + pp[wave * kWave] : (T)0;
+            q[1] = xw2(r1 + (T)0.5 * (r0 + r2));
+            q[2] = xw2(r3 + (T)0.5 * (r2 + r4));
+            q[3] = xw2(r5 + (T)0.5 * (r4 + r6));
+            q[4] = wave < SW - 1 ? pp[(SW + wave) * kWave] + pp[whi * kWave] : (T)0;
+            const T s4h[5] = {s4[0], (T)0.5 * (s4[0] + s4[1]), s4[1], (T)0.5 * (s4[1] + s4[2]), s4[2]};
+#pragma unroll
+            for (int m = 0; m < 5; ++m)
+              {
+                const int L = 4 * wave + m;
+                const bool ok = lane2 && L >= 1 && L <= 31;
+                const T u = (ok ? q[m] * (T)smld[ok ? i2 + 31 * m : 0] : (T)0) + s4h[m];
+                c2[m] = u + (T)0.5 * (ror1(u) + rol1(u));
+              }
+          }
+          ml[0] = (T)0.5 * (c2[0] + c2[1]);
+          ml[1] = c2[1];
+          ml[2] = (T)0.5 * (c2[1] + c2[2]);
+          ml[3] = c2[2];
+          ml[4] = (T)0.5 * (c2[2] + c2[3]);
+          ml[5] = c2[3];
+          ml[6] = (T)0.5 * (c2[3] + c2[4]);
+          ml[7] = c2[4];
         }
   };
 
