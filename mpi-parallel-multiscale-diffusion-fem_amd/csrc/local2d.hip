@@ -633,11 +633,21 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // the partial rows this wave shares with its neighbours: B2 = row 4w (from
   // row y = 8w + 1), T2 = row 4w + 4 (rows 8w + 7, 8w + 8), B4 = row 2w (rows
   // 8w + 1 .. 8w + 3), T4 = row 2w + 2 (rows 8w + 5 .. 8w + 8); lane l = column l + 1
+  // (the lane-dependent addresses and masks below are formed from an opaque
+  // copy of the lane index where used: hoisted out of the loops they would be
+  // kept live and spilled)
+  auto ml_lane = [&]() {
+    int v = lane;
+    asm volatile("" : "+v"(v));
+    return v;
+  };
+  // (the wave index as a scalar: the row conditions below are then uniform branches)
+  const int mwave = __builtin_amdgcn_readfirstlane(wave);
   auto restrict_ml = [&](T (&rv)[R][CPL]) {
     if constexpr (ML)
       if (mlon)
         {
-          T *pp = smlp + wave * kWave + lane;
+          T *pp = smlp + mwave * kWave + ml_lane();
           pp[0] = xw2((T)0.5 * rv[0][0]);
           pp[SW * kWave] = xw2(rv[7][0] + (T)0.5 * rv[6][0]);
           pp[2 * SW * kWave] = xw4(((T)0.75 * rv[0][0] + (T)0.5 * rv[1][0]) + (T)0.25 * rv[2][0]);
@@ -655,26 +665,26 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     if constexpr (ML)
       if (mlon)
         {
-          const T *pp = smlp + lane;
-          const int wlo = wave > 0 ? wave - 1 : 0, whi = wave < SW - 1 ? wave + 1 : SW - 1;
-          const int x = lane + 1;
+          const int xl = ml_lane(), x = xl + 1;
+          const T *pp = smlp + xl;
+          const int wlo = mwave > 0 ? mwave - 1 : 0, whi = mwave < SW - 1 ? mwave + 1 : SW - 1;
           const T r0 = rv[0][0], r1 = rv[1][0], r2 = rv[2][0], r3 = rv[3][0], r4 = rv[4][0], r5 = rv[5][0],
                   r6 = rv[6][0];
           const bool lane2 = (x & 1) == 0 && x <= 62, lane4 = (x & 3) == 0 && x <= 60;
-          const int i2 = (4 * wave - 1) * 31 + (x >> 1) - 1, i4 = kMl2 + (2 * wave - 1) * 15 + (x >> 2) - 1;
+          const int i2 = (4 * mwave - 1) * 31 + (x >> 1) - 1, i4 = kMl2 + (2 * mwave - 1) * 15 + (x >> 2) - 1;
           // level 4, coarse rows 2w .. 2w + 2: y4 at lanes x = 4 K, spread to
           // the level-2 lanes (x = 2 K; zero at odd x)
           T s4[3];
           {
             T q[3];
-            q[0] = wave > 0 ? pp[(3 * SW + wlo) * kWave] + pp[(2 * SW + wave) * kWave] : (T)0;
+            q[0] = mwave > 0 ? pp[(3 * SW + wlo) * kWave] + pp[(2 * SW + mwave) * kWave] : (T)0;
             q[1] = xw4((((((T)0.25 * r0 + (T)0.5 * r1) + (T)0.75 * r2) + r3) + (T)0.75 * r4) + (T)0.5 * r5 +
                        (T)0.25 * r6);
-            q[2] = wave < SW - 1 ? pp[(3 * SW + wave) * kWave] + pp[(2 * SW + whi) * kWave] : (T)0;
+            q[2] = mwave < SW - 1 ? pp[(3 * SW + mwave) * kWave] + pp[(2 * SW + whi) * kWave] : (T)0;
 #pragma unroll
             for (int m = 0; m < 3; ++m)
               {
-                const int L = 2 * wave + m;
+                const int L = 2 * mwave + m;
                 const bool ok = lane4 && L >= 1 && L <= 15;
                 const T y = ok ? q[m] * (T)smld[ok ? i4 + 15 * m : 0] : (T)0;
                 s4[m] = y + (T)0.5 * (ror1(ror1(y)) + rol1(rol1(y)));
@@ -685,16 +695,16 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           T c2[5];
           {
             T q[5];
-            q[0] = wave > 0 ? pp[(SW + wlo) * kWave] + pp[wave * kWave] : (T)0;
+            q[0] = mwave > 0 ? pp[(SW + wlo) * kWave] + pp[mwave * kWave] : (T)0;
             q[1] = xw2(r1 + (T)0.5 * (r0 + r2));
             q[2] = xw2(r3 + (T)0.5 * (r2 + r4));
             q[3] = xw2(r5 + (T)0.5 * (r4 + r6));
-            q[4] = wave < SW - 1 ? pp[(SW + wave) * kWave] + pp[whi * kWave] : (T)0;
+            q[4] = mwave < SW - 1 ? pp[(SW + mwave) * kWave] + pp[whi * kWave] : (T)0;
             const T s4h[5] = {s4[0], (T)0.5 * (s4[0] + s4[1]), s4[1], (T)0.5 * (s4[1] + s4[2]), s4[2]};
 #pragma unroll
             for (int m = 0; m < 5; ++m)
               {
-                const int L = 4 * wave + m;
+                const int L = 4 * mwave + m;
                 const bool ok = lane2 && L >= 1 && L <= 31;
                 const T u = (ok ? q[m] * (T)smld[ok ? i2 + 31 * m : 0] : (T)0) + s4h[m];
                 c2[m] = u + (T)0.5 * (ror1(u) + rol1(u));
