@@ -58,7 +58,7 @@ PEAK_BF16_MFMA_TFLOPS = 2500.0  # dense
 # WRITE_SIZE passes scaled by a plain copy of known size (--copy-calibration):
 # profiles/local_solve_2d_lognormal64_hbm_traffic.md (FP64, 4096 cells) and
 # profiles/local_solve_2d_fp32_grf128_hbm_traffic.md (FP32 energy rule, 16384 cells)
-TRAFFIC_BYTES = {("lognormal64", 0): 2.168743e9, ("surrogate128", 1): 9.704063e9}
+TRAFFIC_BYTES = {("lognormal64", 0): 2.946802e9, ("surrogate128", 1): 9.696504e9}
 METRIC = "coarse-cell bases built/sec: plain FP64 vs batched FP32+surrogate @1e-6 err"
 TAU = 5e-7  # energy stopping rule of the FP32 legs: a 2x margin on the 1e-6 relative energy-norm bar
 WEIGHTS = os.path.join(PKG, "surrogate_weights.npz")
