@@ -65,13 +65,16 @@ WEIGHTS = os.path.join(PKG, "surrogate_weights.npz")
 
 
 def f_ml(nf):
-    """Algorithmic flops per iteration of the intermediate hat levels (spacing 2
-    and 4) of a 2D problem of nf^2 fine cells: separable restriction (row
-    weights over the fine columns, then column weights), diagonal scaling,
-    separable bilinear prolongation and the addition to z."""
+    """Algorithmic flops per iteration of the intermediate hat levels (spacing
+    nf/32 and nf/16: 2 and 4 at 64^2, 4 and 8 at 128^2) of a 2D problem of nf^2
+    fine cells: separable restriction (row weights over the fine columns, then
+    column weights), diagonal scaling, separable bilinear prolongation and the
+    addition to z."""
+    if nf < 64:  # no intermediate levels below 64^2
+        return 0
     n = nf - 1
     f = 2 * n * n  # z += the two levels' values
-    for s in (2, 4):
+    for s in (nf // 32, nf // 16):
         m = nf // s - 1  # interior coarse nodes per side
         f += m * n * 2 * (2 * s - 1) + m * m * 2 * (2 * s - 1)  # restriction
         f += m * m                                               # D_s^-1

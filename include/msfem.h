@@ -116,8 +116,9 @@ typedef struct msfem_solver_opts {
                                  problem, solved exactly) for the 64^2 and 128^2 classes at
                                  the default launch shapes; on 64^2 box cells with per-fine-
                                  cell scalar or tensor coefficients also the intermediate
-                                 hat levels of spacing 2 and 4, scaled by their diagonals
-                                 (additive); 2: the two-level form without the
+                                 hat levels of spacing 2 and 4, on FP32 128^2 classes those
+                                 of spacing 4 and 8, scaled by their diagonals (additive);
+                                 2: the two-level form without the
                                  intermediate levels; 0: Jacobi only.  Same converged
                                  values, fewer iterations. */
   int32_t q1_start;           /* without x0: 1 (default) = the interior starts at the Q1
@@ -191,7 +192,8 @@ void msfem_default_opts(msfem_solver_opts *opts);
 /* 1 if a class of nf fine cells per side with these options, coefficient kind
  * and geometry (general: vertex cells) runs the two-level local solve on this
  * context (its launch-shape table applied), 2 if its preconditioner also takes
- * the intermediate hat levels (spacing 2 and 4), else 0.  No counterpart in the
+ * the intermediate hat levels (spacing 2 and 4 at 64^2, 4 and 8 at 128^2), else
+ * 0.  No counterpart in the
  * reference; for callers that account for the solver's work. */
 int msfem_uses_coarse_space(msfem_ctx *ctx, int dim, int nf, const msfem_solver_opts *opts, int coef_kind,
                             int general);

@@ -124,6 +124,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr int NCACHE = MODE == kModeScalar ? 1 : (MODE == kModeTensor ? 4 : 1);
   // kModeStream keeps q = K p of the own nodes in LDS (thread-private slots)
   constexpr bool QLDS = MODE == kModeStream;
+  // FP32 two-level streamed form: once r is updated, q is dead and the same
+  // slots keep z = M^-1 r from the (r, z) sums to the direction update
+  // instead of forming the preconditioner's terms twice per iteration
+  constexpr bool ZLDS = QLDS && CS && sizeof(T) == 4;
   constexpr int NT = kWave * SW;
   constexpr int PSN = coupling_plane_stride(NF); // node-coupling plane stride (kModeStream)
   // kModeStream, 64^2 (8 nodes per lane, 256 registers): the lane's window
@@ -169,13 +173,21 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // values masked to their lanes.
   constexpr bool ML = CS && NF == 64 && CPL == 1 && SW == 8 && R == 8 && MODE != kModeStream;
   __shared__ T smlp[ML ? 4 * SW * kWave : 1]; // partial rows [B2, T2, B4, T4][wave][lane]
-  __shared__ float smld[ML ? kMlFloats : 1];  // D_2^-1 [31][31], then D_4^-1 [15][15]
+  // The FP32 128^2 streamed form (two node columns per lane) takes the levels
+  // of spacing 4 and 8: each wave restricts its rows to the three spacing-4
+  // coarse rows it touches (B4 and T4 partial, F4 whole) and publishes them
+  // (the odd lanes hold the spacing-4 columns: 32 values per row); the
+  // spacing-8 values follow from the spacing-4 ones (P_8 = P_4 P_48), which
+  // takes the neighbours' F4 rows too.
+  constexpr bool ML8 = CS && NF == 128 && CPL == 2 && SW == 16 && R == 8 && MODE == kModeStream && sizeof(T) == 4;
+  __shared__ T smlq[ML8 ? 3 * SW * 32 : 1];           // [B4, F4, T4][wave][lane / 2]
+  __shared__ float smld[(ML || ML8) ? kMlFloats : 1]; // D_s1^-1 [31][31], then D_s2^-1 [15][15]
 
   const int tid = threadIdx.x, lane = tid & (kWave - 1), wave = tid / kWave;
   const int slot = lane / LW, pl = lane % LW;
   const int prob = xcd_block(blockIdx.x, gridDim.x) * PPW + slot;
   const bool pvalid = prob < a.n_problems;
-  const bool mlon = ML && a.mlinv != nullptr; // (uniform)
+  const bool mlon = (ML || ML8) && a.mlinv != nullptr; // (uniform)
   int cell = 0, basis = 0;
   if (pvalid)
     {
@@ -450,8 +462,30 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // kept live and spilled.
   constexpr bool OPAQUE_CA = WPE >= 4;
   constexpr bool OPAQUE_W = NF == 128;
+  static_assert(!OPAQUE_W || LW == kWave, "128^2 tiles: one problem per wave");
+  // FP32 128^2 tiles (128 registers, all in use): the lane index is read from
+  // the hardware (v_mbcnt) where used instead of kept in a register
+  constexpr bool OPAQUE_L = OPAQUE_W && sizeof(T) == 4;
+  auto lane_here = [&]() -> int {
+    if constexpr (OPAQUE_L)
+      {
+        int v;
+        asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
+        return v;
+      }
+    else
+      return pl;
+  };
+  // (and the wave-dependent LDS slots are formed from an opaque copy of the
+  // wave index where used)
+  auto wave_here = [&]() -> int {
+    int w = wave;
+    if constexpr (OPAQUE_L)
+      asm volatile("" : "+v"(w));
+    return w;
+  };
   auto lane_off = [&]() {
-    int v = (pl & 7) * CPL + 1;
+    int v = (lane_here() & 7) * CPL + 1;
     if constexpr (OPAQUE_W)
       asm volatile("" : "+v"(v));
     return v;
@@ -495,13 +529,14 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           v4[k] = oct_sum_any<T>(v4[k]);
         if ((pl & 7) == 0)
           {
-            int g = cg8; // (OPAQUE_CA: the slot address is formed here, not hoisted and spilled)
+            int g = lane_here() >> 3; // (OPAQUE_CA: the slot address is formed here, not hoisted and spilled)
             if constexpr (OPAQUE_CA)
               asm volatile("" : "+v"(g));
-            spart[spart_at(wave, 0, g, 0)] = v4[0];
-            spart[spart_at(wave, 0, g, 1)] = v4[1];
-            spart[spart_at(wave, 1, g, 0)] = v4[2];
-            spart[spart_at(wave, 1, g, 1)] = v4[3];
+            const int w = wave_here();
+            spart[spart_at(w, 0, g, 0)] = v4[0];
+            spart[spart_at(w, 0, g, 1)] = v4[1];
+            spart[spart_at(w, 1, g, 0)] = v4[2];
+            spart[spart_at(w, 1, g, 1)] = v4[3];
           }
       }
   };
@@ -552,10 +587,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
     T acc = 0;
     if constexpr (WAVE_CS)
       {
-        int lp = pl;
+        int lp = lane_here();
         if constexpr (OPAQUE_CA)
           asm volatile("" : "+v"(lp));
-        T *wrc = swrc + wave * kWave;
+        T *wrc = swrc + wave_here() * kWave;
         if (lp < kCoarseDofs)
           {
             const int jx = lp % kCoarseSide + 1, jy = lp / kCoarseSide + 1;
@@ -600,7 +635,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           const T v = __shfl(acc, src, kWave);
           return (jx >= 1 && jx <= kCoarseSide) ? v : (T)0;
         };
-        const T u00 = uq(cg8, 0), u10 = uq(cg8 + 1, 0), u01 = uq(cg8, 1), u11 = uq(cg8 + 1, 1);
+        const int g8 = lane_here() >> 3;
+        const T u00 = uq(g8, 0), u10 = uq(g8 + 1, 0), u01 = uq(g8, 1), u11 = uq(g8 + 1, 1);
 #pragma unroll
         for (int c = 0; c < CPL; ++c)
           {
@@ -638,6 +674,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // kept live and spilled)
   auto ml_lane = [&]() {
     int v = lane;
+    if constexpr (OPAQUE_L)
+      v = lane_here();
     asm volatile("" : "+v"(v));
     return v;
   };
@@ -719,6 +757,124 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           ml[6] = (T)0.5 * (c2[3] + c2[4]);
           ml[7] = c2[4];
         }
+  };
+
+  // ---- intermediate levels of the 128^2 streamed form (ML8) ---------------------------
+  // Lane l holds columns x = 2 l + 1 (c = 0) and 2 l + 2 (c = 1): the spacing-2
+  // sums v2 = r(2 l + 2) + (r(2 l + 1) + r(2 l + 3)) / 2 sit on every lane, and
+  // the hat-2 weights over them give the spacing-4 restriction on the odd
+  // lanes (x = 4 K, lane 2 K - 1).
+  auto xres4 = [&](T c0, T c1) -> T {
+    const T v2 = c1 + (T)0.5 * (c0 + rol1(c0));
+    return xw2(v2);
+  };
+  auto restrict_ml8 = [&](T (&rv)[R][CPL]) {
+    if constexpr (ML8)
+      if (mlon)
+        {
+          // rows y = 8 w + 1 + i: B4 = coarse row 2 w (i = 0..2), F4 = row 2 w + 1
+          // (i = 0..6), T4 = row 2 w + 2 (i = 4..7)
+          T b[2], f[2], t[2];
+#pragma unroll
+          for (int c = 0; c < 2; ++c)
+            {
+              b[c] = ((T)0.75 * rv[0][c] + (T)0.5 * rv[1][c]) + (T)0.25 * rv[2][c];
+              f[c] = (((((T)0.25 * rv[0][c] + (T)0.5 * rv[1][c]) + (T)0.75 * rv[2][c]) + rv[3][c]) +
+                      (T)0.75 * rv[4][c]) +
+                     (T)0.5 * rv[5][c] + (T)0.25 * rv[6][c];
+              t[c] = (((T)0.25 * rv[4][c] + (T)0.5 * rv[5][c]) + (T)0.75 * rv[6][c]) + rv[7][c];
+            }
+          const T vb = xres4(b[0], b[1]), vf = xres4(f[0], f[1]), vt = xres4(t[0], t[1]);
+          const int xl = ml_lane();
+          if (xl & 1)
+            {
+              T *pp = smlq + mwave * 32 + (xl >> 1);
+              pp[0] = vb;
+              pp[SW * 32] = vf;
+              pp[2 * SW * 32] = vt;
+            }
+        }
+  };
+  // After the restriction barrier: the spacing-4 values y_4 = D_4^-1 (P_4^T r)
+  // of coarse rows 2 w .. 2 w + 2 on the odd lanes, plus the spacing-8 values
+  // y_8 = D_8^-1 (P_8^T r) of coarse rows w, w + 1 interpolated to them
+  // (P_8 = P_4 P_48; P_8^T r = P_48^T (P_4^T r)).  The spacing-8 rows shared by
+  // two waves are formed by both from the same values in the same order.
+  auto ml8_coarse = [&](T (&u4)[3]) {
+#pragma unroll
+    for (int m = 0; m < 3; ++m)
+      u4[m] = 0;
+    if constexpr (ML8)
+      if (mlon)
+        {
+          const int xl = ml_lane();
+          const bool oddl = (xl & 1) != 0;
+          const T *pp = smlq + (xl >> 1);
+          const int wlo = mwave > 0 ? mwave - 1 : 0, whi = mwave < SW - 1 ? mwave + 1 : SW - 1;
+          auto rd = [&](int k, int w) -> T {
+            const T v = pp[(k * SW + w) * 32];
+            return oddl ? v : (T)0;
+          };
+          const T F = rd(1, mwave);
+          T q0 = 0, q2 = 0, Fm = 0, Fp = 0;
+          if (mwave > 0)
+            {
+              q0 = rd(2, wlo) + rd(0, mwave);
+              Fm = rd(1, wlo);
+            }
+          if (mwave < SW - 1)
+            {
+              q2 = rd(2, mwave) + rd(0, whi);
+              Fp = rd(1, whi);
+            }
+          // spacing 8: rows w, w + 1 on the lanes 4 K - 1, spread back to the odd lanes
+          const T r8[2] = {((T)0.5 * Fm + q0) + (T)0.5 * F, ((T)0.5 * F + q2) + (T)0.5 * Fp};
+          T s8[2];
+#pragma unroll
+          for (int m = 0; m < 2; ++m)
+            {
+              const int L = mwave + m;
+              const bool ok = (xl & 3) == 3 && xl <= 59 && L >= 1 && L <= 15;
+              const T x8 = r8[m] + (T)0.5 * (ror1(ror1(r8[m])) + rol1(rol1(r8[m])));
+              const T y = ok ? x8 * (T)smld[ok ? kMl2 + (L - 1) * 15 + ((xl + 1) >> 2) - 1 : 0] : (T)0;
+              s8[m] = y + (T)0.5 * (ror1(ror1(y)) + rol1(rol1(y)));
+            }
+          const T q[3] = {q0, F, q2};
+          const T s8h[3] = {s8[0], (T)0.5 * (s8[0] + s8[1]), s8[1]};
+#pragma unroll
+          for (int m = 0; m < 3; ++m)
+            {
+              const int L = 2 * mwave + m;
+              const bool ok = oddl && xl <= 61 && L >= 1 && L <= 31;
+              u4[m] = (ok ? q[m] * (T)smld[ok ? (L - 1) * 31 + ((xl + 1) >> 1) - 1 : 0] : (T)0) + s8h[m];
+            }
+        }
+  };
+  // the levels' share of z on the lane's two columns of coarse row m (spread
+  // from the odd lanes to every lane's column 1, then to the column 0 midpoints)
+  auto ml8_cols = [&](const T (&u4)[3], T (&v)[3][2]) {
+    if constexpr (ML8)
+#pragma unroll
+      for (int m = 0; m < 3; ++m)
+        {
+          const T c1 = u4[m] + (T)0.5 * (ror1(u4[m]) + rol1(u4[m]));
+          v[m][0] = (T)0.5 * (ror1(c1) + c1);
+          v[m][1] = c1;
+        }
+  };
+  // row i of the wave between coarse rows 2 w (y = 8 w), 2 w + 1 (8 w + 4), 2 w + 2 (8 w + 8)
+  auto ml8_at = [&](const T (&v)[3][2], int i, int c) -> T {
+    if constexpr (ML8)
+      {
+        if (i < 3)
+          return (T)(0.25 * (3 - i)) * v[0][c] + (T)(0.25 * (i + 1)) * v[1][c];
+        if (i == 3)
+          return v[1][c];
+        if (i < 7)
+          return (T)(0.25 * (7 - i)) * v[1][c] + (T)(0.25 * (i - 3)) * v[2][c];
+        return v[2][c];
+      }
+    return (T)0;
   };
 
   T xr[R][CPL], rr[R][CPL], pr[R][CPL], sr[R][CPL];
@@ -812,9 +968,12 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           for (int c = 0; c < CPL; ++c)
             ci0[c] = ci1[c] = 0;
           T cacc = 0; // per-wave coarse sums of the current z (kept for the p update)
+          T u4[3];    // ML8: the intermediate levels' coarse values of the current z (idem)
+          T mv[3][2];
           if constexpr (CS) // irun is uniform (one problem per workgroup): true here
             {
               restrict_r(rr);
+              restrict_ml8(rr);
               __syncthreads(); // restriction partials complete
               if constexpr (WAVE_CS)
                 {
@@ -827,6 +986,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   __syncthreads(); // coarse correction complete
                   interp_cols(ci0, ci1);
                 }
+              ml8_coarse(u4);
+              ml8_cols(u4, mv);
             }
           T v2[2] = {0, 0}; // (r, z), (r, r)
 #pragma unroll
@@ -837,6 +998,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 T z = rr[i][c] * (T)dget(i, c);
                 if constexpr (CS)
                   z += ci0[c] * wya(i) + ci1[c] * wyb(i);
+                if constexpr (ML8)
+                  z += ml8_at(mv, i, c);
                 gown[i * GP + c] = irun ? z : (T)0;
                 v2[0] += rr[i][c] * z;
                 v2[1] += rr[i][c] * rr[i][c];
@@ -886,41 +1049,107 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               const gtptr qg = (gtptr)qb;
               typedef double __attribute__((address_space(1))) * gdptr;
               const gdptr ph = (gdptr)pb;
+              // FP32 128^2: the planes are read as one buffer (raw buffer loads):
+              // the lane's window origin is the only vector offset, formed once
+              // per iteration from the hardware lane index and an opaque wave
+              // index; plane, row and column go into the constant offsets
+              constexpr bool CBUF = OPAQUE_L;
+              __amdgpu_buffer_rsrc_t crs;
+              int cvo = 0;
+              if constexpr (CBUF)
+                {
+                  crs = __builtin_amdgcn_make_buffer_rsrc((void *)cbase, (short)0, 4 * PSN * (int)sizeof(float),
+                                                          0x00020000);
+                  cvo = (wave_here() * (R * NP) + lane_here() * CPL) * (int)sizeof(float);
+                }
               // couplings at window column k (node X0 - 1 + k) of node row Y
+              // (every index lies inside the four padded planes)
               auto cpl = [&](int plane, int Y, int k) -> T {
                 if (plane < NHOIST) // (plane and Y - Y0 are compile-time constants here)
                   return creg[Y - Y0 + 1][HOISTC ? plane : 0][k];
+                if constexpr (CBUF)
+                  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
+                    crs, cvo, (plane * PSN + (Y - Y0 + 1) * NP + k) * (int)sizeof(float), 0));
                 return cp[(unsigned)(plane * PSN + Y * NP + X0 - 1 + k)];
+              };
+              // CBUF: dst[k] = coupling at window column k for k0 <= k <= k1
+              // (k1 - k0 = 1 or 2: one two- or three-dword load), else 0
+              auto crow = [&](int plane, int Y, int k0, int k1, T (&dst)[CPL + 2]) {
+                if constexpr (CBUF)
+                  {
+                    static_assert(CPL == 2, "three-dword rows");
+#pragma unroll
+                    for (int k = 0; k < CPL + 2; ++k)
+                      dst[k] = 0;
+                    const int off = (plane * PSN + (Y - Y0 + 1) * NP + k0) * (int)sizeof(float);
+                    if (k1 - k0 == 2)
+                      {
+                        const auto v = __builtin_amdgcn_raw_buffer_load_b96(crs, cvo, off, 0);
+                        dst[k0] = __uint_as_float(v[0]);
+                        dst[k0 + 1] = __uint_as_float(v[1]);
+                        dst[k0 + 2] = __uint_as_float(v[2]);
+                      }
+                    else
+                      {
+                        const auto v = __builtin_amdgcn_raw_buffer_load_b64(crs, cvo, off, 0);
+                        dst[k0] = __uint_as_float(v[0]);
+                        dst[k0 + 1] = __uint_as_float(v[1]);
+                      }
+                  }
               };
               // couplings of the node row (E, N, NE, NW) and of the row below
               // (pN, pNE, pNW) in a sliding register window (in FP64 this spills
               // a few values, still faster than reloading each node's 8 couplings)
               T w0[CPL + 2], w1[CPL + 2], w2[CPL + 2];
               T pN[CPL + 2], pNE[CPL + 2], pNW[CPL + 2];
-#pragma unroll
-              for (int k = 0; k < CPL + 2; ++k)
+              if constexpr (CBUF)
                 {
-                  w0[k] = gwin[k];
-                  w1[k] = gwin[GP + k];
-                  pN[k] = (k >= 1 && k <= CPL) ? cpl(1, Y0 - 1, k) : (T)0;
-                  pNE[k] = (k <= CPL - 1) ? cpl(2, Y0 - 1, k) : (T)0;
-                  pNW[k] = (k >= 2) ? cpl(3, Y0 - 1, k) : (T)0;
+#pragma unroll
+                  for (int k = 0; k < CPL + 2; ++k)
+                    {
+                      w0[k] = gwin[k];
+                      w1[k] = gwin[GP + k];
+                    }
+                  crow(1, Y0 - 1, 1, CPL, pN);
+                  crow(2, Y0 - 1, 0, CPL - 1, pNE);
+                  crow(3, Y0 - 1, 2, CPL + 1, pNW);
                 }
+              else
+#pragma unroll
+                for (int k = 0; k < CPL + 2; ++k)
+                  {
+                    w0[k] = gwin[k];
+                    w1[k] = gwin[GP + k];
+                    pN[k] = (k >= 1 && k <= CPL) ? cpl(1, Y0 - 1, k) : (T)0;
+                    pNE[k] = (k <= CPL - 1) ? cpl(2, Y0 - 1, k) : (T)0;
+                    pNW[k] = (k >= 2) ? cpl(3, Y0 - 1, k) : (T)0;
+                  }
               T pq[1] = {0};
 #pragma unroll
               for (int i = 0; i < R; ++i)
                 {
                   const int Y = Y0 + i;
                   T E[CPL + 2], N[CPL + 2], NE[CPL + 2], NW[CPL + 2];
-#pragma unroll
-                  for (int k = 0; k < CPL + 2; ++k)
+                  if constexpr (CBUF)
                     {
-                      E[k] = k <= CPL ? cpl(0, Y, k) : (T)0;
-                      N[k] = (k >= 1 && k <= CPL) ? cpl(1, Y, k) : (T)0;
-                      NE[k] = k <= CPL ? cpl(2, Y, k) : (T)0;
-                      NW[k] = k >= 1 ? cpl(3, Y, k) : (T)0;
-                      w2[k] = gwin[(i + 2) * GP + k];
+                      crow(0, Y, 0, CPL, E);
+                      crow(1, Y, 1, CPL, N);
+                      crow(2, Y, 0, CPL, NE);
+                      crow(3, Y, 1, CPL + 1, NW);
+#pragma unroll
+                      for (int k = 0; k < CPL + 2; ++k)
+                        w2[k] = gwin[(i + 2) * GP + k];
                     }
+                  else
+#pragma unroll
+                    for (int k = 0; k < CPL + 2; ++k)
+                      {
+                        E[k] = k <= CPL ? cpl(0, Y, k) : (T)0;
+                        N[k] = (k >= 1 && k <= CPL) ? cpl(1, Y, k) : (T)0;
+                        NE[k] = k <= CPL ? cpl(2, Y, k) : (T)0;
+                        NW[k] = k >= 1 ? cpl(3, Y, k) : (T)0;
+                        w2[k] = gwin[(i + 2) * GP + k];
+                      }
 #pragma unroll
                   for (int c = 0; c < CPL; ++c)
                     {
@@ -951,7 +1180,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     }
                   __builtin_amdgcn_sched_barrier(0);
                 }
-              problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave, slot, pl);
+              problem_sums<T, LW, SW, PPW, 1>(pq, irun, sred2, wave_here(), slot, lane_here());
               v2[0] = 0;
               v2[1] = 0;
               if (irun)
@@ -985,6 +1214,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   if constexpr (CS) // z = D^-1 r + P A_c^-1 P^T r of the new r (uniform branch)
                     {
                       restrict_r(rr);
+                      restrict_ml8(rr);
                       __syncthreads(); // restriction partials complete
                       if constexpr (WAVE_CS)
                         {
@@ -997,29 +1227,46 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                           __syncthreads(); // coarse correction complete
                           interp_cols(ci0, ci1);
                         }
+                      ml8_coarse(u4);
+                      ml8_cols(u4, mv);
 #pragma unroll
                       for (int i = 0; i < R; ++i)
 #pragma unroll
                         for (int c = 0; c < CPL; ++c)
                           {
-                            const T z = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
+                            T z = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
+                            if constexpr (ML8)
+                              z += ml8_at(mv, i, c);
+                            if constexpr (ZLDS) // q of this thread's slot is consumed: keep z there
+                              sq[(i * CPL + c) * NT + tid] = z;
                             v2[0] += rr[i][c] * z;
                             v2[1] += rr[i][c] * rr[i][c];
                           }
                     }
                   ++it_total;
                 }
-              any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave, slot, pl);
+              any = problem_sums<T, LW, SW, PPW, 2>(v2, irun, sred, wave_here(), slot, lane_here());
               if (irun && (v2[1] <= inner_tol2 || it_total >= a.max_iter))
                 irun = false;
               // every lane is past this iteration's stencil (two barriers ago):
               // the own slots of the grid take the next direction in place
               const T beta = irun ? quotient<T>(v2[0], rz) : (T)0;
               rz = v2[0];
+              if constexpr (ZLDS) // z of the new r from this thread's slots (irun: it was formed above)
+                {
+#pragma unroll
+                  for (int i = 0; i < R; ++i)
+#pragma unroll
+                    for (int c = 0; c < CPL; ++c)
+                      gown[i * GP + c] = irun ? sq[(i * CPL + c) * NT + tid] + beta * gown[i * GP + c] : (T)0;
+                  continue;
+                }
               if constexpr (WAVE_CS) // from the kept per-wave coarse sums
                 coarse_wave_interp(cacc, ci0, ci1);
               else if constexpr (CS) // the coarse values are still in LDS (next write: next iteration)
                 interp_cols(ci0, ci1);
+              if constexpr (ML8)
+                ml8_cols(u4, mv);
 #pragma unroll
               for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -1028,6 +1275,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     T z = rr[i][c] * (T)dget(i, c);
                     if constexpr (CS)
                       z += ci0[c] * wya(i) + ci1[c] * wyb(i);
+                    if constexpr (ML8)
+                      z += ml8_at(mv, i, c);
                     gown[i * GP + c] = irun ? z + beta * gown[i * GP + c] : (T)0;
                   }
             }
@@ -1436,19 +1685,21 @@ __device__ inline void setup_element_2d(int coef_kind, const double *cf, int f, 
     fine_element_2d(coef_kind, cf, f, nf, bx, vq, K);
 }
 
-// Intermediate levels of the 64^2 two-level preconditioner: the diagonals of
-// P_s^T K P_s for the bilinear hats of spacing s = 2 (31 x 31 interior coarse
-// nodes) and s = 4 (15 x 15), inverted and stored as FP32, mlinv [cells in
-// class][kMlFloats] = D_2^-1 [L - 1][K - 1], then D_4^-1 [L - 1][K - 1] (node
+// Intermediate levels of the two-level preconditioner: the diagonals of
+// P_s^T K P_s for the bilinear hats of spacing s = s1 = NF / 32 (31 x 31
+// interior coarse nodes) and s = s2 = NF / 16 (15 x 15), i.e. 2 and 4 for the
+// 64^2 tiles, 4 and 8 for 128^2, inverted and stored as FP32, mlinv [cells in
+// class][kMlFloats] = D_s1^-1 [L - 1][K - 1], then D_s2^-1 [L - 1][K - 1] (node
 // (K, L) at fine (s K, s L)).  Each s x s block of fine cells contributes the
 // four diagonal entries w_m^T K_e w_m of its corners m (w_m = the corner's hat
 // at the fine vertices), summed over the block's cells in a fixed order; a
 // node's entry is the sum over its four blocks, in a fixed order.
+template <int NF>
 __global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, const int *cells, int coef_kind,
                                                             const double *box, const double *verts, const double *coef,
                                                             const long long *coef_off, float *mlinv)
 {
-  constexpr int NF = 64, NB2 = NF / 2, NB4 = NF / 4, NBLK = NB2 * NB2 + NB4 * NB4;
+  constexpr int S1 = NF / 32, S2 = NF / 16, NB2 = 32, NB4 = 16, NBLK = NB2 * NB2 + NB4 * NB4;
   __shared__ double sdiag[NBLK][4];
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
@@ -1460,7 +1711,7 @@ __global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, c
   for (int b = t; b < NBLK; b += kSetupThreads)
     {
       const bool l2 = b < NB2 * NB2;
-      const int S = l2 ? 2 : 4, nb = l2 ? NB2 : NB4, bb = l2 ? b : b - NB2 * NB2;
+      const int S = l2 ? S1 : S2, nb = l2 ? NB2 : NB4, bb = l2 ? b : b - NB2 * NB2;
       const int bxi = bb % nb, byi = bb / nb;
       double acc[4] = {0, 0, 0, 0};
       for (int ly = 0; ly < S; ++ly)
@@ -1820,10 +2071,14 @@ hipError_t launch_ml_diag_2d(int n_cells_class, const int *cells, int nf, int co
 {
   if (n_cells_class == 0)
     return hipSuccess;
-  if (nf != 64)
+  if (nf == 64)
+    hipLaunchKernelGGL(ml_diag_2d<64>, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells,
+                       coef_kind, box, verts, coef, coef_off, mlinv);
+  else if (nf == 128)
+    hipLaunchKernelGGL(ml_diag_2d<128>, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells,
+                       coef_kind, box, verts, coef, coef_off, mlinv);
+  else
     return hipErrorInvalidValue;
-  hipLaunchKernelGGL(ml_diag_2d, dim3(n_cells_class), dim3(kSetupThreads), 0, st, n_cells_class, cells, coef_kind, box,
-                     verts, coef, coef_off, mlinv);
   return hipGetLastError();
 }
 

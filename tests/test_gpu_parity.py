@@ -343,6 +343,27 @@ def test_two_level_preconditioner(ctx, kind):
     assert not ctx.uses_coarse_space(2, 32, msfem.default_opts(), case["coef_kind"])
 
 
+def test_intermediate_levels_128(ctx):
+    """128^2 tensor cells, FP32 + refinement (streamed form): the intermediate hat
+    levels of spacing 4 and 8 (coarse_space=1, the default), the two-level form
+    alone (2) and Jacobi (0) reach the oracle's values (1e-10), and the levels take
+    at most 0.75x the two-level form's iterations.  FP64 keeps the two-level form."""
+    say("intermediate levels, tensor 128^2 FP32")
+    case = C.tensor_cells(nc=128, which=np.array([5, 3000, 9001]), n_fine=[128] * 3)
+    ref = C.oracle_build(case)
+    its = {}
+    for cs in (0, 2, 1):
+        out = C.gpu_build(ctx, case, coarse_space=cs, precision=msfem.PREC_FP32_REFINE)
+        check_against_oracle(case, out, ref)
+        its[cs] = out["iters"].mean()
+        assert ctx.uses_coarse_space(2, 128, msfem.default_opts(coarse_space=cs,
+                                                                 precision=msfem.PREC_FP32_REFINE),
+                                     case["coef_kind"]) == {0: 0, 2: 1, 1: 2}[cs]
+    say(f"  mean iterations: Jacobi {its[0]:.1f}, two-level {its[2]:.1f}, with the levels {its[1]:.1f}")
+    assert its[2] < 0.6 * its[0] and its[1] < 0.75 * its[2]
+    assert ctx.uses_coarse_space(2, 128, msfem.default_opts(), case["coef_kind"]) == 1
+
+
 @pytest.mark.parametrize("bc", [msfem.BC_ZERO_DIRICHLET, msfem.BC_REFERENCE])
 def test_coarse_3d_lognormal_full_batch(ctx, bc):
     """The 3D coarse system of the 16^3 x 16^3 lognormal batch
