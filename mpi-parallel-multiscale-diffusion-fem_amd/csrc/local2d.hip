@@ -463,11 +463,10 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr bool OPAQUE_CA = WPE >= 4;
   constexpr bool OPAQUE_W = NF == 128;
   static_assert(!OPAQUE_W || LW == kWave, "128^2 tiles: one problem per wave");
-  // FP32 128^2 tiles (128 registers, all in use): the lane index is read from
-  // the hardware (v_mbcnt) where used instead of kept in a register
-  constexpr bool OPAQUE_L = OPAQUE_W && sizeof(T) == 4;
+  // 128^2 tiles (128 registers, all in use): the lane index is read from the
+  // hardware (v_mbcnt) where used instead of kept in a register
   auto lane_here = [&]() -> int {
-    if constexpr (OPAQUE_L)
+    if constexpr (OPAQUE_W)
       {
         int v;
         asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
@@ -480,7 +479,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // wave index where used)
   auto wave_here = [&]() -> int {
     int w = wave;
-    if constexpr (OPAQUE_L)
+    if constexpr (OPAQUE_W)
       asm volatile("" : "+v"(w));
     return w;
   };
@@ -674,7 +673,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // kept live and spilled)
   auto ml_lane = [&]() {
     int v = lane;
-    if constexpr (OPAQUE_L)
+    if constexpr (OPAQUE_W)
       v = lane_here();
     asm volatile("" : "+v"(v));
     return v;
@@ -1049,33 +1048,44 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
               const gtptr qg = (gtptr)qb;
               typedef double __attribute__((address_space(1))) * gdptr;
               const gdptr ph = (gdptr)pb;
-              // FP32 128^2: the planes are read as one buffer (raw buffer loads):
-              // the lane's window origin is the only vector offset, formed once
-              // per iteration from the hardware lane index and an opaque wave
-              // index; plane, row and column go into the constant offsets
-              constexpr bool CBUF = OPAQUE_L;
+              // 128^2: the planes are read as one buffer (raw buffer loads): the
+              // lane's window origin is the only vector offset, formed once per
+              // iteration from the hardware lane index and an opaque wave index;
+              // plane, row and column go into the constant offsets
+              constexpr bool CBUF = OPAQUE_W;
               __amdgpu_buffer_rsrc_t crs;
               int cvo = 0;
               if constexpr (CBUF)
                 {
-                  crs = __builtin_amdgcn_make_buffer_rsrc((void *)cbase, (short)0, 4 * PSN * (int)sizeof(float),
+                  crs = __builtin_amdgcn_make_buffer_rsrc((void *)cbase, (short)0, 4 * PSN * (int)sizeof(T),
                                                           0x00020000);
-                  cvo = (wave_here() * (R * NP) + lane_here() * CPL) * (int)sizeof(float);
+                  cvo = (wave_here() * (R * NP) + lane_here() * CPL) * (int)sizeof(T);
                 }
+              auto bload = [&](int off) -> T {
+                if constexpr (sizeof(T) == 4)
+                  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(crs, cvo, off, 0));
+                else
+                  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, cvo, off, 0));
+              };
               // couplings at window column k (node X0 - 1 + k) of node row Y
               // (every index lies inside the four padded planes)
               auto cpl = [&](int plane, int Y, int k) -> T {
                 if (plane < NHOIST) // (plane and Y - Y0 are compile-time constants here)
                   return creg[Y - Y0 + 1][HOISTC ? plane : 0][k];
                 if constexpr (CBUF)
-                  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(
-                    crs, cvo, (plane * PSN + (Y - Y0 + 1) * NP + k) * (int)sizeof(float), 0));
+                  return bload((plane * PSN + (Y - Y0 + 1) * NP + k) * (int)sizeof(T));
                 return cp[(unsigned)(plane * PSN + Y * NP + X0 - 1 + k)];
               };
-              // CBUF: dst[k] = coupling at window column k for k0 <= k <= k1
-              // (k1 - k0 = 1 or 2: one two- or three-dword load), else 0
+              // CBUF: dst[k] = coupling at window column k for k0 <= k <= k1, else
+              // 0 (FP32, k1 - k0 = 1 or 2: one two- or three-dword load)
               auto crow = [&](int plane, int Y, int k0, int k1, T (&dst)[CPL + 2]) {
-                if constexpr (CBUF)
+                if constexpr (CBUF && sizeof(T) == 8)
+                  {
+#pragma unroll
+                    for (int k = 0; k < CPL + 2; ++k)
+                      dst[k] = (k >= k0 && k <= k1) ? bload((plane * PSN + (Y - Y0 + 1) * NP + k) * 8) : (T)0;
+                  }
+                else if constexpr (CBUF)
                   {
                     static_assert(CPL == 2, "three-dword rows");
 #pragma unroll
