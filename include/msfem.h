@@ -114,10 +114,10 @@ typedef struct msfem_solver_opts {
   int32_t coarse_space;       /* 1 (default): two-level preconditioner (Jacobi plus the
                                  bilinear coarse space of an 8 x 8 coarse grid per local
                                  problem, solved exactly) for the 64^2 and 128^2 classes at
-                                 the default launch shapes; on 64^2 box cells with per-fine-
-                                 cell scalar or tensor coefficients also the intermediate
-                                 hat levels of spacing 2 and 4, on FP32 128^2 classes those
-                                 of spacing 4 and 8, scaled by their diagonals (additive);
+                                 the default launch shapes; on 64^2 classes also the
+                                 intermediate hat levels of spacing 2 and 4, on FP32 128^2
+                                 classes those of spacing 4 and 8, scaled by their diagonals
+                                 (additive);
                                  2: the two-level form without the
                                  intermediate levels; 0: Jacobi only.  Same converged
                                  values, fewer iterations. */

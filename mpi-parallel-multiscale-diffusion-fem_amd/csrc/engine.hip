@@ -450,12 +450,12 @@ bool class_coarse(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts
 }
 
 // Whether size class nf's two-level solve also takes the intermediate hat
-// levels (coarse_space 1): spacing 2 and 4 on the 64^2 box classes, spacing 4
-// and 8 on the FP32 128^2 classes (streamed form).
+// levels (coarse_space 1): spacing 2 and 4 on the 64^2 classes, spacing 4 and
+// 8 on the FP32 128^2 classes (streamed form).
 bool class_ml(msfem_ctx *ctx, const msfem_batch *in, const msfem_solver_opts *o, int nf)
 {
   const int mode = build_mode(in, nf);
-  const bool fits = (nf == 64 && (mode == kModeScalar || mode == kModeTensor)) ||
+  const bool fits = (nf == 64 && (mode == kModeScalar || mode == kModeTensor || mode == kModeStream)) ||
                     (nf == 128 && mode == kModeStream && o->precision == MSFEM_PREC_FP32_REFINE);
   return o->coarse_space == 1 && fits && class_coarse(ctx, in, o, nf);
 }

@@ -122,12 +122,14 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   constexpr int GR = (SW * R + 2 > NP) ? SW * R + 2 : NP;
   constexpr int GSZ = GP * GR;
   constexpr int NCACHE = MODE == kModeScalar ? 1 : (MODE == kModeTensor ? 4 : 1);
-  // kModeStream keeps q = K p of the own nodes in LDS (thread-private slots)
-  constexpr bool QLDS = MODE == kModeStream;
-  // FP32 two-level streamed form: once r is updated, q is dead and the same
-  // slots keep z = M^-1 r from the (r, z) sums to the direction update
-  // instead of forming the preconditioner's terms twice per iteration
-  constexpr bool ZLDS = QLDS && CS && sizeof(T) == 4;
+  // kModeStream keeps q = K p of the own nodes in LDS (thread-private slots):
+  // FP32, and FP64 in the 64^2 two-level form (the FP64 128^2 grid leaves no
+  // LDS for it: q goes to a global work array there)
+  constexpr bool QLDS = MODE == kModeStream && (sizeof(T) == 4 || (NF == 64 && CS));
+  // two-level streamed form: once r is updated, q is dead and the same slots
+  // keep z = M^-1 r from the (r, z) sums to the direction update instead of
+  // forming the preconditioner's terms twice per iteration
+  constexpr bool ZLDS = QLDS && CS;
   constexpr int NT = kWave * SW;
   constexpr int PSN = coupling_plane_stride(NF); // node-coupling plane stride (kModeStream)
   // kModeStream, 64^2 (8 nodes per lane, 256 registers): the lane's window
@@ -171,7 +173,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
   // with the waves below and above are completed from their published partial
   // rows after the restriction barrier; prolongation by DPP from the coarse
   // values masked to their lanes.
-  constexpr bool ML = CS && NF == 64 && CPL == 1 && SW == 8 && R == 8 && MODE != kModeStream;
+  constexpr bool ML = CS && NF == 64 && CPL == 1 && SW == 8 && R == 8;
   __shared__ T smlp[ML ? 4 * SW * kWave : 1]; // partial rows [B2, T2, B4, T4][wave][lane]
   // The FP32 128^2 streamed form (two node columns per lane) takes the levels
   // of spacing 4 and 8: each wave restricts its rows to the three spacing-4
@@ -969,9 +971,11 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
           T cacc = 0; // per-wave coarse sums of the current z (kept for the p update)
           T u4[3];    // ML8: the intermediate levels' coarse values of the current z (idem)
           T mv[3][2];
+          T mlv[R]; // ML (64^2): the intermediate levels' share of z on the lane's rows
           if constexpr (CS) // irun is uniform (one problem per workgroup): true here
             {
               restrict_r(rr);
+              restrict_ml(rr);
               restrict_ml8(rr);
               __syncthreads(); // restriction partials complete
               if constexpr (WAVE_CS)
@@ -987,6 +991,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                 }
               ml8_coarse(u4);
               ml8_cols(u4, mv);
+              ml_terms(rr, mlv);
             }
           T v2[2] = {0, 0}; // (r, z), (r, r)
 #pragma unroll
@@ -999,6 +1004,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   z += ci0[c] * wya(i) + ci1[c] * wyb(i);
                 if constexpr (ML8)
                   z += ml8_at(mv, i, c);
+                if constexpr (ML)
+                  z += mlv[i];
                 gown[i * GP + c] = irun ? z : (T)0;
                 v2[0] += rr[i][c] * z;
                 v2[1] += rr[i][c] * rr[i][c];
@@ -1173,7 +1180,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                       y += NW[c + 1] * (pC - w2[c]);
                       y += pNW[c + 2] * (pC - w0[c + 2]);
                       const T q = dget(i, c) != 0.0f ? y : (T)0;
-                      if constexpr (sizeof(T) == 4)
+                      if constexpr (QLDS)
                         sq[(i * CPL + c) * NT + tid] = q;
                       else
                         qg[(unsigned)((i * CPL + c) * NT + tid)] = q;
@@ -1202,17 +1209,14 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                     for (int c = 0; c < CPL; ++c)
                       {
                         T q;
-                        if constexpr (sizeof(T) == 4)
-                          {
-                            q = sq[(i * CPL + c) * NT + tid];
-                            xr[i][c] += alpha * gown[i * GP + c];
-                          }
+                        if constexpr (QLDS)
+                          q = sq[(i * CPL + c) * NT + tid];
                         else
-                          {
-                            q = qg[(unsigned)((i * CPL + c) * NT + tid)];
-                            if (dget(i, c) != 0.0f)
-                              ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
-                          }
+                          q = qg[(unsigned)((i * CPL + c) * NT + tid)];
+                        if constexpr (sizeof(T) == 4)
+                          xr[i][c] += alpha * gown[i * GP + c];
+                        else if (dget(i, c) != 0.0f)
+                          ph[(unsigned)((Y0 + i) * NP + X0 + c)] += alpha * gown[i * GP + c];
                         rr[i][c] -= alpha * q;
                         if constexpr (!CS)
                           {
@@ -1224,6 +1228,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                   if constexpr (CS) // z = D^-1 r + P A_c^-1 P^T r of the new r (uniform branch)
                     {
                       restrict_r(rr);
+                      restrict_ml(rr);
                       restrict_ml8(rr);
                       __syncthreads(); // restriction partials complete
                       if constexpr (WAVE_CS)
@@ -1239,6 +1244,7 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                         }
                       ml8_coarse(u4);
                       ml8_cols(u4, mv);
+                      ml_terms(rr, mlv);
 #pragma unroll
                       for (int i = 0; i < R; ++i)
 #pragma unroll
@@ -1247,6 +1253,8 @@ __global__ void __launch_bounds__(kWave *SW, WPE) local_solve_2d(LocalArgs a)
                             T z = rr[i][c] * (T)dget(i, c) + (ci0[c] * wya(i) + ci1[c] * wyb(i));
                             if constexpr (ML8)
                               z += ml8_at(mv, i, c);
+                            if constexpr (ML)
+                              z += mlv[i];
                             if constexpr (ZLDS) // q of this thread's slot is consumed: keep z there
                               sq[(i * CPL + c) * NT + tid] = z;
                             v2[0] += rr[i][c] * z;

@@ -77,6 +77,29 @@ def test_quad_tensor_sinsin_both_precisions(ctx, nf):
     assert (out["status"] == 0).all() and err < 1e-6
 
 
+def test_quad_intermediate_levels(ctx):
+    """64^2 general cells (streamed form): the intermediate hat levels of spacing 2
+    and 4 (coarse_space=1, the default), the two-level form alone (2) and Jacobi (0)
+    reach the oracle's values (FP64 1e-10; FP32 + refinement at 1e-8 within 1e-6
+    energy norm), and the levels take at most 0.75x the two-level form's FP64
+    iterations."""
+    say("quad intermediate levels nf=64")
+    case = C.quad_cells(nc=8, nf=64, which=[9, 27, 45])
+    ref = C.oracle_build(case)
+    its = {}
+    for cs in (0, 2, 1):
+        out = C.gpu_build(ctx, case, coarse_space=cs)
+        check(out, ref)
+        its[cs] = out["iters"].mean()
+        for prec in (msfem.PREC_FP64, msfem.PREC_FP32_REFINE):
+            o = msfem.default_opts(coarse_space=cs, precision=prec)
+            assert ctx.uses_coarse_space(2, 64, o, case["coef_kind"], True) == {0: 0, 2: 1, 1: 2}[cs]
+        out = C.gpu_build(ctx, case, coarse_space=cs, precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+        assert (out["status"] == 0).all() and C.energy_error(case, out["bases"], ref["bases"]) < 1e-6
+    say(f"  mean FP64 iterations: Jacobi {its[0]:.1f}, two-level {its[2]:.1f}, with the levels {its[1]:.1f}")
+    assert its[2] < 0.6 * its[0] and its[1] < 0.75 * its[2]
+
+
 def test_quad_uneven_batch_graph_and_pou(ctx):
     """Mixed resolutions in one batch; graph replay and partition of unity agree with the oracle."""
     say("quad uneven batch, graph and partition of unity")
