@@ -1017,6 +1017,116 @@ __global__ void __launch_bounds__(256) head_bases_kernel(int Cin, int B, int nf,
     }
 }
 
+// The same head from BF16 activations, four nodes per thread: nodes X = 4 p + 1
+// .. 4 p + 4 of row Y (the thread of p = 0 also writes the boundary node X =
+// 0).  Their windows span the pixel columns 4 p .. 4 p + 4 of rows Y - 1 and
+// Y: per channel one 8-byte load (columns 4 p .. 4 p + 3) and one 2-byte load
+// (column 4 p + 4) per row, shared by the four nodes, and one LDS read of the
+// four outputs' weights per tap.  Every node's sums are formed in the order of
+// head_bases_kernel (channel, then tap, ascending; fmaf), so the values are
+// bitwise the same.  Needs nf % 4 == 0, Cin % CU == 0 and 8-byte aligned
+// activations.
+template <int CU>
+__global__ void __launch_bounds__(256) head_bases4_kernel(int Cin, int B, int nf, const uint16_t *in,
+                                                          const uint16_t *w, const float *bias, int relu,
+                                                          float scale, double *bases)
+{
+  constexpr int CO = 4;
+  extern __shared__ float swk[]; // [k][CO]
+  const int K = Cin * 4;
+  for (int t = threadIdx.x; t < K * CO; t += blockDim.x)
+    swk[t] = bf16_to_f32(w[(size_t)(t % CO) * K + t / CO]);
+  __syncthreads();
+  const int np = nf + 1, nq = nf / 4;
+  const size_t per = (size_t)np * np, total = (size_t)B * np * nq, plane = (size_t)B * nf * nf;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total; t += (size_t)gridDim.x * blockDim.x)
+    {
+      const int p = (int)(t % nq);
+      const size_t row = t / nq;
+      const int Y = (int)(row % np), b = (int)(row / np), X0 = 4 * p + 1;
+      const bool yin = Y > 0 && Y < nf, last = p == nq - 1; // last: node X0 + 3 = nf is on the boundary
+      float acc[4][CO];
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+#pragma unroll
+        for (int c = 0; c < CO; ++c)
+          acc[n][c] = 0.0f;
+      if (yin)
+        {
+          const size_t o0 = (size_t)b * nf * nf + (size_t)(Y - 1) * nf + 4 * p; // pixel (Y - 1, 4 p)
+          for (int ci0 = 0; ci0 < Cin; ci0 += CU)
+            {
+              uint2 lo[CU][2];
+              uint16_t hi[CU][2];
+#pragma unroll
+              for (int u = 0; u < CU; ++u)
+#pragma unroll
+                for (int ry = 0; ry < 2; ++ry)
+                  {
+                    const uint16_t *src = in + o0 + (size_t)(ci0 + u) * plane + (size_t)ry * nf;
+                    lo[u][ry] = *(const uint2 *)src;
+                    hi[u][ry] = last ? (uint16_t)0 : src[4]; // (column nf does not exist)
+                  }
+#pragma unroll
+              for (int u = 0; u < CU; ++u)
+                {
+                  float v[2][5]; // [row Y - 1 + ry][column 4 p + j]
+#pragma unroll
+                  for (int ry = 0; ry < 2; ++ry)
+                    {
+                      v[ry][0] = __uint_as_float(lo[u][ry].x << 16);
+                      v[ry][1] = __uint_as_float(lo[u][ry].x & 0xffff0000u);
+                      v[ry][2] = __uint_as_float(lo[u][ry].y << 16);
+                      v[ry][3] = __uint_as_float(lo[u][ry].y & 0xffff0000u);
+                      v[ry][4] = bf16_to_f32(hi[u][ry]);
+                    }
+#pragma unroll
+                  for (int q = 0; q < 4; ++q)
+                    {
+                      const float4 wk = *(const float4 *)(swk + ((ci0 + u) * 4 + q) * CO);
+#pragma unroll
+                      for (int n = 0; n < 4; ++n)
+                        {
+                          const float val = v[q >> 1][n + (q & 1)];
+                          acc[n][0] = fmaf(val, wk.x, acc[n][0]);
+                          acc[n][1] = fmaf(val, wk.y, acc[n][1]);
+                          acc[n][2] = fmaf(val, wk.z, acc[n][2]);
+                          acc[n][3] = fmaf(val, wk.w, acc[n][3]);
+                        }
+                    }
+                }
+            }
+        }
+#pragma unroll
+      for (int n = 0; n < 4; ++n)
+        {
+          const int X = X0 + n;
+          const bool inner = yin && X < nf;
+#pragma unroll
+          for (int c = 0; c < CO; ++c)
+            {
+              double v = nodal_q1(c, 2, X, Y, 0, nf);
+              if (inner)
+                {
+                  float r = acc[n][c];
+                  if (bias)
+                    {
+                      r += bias[c];
+                      if (relu && r < 0.0f)
+                        r = 0.0f;
+                    }
+                  v += (double)(scale * r);
+                }
+              bases[((size_t)b * CO + c) * per + (size_t)Y * np + X] = v;
+            }
+        }
+      if (p == 0)
+#pragma unroll
+        for (int c = 0; c < CO; ++c)
+          bases[((size_t)b * CO + c) * per + (size_t)Y * np] = nodal_q1(c, 2, 0, Y, 0, nf);
+    }
+}
+
 // wp[co][blk][tap][c] = w[co][(32 blk + c) 9 + tap]: the weights of a 3x3
 // convolution regrouped per 32-channel block and tap (conv3x3_bf16_kernel).
 __global__ void conv3x3_pack_kernel(int Cout, int Cin, const uint16_t *w, uint16_t *wp)
@@ -1643,7 +1753,10 @@ int msfem_nn_head_bases(int Cin, int B, int nf, const void *w, const void *in, i
     return MSFEM_ERR_ARG;
   const size_t total = (size_t)B * (nf + 1) * (nf + 1), lds = (size_t)Cin * 4 * 4 * sizeof(float);
   hipStream_t st = (hipStream_t)stream;
-  if (in_bf16)
+  if (in_bf16 && nf % 4 == 0 && Cin % 8 == 0 && ((uintptr_t)in & 7) == 0)
+    hipLaunchKernelGGL(head_bases4_kernel<8>, dim3(grid_for((size_t)B * (nf + 1) * (nf / 4))), dim3(256), lds, st,
+                       Cin, B, nf, (const uint16_t *)in, (const uint16_t *)w, bias, relu, scale, bases);
+  else if (in_bf16)
     hipLaunchKernelGGL(head_bases_kernel<true>, dim3(grid_for(total)), dim3(256), lds, st, Cin, B, nf, in,
                        (const uint16_t *)w, bias, relu, scale, bases);
   else
