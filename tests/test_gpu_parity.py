@@ -149,6 +149,25 @@ def test_reference_default_cells(ctx):
     check_against_oracle(case, out, ref)
 
 
+def test_reference_default_cells_fp32_levels(ctx):
+    """The reference default cells (MatrixCoeff at the quadrature points, 128^2)
+    with FP32 + refinement: the intermediate hat levels of spacing 4 and 8 (the
+    default) and the two-level form alone reach the oracle's values within 1e-6
+    relative energy norm at rel. residual 1e-8, the levels in at most 0.75x the
+    iterations."""
+    say("reference default nf128, fp32 with and without the levels")
+    case = C.reference_cells(which=np.array([0, 27, 45]))
+    ref = C.oracle_build(case)
+    its = {}
+    for cs in (2, 1):
+        out = C.gpu_build(ctx, case, coarse_space=cs, precision=msfem.PREC_FP32_REFINE, rel_tol=1e-8)
+        err = C.energy_error(case, out["bases"], ref["bases"])
+        say(f"  coarse_space={cs}: energy error {err:.2e}, mean iterations {out['iters'].mean():.1f}")
+        assert (out["status"] == 0).all() and err < 1e-6
+        its[cs] = out["iters"].mean()
+    assert its[1] < 0.75 * its[2]
+
+
 def test_partition_of_unity_option(ctx):
     say("partition of unity")
     case = C.lognormal_cells(which=np.array([5, 6, 7]))
