@@ -39,7 +39,7 @@ __device__ __forceinline__ int sym_index(int dim, int i, int j)
 // matrices), which keeps the common instance's registers (and so its occupancy)
 // free of the quadrature-point path.
 template <int DIM, bool QUAD, bool QPK = true>
-__global__ void __launch_bounds__(kElemThreads) element_kernel(ElementArgs a)
+__global__ void __launch_bounds__(kElemThreads, DIM == 2 ? 2 : 1) element_kernel(ElementArgs a)
 {
   constexpr int NV = 1 << DIM, NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
   constexpr int NVAL = NV * NV + NV;

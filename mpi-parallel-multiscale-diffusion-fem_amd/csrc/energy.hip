@@ -118,32 +118,48 @@ __device__ __forceinline__ double jint1(int bit, int j, int nf)
   return 0.5 * (b * b - a * a);
 }
 
-// Fixed-order block reduction (sum or min) of kEnergyThreads values.
-__device__ __forceinline__ double block_reduce(double v, double *red, bool is_min)
+// Fixed-order block reduction of NV values at once over kEnergyThreads
+// threads (values 0 and 1 by minimum, the others by sum), one barrier per
+// tree level.
+template <int NV>
+__device__ __forceinline__ void block_reduce_n(double (&v)[NV], double (*red)[kEnergyThreads])
 {
   const int tid = threadIdx.x;
-  red[tid] = v;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    red[k][tid] = v[k];
   __syncthreads();
   for (int s = kEnergyThreads / 2; s > 0; s >>= 1)
     {
       if (tid < s)
-        red[tid] = is_min ? fmin(red[tid], red[tid + s]) : red[tid] + red[tid + s];
+#pragma unroll
+        for (int k = 0; k < NV; ++k)
+          red[k][tid] = k < 2 ? fmin(red[k][tid], red[k][tid + s]) : red[k][tid] + red[k][tid + s];
       __syncthreads();
     }
-  const double r = red[0];
-  __syncthreads();
-  return r;
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    v[k] = red[k][0];
 }
 
 // One workgroup per cell of a size class: lam_lb[cell] and, when tau > 0,
 // ptol[cell][i] = tau sqrt(lam_lb e_lb_i).
+// (resolutions up to kEnergyMaxNF per side tabulate the interval integrals
+// and the mode cosines once per cell instead of per fine cell and mode: the
+// divisions and cospi calls were most of the kernel's time)
+constexpr int kEnergyMaxNF = 1024;
+
 template <int DIM>
 __global__ void __launch_bounds__(kEnergyThreads)
   energy_terms_kernel(int n_cells_class, const int *cells, int nf, int kind, const double *box, const double *coef,
                       const long long *coef_off, double tau, double *lam_lb, double *ptol)
 {
   constexpr int NB = 1 << DIM;
-  __shared__ double red[kEnergyThreads];
+  __shared__ double redn[2 + NB][kEnergyThreads];
+  // dynamic: [jint2 bit 0, jint2 bit 1, jint1 bit 0, jint1 bit 1][nf], then
+  // the mode factors k_a(j), m_a(j) per axis [DIM][2][nf] (j = 1 .. nf - 1);
+  // none beyond kEnergyMaxNF
+  extern __shared__ double stab[];
   const int k = blockIdx.x;
   if (k >= n_cells_class)
     return;
@@ -159,6 +175,43 @@ __global__ void __launch_bounds__(kEnergyThreads)
   int nfd = nf * nf;
   if (DIM == 3)
     nfd *= nf;
+  const bool tab = nf <= kEnergyMaxNF; // (uniform)
+  double *skm = stab + 4 * nf;
+  // mode factors along axis a: k = (2 - 2 cos t) / h_a, m = h_a (4 + 2 cos t) / 6, t = j pi / nf
+  auto ha = [&](int a) -> double { return a == 0 ? h[0] : a == 1 ? h[1] : h[2]; }; // (no indexed array)
+  auto kfac = [&](int a, int j) -> double { return (2.0 - 2.0 * cospi((double)j / nf)) / ha(a); };
+  auto mfac = [&](int a, int j) -> double { return ha(a) * (4.0 + 2.0 * cospi((double)j / nf)) / 6.0; };
+  if (tab)
+    {
+      for (int t = threadIdx.x; t < 4 * nf; t += kEnergyThreads)
+        {
+          const int w = t / nf, j = t % nf;
+          stab[t] = w < 2 ? jint2(w, j, nf) : jint1(w - 2, j, nf);
+        }
+      for (int t = threadIdx.x; t < DIM * 2 * nf; t += kEnergyThreads)
+        {
+          const int a = t / (2 * nf), km = t / nf % 2, j = t % nf;
+          skm[t] = j == 0 ? 0.0 : km == 0 ? kfac(a, j) : mfac(a, j);
+        }
+      __syncthreads();
+    }
+  auto J2 = [&](int bit, int j) -> double { return tab ? stab[bit * nf + j] : jint2(bit, j, nf); };
+  auto J1 = [&](int bit, int j) -> double { return tab ? stab[(2 + bit) * nf + j] : jint1(bit, j, nf); };
+  // the constant factors of G (2D: gxx = H_y / (H_x nf) J2_y, gyy = H_x / (H_y nf) J2_x;
+  // 3D: gxx = H_y H_z / (H_x nf) J2_y J2_z, ...)
+  double cg[3];
+  if (DIM == 2)
+    {
+      cg[0] = H[1] / (H[0] * nf);
+      cg[1] = H[0] / (H[1] * nf);
+      cg[2] = 0;
+    }
+  else
+    {
+      cg[0] = H[1] * H[2] / (H[0] * nf);
+      cg[1] = H[0] * H[2] / (H[1] * nf);
+      cg[2] = H[0] * H[1] / (H[2] * nf);
+    }
   double lam = 1e300, D[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i)
@@ -172,28 +225,29 @@ __global__ void __launch_bounds__(kEnergyThreads)
       for (int i = 0; i < NB; ++i)
         {
           int bit[3];
-          double s[3], J2[3], J1[3];
+          double s[3], j2[3], j1[3];
+#pragma unroll
           for (int a = 0; a < DIM; ++a)
             {
               bit[a] = (i >> a) & 1;
               s[a] = bit[a] ? 1.0 : -1.0;
-              J2[a] = jint2(bit[a], j[a], nf);
-              J1[a] = jint1(bit[a], j[a], nf);
+              j2[a] = J2(bit[a], j[a]);
+              j1[a] = J1(bit[a], j[a]);
             }
           // G[k][m] = int over the fine cell of d_k N_i d_m N_i
           double g = 0;
           if (DIM == 2)
             {
-              const double gxx = H[1] * J2[1] / (H[0] * nf), gyy = H[0] * J2[0] / (H[1] * nf);
-              const double gxy = s[0] * s[1] * J1[0] * J1[1];
+              const double gxx = cg[0] * j2[1], gyy = cg[1] * j2[0];
+              const double gxy = s[0] * s[1] * j1[0] * j1[1];
               g = B[0] * gxx + 2.0 * B[1] * gxy + B[2] * gyy;
             }
           else
             {
               // scalar coefficients only in 3D: A^-1 = I / lam
-              const double gxx = H[1] * H[2] * J2[1] * J2[2] / (H[0] * nf);
-              const double gyy = H[0] * H[2] * J2[0] * J2[2] / (H[1] * nf);
-              const double gzz = H[0] * H[1] * J2[0] * J2[1] / (H[2] * nf);
+              const double gxx = cg[0] * j2[1] * j2[2];
+              const double gyy = cg[1] * j2[0] * j2[2];
+              const double gzz = cg[2] * j2[0] * j2[1];
               g = B[0] * (gxx + gyy + gzz);
             }
           D[i] += g;
@@ -207,11 +261,11 @@ __global__ void __launch_bounds__(kEnergyThreads)
     {
       const int jm[3] = {q % nm + 1, (q / nm) % nm + 1, DIM == 3 ? q / (nm * nm) + 1 : 1};
       double kk[3], mm[3];
+#pragma unroll
       for (int a = 0; a < DIM; ++a)
         {
-          const double c = cospi((double)jm[a] / nf);
-          kk[a] = (2.0 - 2.0 * c) / h[a];
-          mm[a] = h[a] * (4.0 + 2.0 * c) / 6.0;
+          kk[a] = tab ? skm[(2 * a) * nf + jm[a]] : kfac(a, jm[a]);
+          mm[a] = tab ? skm[(2 * a + 1) * nf + jm[a]] : mfac(a, jm[a]);
         }
       double v;
       if (DIM == 2)
@@ -220,12 +274,18 @@ __global__ void __launch_bounds__(kEnergyThreads)
         v = kk[0] * mm[1] * mm[2] + mm[0] * kk[1] * mm[2] + mm[0] * mm[1] * kk[2];
       mu = fmin(mu, v);
     }
-  lam = block_reduce(lam, red, true);
-  mu = block_reduce(mu, red, true);
+  // one fixed-order tree over the workgroup for all 2 + NB values (min, min, sums)
+  double vals[2 + NB] = {lam, mu};
+#pragma unroll
+  for (int i = 0; i < NB; ++i)
+    vals[2 + i] = D[i];
+  block_reduce_n<2 + NB>(vals, redn);
+  lam = vals[0];
+  mu = vals[1];
   double Dsum[NB];
 #pragma unroll
   for (int i = 0; i < NB; ++i)
-    Dsum[i] = block_reduce(D[i], red, false);
+    Dsum[i] = vals[2 + i];
   const double llb = lam * mu;
   if (threadIdx.x == 0)
     {
@@ -315,11 +375,12 @@ hipError_t launch_energy_terms(int dim, int n_cells_class, const int *cells, int
 {
   if (n_cells_class == 0)
     return hipSuccess;
+  const size_t lds = nf <= kEnergyMaxNF ? (size_t)(4 + 2 * dim) * nf * sizeof(double) : 0;
   if (dim == 2)
-    hipLaunchKernelGGL(energy_terms_kernel<2>, dim3(n_cells_class), dim3(kEnergyThreads), 0, st, n_cells_class,
+    hipLaunchKernelGGL(energy_terms_kernel<2>, dim3(n_cells_class), dim3(kEnergyThreads), lds, st, n_cells_class,
                        cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
   else
-    hipLaunchKernelGGL(energy_terms_kernel<3>, dim3(n_cells_class), dim3(kEnergyThreads), 0, st, n_cells_class,
+    hipLaunchKernelGGL(energy_terms_kernel<3>, dim3(n_cells_class), dim3(kEnergyThreads), lds, st, n_cells_class,
                        cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
   return hipGetLastError();
 }

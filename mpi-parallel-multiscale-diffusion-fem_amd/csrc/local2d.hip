@@ -34,6 +34,8 @@
 // the inner CG in type T solves K d = r (T = double: to the final tolerance,
 // normally one cycle; T = float: to inner_rel_tol, refined in FP64 until
 // converged).
+#include <utility>
+
 #include "msfem_device.hpp"
 #include "msfem_kernels.hpp"
 
@@ -1666,9 +1668,8 @@ __global__ void quad_dirichlet_2d(int n_cells_class, const int *cells, int nf, c
 }
 
 // Two-level setup: A_c = P^T K P over the 49 interior nodes of the 8 x 8
-// coarse grid of an NF^2 tile (coarse spacing NF / 8), then A_c^-1
-// (Gauss-Jordan; A_c is SPD, no pivoting), one workgroup of kSetupThreads per
-// cell.  K_c of every fine cell is its QGauss(2) element matrix (box cells or
+// coarse grid of an NF^2 tile (coarse spacing NF / 8), one workgroup of
+// kSetupThreads per cell, then A_c^-1 in place (coarse_invert_2d).  K_c of every fine cell is its QGauss(2) element matrix (box cells or
 // general quadrilaterals); each fine cell lies in one coarse block, whose 4 x 4
 // block matrix sum_c W^T K_c W (W: bilinear weights of the fine cell's
 // vertices to the block corners, in the tile's index space) is summed by 4
@@ -1719,6 +1720,11 @@ __global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, c
 {
   constexpr int S1 = NF / 32, S2 = NF / 16, NB2 = 32, NB4 = 16, NBLK = NB2 * NB2 + NB4 * NB4;
   __shared__ double sdiag[NBLK][4];
+  // box cells with per-fine-cell scalar or tensor coefficients: K_e = ka Kxx +
+  // kc Kyy + kb Kxy (setup_element_2d), so w_m^T K_e w_m = ka gxx + kc gyy + kb
+  // gxy with g the same for every cell: tabulated per level, fine position in
+  // the block and corner, [level][component][ly S + lx][m]
+  __shared__ double sg[2][3][S2 * S2][4];
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
     return;
@@ -1726,31 +1732,82 @@ __global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, c
   const double *bx = verts ? nullptr : box + (size_t)cell * 4;
   const double *vq = verts ? verts + (size_t)cell * 8 : nullptr;
   const double *cf = coef + coef_off[cell];
+  const bool fast = !vq && coef_kind <= 1; // (uniform)
+  if (fast)
+    {
+      for (int e = t; e < 2 * 3 * S2 * S2 * 4; e += kSetupThreads)
+        {
+          const int m = e % 4, pos = e / 4 % (S2 * S2), comp = e / (4 * S2 * S2) % 3, lev = e / (12 * S2 * S2);
+          const int S = lev ? S2 : S1, lx = pos % S2, ly = pos / S2;
+          double g = 0;
+          if (lx < S && ly < S)
+            {
+              double w[4];
+              for (int v = 0; v < 4; ++v)
+                {
+                  const double px = (double)(lx + (v & 1)) / S, py = (double)(ly + (v >> 1)) / S;
+                  w[v] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
+                }
+              for (int u = 0; u < 4; ++u)
+                for (int v = 0; v < 4; ++v)
+                  {
+                    const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
+                    const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
+                    const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
+                    const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
+                    const double k = comp == 0 ? sx * ny : comp == 1 ? sy * nx : su * tv + tu * sv;
+                    g += w[u] * k * w[v];
+                  }
+            }
+          sg[lev][comp][ly * S2 + lx][m] = g;
+        }
+      __syncthreads();
+    }
+  const double rho = fast ? (bx[3] - bx[2]) / (bx[1] - bx[0]) : 1.0;
   for (int b = t; b < NBLK; b += kSetupThreads)
     {
       const bool l2 = b < NB2 * NB2;
       const int S = l2 ? S1 : S2, nb = l2 ? NB2 : NB4, bb = l2 ? b : b - NB2 * NB2;
       const int bxi = bb % nb, byi = bb / nb;
       double acc[4] = {0, 0, 0, 0};
-      for (int ly = 0; ly < S; ++ly)
-        for (int lx = 0; lx < S; ++lx)
-          {
-            double K[4][4];
-            setup_element_2d(coef_kind, cf, (byi * S + ly) * NF + bxi * S + lx, NF, bx, vq, K);
-            for (int m = 0; m < 4; ++m)
+      if (fast)
+        {
+          const int lev = l2 ? 0 : 1;
+          for (int ly = 0; ly < S; ++ly)
+            for (int lx = 0; lx < S; ++lx)
               {
-                double w[4]; // corner m's hat at the fine cell's vertices
-                for (int v = 0; v < 4; ++v)
-                  {
-                    const double px = (double)(lx + (v & 1)) / S, py = (double)(ly + (v >> 1)) / S;
-                    w[v] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
-                  }
-                double q = 0;
-                for (int u = 0; u < 4; ++u)
-                  q += w[u] * (((K[u][0] * w[0] + K[u][1] * w[1]) + K[u][2] * w[2]) + K[u][3] * w[3]);
-                acc[m] += q;
+                const int f = (byi * S + ly) * NF + bxi * S + lx;
+                const double a = coef_kind == 0 ? cf[f] : cf[3 * f], bq = coef_kind == 0 ? 0.0 : cf[3 * f + 1];
+                const double c = coef_kind == 0 ? cf[f] : cf[3 * f + 2];
+                const double ka = a * rho * (1.0 / 6.0), kc = c / rho * (1.0 / 6.0), kb = 0.25 * bq;
+                const double *g0 = sg[lev][0][ly * S2 + lx], *g1 = sg[lev][1][ly * S2 + lx],
+                             *g2 = sg[lev][2][ly * S2 + lx];
+                for (int m = 0; m < 4; ++m)
+                  acc[m] += (ka * g0[m] + kc * g1[m]) + kb * g2[m];
               }
-          }
+        }
+      else
+        {
+        for (int ly = 0; ly < S; ++ly)
+          for (int lx = 0; lx < S; ++lx)
+            {
+              double K[4][4];
+              setup_element_2d(coef_kind, cf, (byi * S + ly) * NF + bxi * S + lx, NF, bx, vq, K);
+              for (int m = 0; m < 4; ++m)
+                {
+                  double w[4]; // corner m's hat at the fine cell's vertices
+                  for (int v = 0; v < 4; ++v)
+                    {
+                      const double px = (double)(lx + (v & 1)) / S, py = (double)(ly + (v >> 1)) / S;
+                      w[v] = ((m & 1) ? px : 1.0 - px) * ((m & 2) ? py : 1.0 - py);
+                    }
+                  double q = 0;
+                  for (int u = 0; u < 4; ++u)
+                    q += w[u] * (((K[u][0] * w[0] + K[u][1] * w[1]) + K[u][2] * w[2]) + K[u][3] * w[3]);
+                  acc[m] += q;
+                }
+            }
+        }
       for (int m = 0; m < 4; ++m)
         sdiag[b][m] = acc[m];
     }
@@ -1769,6 +1826,52 @@ __global__ void __launch_bounds__(kSetupThreads) ml_diag_2d(int n_cells_class, c
     }
 }
 
+// One Gauss-Jordan pivot step of the register-resident inversion in
+// coarse_setup_2d (lane i holds row i; K is a compile-time constant, so every
+// register index is): the owner of row K publishes it in rb, every lane reads
+// it back, then a_ij -= (a_iK p) a_Kj, a_iK = -a_iK p (i != K), a_Kj *= p,
+// a_KK = p with p = 1 / a_KK (to within an ulp).
+template <int K, int N>
+__device__ __forceinline__ void gj_pivot(double (&a)[N], int i, double (*srow)[N + 1])
+{
+  double *rb = srow[K & 1];
+  if (i == K)
+    {
+#pragma unroll
+      for (int j = 0; j + 1 < N; j += 2)
+        *(double2 *)&rb[j] = make_double2(a[j], a[j + 1]);
+      if (N % 2)
+        rb[N - 1] = a[N - 1];
+    }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // p = 1 / a_KK: the hardware estimate refined by two Newton steps (a_KK > 0
+  // and normal: A_c is SPD), shorter than an IEEE division on the pivot chain
+  const double p = reciprocal<double>(rb[K]);
+  // a_ij <- sc a_ij - f a_Kj with (sc, f) = (1, a_iK p), and (p, 0) on row K
+  // itself (where a_Kj is the lane's own value): one multiply and one FMA per
+  // entry, no select
+  const bool piv = i == K;
+  const double f = piv ? 0.0 : a[K] * p, sc = piv ? p : 1.0;
+#pragma unroll
+  for (int j = 0; j + 1 < N; j += 2)
+    {
+      const double2 v = *(const double2 *)&rb[j];
+      a[j] = fma(-f, v.x, sc * a[j]);
+      a[j + 1] = fma(-f, v.y, sc * a[j + 1]);
+    }
+  if (N % 2)
+    a[N - 1] = fma(-f, rb[N - 1], sc * a[N - 1]);
+  a[K] = piv ? p : -f;
+}
+
+template <int N, int... K>
+__device__ __forceinline__ void gj_all(double (&a)[N], int i, double (*srow)[N + 1], std::integer_sequence<int, K...>)
+{
+  (gj_pivot<K, N>(a, i, srow), ...);
+}
+
 template <int NF>
 __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_class, const int *cells, int coef_kind,
                                                                 const double *box, const double *verts,
@@ -1780,7 +1883,6 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
   constexpr int SHN = NB * NB * 4 * 16 > N * N ? NB * NB * 4 * 16 : N * N;
   __shared__ double sh[SHN];           // block partials [64][4][16], then A_c, inverted in place
   __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
-  __shared__ double fac[2][N], rowk[2][N]; // Gauss-Jordan: column k, row k (double-buffered)
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
     return;
@@ -1840,57 +1942,42 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
       sh[e] = v;
     }
   __syncthreads();
-  // Gauss-Jordan in place (A_c is SPD: no pivoting): with p = 1 / a_kk,
-  // a_ij -= a_ik p a_kj (i, j != k), a_kj *= p, a_ik *= -p, a_kk = p.  Thread t
-  // keeps entries e = t + kSetupThreads m in registers; per pivot the owners of
-  // column k and row k publish them into a double-buffered pair (one barrier
-  // per pivot: buffer k & 1 is rewritten at pivot k + 2, after every thread
-  // has passed barrier k + 1, i.e. finished reading it at pivot k).
-  constexpr int EM = (N * N + kSetupThreads - 1) / kSetupThreads;
-  double v[EM];
-#pragma unroll
-  for (int m = 0; m < EM; ++m)
-    {
-      const int e = t + kSetupThreads * m;
-      v[m] = e < N * N ? sh[e] : 0.0;
-    }
-  for (int k = 0; k < N; ++k)
-    {
-      double *fk = fac[k & 1], *rk = rowk[k & 1];
-#pragma unroll
-      for (int m = 0; m < EM; ++m)
-        {
-          const int e = t + kSetupThreads * m, i = e / N, j = e % N;
-          if (e < N * N && j == k)
-            fk[i] = v[m];
-          if (e < N * N && i == k)
-            rk[j] = v[m];
-        }
-      __syncthreads();
-      const double p = 1.0 / fk[k]; // one division per pivot, not per entry
-#pragma unroll
-      for (int m = 0; m < EM; ++m)
-        {
-          const int e = t + kSetupThreads * m, i = e / N, j = e % N;
-          if (e < N * N)
-            {
-              if (i == k)
-                v[m] = j == k ? p : rk[j] * p;
-              else if (j == k)
-                v[m] = -(fk[i] * p);
-              else
-                v[m] = v[m] - (fk[i] * p) * rk[j];
-            }
-        }
-    }
   double *out = cinv + (size_t)cc * N * N;
+  for (int e = t; e < N * N; e += kSetupThreads)
+    out[e] = sh[e];
+}
+
+// In-place inversion of the coarse matrices written by coarse_setup_2d, one
+// wave per cell: Gauss-Jordan without pivoting (A_c is SPD) with p = 1 / a_kk,
+// a_ij -= (a_ik p) a_kj (i, j != k), a_kj *= p, a_ik *= -p, a_kk = p.  Lane i
+// keeps row i in registers (every index a compile-time constant: the pivots
+// are unrolled by gj_all); per pivot the owner of row k publishes it in the
+// wave's LDS slots and every lane reads it back (the wave's LDS operations run
+// in order, so a wave-level fence suffices).  No workgroup barrier, and small
+// enough (registers, LDS) for several waves per SIMD to overlap their pivot
+// chains.
+__global__ void __launch_bounds__(256) coarse_invert_2d(int n_cells_class, double *cinv)
+{
+  constexpr int N = kCoarseDofs, WPB = 256 / kWave;
+  __shared__ __align__(16) double srow[WPB][2][N + 1];
+  const int wave = threadIdx.x / kWave, i = threadIdx.x % kWave;
+  const int cc = blockIdx.x * WPB + wave;
+  if (cc >= n_cells_class)
+    return;
+  // lane i takes column i of A_c as its row (A_c is symmetric, up to the
+  // rounding of its assembly; the loads and stores are then coalesced) and
+  // stores its row of the inverse as column i: the result is the transpose of
+  // the inverse of A_c^T, i.e. A_c^-1 to rounding
+  double *m = cinv + (size_t)cc * N * N;
+  double a[N];
 #pragma unroll
-  for (int m = 0; m < EM; ++m)
-    {
-      const int e = t + kSetupThreads * m;
-      if (e < N * N)
-        out[e] = v[m];
-    }
+  for (int j = 0; j < N; ++j)
+    a[j] = i < N ? m[j * N + i] : 0.0;
+  gj_all(a, i, srow[wave], std::make_integer_sequence<int, N>{});
+  if (i < N)
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+      m[j * N + i] = a[j];
 }
 
 // ---- launch table -----------------------------------------------------------------
@@ -2080,6 +2167,7 @@ hipError_t launch_coarse_setup_2d(int n_cells_class, const int *cells, int nf, i
                        coef_kind, box, verts, coef, coef_off, cinv);
   else
     return hipErrorInvalidValue;
+  hipLaunchKernelGGL(coarse_invert_2d, dim3((n_cells_class + 3) / 4), dim3(256), 0, st, n_cells_class, cinv);
   return hipGetLastError();
 }
 
