@@ -1880,9 +1880,15 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
 {
   constexpr int H = NF / 8, NB = kCoarseSide + 1, N = kCoarseDofs;
   static_assert(H % 4 == 0 && NB * H == NF, "four threads per coarse block");
-  constexpr int SHN = NB * NB * 4 * 16 > N * N ? NB * NB * 4 * 16 : N * N;
-  __shared__ double sh[SHN];           // block partials [64][4][16], then A_c, inverted in place
+  constexpr int SHN = NB * NB * 4 * 10 > N * N ? NB * NB * 4 * 10 : N * N;
+  __shared__ double sh[SHN];           // block partials [64][4][10] (upper triangles), then A_c
   __shared__ double kb[NB * NB * 16];  // block matrices [64][4][4]
+  // 64^2 box cells with per-fine-cell scalar or tensor coefficients: K_e = ka
+  // Kxx + kc Kyy + kb Kxy (setup_element_2d), so W^T K_e W = ka Gxx + kc Gyy +
+  // kb Gxy with G the same for every cell, tabulated per fine position in the
+  // block: [component][ly H + lx][upper-triangle entry]
+  constexpr bool GT = NF == 64;
+  __shared__ double sgt[GT ? 3 : 1][GT ? H * H : 1][10];
   const int cc = blockIdx.x, t = threadIdx.x;
   if (cc >= n_cells_class)
     return;
@@ -1890,9 +1896,68 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
   const double *bx = verts ? nullptr : box + (size_t)cell * 4;
   const double *vq = verts ? verts + (size_t)cell * 8 : nullptr;
   const double *cf = coef + coef_off[cell];
+  // upper-triangle entry e of (m, n), m <= n: 0..3 row 0, 4..6 row 1, 7..8 row 2, 9
+  auto tri = [](int m, int n) { return m * 4 - m * (m - 1) / 2 + (n - m); };
+  const bool fast = GT && !vq && coef_kind <= 1; // (uniform)
+  if (fast)
+    {
+#pragma unroll 1
+      for (int e = t; e < 3 * H * H * 10; e += kSetupThreads)
+        {
+          const int ent = e % 10, pos = e / 10 % (H * H), comp = e / (10 * H * H);
+          const int m = ent < 4 ? 0 : ent < 7 ? 1 : ent < 9 ? 2 : 3, n = m + ent - tri(m, m);
+          const int lx = pos % H, ly = pos / H;
+          // block corner c's hat at vertex v of the fine cell
+          auto wt = [&](int v, int c) -> double {
+            const double px = (double)(lx + (v & 1)) / H, py = (double)(ly + (v >> 1)) / H;
+            return ((c & 1) ? px : 1.0 - px) * ((c & 2) ? py : 1.0 - py);
+          };
+          double g = 0;
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+              {
+                const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
+                const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
+                const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
+                const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
+                const double k = comp == 0 ? sx * ny : comp == 1 ? sy * nx : su * tv + tu * sv;
+                g += wt(u, m) * k * wt(v, n);
+              }
+          sgt[comp][pos][ent] = g;
+        }
+      __syncthreads();
+    }
   {
     const int b = t >> 2, part = t & 3, bxi = b % NB, byi = b / NB;
     double acc[4][4] = {};
+    if (fast)
+      {
+        const double rho = (bx[3] - bx[2]) / (bx[1] - bx[0]);
+        double at[10] = {};
+#pragma unroll 1
+        for (int ly = part * (H / 4); ly < (part + 1) * (H / 4); ++ly)
+#pragma unroll 2
+          for (int lx = 0; lx < H; ++lx)
+            {
+              const int f = (byi * H + ly) * NF + bxi * H + lx;
+              const double a = coef_kind == 0 ? cf[f] : cf[3 * f], bq = coef_kind == 0 ? 0.0 : cf[3 * f + 1];
+              const double c = coef_kind == 0 ? cf[f] : cf[3 * f + 2];
+              const double ka = a * rho * (1.0 / 6.0), kc = c / rho * (1.0 / 6.0), kbq = 0.25 * bq;
+              const double *g0 = sgt[0][ly * H + lx], *g1 = sgt[GT ? 1 : 0][ly * H + lx],
+                           *g2 = sgt[GT ? 2 : 0][ly * H + lx];
+#pragma unroll
+              for (int e = 0; e < 10; ++e)
+                at[e] += (ka * g0[e] + kc * g1[e]) + kbq * g2[e];
+            }
+#pragma unroll
+        for (int m = 0; m < 4; ++m)
+#pragma unroll
+          for (int n = m; n < 4; ++n)
+            acc[m][n] = at[tri(m, n)];
+      }
+    else
     for (int ly = part * (H / 4); ly < (part + 1) * (H / 4); ++ly)
       for (int lx = 0; lx < H; ++lx)
         {
@@ -1913,15 +1978,20 @@ __global__ void __launch_bounds__(kSetupThreads) coarse_setup_2d(int n_cells_cla
             for (int n = 0; n < 4; ++n)
               acc[m][n] += ((Wt[0][m] * Q[0][n] + Wt[1][m] * Q[1][n]) + Wt[2][m] * Q[2][n]) + Wt[3][m] * Q[3][n];
         }
-    for (int m = 0; m < 16; ++m)
-      sh[(b * 4 + part) * 16 + m] = acc[m >> 2][m & 3];
+#pragma unroll
+    for (int m = 0; m < 4; ++m)
+#pragma unroll
+      for (int n = m; n < 4; ++n)
+        sh[(b * 4 + part) * 10 + tri(m, n)] = acc[m][n];
   }
   __syncthreads();
+  // block matrices, symmetric: the upper triangle's sums, mirrored
   for (int e = t; e < NB * NB * 16; e += kSetupThreads)
     {
-      const int b = e >> 4, m = e & 15;
-      kb[e] = ((sh[(b * 4 + 0) * 16 + m] + sh[(b * 4 + 1) * 16 + m]) + sh[(b * 4 + 2) * 16 + m]) +
-              sh[(b * 4 + 3) * 16 + m];
+      const int b = e >> 4, m = (e >> 2) & 3, n = e & 3;
+      const int k = m <= n ? tri(m, n) : tri(n, m);
+      kb[e] = ((sh[(b * 4 + 0) * 10 + k] + sh[(b * 4 + 1) * 10 + k]) + sh[(b * 4 + 2) * 10 + k]) +
+              sh[(b * 4 + 3) * 10 + k];
     }
   __syncthreads();
   // A_c(I, J) = sum over the blocks holding both coarse nodes
