@@ -33,6 +33,13 @@ __device__ __forceinline__ int sym_index(int dim, int i, int j)
   return 5;
 }
 
+// Upper-triangle index of (i, j), i <= j, of an nv x nv symmetric matrix.
+template <int NVV>
+__device__ __forceinline__ constexpr int symi_of(int i, int j)
+{
+  return i * NVV - i * (i - 1) / 2 + (j - i);
+}
+
 // QUAD: general cells (a.verts: quadrilaterals, hexahedra), geometry per fine cell
 // QPK: the coefficient kind may be one given at the quadrature points; false
 // compiles only the per-fine-cell kinds (their Gauss sums are folded into unit
@@ -42,9 +49,12 @@ template <int DIM, bool QUAD, bool QPK = true>
 __global__ void __launch_bounds__(kElemThreads, DIM == 2 ? 2 : 1) element_kernel(ElementArgs a)
 {
   constexpr int NV = 1 << DIM, NQ = 1 << DIM, NS = DIM == 2 ? 3 : 6;
-  constexpr int NVAL = NV * NV + NV;
+  // A^K is symmetric: the running sums hold its upper triangle (NSYM entries,
+  // index symi(i, j), i <= j), then F^K
+  constexpr int NSYM = NV * (NV + 1) / 2, NVAL = NSYM + NV;
   __shared__ double sM[NS][NV * NV]; // unit element matrices of each tensor component
   __shared__ double sred[kElemThreads / kWave][NVAL];
+  auto symi = [](int i, int j) { return symi_of<NV>(i, j); };
   // f = prod_k sin(pi x_k) is separable: sin(pi x_k) at every fine cell's two
   // Gauss coordinates along each axis, tabulated once per cell (box cells)
   __shared__ double ssin[QUAD ? 1 : DIM][QUAD ? 1 : 2 * kElemMaxNF];
@@ -257,13 +267,13 @@ __global__ void __launch_bounds__(kElemThreads, DIM == 2 ? 2 : 1) element_kernel
               y[aa] = s;
             }
 #pragma unroll
-          for (int i = 0; i < NV; ++i)
+          for (int i = 0; i <= j; ++i)
             {
               double s = 0;
 #pragma unroll
               for (int aa = 0; aa < NV; ++aa)
                 s += pv[i][aa] * y[aa];
-              acc[i * NV + j] += s;
+              acc[symi(i, j)] += s;
             }
         }
 #pragma unroll
@@ -273,7 +283,7 @@ __global__ void __launch_bounds__(kElemThreads, DIM == 2 ? 2 : 1) element_kernel
 #pragma unroll
           for (int aa = 0; aa < NV; ++aa)
             s += pv[i][aa] * be[aa];
-          acc[NV * NV + i] += s;
+          acc[NSYM + i] += s;
         }
     }
 
@@ -286,11 +296,18 @@ __global__ void __launch_bounds__(kElemThreads, DIM == 2 ? 2 : 1) element_kernel
         sred[wave][v] = s;
     }
   __syncthreads();
-  for (int v = threadIdx.x; v < NVAL; v += blockDim.x)
+  for (int v = threadIdx.x; v < NV * NV + NV; v += blockDim.x)
     {
+      // entry (i, j) of A^K from the upper triangle, then F^K
+      int k = NSYM + v - NV * NV;
+      if (v < NV * NV)
+        {
+          const int i = v / NV, j = v % NV;
+          k = i <= j ? symi(i, j) : symi(j, i);
+        }
       double s = 0;
       for (int w = 0; w < kElemThreads / kWave; ++w)
-        s += sred[w][v];
+        s += sred[w][k];
       if (v < NV * NV)
         a.Ae[(size_t)cell * NV * NV + v] = s;
       else
