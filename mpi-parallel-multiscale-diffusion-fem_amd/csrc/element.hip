@@ -469,6 +469,178 @@ __global__ void __launch_bounds__(kElemThreads) element3d_kernel(ElementArgs a)
     }
 }
 
+// 2D box cells of 64^2 fine cells with per-fine-cell scalar (KIND 0) or
+// tensor (KIND 1) coefficients, the same A^K and F^K as element_kernel: one
+// wave per cell, lane = fine column fx sweeping the 64 rows, so the bases at
+// a row of nodes are loaded once (two nodes per lane) and kept for the next
+// row, instead of four times per node.  K_e is the exact QGauss(2) box
+// element matrix ka Kxx + kc Kyy + kb Kxy (ka = a rho / 6, kc = c / (6 rho),
+// kb = b / 4, rho = h_y / h_x; setup_element_2d's form), the load vector
+// sum_q psi_a(q) f(q) jxw.  The upper triangle of A^K and F^K are summed per
+// lane over its column, then over the wave in a fixed order.
+template <int KIND>
+__global__ void __launch_bounds__(256) element_rows_kernel(ElementArgs a)
+{
+  constexpr int NF = 64, NP = NF + 1, NV = 4, NSYM = 10, NVAL = NSYM + NV;
+  __shared__ double ssy[4][2 * NF]; // per wave: sin(pi y) at the two Gauss rows of every fine row
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int cc = blockIdx.x * 4 + wave;
+  if (cc >= a.n_cells_class)
+    return;
+  const int cell = a.cells[cc];
+  const double *bx = a.box + (size_t)cell * 4;
+  const double hx = (bx[1] - bx[0]) / NF, hy = (bx[3] - bx[2]) / NF, rho = hy / hx;
+  const double jxw = hx * hy / 4.0, gs = 0.5 / sqrt(3.0);
+  const double *cf = a.coef + a.coef_off[cell];
+  const double *phi = a.bases + a.bas_off[cell];
+  constexpr size_t nn = (size_t)NP * NP;
+  const int fx = lane;
+  // sin-sin load: the column's two Gauss x-points, the rows' from the table
+  double sxq[2] = {1.0, 1.0};
+  if (a.rhs_kind == 1)
+    {
+      for (int k = 0; k < 2; ++k)
+        {
+          const double xi = k ? 0.5 + gs : 0.5 - gs;
+          sxq[k] = sin(M_PI * (bx[0] + (bx[1] - bx[0]) * ((double)fx / NF) + hx * xi));
+          ssy[wave][2 * lane + k] = sin(M_PI * (bx[2] + (bx[3] - bx[2]) * ((double)lane / NF) + hy * xi));
+        }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  auto shape = [&](int aa, int q) -> double { // psi_a at Gauss point q (bit 0: x, bit 1: y)
+    const double xi = (q & 1) ? 0.5 + gs : 0.5 - gs, eta = (q & 2) ? 0.5 + gs : 0.5 - gs;
+    return ((aa & 1) ? xi : 1.0 - xi) * ((aa & 2) ? eta : 1.0 - eta);
+  };
+  double acc[NVAL];
+#pragma unroll
+  for (int v = 0; v < NVAL; ++v)
+    acc[v] = 0;
+  // bases at the nodes (fx, fy) and (fx + 1, fy) of the current bottom row
+  double bl[NV], br[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i)
+    {
+      bl[i] = phi[(size_t)i * nn + fx];
+      br[i] = phi[(size_t)i * nn + fx + 1];
+    }
+#pragma unroll 1
+  for (int fy = 0; fy < NF; ++fy)
+    {
+      double tl[NV], tr[NV];
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        {
+          tl[i] = phi[(size_t)i * nn + (size_t)(fy + 1) * NP + fx];
+          tr[i] = phi[(size_t)i * nn + (size_t)(fy + 1) * NP + fx + 1];
+        }
+      const int f = fy * NF + fx;
+      double ka, kc, kb = 0.0;
+      if constexpr (KIND == 0)
+        {
+          const double al = cf[f];
+          ka = al * rho * (1.0 / 6.0);
+          kc = al / rho * (1.0 / 6.0);
+        }
+      else
+        {
+          ka = cf[3 * f] * rho * (1.0 / 6.0);
+          kc = cf[3 * f + 2] / rho * (1.0 / 6.0);
+          kb = 0.25 * cf[3 * f + 1];
+        }
+      // K_e[u][v] (vertex u = (u & 1, u >> 1))
+      auto Ke = [&](int u, int v) -> double {
+        const double sx = ((u ^ v) & 1) ? -1.0 : 1.0, sy = ((u ^ v) & 2) ? -1.0 : 1.0;
+        const double ny = ((u ^ v) & 2) ? 1.0 : 2.0, nx = ((u ^ v) & 1) ? 1.0 : 2.0;
+        const double su = (u & 1) ? 1.0 : -1.0, tu = (u & 2) ? 1.0 : -1.0;
+        const double sv = (v & 1) ? 1.0 : -1.0, tv = (v & 2) ? 1.0 : -1.0;
+        return ka * (sx * ny) + kc * (sy * nx) + (KIND == 0 ? 0.0 : kb * (su * tv + tu * sv));
+      };
+      double pv[NV][NV]; // [basis][vertex]
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        {
+          pv[i][0] = bl[i];
+          pv[i][1] = br[i];
+          pv[i][2] = tl[i];
+          pv[i][3] = tr[i];
+        }
+#pragma unroll
+      for (int j = 0; j < NV; ++j)
+        {
+          double y[NV];
+#pragma unroll
+          for (int u = 0; u < NV; ++u)
+            {
+              double t = 0;
+#pragma unroll
+              for (int v = 0; v < NV; ++v)
+                t += Ke(u, v) * pv[j][v];
+              y[u] = t;
+            }
+#pragma unroll
+          for (int i = 0; i <= j; ++i)
+            {
+              double t = 0;
+#pragma unroll
+              for (int u = 0; u < NV; ++u)
+                t += pv[i][u] * y[u];
+              acc[symi_of<NV>(i, j)] += t;
+            }
+        }
+      // load vector of the fine cell
+      double be[NV] = {0, 0, 0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        {
+          const double fv = a.rhs_kind == 1 ? sxq[q & 1] * ssy[wave][2 * fy + (q >> 1)] : a.rhs_value;
+#pragma unroll
+          for (int u = 0; u < NV; ++u)
+            be[u] += shape(u, q) * fv * jxw;
+        }
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        {
+          double t = 0;
+#pragma unroll
+          for (int u = 0; u < NV; ++u)
+            t += pv[i][u] * be[u];
+          acc[NSYM + i] += t;
+        }
+#pragma unroll
+      for (int i = 0; i < NV; ++i)
+        {
+          bl[i] = tl[i];
+          br[i] = tr[i];
+        }
+    }
+#pragma unroll
+  for (int v = 0; v < NVAL; ++v)
+    acc[v] = group_sum<kWave>(acc[v]);
+  if (lane < NV * NV + NV)
+    {
+      // lane v writes A^K entry v (from the upper triangle) or F^K entry v - 16
+      double out = 0;
+#pragma unroll
+      for (int v = 0; v < NV * NV + NV; ++v)
+        if (lane == v)
+          {
+            int k = NSYM + v - NV * NV;
+            if (v < NV * NV)
+              {
+                const int i = v / NV, j = v % NV;
+                k = i <= j ? symi_of<NV>(i, j) : symi_of<NV>(j, i);
+              }
+            out = acc[k];
+          }
+      if (lane < NV * NV)
+        a.Ae[(size_t)cell * NV * NV + lane] = out;
+      else
+        a.Fe[(size_t)cell * NV + lane - NV * NV] = out;
+    }
+}
+
 __global__ void pou_fixup_kernel(int n_cells_class, const int *cells, int nb, size_t nn, double *bases,
                                  const long long *bas_off)
 {
@@ -495,6 +667,13 @@ hipError_t launch_element(const ElementArgs &a, hipStream_t st)
     hipLaunchKernelGGL((element_kernel<2, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.verts) // general hexahedra
     hipLaunchKernelGGL((element_kernel<3, true>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
+  else if (a.dim == 2 && a.coef_kind <= 1 && a.nf == 64 && a.nb == 4) // per-fine-cell coefficient, 64^2
+    {
+      if (a.coef_kind == 0)
+        hipLaunchKernelGGL((element_rows_kernel<0>), dim3((a.n_cells_class + 3) / 4), dim3(256), 0, st, a);
+      else
+        hipLaunchKernelGGL((element_rows_kernel<1>), dim3((a.n_cells_class + 3) / 4), dim3(256), 0, st, a);
+    }
   else if (a.dim == 2 && a.coef_kind <= 1) // per-fine-cell coefficient
     hipLaunchKernelGGL((element_kernel<2, false, false>), dim3(a.n_cells_class), dim3(kElemThreads), 0, st, a);
   else if (a.dim == 2)
