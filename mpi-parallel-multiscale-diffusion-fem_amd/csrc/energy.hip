@@ -142,27 +142,42 @@ __device__ __forceinline__ void block_reduce_n(double (&v)[NV], double (*red)[kE
     v[k] = red[k][0];
 }
 
-// One workgroup per cell of a size class: lam_lb[cell] and, when tau > 0,
-// ptol[cell][i] = tau sqrt(lam_lb e_lb_i).
+// TPC threads per cell of a size class (kEnergyThreads: one workgroup; kWave:
+// one wave, 4 cells per workgroup, the reductions inside the wave and no
+// workgroup barrier): lam_lb[cell] and, when tau > 0, ptol[cell][i] = tau
+// sqrt(lam_lb e_lb_i).
 // (resolutions up to kEnergyMaxNF per side tabulate the interval integrals
 // and the mode cosines once per cell instead of per fine cell and mode: the
 // divisions and cospi calls were most of the kernel's time)
 constexpr int kEnergyMaxNF = 1024;
 
-template <int DIM>
+template <int DIM, int TPC>
 __global__ void __launch_bounds__(kEnergyThreads)
   energy_terms_kernel(int n_cells_class, const int *cells, int nf, int kind, const double *box, const double *coef,
                       const long long *coef_off, double tau, double *lam_lb, double *ptol)
 {
-  constexpr int NB = 1 << DIM;
-  __shared__ double redn[2 + NB][kEnergyThreads];
-  // dynamic: [jint2 bit 0, jint2 bit 1, jint1 bit 0, jint1 bit 1][nf], then
-  // the mode factors k_a(j), m_a(j) per axis [DIM][2][nf] (j = 1 .. nf - 1);
-  // none beyond kEnergyMaxNF
-  extern __shared__ double stab[];
-  const int k = blockIdx.x;
+  constexpr int NB = 1 << DIM, CPB = kEnergyThreads / TPC;
+  static_assert(TPC == kEnergyThreads || TPC == kWave, "a workgroup or a wave per cell");
+  __shared__ double redn[TPC == kEnergyThreads ? 2 + NB : 1][kEnergyThreads];
+  // dynamic, per cell of the workgroup: [jint2 bit 0, jint2 bit 1, jint1 bit 0,
+  // jint1 bit 1][nf], then the mode factors k_a(j), m_a(j) per axis [DIM][2][nf]
+  // (j = 1 .. nf - 1); none beyond kEnergyMaxNF
+  extern __shared__ double stab_all[];
+  const int tid = threadIdx.x % TPC, slot = threadIdx.x / TPC;
+  const int k = blockIdx.x * CPB + slot;
   if (k >= n_cells_class)
     return;
+  double *stab = stab_all + (size_t)slot * (4 + 2 * DIM) * nf;
+  auto sync = [&]() {
+    if constexpr (TPC == kEnergyThreads)
+      __syncthreads();
+    else
+      {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+  };
   const int cell = cells[k];
   const double *bx = box + (size_t)cell * 2 * DIM;
   const double *cf = coef + coef_off[cell];
@@ -183,17 +198,17 @@ __global__ void __launch_bounds__(kEnergyThreads)
   auto mfac = [&](int a, int j) -> double { return ha(a) * (4.0 + 2.0 * cospi((double)j / nf)) / 6.0; };
   if (tab)
     {
-      for (int t = threadIdx.x; t < 4 * nf; t += kEnergyThreads)
+      for (int t = tid; t < 4 * nf; t += TPC)
         {
           const int w = t / nf, j = t % nf;
           stab[t] = w < 2 ? jint2(w, j, nf) : jint1(w - 2, j, nf);
         }
-      for (int t = threadIdx.x; t < DIM * 2 * nf; t += kEnergyThreads)
+      for (int t = tid; t < DIM * 2 * nf; t += TPC)
         {
           const int a = t / (2 * nf), km = t / nf % 2, j = t % nf;
           skm[t] = j == 0 ? 0.0 : km == 0 ? kfac(a, j) : mfac(a, j);
         }
-      __syncthreads();
+      sync();
     }
   auto J2 = [&](int bit, int j) -> double { return tab ? stab[bit * nf + j] : jint2(bit, j, nf); };
   auto J1 = [&](int bit, int j) -> double { return tab ? stab[(2 + bit) * nf + j] : jint1(bit, j, nf); };
@@ -216,7 +231,7 @@ __global__ void __launch_bounds__(kEnergyThreads)
 #pragma unroll
   for (int i = 0; i < NB; ++i)
     D[i] = 0;
-  for (int f = threadIdx.x; f < nfd; f += kEnergyThreads)
+  for (int f = tid; f < nfd; f += TPC)
     {
       double B[3];
       lam = fmin(lam, cell_lambda<DIM>(kind, cf, f, B));
@@ -257,7 +272,7 @@ __global__ void __launch_bounds__(kEnergyThreads)
   double mu = 1e300;
   const int nm = nf - 1;
   const int nmodes = DIM == 2 ? nm * nm : nm * nm * nm;
-  for (int q = threadIdx.x; q < nmodes; q += kEnergyThreads)
+  for (int q = tid; q < nmodes; q += TPC)
     {
       const int jm[3] = {q % nm + 1, (q / nm) % nm + 1, DIM == 3 ? q / (nm * nm) + 1 : 1};
       double kk[3], mm[3];
@@ -274,20 +289,35 @@ __global__ void __launch_bounds__(kEnergyThreads)
         v = kk[0] * mm[1] * mm[2] + mm[0] * kk[1] * mm[2] + mm[0] * mm[1] * kk[2];
       mu = fmin(mu, v);
     }
-  // one fixed-order tree over the workgroup for all 2 + NB values (min, min, sums)
-  double vals[2 + NB] = {lam, mu};
-#pragma unroll
-  for (int i = 0; i < NB; ++i)
-    vals[2 + i] = D[i];
-  block_reduce_n<2 + NB>(vals, redn);
-  lam = vals[0];
-  mu = vals[1];
   double Dsum[NB];
+  if constexpr (TPC == kEnergyThreads)
+    {
+      // one fixed-order tree over the workgroup for all 2 + NB values (min, min, sums)
+      double vals[2 + NB] = {lam, mu};
 #pragma unroll
-  for (int i = 0; i < NB; ++i)
-    Dsum[i] = vals[2 + i];
+      for (int i = 0; i < NB; ++i)
+        vals[2 + i] = D[i];
+      block_reduce_n<2 + NB>(vals, redn);
+      lam = vals[0];
+      mu = vals[1];
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        Dsum[i] = vals[2 + i];
+    }
+  else
+    {
+      // over the wave: butterfly minima (exact), fixed-order DPP sums
+      for (int off = kWave / 2; off > 0; off >>= 1)
+        {
+          lam = fmin(lam, __shfl_xor(lam, off, kWave));
+          mu = fmin(mu, __shfl_xor(mu, off, kWave));
+        }
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        Dsum[i] = group_sum<kWave>(D[i]);
+    }
   const double llb = lam * mu;
-  if (threadIdx.x == 0)
+  if (tid == 0)
     {
       lam_lb[cell] = llb;
       if (ptol)
@@ -376,12 +406,23 @@ hipError_t launch_energy_terms(int dim, int n_cells_class, const int *cells, int
   if (n_cells_class == 0)
     return hipSuccess;
   const size_t lds = nf <= kEnergyMaxNF ? (size_t)(4 + 2 * dim) * nf * sizeof(double) : 0;
-  if (dim == 2)
-    hipLaunchKernelGGL(energy_terms_kernel<2>, dim3(n_cells_class), dim3(kEnergyThreads), lds, st, n_cells_class,
+  // up to 128 fine cells per side one wave per cell (4 cells per workgroup,
+  // their tables side by side), beyond that a workgroup per cell
+  const bool wave = nf <= 128;
+  const unsigned grid = wave ? (unsigned)((n_cells_class + 3) / 4) : (unsigned)n_cells_class;
+  const size_t wl = wave ? 4 * lds : lds;
+  if (dim == 2 && wave)
+    hipLaunchKernelGGL((energy_terms_kernel<2, kWave>), dim3(grid), dim3(kEnergyThreads), wl, st, n_cells_class,
+                       cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
+  else if (dim == 2)
+    hipLaunchKernelGGL((energy_terms_kernel<2, kEnergyThreads>), dim3(grid), dim3(kEnergyThreads), wl, st,
+                       n_cells_class, cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
+  else if (wave)
+    hipLaunchKernelGGL((energy_terms_kernel<3, kWave>), dim3(grid), dim3(kEnergyThreads), wl, st, n_cells_class,
                        cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
   else
-    hipLaunchKernelGGL(energy_terms_kernel<3>, dim3(n_cells_class), dim3(kEnergyThreads), lds, st, n_cells_class,
-                       cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
+    hipLaunchKernelGGL((energy_terms_kernel<3, kEnergyThreads>), dim3(grid), dim3(kEnergyThreads), wl, st,
+                       n_cells_class, cells, nf, coef_kind, box, coef, coef_off, tau, lam_lb, ptol);
   return hipGetLastError();
 }
 
