@@ -231,43 +231,79 @@ __global__ void __launch_bounds__(kEnergyThreads)
 #pragma unroll
   for (int i = 0; i < NB; ++i)
     D[i] = 0;
-  for (int f = tid; f < nfd; f += TPC)
+  if constexpr (DIM == 2)
     {
-      double B[3];
-      lam = fmin(lam, cell_lambda<DIM>(kind, cf, f, B));
-      const int j[3] = {f % nf, (f / nf) % nf, DIM == 3 ? f / (nf * nf) : 0};
+      // D_i = sum_f B0 cg0 J2(by, jy) + B2 cg1 J2(bx, jx) + 2 B1 sx sy J1(bx, jx) J1(by, jy)
+      // (bit bx, by of basis i; sx, sy = +-1 by bit): separable, so the fine
+      // cells only accumulate P[b] = sum B0 J2(b, jy), Q[b] = sum B2 J2(b, jx)
+      // and R[bx][by] = sum B1 J1(bx, jx) J1(by, jy)
+      double P[2] = {0, 0}, Q[2] = {0, 0}, Rm[2][2] = {{0, 0}, {0, 0}};
+      for (int f = tid; f < nfd; f += TPC)
+        {
+          double B[3];
+          lam = fmin(lam, cell_lambda<DIM>(kind, cf, f, B));
+          const int jx = f % nf, jy = f / nf;
+#pragma unroll
+          for (int b = 0; b < 2; ++b)
+            {
+              P[b] += B[0] * J2(b, jy);
+              Q[b] += B[2] * J2(b, jx);
+            }
+          if (B[1] != 0.0)
+            {
+              const double x0 = J1(0, jx), x1 = J1(1, jx), y0 = J1(0, jy), y1 = J1(1, jy);
+              Rm[0][0] += B[1] * (x0 * y0);
+              Rm[0][1] += B[1] * (x0 * y1);
+              Rm[1][0] += B[1] * (x1 * y0);
+              Rm[1][1] += B[1] * (x1 * y1);
+            }
+        }
 #pragma unroll
       for (int i = 0; i < NB; ++i)
         {
-          int bit[3];
-          double s[3], j2[3], j1[3];
-#pragma unroll
-          for (int a = 0; a < DIM; ++a)
-            {
-              bit[a] = (i >> a) & 1;
-              s[a] = bit[a] ? 1.0 : -1.0;
-              j2[a] = J2(bit[a], j[a]);
-              j1[a] = J1(bit[a], j[a]);
-            }
-          // G[k][m] = int over the fine cell of d_k N_i d_m N_i
-          double g = 0;
-          if (DIM == 2)
-            {
-              const double gxx = cg[0] * j2[1], gyy = cg[1] * j2[0];
-              const double gxy = s[0] * s[1] * j1[0] * j1[1];
-              g = B[0] * gxx + 2.0 * B[1] * gxy + B[2] * gyy;
-            }
-          else
-            {
-              // scalar coefficients only in 3D: A^-1 = I / lam
-              const double gxx = cg[0] * j2[1] * j2[2];
-              const double gyy = cg[1] * j2[0] * j2[2];
-              const double gzz = cg[2] * j2[0] * j2[1];
-              g = B[0] * (gxx + gyy + gzz);
-            }
-          D[i] += g;
+          const int bxi = i & 1, byi = (i >> 1) & 1;
+          const double sxy = (bxi == byi) ? 1.0 : -1.0;
+          D[i] = cg[0] * P[byi] + cg[1] * Q[bxi] + 2.0 * sxy * Rm[bxi][byi];
         }
     }
+  else
+    for (int f = tid; f < nfd; f += TPC)
+      {
+        double B[3];
+        lam = fmin(lam, cell_lambda<DIM>(kind, cf, f, B));
+        const int j[3] = {f % nf, (f / nf) % nf, DIM == 3 ? f / (nf * nf) : 0};
+  #pragma unroll
+        for (int i = 0; i < NB; ++i)
+          {
+            int bit[3];
+            double s[3], j2[3], j1[3];
+  #pragma unroll
+            for (int a = 0; a < DIM; ++a)
+              {
+                bit[a] = (i >> a) & 1;
+                s[a] = bit[a] ? 1.0 : -1.0;
+                j2[a] = J2(bit[a], j[a]);
+                j1[a] = J1(bit[a], j[a]);
+              }
+            // G[k][m] = int over the fine cell of d_k N_i d_m N_i
+            double g = 0;
+            if (DIM == 2)
+              {
+                const double gxx = cg[0] * j2[1], gyy = cg[1] * j2[0];
+                const double gxy = s[0] * s[1] * j1[0] * j1[1];
+                g = B[0] * gxx + 2.0 * B[1] * gxy + B[2] * gyy;
+              }
+            else
+              {
+                // scalar coefficients only in 3D: A^-1 = I / lam
+                const double gxx = cg[0] * j2[1] * j2[2];
+                const double gyy = cg[1] * j2[0] * j2[2];
+                const double gzz = cg[2] * j2[0] * j2[1];
+                g = B[0] * (gxx + gyy + gzz);
+              }
+            D[i] += g;
+          }
+      }
   // mu_min over the separable modes j = 1 .. nf - 1 per axis
   double mu = 1e300;
   const int nm = nf - 1;
